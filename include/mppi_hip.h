@@ -1,0 +1,172 @@
+/* mppi_hip.h — C ABI of the MI355X MPPI backend.
+ *
+ * This is the seam that replaces the reference's MetalBackend class
+ * (src/metal_backend.hpp:22-108), one entry point per method, with plain
+ * pointers and sizes so it can be bound from C, C++ (the MPPIDriver facade in
+ * include/mppi_metal/mppi_driver.hpp), Python ctypes or any other FFI.
+ *
+ * Conventions
+ *   - Every function returns an int: MPPI_HIP_OK (0) or a negative
+ *     mppi_hip_status.  The message of the last failure on a context is
+ *     returned by mppi_hip_last_error(ctx); failures before a context exists
+ *     (mppi_hip_create) are reported by mppi_hip_last_error(NULL), which is
+ *     per calling thread.
+ *   - All host pointers are caller-owned; nothing is retained after a call
+ *     returns (parameter blobs are copied at bind time).
+ *   - A context owns one HIP stream on one device.  Calls on one context are
+ *     not thread-safe; distinct contexts may be used from distinct threads
+ *     (the reference's concurrency model: one driver per host thread).
+ *   - Calls that return results (step, simulate, simulate_batch) block until
+ *     the device work has finished, like the reference's waitUntilCompleted
+ *     (src/metal_backend.mm:719-720).
+ */
+#ifndef MPPI_HIP_H
+#define MPPI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum mppi_hip_status {
+    MPPI_HIP_OK = 0,
+    MPPI_HIP_E_INVALID = -1,          /* bad argument, shape or size */
+    MPPI_HIP_E_NOT_INITIALIZED = -2,  /* no plugin linked yet */
+    MPPI_HIP_E_PLUGIN = -3,           /* plugin source missing or callable not found */
+    MPPI_HIP_E_COMPILE = -4,          /* hipRTC compilation or module load failed */
+    MPPI_HIP_E_DEVICE = -5,           /* HIP runtime error (no device, launch failure) */
+    MPPI_HIP_E_NO_MEMORY = -6         /* device or pinned allocation failed */
+};
+
+enum mppi_hip_covariance { MPPI_HIP_COV_DIAGONAL = 0, MPPI_HIP_COV_FULL = 1 };
+enum mppi_hip_numeric_mode { MPPI_HIP_FP32 = 0, MPPI_HIP_FP16 = 1, MPPI_HIP_MIXED = 2 };
+
+typedef struct mppi_hip_ctx mppi_hip_ctx;
+
+/* Problem dimensions and noise model; the fields of the reference's
+ * DriverConfig (include/mppi_metal/mppi_types.hpp:46-62). */
+typedef struct mppi_hip_config {
+    uint32_t state_dim;
+    uint32_t control_dim;
+    uint32_t horizon;
+    uint32_t sample_count;
+    float lambda;
+    uint32_t covariance_form;   /* mppi_hip_covariance */
+    /* DIAGONAL: control_dim standard deviations (epsilon_d = sigma_d * z_d).
+     * FULL: control_dim x control_dim covariance, row-major, symmetric
+     * positive definite; epsilon = L z with L the lower Cholesky factor. */
+    const float* sigma;
+    const float* u_min;         /* control_dim */
+    const float* u_max;         /* control_dim */
+    uint32_t numeric_mode;      /* mppi_hip_numeric_mode; only FP32 is implemented */
+    int32_t device;             /* HIP device ordinal; -1 = current device */
+} mppi_hip_config;
+
+/* Create a context: copies the configuration, selects the device and creates
+ * the stream.  Replaces MetalBackend construction plus the device part of
+ * MetalBackend::initialize (src/metal_backend.mm:225-231). */
+int mppi_hip_create(const mppi_hip_config* config, mppi_hip_ctx** out_ctx);
+
+/* Release all device resources of a context.  Replaces ~MetalBackend and
+ * MetalBackend::reset (src/metal_backend.mm:1130). */
+void mppi_hip_destroy(mppi_hip_ctx* ctx);
+
+/* Message of the last failure on ctx (or of the calling thread if ctx is
+ * NULL).  Valid until the next call on the same context / thread. */
+const char* mppi_hip_last_error(const mppi_hip_ctx* ctx);
+
+/* Compile the plugin at plugin_path with the library kernels through hipRTC
+ * and load the code object.  dynamics is required; stage_cost and
+ * terminal_cost may be NULL or "" (the library's zero costs are used).
+ * model_bytes / cost_bytes are the ParamLayout sizes; seed is the 64-bit
+ * RngConfig::seed (all 64 bits are used).
+ * Replaces MetalBackend::initialize (src/metal_backend.mm:225-568): library
+ * and user metallib loading, callable lookup, MTLLinkedFunctions and the
+ * visible function tables. */
+int mppi_hip_link_plugin(mppi_hip_ctx* ctx, const char* plugin_path, const char* dynamics,
+                         const char* stage_cost, const char* terminal_cost,
+                         size_t model_bytes, size_t cost_bytes, uint64_t seed);
+
+/* Same as mppi_hip_link_plugin, with the plugin given as source text;
+ * source_name is used in diagnostics.  No reference counterpart (the
+ * reference only loads plugins from files); used by callers that generate
+ * plugins. */
+int mppi_hip_link_plugin_source(mppi_hip_ctx* ctx, const char* source, const char* source_name,
+                                const char* dynamics, const char* stage_cost,
+                                const char* terminal_cost, size_t model_bytes,
+                                size_t cost_bytes, uint64_t seed);
+
+/* Copy the model / cost parameter blobs to the device for later steps.
+ * Sizes must equal the linked layout (0 means "none").  Replaces
+ * MetalBackend::bind_params (src/metal_backend.mm:574-598). */
+int mppi_hip_bind_params(mppi_hip_ctx* ctx, const void* model, size_t model_bytes,
+                         const void* cost, size_t cost_bytes);
+
+/* One blocking MPPI iteration: rollout, softmax weights, weighted-noise
+ * reduce.  x0: state_dim floats; u_nominal: horizon*control_dim floats;
+ * overrides with bytes == 0 use the bound blobs; u_out: horizon*control_dim
+ * floats; diag3 (may be NULL): best cost, mean cost, effective sample size.
+ * step_index selects the noise stream (low 32 bits are used).
+ * Replaces MetalBackend::dispatch_rollout (src/metal_backend.mm:604-741). */
+int mppi_hip_step(mppi_hip_ctx* ctx, uint64_t step_index, const float* x0,
+                  const float* u_nominal, const void* model_override, size_t model_override_bytes,
+                  const void* cost_override, size_t cost_override_bytes, float* u_out,
+                  float* diag3);
+
+/* num_steps closed-loop steps on the device: each step runs the MPPI
+ * iteration, logs (state, first optimal control, best cost), steps the plant
+ * with the plugin dynamics and shifts the warm start (hold last).  The noise
+ * stream of step i is start_step + i.  Output pointers may be NULL.
+ * states_out: num_steps*state_dim, controls_out: num_steps*control_dim,
+ * costs_out: num_steps.
+ * Replaces MetalBackend::dispatch_simulation (src/metal_backend.mm:747-929). */
+int mppi_hip_simulate(mppi_hip_ctx* ctx, uint32_t num_steps, uint64_t start_step,
+                      const float* x0, const float* u_nominal, const void* model_override,
+                      size_t model_override_bytes, const void* cost_override,
+                      size_t cost_override_bytes, float* states_out, float* controls_out,
+                      float* costs_out);
+
+/* num_agents independent closed loops sharing the linked plugin, the bound
+ * parameters, K and H.  x0_packed: num_agents*state_dim; u_nominal_packed:
+ * num_agents*horizon*control_dim, or NULL for all zeros.  Outputs (each may be
+ * NULL) are indexed [step][agent][element]:
+ * states_out num_steps*num_agents*state_dim, controls_out
+ * num_steps*num_agents*control_dim, costs_out num_steps*num_agents;
+ * terminal_out num_agents*state_dim is the state after the last plant step.
+ * Agent a draws its noise from counter word 3 = agent a (the key is the seed
+ * for every agent).
+ * Replaces MetalBackend::dispatch_batch_simulation
+ * (src/metal_backend.mm:935-1124). */
+int mppi_hip_simulate_batch(mppi_hip_ctx* ctx, uint32_t num_agents, uint32_t num_steps,
+                            uint64_t start_step, const float* x0_packed,
+                            const float* u_nominal_packed, float* states_out,
+                            float* controls_out, float* costs_out, float* terminal_out);
+
+/* Inspection of the last iteration for agent `agent` (0 for step/simulate):
+ * its K sample costs, or its K normalised weights.  No reference counterpart
+ * (the reference keeps these in private Metal buffers); used by parity tests. */
+int mppi_hip_read_costs(mppi_hip_ctx* ctx, uint32_t agent, float* out, size_t count);
+int mppi_hip_read_weights(mppi_hip_ctx* ctx, uint32_t agent, float* out, size_t count);
+
+/* Launch-shape knobs of the reduce kernel (samples per thread); 0 restores
+ * the built-in choice.  No reference counterpart; used by the autotuner. */
+int mppi_hip_set_reduce_samples_per_thread(mppi_hip_ctx* ctx, uint32_t spt);
+
+/* Device timing: when enabled, every rollout and reduce launch is bracketed by
+ * HIP events on the context's stream and their durations are summed.
+ * mppi_hip_profile_read returns the sums in milliseconds and the launch count
+ * and clears them.  No reference counterpart. */
+int mppi_hip_profile_enable(mppi_hip_ctx* ctx, int enable);
+int mppi_hip_profile_read(mppi_hip_ctx* ctx, double* rollout_ms, double* reduce_ms,
+                          uint64_t* launches);
+
+/* Library version string. */
+const char* mppi_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPPI_HIP_H */

@@ -1,0 +1,241 @@
+// mppi_math.h — single-precision elementary functions used identically by the
+// GPU rollout (compiled by hipRTC/hipcc for gfx950) and by the CPU oracle
+// (compiled by g++).
+//
+// Why these exist: the MPPI weights are exp(-(c - min)/lambda), so a cost that
+// differs by one ulp between two builds changes the weights.  Vendor libm
+// (glibc on the host, OCML on the GPU) give different last-bit results, so
+// every transcendental on the rollout path is written here from +, -, *, fma,
+// correctly rounded / and sqrt, comparisons and bit manipulation only.  Each of
+// those operations is IEEE-754 exact-rounded on both targets, so the same
+// source gives the same bits on both, provided floating-point contraction is
+// off (-ffp-contract=off on both compilers; the library passes it to hipRTC).
+//
+// Methods (all public, textbook):
+//   exp   : Cody-Waite reduction x = k*ln2 + r with ln2 split in two parts,
+//           |r| <= ln2/2, Taylor polynomial of degree 7, scaling by 2^k.
+//   log   : x = 2^e * m, m in [sqrt(1/2), sqrt(2)); log(m) = 2*atanh(s) with
+//           s = (m-1)/(m+1), odd series to s^11.
+//   sin/cos: Cody-Waite reduction by pi/2 with a three-part pi/2 (the fdlibm
+//           split), Taylor polynomials on [-pi/4, pi/4], quadrant selection.
+//   sincospi: exact reduction of x to [-1/4, 1/4] (multiples of 1/2 are exact
+//           in binary), then the same polynomials.
+//   fmod  : exact long division by repeated subtraction of y*2^j (each step
+//           is exact by Sterbenz's lemma), as IEEE fmod requires.
+//   tanh  : via exp for |x| >= 0.55 and an odd Taylor series below.
+// Accuracy against FP64 libm is measured in tests/test_oracle_math.py.
+#ifndef MPPI_HIP_MATH_H
+#define MPPI_HIP_MATH_H
+
+#if defined(__HIPCC_RTC__) || defined(__HIPCC__) || defined(__HIP__)
+#define MPPI_HD __host__ __device__ inline
+#else
+#define MPPI_HD inline
+#endif
+
+#if !defined(__HIPCC_RTC__)
+#include <stdint.h>
+#endif
+
+#if defined(__clang__)
+#define MPPI_UNROLL _Pragma("unroll")
+#else
+#define MPPI_UNROLL
+#endif
+
+MPPI_HD uint32_t mppi_f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
+MPPI_HD float mppi_u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
+
+MPPI_HD float mppi_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+MPPI_HD float mppi_sqrtf(float x) { return __builtin_sqrtf(x); }
+MPPI_HD float mppi_fabsf(float x) { return mppi_u2f(mppi_f2u(x) & 0x7fffffffu); }
+
+// Clamp with explicit comparisons (NaN passes through unchanged on both
+// targets, unlike min/max instructions whose NaN rules differ).
+MPPI_HD float mppi_clampf(float x, float lo, float hi) {
+    return x < lo ? lo : (x > hi ? hi : x);
+}
+
+// Round to nearest integer (ties to even) for |x| < 2^22.
+MPPI_HD float mppi_rintf(float x) {
+    const float magic = 12582912.0f;  // 1.5 * 2^23
+    float t = x + magic;
+    return t - magic;
+}
+
+// 2^k as a float for k in [-126, 127].
+MPPI_HD float mppi_pow2i(int k) { return mppi_u2f(uint32_t(k + 127) << 23); }
+
+// ---------------------------------------------------------------------------
+// exp
+// ---------------------------------------------------------------------------
+MPPI_HD float mppi_expf(float x) {
+    if (!(x == x)) return x;                       // NaN
+    if (x > 88.72283935546875f) return mppi_u2f(0x7f800000u);  // +inf
+    if (x < -103.972084045410156f) return 0.0f;    // below the smallest subnormal
+    const float log2e = 1.44269502162933349609375f;
+    const float ln2_hi = 0.693145751953125f;       // 0x3f317200: 16 trailing zero bits
+    const float ln2_lo = 1.428606765330187045e-06f;
+    float kf = mppi_rintf(x * log2e);
+    float r = mppi_fma(-kf, ln2_hi, x);
+    r = mppi_fma(-kf, ln2_lo, r);
+    // Taylor coefficients 1/n!, n = 7..0.
+    float p = 1.98412698412698412698e-4f;
+    p = mppi_fma(p, r, 1.38888888888888888889e-3f);
+    p = mppi_fma(p, r, 8.33333333333333333333e-3f);
+    p = mppi_fma(p, r, 4.16666666666666666667e-2f);
+    p = mppi_fma(p, r, 1.66666666666666666667e-1f);
+    p = mppi_fma(p, r, 0.5f);
+    p = mppi_fma(p, r, 1.0f);
+    p = mppi_fma(p, r, 1.0f);
+    int k = int(kf);
+    if (k > 127) return (p * 2.0f) * mppi_pow2i(k - 1);
+    if (k < -126) return (p * mppi_pow2i(-126)) * mppi_pow2i(k + 126);
+    return p * mppi_pow2i(k);
+}
+
+// ---------------------------------------------------------------------------
+// log
+// ---------------------------------------------------------------------------
+MPPI_HD float mppi_logf(float x) {
+    if (!(x == x)) return x;
+    if (x < 0.0f) return mppi_u2f(0x7fc00000u);    // NaN
+    if (x == 0.0f) return mppi_u2f(0xff800000u);   // -inf
+    uint32_t ix = mppi_f2u(x);
+    if (ix == 0x7f800000u) return x;               // +inf
+    int e = 0;
+    if (ix < 0x00800000u) {                        // subnormal: scale by 2^25
+        x = x * 33554432.0f;
+        ix = mppi_f2u(x);
+        e = -25;
+    }
+    e += int(ix >> 23) - 127;
+    uint32_t mi = (ix & 0x007fffffu) | 0x3f800000u;  // m in [1, 2)
+    if (mi > 0x3fb504f3u) {                        // m > sqrt(2): halve it
+        mi -= 0x00800000u;
+        e += 1;
+    }
+    float m = mppi_u2f(mi);
+    float f = m - 1.0f;                            // exact (Sterbenz)
+    float s = f / (2.0f + f);
+    float s2 = s * s;
+    // 2*atanh(s) = 2s + 2s^3/3 + 2s^5/5 + ... ; polynomial in s2 of the tail.
+    float p = 0.181818181818181818f;               // 2/11
+    p = mppi_fma(p, s2, 0.222222222222222222f);    // 2/9
+    p = mppi_fma(p, s2, 0.285714285714285714f);    // 2/7
+    p = mppi_fma(p, s2, 0.4f);                     // 2/5
+    p = mppi_fma(p, s2, 0.666666666666666667f);    // 2/3
+    float lm = mppi_fma(s * s2, p, 2.0f * s);
+    const float ln2_hi = 0.693145751953125f;
+    const float ln2_lo = 1.428606765330187045e-06f;
+    float ef = float(e);
+    return mppi_fma(ef, ln2_hi, mppi_fma(ef, ln2_lo, lm));
+}
+
+// ---------------------------------------------------------------------------
+// sin / cos kernels on [-pi/4, pi/4]
+// ---------------------------------------------------------------------------
+MPPI_HD float mppi_sin_poly(float r) {
+    float r2 = r * r;
+    float p = 2.75573192239858906526e-6f;          // 1/9!
+    p = mppi_fma(p, r2, -1.98412698412698412698e-4f);  // -1/7!
+    p = mppi_fma(p, r2, 8.33333333333333333333e-3f);   // 1/5!
+    p = mppi_fma(p, r2, -1.66666666666666666667e-1f);  // -1/3!
+    return mppi_fma(r * r2, p, r);
+}
+
+MPPI_HD float mppi_cos_poly(float r) {
+    float r2 = r * r;
+    float p = -2.75573192239858906526e-7f;         // -1/10!
+    p = mppi_fma(p, r2, 2.48015873015873015873e-5f);   // 1/8!
+    p = mppi_fma(p, r2, -1.38888888888888888889e-3f);  // -1/6!
+    p = mppi_fma(p, r2, 4.16666666666666666667e-2f);   // 1/4!
+    p = mppi_fma(p, r2, -0.5f);
+    return mppi_fma(r2, p, 1.0f);
+}
+
+// Select (sin, cos) of r + q*pi/2 from the kernels' values.
+MPPI_HD void mppi_quadrant(int q, float s, float c, float* so, float* co) {
+    switch (q & 3) {
+        case 0: *so = s;  *co = c;  break;
+        case 1: *so = c;  *co = -s; break;
+        case 2: *so = -s; *co = -c; break;
+        default: *so = -c; *co = s; break;
+    }
+}
+
+// sin(x), cos(x).  Accurate for |x| <= 1e5 (the three-part pi/2 keeps the
+// reduced argument exact enough there); deterministic everywhere finite.
+MPPI_HD void mppi_sincosf(float x, float* s, float* c) {
+    const float two_over_pi = 0.636619772367581343076f;
+    const float pio2_1 = 1.57079625129699707031e+00f;  // 0x3fc90fda
+    const float pio2_2 = 7.54978941586159635335e-08f;  // 0x33a22168
+    const float pio2_3 = 5.39030252995776476554e-15f;  // 0x27c234c4
+    float kf = mppi_rintf(x * two_over_pi);
+    float r = mppi_fma(-kf, pio2_1, x);
+    r = mppi_fma(-kf, pio2_2, r);
+    r = mppi_fma(-kf, pio2_3, r);
+    mppi_quadrant(int(kf), mppi_sin_poly(r), mppi_cos_poly(r), s, c);
+}
+
+MPPI_HD float mppi_sinf(float x) { float s, c; mppi_sincosf(x, &s, &c); return s; }
+MPPI_HD float mppi_cosf(float x) { float s, c; mppi_sincosf(x, &s, &c); return c; }
+
+// sin(pi*x), cos(pi*x) for |x| < 2^21.
+MPPI_HD void mppi_sincospif(float x, float* s, float* c) {
+    float n = mppi_rintf(x * 2.0f);                // exact scaling
+    float f = x - n * 0.5f;                        // exact: f in [-1/4, 1/4]
+    const float pi_f = 3.14159274101257324219f;
+    mppi_quadrant(int(n), mppi_sin_poly(f * pi_f), mppi_cos_poly(f * pi_f), s, c);
+}
+
+// ---------------------------------------------------------------------------
+// fmod: IEEE remainder toward zero, exact.
+// ---------------------------------------------------------------------------
+MPPI_HD float mppi_fmodf(float x, float y) {
+    float ax = mppi_fabsf(x), ay = mppi_fabsf(y);
+    if (!(ax == ax) || !(ay == ay) || ay == 0.0f || ax == mppi_u2f(0x7f800000u))
+        return mppi_u2f(0x7fc00000u);
+    if (ax < ay) return x;
+    // Binary long division: yj runs over ay*2^i from the largest value with
+    // ax < 2*yj down to ay.  With yj <= r < 2*yj each r - yj is exact
+    // (Sterbenz), and halving yj is exact, so the remainder is exact.
+    float yj = ay;
+    while (!(yj > ax * 0.5f)) yj = yj * 2.0f;
+    float r = ax;
+    for (;;) {
+        if (yj <= r) r = r - yj;
+        if (yj == ay) break;
+        yj = yj * 0.5f;
+    }
+    return x < 0.0f ? -r : r;
+}
+
+// ---------------------------------------------------------------------------
+// tanh
+// ---------------------------------------------------------------------------
+MPPI_HD float mppi_tanhf(float x) {
+    float ax = mppi_fabsf(x);
+    float r;
+    if (ax < 0.55f) {
+        // tanh(x) = x - x^3/3 + 2x^5/15 - 17x^7/315 + 62x^9/2835
+        //         - 1382x^11/155925 + 21844x^13/6081075 - 929569x^15/638512875
+        float x2 = ax * ax;
+        float p = -1.45583438705131826e-3f;
+        p = mppi_fma(p, x2, 3.59212803657248102e-3f);
+        p = mppi_fma(p, x2, -8.86323552990219657e-3f);
+        p = mppi_fma(p, x2, 2.18694885361552028e-2f);
+        p = mppi_fma(p, x2, -5.39682539682539683e-2f);
+        p = mppi_fma(p, x2, 1.33333333333333333e-1f);
+        p = mppi_fma(p, x2, -3.33333333333333333e-1f);
+        r = mppi_fma(ax * x2, p, ax);
+    } else if (ax < 9.0f) {
+        float e = mppi_expf(2.0f * ax);
+        r = 1.0f - 2.0f / (e + 1.0f);
+    } else {
+        r = 1.0f;
+    }
+    return x < 0.0f ? -r : r;
+}
+
+#endif  // MPPI_HIP_MATH_H

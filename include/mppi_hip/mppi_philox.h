@@ -1,0 +1,108 @@
+// mppi_philox.h — counter-based random numbers shared by the GPU kernels and
+// the CPU oracle.
+//
+// Philox4x32-10 and Philox2x32-10 are the generators of Salmon, Moraes, Dror
+// and Shaw, "Parallel random numbers: as easy as 1, 2, 3" (SC'11), as
+// published with Random123: multipliers 0xD2511F53 / 0xCD9E8D57 (4x32) and
+// 0xD256D193 (2x32), Weyl key increments 0x9E3779B9 / 0xBB67AE85, ten rounds.
+// The 2x32 form is the reference's generator (shaders/philox.h:19-36), kept
+// for compatibility checks; the library's noise stream uses the 4x32 form.
+//
+// Noise stream contract (replaces the reference's flat u32 counter,
+// shaders/mppi_rollout.metal:73-87, which overflows at K*H*P >= 2^31):
+//   key     = the 64-bit seed (lo word, hi word)
+//   counter = (block, sample, step, tag << 28 | agent)
+// Sample k of agent a at MPPI step s draws the normals of its H*C control
+// noise values as one sequence n = t*C + d; normal n comes from counter block
+// b = n / 4, lane j = n % 4: the block's words (w0,w1) and (w2,w3) are two
+// Box-Muller pairs giving (z0, z1) and (z2, z3).
+#ifndef MPPI_HIP_PHILOX_H
+#define MPPI_HIP_PHILOX_H
+
+#include "mppi_math.h"
+
+struct mppi_u32x4 { uint32_t x, y, z, w; };
+struct mppi_f32x4 { float x, y, z, w; };
+
+// Stream tags (top 4 bits of counter word 3).
+#define MPPI_TAG_CONTROL_NOISE 0u   // rollout control noise
+#define MPPI_TAG_DYNAMICS      1u   // stochastic dynamics inside the rollout
+#define MPPI_TAG_PLANT         2u   // closed-loop plant step (simulate)
+#define MPPI_TAG_SCENE         3u   // synthetic scene generation
+#define MPPI_TAG_SURROGATE     4u   // surrogate training data
+#define MPPI_AGENT_MASK        0x0fffffffu
+
+MPPI_HD uint32_t mppi_mulhilo32(uint32_t a, uint32_t b, uint32_t* hi) {
+    uint64_t p = uint64_t(a) * uint64_t(b);
+    *hi = uint32_t(p >> 32);
+    return uint32_t(p);
+}
+
+MPPI_HD mppi_u32x4 mppi_philox4x32_10(mppi_u32x4 c, uint32_t k0, uint32_t k1) {
+MPPI_UNROLL
+    for (int i = 0; i < 10; ++i) {
+        uint32_t hi0, hi1;
+        uint32_t lo0 = mppi_mulhilo32(0xD2511F53u, c.x, &hi0);
+        uint32_t lo1 = mppi_mulhilo32(0xCD9E8D57u, c.z, &hi1);
+        mppi_u32x4 n;
+        n.x = hi1 ^ c.y ^ k0;
+        n.y = lo1;
+        n.z = hi0 ^ c.w ^ k1;
+        n.w = lo0;
+        c = n;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// Reference-compatible Philox2x32-10 (counter (x, y), 32-bit key).
+MPPI_HD void mppi_philox2x32_10(uint32_t* x, uint32_t* y, uint32_t key) {
+MPPI_UNROLL
+    for (int i = 0; i < 10; ++i) {
+        uint32_t hi;
+        uint32_t lo = mppi_mulhilo32(0xD256D193u, *x, &hi);
+        uint32_t nx = hi ^ key ^ *y;
+        *y = lo;
+        *x = nx;
+        key += 0x9E3779B9u;
+    }
+}
+
+// Uniform in (0, 1): the top 23 bits j of x give (2j + 1) * 2^-24, which is
+// exact in binary32, symmetric about 1/2, never 0 and never 1.
+MPPI_HD float mppi_u01(uint32_t x) {
+    return float((x >> 8) | 1u) * 5.9604644775390625e-8f;  // 2^-24
+}
+
+// Box-Muller: one pair of uniform words -> two standard normals.
+MPPI_HD void mppi_box_muller(uint32_t a, uint32_t b, float* z0, float* z1) {
+    float u1 = mppi_u01(a);
+    float u2 = mppi_u01(b);
+    float r = mppi_sqrtf(-2.0f * mppi_logf(u1));
+    float s, c;
+    mppi_sincospif(2.0f * u2, &s, &c);
+    *z0 = r * c;
+    *z1 = r * s;
+}
+
+// Four standard normals from one Philox4x32-10 block.
+MPPI_HD mppi_f32x4 mppi_normal4(mppi_u32x4 ctr, uint32_t k0, uint32_t k1) {
+    mppi_u32x4 w = mppi_philox4x32_10(ctr, k0, k1);
+    mppi_f32x4 z;
+    mppi_box_muller(w.x, w.y, &z.x, &z.y);
+    mppi_box_muller(w.z, w.w, &z.z, &z.w);
+    return z;
+}
+
+MPPI_HD mppi_u32x4 mppi_noise_counter(uint32_t block, uint32_t sample, uint32_t step,
+                                      uint32_t tag, uint32_t agent) {
+    mppi_u32x4 c;
+    c.x = block;
+    c.y = sample;
+    c.z = step;
+    c.w = (tag << 28) | (agent & MPPI_AGENT_MASK);
+    return c;
+}
+
+#endif  // MPPI_HIP_PHILOX_H

@@ -1,0 +1,417 @@
+// mppi_kernels.hip — library-owned MPPI kernels for gfx950.
+//
+// This file is not compiled on its own: the library hands it to hipRTC
+// together with the user's plugin source (as the in-memory header
+// "mppi_user_plugin.hip") and these JIT-time definitions:
+//   MPPI_SDIM, MPPI_CDIM            state and control dimensions
+//   MPPI_DYNAMICS                   name of the dynamics callable
+//   MPPI_STAGE_COST, MPPI_TERMINAL_COST  cost callables (or the defaults)
+//   MPPI_COV_FULL                   0: diagonal sigma, 1: lower Cholesky factor
+// so the plugin's callables are direct, inlined calls (the reference reaches
+// them through a visible function table at every timestep,
+// shaders/mppi_rollout.metal:90-91).
+//
+// One MPPI step is two launches:
+//   mppi_rollout  one sample per lane; control noise from Philox4x32-10 in
+//                 registers (never stored); clamp; dynamics; stage cost on the
+//                 post-dynamics state; terminal cost; FP32 cost accumulation.
+//                 Each workgroup also reduces its 256 costs to
+//                 (min, sum exp, sum cost, sum exp^2) with wave shuffles and
+//                 LDS, and the last workgroup of an agent combines those
+//                 partials into the softmax normaliser (log-sum-exp form).
+//                 Reference: mppi_rollout (mppi_rollout.metal:36-96) and the
+//                 min / sum passes of mppi_compute_weights
+//                 (mppi_reduce.metal:14-121).
+//   mppi_reduce   weighted-noise average, parallel over samples: every lane
+//                 regenerates its noise blocks from the same counters and
+//                 accumulates w_k * z_k; workgroup partials are summed by the
+//                 last workgroup, which writes u_next, the diagnostics and,
+//                 in closed-loop mode, steps the plant and shifts the warm
+//                 start.  Reference: mppi_reduce (mppi_reduce.metal:130-168)
+//                 and mppi_propagate_and_shift (mppi_reduce.metal:175-231).
+// Cross-block sums (normaliser and weighted noise) run in FP64 when K >= 2^18.
+#if !defined(__HIPCC_RTC__)
+#include <hip/hip_runtime.h>   // hipRTC provides these builtins itself
+#endif
+#include "mppi_core.h"
+#include "mppi_launch.h"
+#include "mppi_user_plugin.hip"
+
+#ifndef MPPI_STAGE_COST
+#define MPPI_STAGE_COST mppi_default_stage_cost
+#endif
+#ifndef MPPI_TERMINAL_COST
+#define MPPI_TERMINAL_COST mppi_default_terminal_cost
+#endif
+#ifndef MPPI_COV_FULL
+#define MPPI_COV_FULL 0
+#endif
+
+#define SDIM MPPI_SDIM
+#define CDIM MPPI_CDIM
+#define MPPI_BLOCK 256
+#define MPPI_WAVES (MPPI_BLOCK / 64)
+#define MPPI_REDUCE_BPG 8                 // noise blocks per reduce workgroup
+#define MPPI_FP64_MIN_K (1u << 18)
+
+constexpr int mppi_gcd(int a, int b) { return b == 0 ? a : mppi_gcd(b, a % b); }
+// Timesteps whose C*G noise values fill whole Philox blocks.
+constexpr int MPPI_G = 4 / mppi_gcd(CDIM, 4);
+constexpr int MPPI_NG = MPPI_G * CDIM;     // normals per group (multiple of 4)
+constexpr int MPPI_BG = MPPI_NG / 4;       // Philox blocks per group
+
+__device__ inline float mppi_wave_min(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        float w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+__device__ inline float mppi_wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ inline bool mppi_isfinite(float x) {
+    return (mppi_f2u(x) & 0x7f800000u) != 0x7f800000u;
+}
+
+// Control for timestep t from the normals z[0..CDIM): u = clamp(u_nom + eps).
+__device__ inline void mppi_make_control(const float* __restrict__ un, const float* z,
+                                         const float* __restrict__ scale,
+                                         const float* __restrict__ lo,
+                                         const float* __restrict__ hi, float* u) {
+#if MPPI_COV_FULL
+#pragma unroll
+    for (int i = 0; i < CDIM; ++i) {
+        float e = 0.0f;
+#pragma unroll
+        for (int j = 0; j <= i; ++j) e = mppi_fma(scale[i * CDIM + j], z[j], e);
+        u[i] = mppi_clampf(un[i] + e, lo[i], hi[i]);
+    }
+#else
+#pragma unroll
+    for (int i = 0; i < CDIM; ++i) {
+        float e = z[i] * scale[i];
+        u[i] = mppi_clampf(un[i] + e, lo[i], hi[i]);
+    }
+#endif
+}
+
+
+// Combine the per-workgroup (min, sum e, sum c, sum e^2) partials of one agent
+// into the softmax normaliser: m = min m_g, S = sum_g S_g exp(-(m_g - m)/lambda),
+// Q likewise with the squared factor, ESS = S^2 / Q (= 1 / sum w^2).
+// T is the accumulation type: float, or double when K >= 2^18.
+template <typename T>
+__device__ void mppi_combine_stats(const mppi_rollout_args& a, uint32_t agent) {
+    __shared__ float s_min[MPPI_WAVES];
+    __shared__ T s_sum[3][MPPI_WAVES];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const float* st = a.blk_stats + size_t(agent) * a.n_blk * 4;
+    float mg = __builtin_inff();
+    for (uint32_t i = threadIdx.x; i < a.n_blk; i += MPPI_BLOCK) {
+        float v = st[i * 4];
+        mg = v < mg ? v : mg;
+    }
+    mg = mppi_wave_min(mg);
+    if (lane == 0) s_min[wave] = mg;
+    __syncthreads();
+    mg = s_min[0];
+#pragma unroll
+    for (int w = 1; w < MPPI_WAVES; ++w) mg = s_min[w] < mg ? s_min[w] : mg;
+    T S = T(0), Cs = T(0), Q = T(0);
+    for (uint32_t i = threadIdx.x; i < a.n_blk; i += MPPI_BLOCK) {
+        float bm = st[i * 4 + 0];
+        float f = (mppi_isfinite(bm) && mppi_isfinite(mg)) ? mppi_expf(-(bm - mg) / a.lambda) : 0.0f;
+        S += T(st[i * 4 + 1]) * T(f);
+        Cs += T(st[i * 4 + 2]);
+        Q += T(st[i * 4 + 3]) * (T(f) * T(f));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        S += __shfl_xor(S, o, 64);
+        Cs += __shfl_xor(Cs, o, 64);
+        Q += __shfl_xor(Q, o, 64);
+    }
+    if (lane == 0) { s_sum[0][wave] = S; s_sum[1][wave] = Cs; s_sum[2][wave] = Q; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        S = s_sum[0][0]; Cs = s_sum[1][0]; Q = s_sum[2][0];
+#pragma unroll
+        for (int w = 1; w < MPPI_WAVES; ++w) { S += s_sum[0][w]; Cs += s_sum[1][w]; Q += s_sum[2][w]; }
+        if (!mppi_isfinite(mg)) {
+            // Every cost is NaN or infinite: fall back to uniform weights.
+            S = T(a.K);
+            Q = T(a.K);
+        }
+        mppi_agent_state* ag = &a.agents[agent];
+        ag->min_cost = mg;
+        ag->weight_sum = double(S);
+        ag->mean_cost = double(Cs / T(a.K));
+        ag->ess = double((S * S) / Q);
+        ag->rollout_arrivals = 0u;
+        __threadfence();
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(MPPI_BLOCK)
+mppi_rollout(mppi_rollout_args a) {
+    const uint32_t agent = blockIdx.y;
+    const uint32_t k = blockIdx.x * MPPI_BLOCK + threadIdx.x;
+    const uint32_t K = a.K, H = a.H;
+    const bool active = k < K;
+    const uint32_t step = a.agents[agent].step;
+    const uint32_t tag_agent = (MPPI_TAG_CONTROL_NOISE << 28) | (agent & MPPI_AGENT_MASK);
+    const uint32_t dyn_tag = (MPPI_TAG_DYNAMICS << 28) | (agent & MPPI_AGENT_MASK);
+    const float* __restrict__ un = a.u_nom + size_t(agent) * H * CDIM;
+
+    float total = 0.0f;
+    if (active) {
+        float x[SDIM];
+#pragma unroll
+        for (int i = 0; i < SDIM; ++i) x[i] = a.x0[size_t(agent) * SDIM + i];
+        for (uint32_t t0 = 0; t0 < H; t0 += MPPI_G) {
+            float z[MPPI_NG];
+            const uint32_t b0 = (t0 * CDIM) >> 2;
+#pragma unroll
+            for (int q = 0; q < MPPI_BG; ++q) {
+                mppi_f32x4 v = mppi_normal4(
+                    mppi_u32x4{b0 + q, k, step, tag_agent}, a.key0, a.key1);
+                z[4 * q + 0] = v.x; z[4 * q + 1] = v.y;
+                z[4 * q + 2] = v.z; z[4 * q + 3] = v.w;
+            }
+#pragma unroll
+            for (int g = 0; g < MPPI_G; ++g) {
+                const uint32_t t = t0 + g;
+                if (t < H) {
+                    float u[CDIM];
+                    mppi_make_control(un + t * CDIM, z + g * CDIM, a.noise_scale,
+                                      a.u_min, a.u_max, u);
+                    mppi_rng rng{t << 8, k, step, dyn_tag, a.key0, a.key1};
+                    MPPI_DYNAMICS(mppi_state_ref{x}, u, mppi_blob_ref{a.model}, rng);
+                    total += MPPI_STAGE_COST(mppi_cstate_ref{x}, u, mppi_blob_ref{a.model},
+                                             mppi_blob_ref{a.cost});
+                }
+            }
+        }
+        total += MPPI_TERMINAL_COST(mppi_cstate_ref{x}, mppi_blob_ref{a.model},
+                                    mppi_blob_ref{a.cost});
+        a.costs[size_t(agent) * K + k] = total;
+    }
+
+    // ---- workgroup softmax partials over this block's costs ----
+    __shared__ float s_red[4][MPPI_WAVES];
+    __shared__ int s_last;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float c = (active && mppi_isfinite(total)) ? total : __builtin_inff();
+    float m = mppi_wave_min(c);
+    if (lane == 0) s_red[0][wave] = m;
+    __syncthreads();
+    m = s_red[0][0];
+#pragma unroll
+    for (int w = 1; w < MPPI_WAVES; ++w) m = s_red[0][w] < m ? s_red[0][w] : m;
+    float e = (mppi_isfinite(c) && mppi_isfinite(m)) ? mppi_expf(-(c - m) / a.lambda) : 0.0f;
+    float se = mppi_wave_sum(e);
+    float sq = mppi_wave_sum(e * e);
+    float sc = mppi_wave_sum(active ? total : 0.0f);
+    if (lane == 0) { s_red[1][wave] = se; s_red[2][wave] = sc; s_red[3][wave] = sq; }
+    __syncthreads();
+    float* bs = a.blk_stats + (size_t(agent) * a.n_blk + blockIdx.x) * 4;
+    if (threadIdx.x == 0) {
+        float S = 0.0f, Cs = 0.0f, Q = 0.0f;
+#pragma unroll
+        for (int w = 0; w < MPPI_WAVES; ++w) { S += s_red[1][w]; Cs += s_red[2][w]; Q += s_red[3][w]; }
+        bs[0] = m; bs[1] = S; bs[2] = Cs; bs[3] = Q;
+        __threadfence();
+        uint32_t prev = atomicAdd(&a.agents[agent].rollout_arrivals, 1u);
+        s_last = (prev == a.n_blk - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+
+    // ---- last workgroup of this agent: combine the partials ----
+    __threadfence();
+    if (K >= MPPI_FP64_MIN_K) mppi_combine_stats<double>(a, agent);
+    else mppi_combine_stats<float>(a, agent);
+}
+
+
+// Sum the per-workgroup weighted-noise partials of one agent over all sample
+// chunks and divide by the normaliser: zs[n] = (sum_k w~_k z_k(n)) / S for the
+// NB*4 noise slots n.  Columns are spread over all threads in row groups so
+// the loads stay coalesced and several are in flight per thread.
+template <typename T>
+__device__ void mppi_sum_partials(const mppi_reduce_args& a, uint32_t agent, float* zs) {
+    __shared__ T s_col[MPPI_BLOCK];
+    const uint32_t W = a.NB * 4;
+    const T S = T(a.agents[agent].weight_sum);
+    const float* pb = a.partials + size_t(agent) * a.n_chunks * W;
+    if (W <= MPPI_BLOCK) {
+        const uint32_t R = MPPI_BLOCK / W;
+        const uint32_t c = threadIdx.x % W, r = threadIdx.x / W;
+        if (r < R) {
+            T s = T(0);
+            for (uint32_t ch = r; ch < a.n_chunks; ch += R) s += T(pb[size_t(ch) * W + c]);
+            s_col[r * W + c] = s;
+        }
+        __syncthreads();
+        for (uint32_t n = threadIdx.x; n < W; n += MPPI_BLOCK) {
+            T s = T(0);
+            for (uint32_t q = 0; q < R; ++q) s += s_col[q * W + n];
+            zs[n] = float(s / S);
+        }
+    } else {
+        for (uint32_t n = threadIdx.x; n < W; n += MPPI_BLOCK) {
+            T s = T(0);
+            for (uint32_t ch = 0; ch < a.n_chunks; ++ch) s += T(pb[size_t(ch) * W + n]);
+            zs[n] = float(s / S);
+        }
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(MPPI_BLOCK)
+mppi_reduce(mppi_reduce_args a) {
+    const uint32_t agent = blockIdx.z;
+    const uint32_t chunk = blockIdx.x, grp = blockIdx.y;
+    const uint32_t K = a.K, H = a.H, NB = a.NB;
+    const uint32_t step = a.agents[agent].step;
+    const double mind = a.agents[agent].min_cost;
+    const float m = float(mind);
+    const float Sf = float(a.agents[agent].weight_sum);
+    const bool uniform = !mppi_isfinite(m);
+    const uint32_t tag_agent = (MPPI_TAG_CONTROL_NOISE << 28) | (agent & MPPI_AGENT_MASK);
+    const uint32_t b_begin = grp * MPPI_REDUCE_BPG;
+
+    float acc[MPPI_REDUCE_BPG][4];
+#pragma unroll
+    for (int q = 0; q < MPPI_REDUCE_BPG; ++q)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[q][j] = 0.0f;
+
+    const uint32_t k_base = chunk * MPPI_BLOCK * a.spt + threadIdx.x;
+    for (uint32_t s = 0; s < a.spt; ++s) {
+        const uint32_t k = k_base + s * MPPI_BLOCK;
+        if (k >= K) break;
+        const float c = a.costs[size_t(agent) * K + k];
+        float w;
+        if (uniform) w = 1.0f;
+        else w = mppi_isfinite(c) ? mppi_expf(-(c - m) / a.lambda) : 0.0f;
+        if (grp == 0 && a.weights) a.weights[size_t(agent) * K + k] = w / Sf;
+#pragma unroll
+        for (int q = 0; q < MPPI_REDUCE_BPG; ++q) {
+            if (b_begin + q < NB) {
+                mppi_f32x4 z = mppi_normal4(mppi_u32x4{b_begin + q, k, step, tag_agent},
+                                            a.key0, a.key1);
+                acc[q][0] = mppi_fma(w, z.x, acc[q][0]);
+                acc[q][1] = mppi_fma(w, z.y, acc[q][1]);
+                acc[q][2] = mppi_fma(w, z.z, acc[q][2]);
+                acc[q][3] = mppi_fma(w, z.w, acc[q][3]);
+            }
+        }
+    }
+
+    // ---- workgroup sum of the 4*BPG accumulators (transpose through LDS) ----
+    constexpr int NV = 4 * MPPI_REDUCE_BPG;        // 32 values per thread
+    __shared__ float s_acc[MPPI_BLOCK * (NV + 1)];
+    __shared__ int s_last;
+#pragma unroll
+    for (int q = 0; q < MPPI_REDUCE_BPG; ++q)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) s_acc[threadIdx.x * (NV + 1) + q * 4 + j] = acc[q][j];
+    __syncthreads();
+    {
+        // 256 threads: value v = tid % NV, segment = tid / NV sums 256*NV/256 rows
+        const int v = threadIdx.x % NV, seg = threadIdx.x / NV;
+        constexpr int NSEG = MPPI_BLOCK / NV;      // 8
+        constexpr int ROWS = MPPI_BLOCK / NSEG;    // 32
+        float sum = 0.0f;
+        for (int r = 0; r < ROWS; ++r) sum += s_acc[(seg * ROWS + r) * (NV + 1) + v];
+        __syncthreads();
+        s_acc[seg * NV + v] = sum;
+        __syncthreads();
+        if (threadIdx.x < NV) {
+            float tot = 0.0f;
+#pragma unroll
+            for (int g = 0; g < NSEG; ++g) tot += s_acc[g * NV + threadIdx.x];
+            const uint32_t b = b_begin + threadIdx.x / 4;
+            if (b < NB)
+                a.partials[((size_t(agent) * a.n_chunks + chunk) * NB + b) * 4 + (threadIdx.x & 3)] = tot;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        uint32_t prev = atomicAdd(&a.agents[agent].reduce_arrivals, 1u);
+        s_last = (prev == a.n_chunks * a.n_groups - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+
+    // ---- last workgroup of this agent: finalize u_next ----
+    __threadfence();
+    const uint32_t HC = H * CDIM;
+    float* zs = s_acc;                                  // reuse: HC <= 4096 floats
+    if (K >= MPPI_FP64_MIN_K) mppi_sum_partials<double>(a, agent, zs);
+    else mppi_sum_partials<float>(a, agent, zs);
+    __syncthreads();
+    float* uo = a.u_out + size_t(agent) * HC;
+    float* un = a.u_nom + size_t(agent) * HC;
+    for (uint32_t n = threadIdx.x; n < HC; n += MPPI_BLOCK) {
+        const uint32_t t = n / CDIM, i = n % CDIM;
+#if MPPI_COV_FULL
+        float e = 0.0f;
+        for (uint32_t j = 0; j <= i; ++j) e = mppi_fma(a.noise_scale[i * CDIM + j], zs[t * CDIM + j], e);
+#else
+        float e = a.noise_scale[i] * zs[n];
+#endif
+        uo[n] = mppi_clampf(un[n] + e, a.u_min[i], a.u_max[i]);
+    }
+    __syncthreads();
+    mppi_agent_state* ag = &a.agents[agent];
+    if (threadIdx.x == 0 && a.diag) {
+        a.diag[agent * 3 + 0] = float(ag->min_cost);
+        a.diag[agent * 3 + 1] = float(ag->mean_cost);
+        a.diag[agent * 3 + 2] = float(ag->ess);
+    }
+    if (a.closed_loop) {
+        if (threadIdx.x == 0) {
+            const uint32_t row = ag->hist;
+            float* xs = a.x_state + size_t(agent) * SDIM;
+            float x[SDIM];
+            float u[CDIM];
+#pragma unroll
+            for (int i = 0; i < SDIM; ++i) x[i] = xs[i];
+#pragma unroll
+            for (int i = 0; i < CDIM; ++i) u[i] = uo[i];
+            if (a.hist_states)
+                for (int i = 0; i < SDIM; ++i)
+                    a.hist_states[(size_t(row) * a.N + agent) * SDIM + i] = x[i];
+            if (a.hist_controls)
+                for (int i = 0; i < CDIM; ++i)
+                    a.hist_controls[(size_t(row) * a.N + agent) * CDIM + i] = u[i];
+            if (a.hist_costs) a.hist_costs[size_t(row) * a.N + agent] = float(ag->min_cost);
+            mppi_rng rng{0u, 0xffffffffu, step,
+                         (MPPI_TAG_PLANT << 28) | (agent & MPPI_AGENT_MASK), a.key0, a.key1};
+            MPPI_DYNAMICS(mppi_state_ref{x}, u, mppi_blob_ref{a.model}, rng);
+#pragma unroll
+            for (int i = 0; i < SDIM; ++i) xs[i] = x[i];
+            if (a.terminal)
+                for (int i = 0; i < SDIM; ++i) a.terminal[size_t(agent) * SDIM + i] = x[i];
+            ag->hist = row + 1;
+            ag->step = step + 1;
+        }
+        for (uint32_t n = threadIdx.x; n < HC; n += MPPI_BLOCK) {
+            const uint32_t t = n / CDIM, i = n % CDIM;
+            const uint32_t src = (t + 1 < H ? t + 1 : H - 1) * CDIM + i;
+            un[n] = uo[src];
+        }
+    }
+    if (threadIdx.x == 0) {
+        ag->reduce_arrivals = 0u;
+        __threadfence();
+    }
+}

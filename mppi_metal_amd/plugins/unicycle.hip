@@ -1,0 +1,100 @@
+// Kinematic unicycle ("nonholonomic vehicle"), the model of the reference's
+// example plugin (examples/nonholonomic_vehicle/nonholonomic_vehicle.metal:24-91),
+// plus a circular-obstacle hinge penalty for the benchmark scenes.
+//
+// State [x, y, theta, v], control [a, omega].
+//   x += v cos(theta) dt; y += v sin(theta) dt; theta += omega dt; v += a dt;
+//   theta wrapped to [-pi, pi) with fmod; v clamped to [-v_max, v_max].
+// Model params: UnicycleModel {v_max, dt} (8 bytes, as the reference's).
+// Cost params:  UnicycleCost, whose first 24 bytes are the reference's
+//   CostParams {x_goal, y_goal, goal_radius, pos_weight, heading_weight,
+//   control_weight}; the obstacle fields follow.
+//
+// Callables:
+//   mppi_dynamics                       — dynamics
+//   mppi_stage_cost / mppi_terminal_cost — the reference's goal costs; they
+//       read only the first 24 bytes, so the reference's CostParams works.
+//   unicycle_stage_cost_obstacles       — goal cost plus
+//       obstacle_weight * sum_i max(0, r_i - |p - c_i|)
+//   unicycle_terminal_cost              — 10 * pos_weight * |p - g|^2
+#include "mppi_core.h"
+
+#define UNICYCLE_MAX_OBSTACLES 32
+
+struct UnicycleModel {
+    float v_max;
+    float dt;
+};
+
+struct UnicycleCost {
+    float x_goal;
+    float y_goal;
+    float goal_radius;
+    float pos_weight;
+    float heading_weight;
+    float control_weight;
+    float obstacle_weight;
+    uint32_t n_obstacles;
+    float obstacles[UNICYCLE_MAX_OBSTACLES][3];  // (cx, cy, radius)
+};
+
+MPPI_CALLABLE void mppi_dynamics(float* state, const float* control,
+                                 const uint8_t* model_params_raw, mppi_rng rng) {
+    const UnicycleModel* params = reinterpret_cast<const UnicycleModel*>(model_params_raw);
+    float dt = params->dt;
+    float theta = state[2];
+    float v = state[3];
+    float s, c;
+    mppi_sincosf(theta, &s, &c);
+    state[0] += v * c * dt;
+    state[1] += v * s * dt;
+    state[2] += control[1] * dt;
+    state[3] += control[0] * dt;
+    const float pi = 3.1415926535f;
+    float th = mppi_fmodf(state[2] + pi, 2.0f * pi);
+    if (th < 0.0f) th += 2.0f * pi;
+    state[2] = th - pi;
+    if (state[3] > params->v_max) state[3] = params->v_max;
+    if (state[3] < -params->v_max) state[3] = -params->v_max;
+}
+
+MPPI_CALLABLE float mppi_stage_cost(const float* state, const float* control,
+                                    const uint8_t* model_params, const uint8_t* cost_params_raw) {
+    const UnicycleCost* p = reinterpret_cast<const UnicycleCost*>(cost_params_raw);
+    float dx = state[0] - p->x_goal;
+    float dy = state[1] - p->y_goal;
+    float cost = p->pos_weight * (dx * dx + dy * dy);
+    cost += p->control_weight * (control[0] * control[0] + control[1] * control[1]);
+    return cost;
+}
+
+MPPI_CALLABLE float unicycle_stage_cost_obstacles(const float* state, const float* control,
+                                                  const uint8_t* model_params,
+                                                  const uint8_t* cost_params_raw) {
+    const UnicycleCost* p = reinterpret_cast<const UnicycleCost*>(cost_params_raw);
+    float cost = mppi_stage_cost(state, control, model_params, cost_params_raw);
+    float pen = 0.0f;
+    uint32_t n = p->n_obstacles < UNICYCLE_MAX_OBSTACLES ? p->n_obstacles : UNICYCLE_MAX_OBSTACLES;
+    for (uint32_t i = 0; i < n; ++i) {
+        float ox = state[0] - p->obstacles[i][0];
+        float oy = state[1] - p->obstacles[i][1];
+        float r = p->obstacles[i][2];
+        float d2 = ox * ox + oy * oy;
+        // max(0, r - |p - c|); the square root is only needed inside.
+        if (d2 < r * r) pen += r - mppi_sqrtf(d2);
+    }
+    return cost + p->obstacle_weight * pen;
+}
+
+MPPI_CALLABLE float mppi_terminal_cost(const float* state, const uint8_t* model_params,
+                                       const uint8_t* cost_params_raw) {
+    const UnicycleCost* p = reinterpret_cast<const UnicycleCost*>(cost_params_raw);
+    float dx = state[0] - p->x_goal;
+    float dy = state[1] - p->y_goal;
+    return p->pos_weight * (dx * dx + dy * dy) * 10.0f;
+}
+
+MPPI_CALLABLE float unicycle_terminal_cost(const float* state, const uint8_t* model_params,
+                                           const uint8_t* cost_params_raw) {
+    return mppi_terminal_cost(state, model_params, cost_params_raw);
+}

@@ -1,0 +1,614 @@
+// mppi_oracle.cpp — TEST INFRASTRUCTURE: the CPU oracle of the MPPI step.
+//
+// Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load
+// this library, and only as the checker or the timed CPU baseline.  The
+// product (mppi_metal_amd/libmppi_hip.so) never calls it.
+//
+// It restates, in scalar C++, the reference's MPPI iteration:
+//   rollout        shaders/mppi_rollout.metal:36-96 (one sample: noise, clamp,
+//                  dynamics, stage cost on the post-dynamics state, terminal)
+//   weights        shaders/mppi_reduce.metal:14-121 (min, exp(-(c-min)/lambda),
+//                  normalise, mean cost, ESS = 1/sum w^2)
+//   reduce         shaders/mppi_reduce.metal:130-168 (u = clamp(u_nom +
+//                  sum_k w_k eps_k), k ascending, unclamped noise)
+//   propagate/shift shaders/mppi_reduce.metal:175-231 and the batch variant
+//                  :413-488 (history, plant step, hold-last shift)
+//   driver loop    src/mppi_driver.cpp:326-339 (host warm start), 343-421,
+//                  423-491
+// with the noise stream of the new build (Philox4x32-10, counter
+// (block, sample, step, tag<<28 | agent), 64-bit key = seed) in place of the
+// reference's flat Philox2x32 counter (shaders/mppi_rollout.metal:73-87).
+//
+// Two precisions:
+//   FP32  the plugin callables are the bundled plugin sources compiled for
+//         the host, the transcendentals are include/mppi_hip/mppi_math.h;
+//         every operation is in the order the reference states it, so sample
+//         costs are expected to match the GPU bit for bit.
+//   FP64  the same algorithm in double with libm transcendentals and models
+//         restated by hand in double.  It shares nothing with the GPU path
+//         but the integer Philox stream, and is the yardstick each FP32
+//         result's error is stated against.
+// The Philox generator here is this directory's own (oracle_philox.h).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "mppi_core.h"
+#include "oracle_philox.h"
+
+namespace plug_di {
+#include "../mppi_metal_amd/plugins/double_integrator.hip"
+}
+namespace plug_uni {
+#include "../mppi_metal_amd/plugins/unicycle.hip"
+}
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Numeric traits: the FP32 path uses the shared elementary functions, the FP64
+// path libm.
+// ---------------------------------------------------------------------------
+template <typename T> struct Num;
+template <> struct Num<float> {
+    static float log(float x) { return mppi_logf(x); }
+    static float sqrt(float x) { return mppi_sqrtf(x); }
+    static float exp(float x) { return mppi_expf(x); }
+    static void sincospi(float x, float* s, float* c) { mppi_sincospif(x, s, c); }
+};
+template <> struct Num<double> {
+    static double log(double x) { return std::log(x); }
+    static double sqrt(double x) { return std::sqrt(x); }
+    static double exp(double x) { return std::exp(x); }
+    static void sincospi(double x, double* s, double* c) {
+        *s = std::sin(M_PI * x);
+        *c = std::cos(M_PI * x);
+    }
+};
+
+template <typename T> T u01(uint32_t x) { return T((x >> 8) | 1u) * T(5.9604644775390625e-8); }
+
+// Four standard normals of counter block (block, sample, step, tag<<28|agent).
+template <typename T>
+void normals4(uint32_t block, uint32_t sample, uint32_t step, uint32_t tag, uint32_t agent,
+              uint64_t seed, T z[4]) {
+    const uint32_t ctr[4] = {block, sample, step, (tag << 28) | (agent & 0x0fffffffu)};
+    const uint32_t key[2] = {uint32_t(seed), uint32_t(seed >> 32)};
+    uint32_t w[4];
+    oracle_philox4x32_10(ctr, key, w);
+    for (int p = 0; p < 2; ++p) {
+        T u1 = u01<T>(w[2 * p]), u2 = u01<T>(w[2 * p + 1]);
+        T r = Num<T>::sqrt(T(-2) * Num<T>::log(u1));
+        T s, c;
+        Num<T>::sincospi(T(2) * u2, &s, &c);
+        z[2 * p] = r * c;
+        z[2 * p + 1] = r * s;
+    }
+}
+
+template <typename T> T clampv(T x, T lo, T hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// ---------------------------------------------------------------------------
+// Models.  FP32: the plugin sources.  FP64: hand restatements of the same
+// formulas.
+// ---------------------------------------------------------------------------
+struct RngCtx {
+    uint32_t t, sample, step, tag, agent;
+    uint64_t seed;
+};
+
+template <typename T>
+using DynFn = void (*)(T* x, const T* u, const uint8_t* model, const RngCtx& rng);
+template <typename T>
+using StageFn = T (*)(const T* x, const T* u, const uint8_t* model, const uint8_t* cost);
+template <typename T>
+using TermFn = T (*)(const T* x, const uint8_t* model, const uint8_t* cost);
+
+mppi_rng to_rng(const RngCtx& r) {
+    return mppi_rng{r.t << 8, r.sample, r.step, (r.tag << 28) | (r.agent & 0x0fffffffu),
+                    uint32_t(r.seed), uint32_t(r.seed >> 32)};
+}
+
+template <typename T> T zero_stage(const T*, const T*, const uint8_t*, const uint8_t*) { return T(0); }
+template <typename T> T zero_term(const T*, const uint8_t*, const uint8_t*) { return T(0); }
+
+// ---- double integrator (shaders/test_user_dynamics.metal:8-43) ----
+void di_dyn_d(double* s, const double* u, const uint8_t*, const RngCtx&) {
+    const double dt = double(0.1f);
+    s[0] += s[2] * dt;
+    s[1] += s[3] * dt;
+    s[2] += u[0] * dt;
+    s[3] += u[1] * dt;
+}
+double di_cost_d(const double* s, const double*, const uint8_t*, const uint8_t*) {
+    return s[0] * s[0] + s[1] * s[1];
+}
+double di_term_d(const double* s, const uint8_t*, const uint8_t*) { return s[0] * s[0] + s[1] * s[1]; }
+
+// ---- unicycle (examples/nonholonomic_vehicle/nonholonomic_vehicle.metal:24-91) ----
+struct UniModel { float v_max, dt; };
+struct UniCost {
+    float x_goal, y_goal, goal_radius, pos_weight, heading_weight, control_weight;
+    float obstacle_weight;
+    uint32_t n_obstacles;
+    float obstacles[32][3];
+};
+void uni_dyn_d(double* s, const double* u, const uint8_t* mp, const RngCtx&) {
+    const UniModel* p = reinterpret_cast<const UniModel*>(mp);
+    const double dt = p->dt, th = s[2], v = s[3];
+    s[0] += v * std::cos(th) * dt;
+    s[1] += v * std::sin(th) * dt;
+    s[2] += u[1] * dt;
+    s[3] += u[0] * dt;
+    const double pi = double(3.1415926535f);
+    double w = std::fmod(s[2] + pi, 2.0 * pi);
+    if (w < 0) w += 2.0 * pi;
+    s[2] = w - pi;
+    if (s[3] > p->v_max) s[3] = p->v_max;
+    if (s[3] < -p->v_max) s[3] = -p->v_max;
+}
+double uni_stage_d(const double* s, const double* u, const uint8_t*, const uint8_t* cp) {
+    const UniCost* p = reinterpret_cast<const UniCost*>(cp);
+    double dx = s[0] - p->x_goal, dy = s[1] - p->y_goal;
+    return p->pos_weight * (dx * dx + dy * dy) + p->control_weight * (u[0] * u[0] + u[1] * u[1]);
+}
+double uni_stage_obs_d(const double* s, const double* u, const uint8_t* mp, const uint8_t* cp) {
+    const UniCost* p = reinterpret_cast<const UniCost*>(cp);
+    double c = uni_stage_d(s, u, mp, cp), pen = 0.0;
+    uint32_t n = std::min<uint32_t>(p->n_obstacles, 32);
+    for (uint32_t i = 0; i < n; ++i) {
+        double ox = s[0] - p->obstacles[i][0], oy = s[1] - p->obstacles[i][1];
+        pen += std::max(0.0, double(p->obstacles[i][2]) - std::sqrt(ox * ox + oy * oy));
+    }
+    return c + p->obstacle_weight * pen;
+}
+double uni_term_d(const double* s, const uint8_t*, const uint8_t* cp) {
+    const UniCost* p = reinterpret_cast<const UniCost*>(cp);
+    double dx = s[0] - p->x_goal, dy = s[1] - p->y_goal;
+    return p->pos_weight * (dx * dx + dy * dy) * 10.0;
+}
+
+struct Callable {
+    const char* model;
+    const char* name;
+    int role;  // 0 dynamics, 1 stage, 2 terminal
+    void* f32;
+    void* f64;
+};
+
+#define FP(x) reinterpret_cast<void*>(x)
+const Callable kCallables[] = {
+    {"double_integrator", "mppi_dynamics", 0,
+     FP(+[](float* x, const float* u, const uint8_t* m, const RngCtx& r) {
+         plug_di::mppi_dynamics(mppi_state_ref{x}, u, m, to_rng(r));
+     }),
+     FP(&di_dyn_d)},
+    {"double_integrator", "mppi_stage_cost", 1,
+     FP(+[](const float* x, const float* u, const uint8_t* m, const uint8_t* c) {
+         return plug_di::mppi_stage_cost(x, u, m, c);
+     }),
+     FP(&di_cost_d)},
+    {"double_integrator", "mppi_terminal_cost", 2,
+     FP(+[](const float* x, const uint8_t* m, const uint8_t* c) {
+         return plug_di::mppi_terminal_cost(x, m, c);
+     }),
+     FP(&di_term_d)},
+    {"unicycle", "mppi_dynamics", 0,
+     FP(+[](float* x, const float* u, const uint8_t* m, const RngCtx& r) {
+         plug_uni::mppi_dynamics(x, u, m, to_rng(r));
+     }),
+     FP(&uni_dyn_d)},
+    {"unicycle", "mppi_stage_cost", 1,
+     FP(+[](const float* x, const float* u, const uint8_t* m, const uint8_t* c) {
+         return plug_uni::mppi_stage_cost(x, u, m, c);
+     }),
+     FP(&uni_stage_d)},
+    {"unicycle", "unicycle_stage_cost_obstacles", 1,
+     FP(+[](const float* x, const float* u, const uint8_t* m, const uint8_t* c) {
+         return plug_uni::unicycle_stage_cost_obstacles(x, u, m, c);
+     }),
+     FP(&uni_stage_obs_d)},
+    {"unicycle", "mppi_terminal_cost", 2,
+     FP(+[](const float* x, const uint8_t* m, const uint8_t* c) {
+         return plug_uni::mppi_terminal_cost(x, m, c);
+     }),
+     FP(&uni_term_d)},
+    {"unicycle", "unicycle_terminal_cost", 2,
+     FP(+[](const float* x, const uint8_t* m, const uint8_t* c) {
+         return plug_uni::unicycle_terminal_cost(x, m, c);
+     }),
+     FP(&uni_term_d)},
+};
+#undef FP
+
+const Callable* find_callable(const char* model, const char* name, int role) {
+    for (const Callable& c : kCallables)
+        if (c.role == role && std::strcmp(c.model, model) == 0 && std::strcmp(c.name, name) == 0)
+            return &c;
+    return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Problem description shared by all oracle entry points.
+struct oracle_problem {
+    uint32_t D, C, H, K;
+    float lambda;
+    uint32_t cov_full;        // 0: sigma = C std devs; 1: sigma = C*C covariance
+    const float* sigma;
+    const float* u_min;
+    const float* u_max;
+    uint64_t seed;
+    const char* model;        // "double_integrator" | "unicycle"
+    const char* dynamics;
+    const char* stage_cost;   // NULL or "" -> zero cost
+    const char* terminal_cost;
+    const void* model_params;
+    const void* cost_params;
+};
+
+}  // extern "C"
+
+namespace {
+
+template <typename T> struct Resolved {
+    DynFn<T> dyn = nullptr;
+    StageFn<T> stage = nullptr;
+    TermFn<T> term = nullptr;
+    std::vector<T> scale;     // sigma or lower Cholesky factor
+    std::vector<T> lo, hi;
+    const uint8_t* model = nullptr;
+    const uint8_t* cost = nullptr;
+    bool ok = false;
+};
+
+// Lower Cholesky factor, computed in double and rounded to float exactly as
+// the library's host code does before upload.
+bool cholesky_f(const float* a, uint32_t n, std::vector<float>* out) {
+    std::vector<double> L(size_t(n) * n, 0.0);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t j = 0; j <= i; ++j) {
+            double s = double(a[i * n + j]);
+            for (uint32_t k = 0; k < j; ++k) s -= L[i * n + k] * L[j * n + k];
+            if (i == j) {
+                if (!(s > 0.0)) return false;
+                L[i * n + i] = std::sqrt(s);
+            } else {
+                L[i * n + j] = s / L[j * n + j];
+            }
+        }
+    out->assign(size_t(n) * n, 0.0f);
+    for (size_t i = 0; i < L.size(); ++i) (*out)[i] = float(L[i]);
+    return true;
+}
+
+template <typename T> Resolved<T> resolve(const oracle_problem* p) {
+    Resolved<T> r;
+    const bool f32 = sizeof(T) == 4;
+    const Callable* d = find_callable(p->model, p->dynamics, 0);
+    if (!d) return r;
+    r.dyn = reinterpret_cast<DynFn<T>>(f32 ? d->f32 : d->f64);
+    if (p->stage_cost && *p->stage_cost) {
+        const Callable* s = find_callable(p->model, p->stage_cost, 1);
+        if (!s) return r;
+        r.stage = reinterpret_cast<StageFn<T>>(f32 ? s->f32 : s->f64);
+    } else {
+        r.stage = &zero_stage<T>;
+    }
+    if (p->terminal_cost && *p->terminal_cost) {
+        const Callable* t = find_callable(p->model, p->terminal_cost, 2);
+        if (!t) return r;
+        r.term = reinterpret_cast<TermFn<T>>(f32 ? t->f32 : t->f64);
+    } else {
+        r.term = &zero_term<T>;
+    }
+    if (p->cov_full) {
+        std::vector<float> L;
+        if (!cholesky_f(p->sigma, p->C, &L)) return r;
+        r.scale.assign(L.begin(), L.end());
+    } else {
+        r.scale.assign(p->sigma, p->sigma + p->C);
+    }
+    r.lo.assign(p->u_min, p->u_min + p->C);
+    r.hi.assign(p->u_max, p->u_max + p->C);
+    static const uint8_t zeros[256] = {0};
+    r.model = p->model_params ? static_cast<const uint8_t*>(p->model_params) : zeros;
+    r.cost = p->cost_params ? static_cast<const uint8_t*>(p->cost_params) : zeros;
+    r.ok = true;
+    return r;
+}
+
+// Noise eps(t, .) of sample k: unclamped, already scaled (sigma z or L z).
+template <typename T>
+void noise_eps(const oracle_problem* p, const Resolved<T>& r, const T* z_all, uint32_t t, T* eps) {
+    const uint32_t C = p->C;
+    const T* z = z_all + size_t(t) * C;
+    if (p->cov_full) {
+        for (uint32_t i = 0; i < C; ++i) {
+            T e = T(0);
+            for (uint32_t j = 0; j <= i; ++j) {
+                if (sizeof(T) == 4) e = T(mppi_fma(float(r.scale[i * C + j]), float(z[j]), float(e)));
+                else e = std::fma(r.scale[i * C + j], z[j], e);
+            }
+            eps[i] = e;
+        }
+    } else {
+        for (uint32_t i = 0; i < C; ++i) eps[i] = z[i] * r.scale[i];
+    }
+}
+
+// All H*C normals of sample k (normal n = t*C + d from block n/4, lane n%4).
+template <typename T>
+void sample_normals(const oracle_problem* p, uint32_t k, uint32_t step, uint32_t agent, T* z) {
+    const uint32_t HC = p->H * p->C;
+    T blk[4];
+    for (uint32_t n = 0; n < HC; ++n) {
+        if ((n & 3) == 0) normals4<T>(n >> 2, k, step, MPPI_TAG_CONTROL_NOISE, agent, p->seed, blk);
+        z[n] = blk[n & 3];
+    }
+}
+
+// Rollout of samples [k0, k1) (mppi_rollout.metal:36-96).
+template <typename T>
+void rollout_range(const oracle_problem* p, const Resolved<T>& r, uint32_t step, uint32_t agent,
+                   const float* x0, const T* u_nom, uint32_t k0, uint32_t k1, T* costs) {
+    const uint32_t D = p->D, C = p->C, H = p->H;
+    std::vector<T> z(size_t(H) * C), x(D), u(C), eps(C);
+    for (uint32_t k = k0; k < k1; ++k) {
+        sample_normals<T>(p, k, step, agent, z.data());
+        for (uint32_t i = 0; i < D; ++i) x[i] = T(x0[i]);
+        T total = T(0);
+        for (uint32_t t = 0; t < H; ++t) {
+            noise_eps<T>(p, r, z.data(), t, eps.data());
+            for (uint32_t i = 0; i < C; ++i)
+                u[i] = clampv<T>(u_nom[t * C + i] + eps[i], r.lo[i], r.hi[i]);
+            RngCtx rc{t, k, step, MPPI_TAG_DYNAMICS, agent, p->seed};
+            r.dyn(x.data(), u.data(), r.model, rc);
+            total += r.stage(x.data(), u.data(), r.model, r.cost);
+        }
+        total += r.term(x.data(), r.model, r.cost);
+        costs[k] = total;
+    }
+}
+
+template <typename T>
+void rollout(const oracle_problem* p, const Resolved<T>& r, uint32_t step, uint32_t agent,
+             const float* x0, const T* u_nom, T* costs, int threads) {
+    const uint32_t K = p->K;
+    if (threads <= 1 || K < 256) {
+        rollout_range<T>(p, r, step, agent, x0, u_nom, 0, K, costs);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const uint32_t per = (K + threads - 1) / threads;
+    for (int i = 0; i < threads; ++i) {
+        uint32_t a = std::min<uint32_t>(K, i * per), b = std::min<uint32_t>(K, a + per);
+        if (a >= b) break;
+        pool.emplace_back([=, &r] { rollout_range<T>(p, r, step, agent, x0, u_nom, a, b, costs); });
+    }
+    for (auto& th : pool) th.join();
+}
+
+bool finite(double x) { return std::isfinite(x); }
+
+// Softmax weights and diagnostics (mppi_reduce.metal:14-121).  Non-finite
+// costs get weight 0; if every cost is non-finite the weights are uniform.
+template <typename T>
+void weights(const oracle_problem* p, const T* costs, T* w, T diag[3]) {
+    const uint32_t K = p->K;
+    T m = T(INFINITY);
+    for (uint32_t k = 0; k < K; ++k) {
+        T c = finite(double(costs[k])) ? costs[k] : T(INFINITY);
+        m = c < m ? c : m;
+    }
+    T sum_w = T(0), sum_c = T(0);
+    const bool uniform = !finite(double(m));
+    for (uint32_t k = 0; k < K; ++k) {
+        T e;
+        if (uniform) e = T(1);
+        else e = finite(double(costs[k])) ? Num<T>::exp(-(costs[k] - m) / T(p->lambda)) : T(0);
+        w[k] = e;
+        sum_w += e;
+        sum_c += costs[k];
+    }
+    T sum_w2 = T(0);
+    for (uint32_t k = 0; k < K; ++k) {
+        w[k] = w[k] / sum_w;
+        sum_w2 += w[k] * w[k];
+    }
+    diag[0] = m;
+    diag[1] = sum_c / T(K);
+    diag[2] = T(1) / sum_w2;
+}
+
+// Weighted-noise average (mppi_reduce.metal:130-168): for each (t, d),
+// u = clamp(u_nom + sum_{k=0..K-1} w_k * eps_k(t, d)), k ascending.
+template <typename T>
+void reduce(const oracle_problem* p, const Resolved<T>& r, uint32_t step, uint32_t agent,
+            const T* w, const T* u_nom, T* u_out) {
+    const uint32_t C = p->C, H = p->H, HC = H * C;
+    std::vector<T> acc(HC, T(0)), z(HC), eps(C);
+    for (uint32_t k = 0; k < p->K; ++k) {
+        sample_normals<T>(p, k, step, agent, z.data());
+        for (uint32_t t = 0; t < H; ++t) {
+            noise_eps<T>(p, r, z.data(), t, eps.data());
+            for (uint32_t i = 0; i < C; ++i) acc[t * C + i] += w[k] * eps[i];
+        }
+    }
+    for (uint32_t n = 0; n < HC; ++n) {
+        const uint32_t i = n % C;
+        u_out[n] = clampv<T>(u_nom[n] + acc[n], r.lo[i], r.hi[i]);
+    }
+}
+
+template <typename T>
+int step_impl(const oracle_problem* p, uint32_t step, uint32_t agent, const float* x0,
+              const T* u_nom, T* u_out, T* diag3, T* costs_out, T* weights_out, int threads) {
+    Resolved<T> r = resolve<T>(p);
+    if (!r.ok) return -1;
+    std::vector<T> costs(p->K), w(p->K);
+    T diag[3];
+    rollout<T>(p, r, step, agent, x0, u_nom, costs.data(), threads);
+    weights<T>(p, costs.data(), w.data(), diag);
+    reduce<T>(p, r, step, agent, w.data(), u_nom, u_out);
+    if (diag3) std::copy(diag, diag + 3, diag3);
+    if (costs_out) std::copy(costs.begin(), costs.end(), costs_out);
+    if (weights_out) std::copy(w.begin(), w.end(), weights_out);
+    return 0;
+}
+
+// Closed loop for n_agents agents over T steps: MPPI iteration, history,
+// plant step with the dynamics (counter (0, 0xffffffff, step, PLANT|agent)),
+// hold-last shift (mppi_reduce.metal:175-231, :413-488).
+template <typename T>
+int simulate_impl(const oracle_problem* p, uint32_t T_steps, uint32_t start_step, uint32_t n_agents,
+                  const float* x0_packed, const float* u_nom_packed, float* states_out,
+                  float* controls_out, float* costs_out, float* terminal_out, int threads) {
+    Resolved<T> r = resolve<T>(p);
+    if (!r.ok) return -1;
+    const uint32_t D = p->D, C = p->C, H = p->H, HC = H * C;
+    for (uint32_t a = 0; a < n_agents; ++a) {
+        std::vector<float> x(x0_packed + size_t(a) * D, x0_packed + size_t(a + 1) * D);
+        std::vector<T> un(HC, T(0)), uo(HC), costs(p->K), w(p->K), xs(D);
+        if (u_nom_packed)
+            for (uint32_t n = 0; n < HC; ++n) un[n] = T(u_nom_packed[size_t(a) * HC + n]);
+        for (uint32_t s = 0; s < T_steps; ++s) {
+            const uint32_t step = start_step + s;
+            T diag[3];
+            rollout<T>(p, r, step, a, x.data(), un.data(), costs.data(), threads);
+            weights<T>(p, costs.data(), w.data(), diag);
+            reduce<T>(p, r, step, a, w.data(), un.data(), uo.data());
+            const size_t row = size_t(s) * n_agents + a;
+            if (states_out) for (uint32_t i = 0; i < D; ++i) states_out[row * D + i] = x[i];
+            if (controls_out) for (uint32_t i = 0; i < C; ++i) controls_out[row * C + i] = float(uo[i]);
+            if (costs_out) costs_out[row] = float(diag[0]);
+            for (uint32_t i = 0; i < D; ++i) xs[i] = T(x[i]);
+            RngCtx rc{0, 0xffffffffu, step, MPPI_TAG_PLANT, a, p->seed};
+            r.dyn(xs.data(), uo.data(), r.model, rc);
+            for (uint32_t i = 0; i < D; ++i) x[i] = float(xs[i]);
+            for (uint32_t n = 0; n < HC; ++n) {
+                const uint32_t t = n / C, i = n % C;
+                un[n] = uo[(t + 1 < H ? t + 1 : H - 1) * C + i];
+            }
+        }
+        if (terminal_out) for (uint32_t i = 0; i < D; ++i) terminal_out[size_t(a) * D + i] = x[i];
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void oracle_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    oracle_philox4x32_10(ctr, key, out);
+}
+void oracle_philox2x32(const uint32_t ctr[2], uint32_t key, uint32_t out[2]) {
+    oracle_philox2x32_10(ctr, key, out);
+}
+void oracle_normals4(uint32_t block, uint32_t sample, uint32_t step, uint32_t tag, uint32_t agent,
+                     uint64_t seed, float out[4]) {
+    normals4<float>(block, sample, step, tag, agent, seed, out);
+}
+void oracle_normals4_64(uint32_t block, uint32_t sample, uint32_t step, uint32_t tag,
+                        uint32_t agent, uint64_t seed, double out[4]) {
+    normals4<double>(block, sample, step, tag, agent, seed, out);
+}
+
+// The shared single-precision elementary functions, compiled for the host,
+// so tests can bound their error against FP64 libm.
+// fn: 0 exp, 1 log, 2 sin, 3 cos, 4 sin(pi x), 5 cos(pi x), 6 tanh, 7 sqrt
+void oracle_mathf(int fn, const float* x, float* y, size_t n) {
+    for (size_t i = 0; i < n; ++i) {
+        float s, c;
+        switch (fn) {
+            case 0: y[i] = mppi_expf(x[i]); break;
+            case 1: y[i] = mppi_logf(x[i]); break;
+            case 2: y[i] = mppi_sinf(x[i]); break;
+            case 3: y[i] = mppi_cosf(x[i]); break;
+            case 4: mppi_sincospif(x[i], &s, &c); y[i] = s; break;
+            case 5: mppi_sincospif(x[i], &s, &c); y[i] = c; break;
+            case 6: y[i] = mppi_tanhf(x[i]); break;
+            default: y[i] = mppi_sqrtf(x[i]); break;
+        }
+    }
+}
+void oracle_fmodf(const float* x, const float* y, float* out, size_t n) {
+    for (size_t i = 0; i < n; ++i) out[i] = mppi_fmodf(x[i], y[i]);
+}
+
+int oracle_rollout(const oracle_problem* p, uint32_t step, uint32_t agent, const float* x0,
+                   const float* u_nom, float* costs, int threads) {
+    Resolved<float> r = resolve<float>(p);
+    if (!r.ok) return -1;
+    rollout<float>(p, r, step, agent, x0, u_nom, costs, threads);
+    return 0;
+}
+
+int oracle_rollout64(const oracle_problem* p, uint32_t step, uint32_t agent, const float* x0,
+                     const double* u_nom, double* costs, int threads) {
+    Resolved<double> r = resolve<double>(p);
+    if (!r.ok) return -1;
+    rollout<double>(p, r, step, agent, x0, u_nom, costs, threads);
+    return 0;
+}
+
+int oracle_weights(const oracle_problem* p, const float* costs, float* w, float* diag3) {
+    weights<float>(p, costs, w, diag3);
+    return 0;
+}
+
+int oracle_weights64(const oracle_problem* p, const double* costs, double* w, double* diag3) {
+    weights<double>(p, costs, w, diag3);
+    return 0;
+}
+
+int oracle_reduce(const oracle_problem* p, uint32_t step, uint32_t agent, const float* w,
+                  const float* u_nom, float* u_out) {
+    Resolved<float> r = resolve<float>(p);
+    if (!r.ok) return -1;
+    reduce<float>(p, r, step, agent, w, u_nom, u_out);
+    return 0;
+}
+
+int oracle_reduce64(const oracle_problem* p, uint32_t step, uint32_t agent, const double* w,
+                    const double* u_nom, double* u_out) {
+    Resolved<double> r = resolve<double>(p);
+    if (!r.ok) return -1;
+    reduce<double>(p, r, step, agent, w, u_nom, u_out);
+    return 0;
+}
+
+int oracle_step(const oracle_problem* p, uint32_t step, uint32_t agent, const float* x0,
+                const float* u_nom, float* u_out, float* diag3, float* costs_out,
+                float* weights_out, int threads) {
+    return step_impl<float>(p, step, agent, x0, u_nom, u_out, diag3, costs_out, weights_out, threads);
+}
+
+int oracle_step64(const oracle_problem* p, uint32_t step, uint32_t agent, const float* x0,
+                  const double* u_nom, double* u_out, double* diag3, double* costs_out,
+                  double* weights_out, int threads) {
+    return step_impl<double>(p, step, agent, x0, u_nom, u_out, diag3, costs_out, weights_out, threads);
+}
+
+int oracle_simulate(const oracle_problem* p, uint32_t T, uint32_t start_step, uint32_t n_agents,
+                    const float* x0_packed, const float* u_nom_packed, float* states_out,
+                    float* controls_out, float* costs_out, float* terminal_out, int threads) {
+    return simulate_impl<float>(p, T, start_step, n_agents, x0_packed, u_nom_packed, states_out,
+                                controls_out, costs_out, terminal_out, threads);
+}
+
+int oracle_simulate64(const oracle_problem* p, uint32_t T, uint32_t start_step, uint32_t n_agents,
+                      const float* x0_packed, const float* u_nom_packed, float* states_out,
+                      float* controls_out, float* costs_out, float* terminal_out, int threads) {
+    return simulate_impl<double>(p, T, start_step, n_agents, x0_packed, u_nom_packed, states_out,
+                                 controls_out, costs_out, terminal_out, threads);
+}
+
+}  // extern "C"

@@ -1,0 +1,581 @@
+// Port of the reference's behavioural driver tests
+// (tests/mppi_driver_tests.cpp:78-1038, 46 tests) to the HIP-backed
+// MPPIDriver.  Same fixture (D=4, C=2, H=10, K=64, lambda=1, sigma=[1,1],
+// bounds +-1; :42-72), same plugin model (the double integrator,
+// mppi_metal_amd/plugins/double_integrator.hip), same assertions and error
+// key words.
+//
+//   mppi_driver_tests              run all 46 tests (needs a GPU)
+//   mppi_driver_tests --cpu-only   run the 21 tests that never reach the
+//                                  device (validation, reset, pre-init)
+// The plugin path comes from MPPI_TEST_PLUGIN, else the in-tree plugin next
+// to this binary.  Exit status 0 iff every executed check passed.
+#include "mppi_metal/mppi_driver.hpp"
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <iostream>
+#include <string>
+#include <unistd.h>
+#include <vector>
+
+namespace {
+
+int g_pass = 0, g_fail = 0;
+std::string g_plugin;
+
+bool check(bool ok, const std::string& what) {
+    (ok ? std::cout : std::cerr) << (ok ? "[PASS] " : "[FAIL] ") << what << '\n';
+    ok ? ++g_pass : ++g_fail;
+    return ok;
+}
+bool contains(const std::string& s, const char* sub) { return s.find(sub) != std::string::npos; }
+
+using namespace mppi_metal;
+
+struct Fixture {
+    std::vector<float> sigma{1.0f, 1.0f}, sigma_inv{1.0f, 1.0f}, u_min{-1.0f, -1.0f}, u_max{1.0f, 1.0f};
+    DriverConfig config;
+    Fixture() {
+        config.state_dim = 4;
+        config.control_dim = 2;
+        config.horizon = 10;
+        config.sample_count = 64;
+        config.lambda = 1.0f;
+        config.noise.covariance_form = ControlNoiseConfig::CovarianceForm::kDiagonal;
+        config.noise.sigma = sigma.data();
+        config.noise.sigma_inv = sigma_inv.data();
+        config.noise.control_dim = 2;
+        config.bounds.u_min = u_min.data();
+        config.bounds.u_max = u_max.data();
+        config.bounds.control_dim = 2;
+    }
+};
+
+// initialize() on a config mutated by `mutate` must fail, naming `key`.
+void expect_init_fails(const std::function<void(Fixture&)>& mutate, const char* key,
+                       const std::string& what) {
+    Fixture f;
+    mutate(f);
+    MPPIDriver d(f.config);
+    std::string e;
+    ModelPluginSpec m;
+    check(!d.initialize(m, &e), "initialize should fail for " + what);
+    check(contains(e, key), std::string("error should mention ") + key);
+}
+
+MPPIDriver make_driver(const ParamLayout& layout = {}, const std::string& stage = "") {
+    Fixture f;
+    MPPIDriver d(f.config);
+    ModelPluginSpec m;
+    m.user_metallib_path = g_plugin;
+    m.param_layout = layout;
+    if (!stage.empty()) m.callables.stage_cost = stage;
+    std::string e;
+    if (!d.initialize(m, &e)) std::cerr << "[SETUP] initialize failed: " << e << '\n';
+    return d;
+}
+
+// ---------------- GPU-free tests ----------------
+void dims_and_noise_and_bounds() {
+    expect_init_fails([](Fixture& f) { f.config.state_dim = 0; }, "state_dim", "state_dim == 0");
+    expect_init_fails([](Fixture& f) { f.config.control_dim = 0; }, "control_dim", "control_dim == 0");
+    expect_init_fails([](Fixture& f) { f.config.horizon = 0; }, "horizon", "horizon == 0");
+    expect_init_fails([](Fixture& f) { f.config.sample_count = 0; }, "sample_count", "sample_count == 0");
+    expect_init_fails([](Fixture& f) { f.config.noise.control_dim = 99; }, "control_dim",
+                      "noise.control_dim mismatch");
+    expect_init_fails([](Fixture& f) { f.config.noise.sigma = nullptr; }, "sigma", "null sigma");
+    expect_init_fails([](Fixture& f) { f.config.noise.sigma_inv = nullptr; }, "sigma_inv", "null sigma_inv");
+    expect_init_fails([](Fixture& f) { f.config.bounds.control_dim = 99; }, "control_dim",
+                      "bounds.control_dim mismatch");
+    expect_init_fails([](Fixture& f) { f.config.bounds.u_min = nullptr; }, "u_min", "null u_min");
+    expect_init_fails([](Fixture& f) { f.config.bounds.u_max = nullptr; }, "u_max", "null u_max");
+    expect_init_fails(
+        [](Fixture& f) {
+            f.u_min[1] = 5.0f;
+            f.u_max[1] = -5.0f;
+        },
+        "u_min", "u_min > u_max");
+}
+
+void null_error_pointer() {
+    Fixture f;
+    f.config.state_dim = 0;
+    MPPIDriver d(f.config);
+    ModelPluginSpec m;
+    check(!d.initialize(m, nullptr), "initialize should fail with null error pointer without crashing");
+}
+
+void empty_plugin_path() {
+    Fixture f;
+    MPPIDriver d(f.config);
+    std::string e;
+    ModelPluginSpec m;
+    m.user_metallib_path = "";
+    check(!d.initialize(m, &e), "initialize should fail for empty plugin path");
+}
+
+void reset_tests() {
+    {
+        Fixture f;
+        MPPIDriver d(f.config);
+        std::string e;
+        check(d.reset(&e), "reset() with no args should succeed");
+    }
+    {
+        Fixture f;
+        MPPIDriver d(f.config);
+        check(d.reset(nullptr), "reset(nullptr) should succeed without crashing");
+    }
+    {
+        Fixture f;
+        MPPIDriver d(f.config);
+        std::vector<float> nominal(20, 0.5f);
+        std::string e;
+        check(d.reset(ControlSequenceView{nominal.data(), 10, 2}, &e),
+              "reset(valid ControlSequenceView) should succeed");
+    }
+    {
+        Fixture f;
+        MPPIDriver d(f.config);
+        std::vector<float> nominal(100, 0.0f);
+        std::string e;
+        check(!d.reset(ControlSequenceView{nominal.data(), 11, 2}, &e), "reset should fail for horizon mismatch");
+        check(contains(e, "horizon"), "error should mention horizon");
+    }
+    {
+        Fixture f;
+        MPPIDriver d(f.config);
+        std::vector<float> nominal(100, 0.0f);
+        std::string e;
+        check(!d.reset(ControlSequenceView{nominal.data(), 10, 3}, &e),
+              "reset should fail for control_dim mismatch");
+        check(contains(e, "control_dim"), "error should mention control_dim");
+    }
+    {
+        Fixture f;
+        MPPIDriver d(f.config);
+        std::string e;
+        check(!d.reset(ControlSequenceView{nullptr, 10, 2}, &e),
+              "reset should fail for null data with nonzero dims");
+        check(contains(e, "null"), "error should mention null data");
+    }
+    {
+        Fixture f;
+        MPPIDriver d(f.config);
+        check(!d.reset(ControlSequenceView{nullptr, 999, 999}, nullptr),
+              "reset with bad shape and null error should fail without crashing");
+    }
+}
+
+void pre_init_calls() {
+    {
+        Fixture f;
+        MPPIDriver d(f.config);
+        ProblemParams p;
+        std::string e;
+        check(!d.set_problem_params(p, &e), "set_problem_params should fail before initialize");
+        check(contains(e, "not initialized"), "error should mention not initialized");
+    }
+    {
+        Fixture f;
+        MPPIDriver d(f.config);
+        std::vector<float> x0(4, 0.0f), u(20, 0.0f);
+        StepInputs in;
+        in.x0 = {x0.data(), 4};
+        StepDiagnostics diag;
+        std::string e;
+        check(!d.step(in, MutableControlSequenceView{u.data(), 10, 2}, &diag, &e),
+              "step should fail before initialize");
+    }
+}
+
+// ---------------- GPU tests ----------------
+void init_tests() {
+    {
+        Fixture f;
+        MPPIDriver d(f.config);
+        ModelPluginSpec m;
+        m.user_metallib_path = g_plugin;
+        std::string e;
+        check(d.initialize(m, &e), "initialize should succeed with valid config + valid plugin: " + e);
+    }
+    {
+        DriverConfig config;
+        config.state_dim = 4;
+        config.control_dim = 2;
+        config.horizon = 10;
+        config.sample_count = 64;
+        {
+            std::vector<float> sigma{1, 1}, sigma_inv{1, 1}, lo{-1, -1}, hi{1, 1};
+            config.noise.sigma = sigma.data();
+            config.noise.sigma_inv = sigma_inv.data();
+            config.noise.control_dim = 2;
+            config.bounds.u_min = lo.data();
+            config.bounds.u_max = hi.data();
+            config.bounds.control_dim = 2;
+        }
+        MPPIDriver d(config);  // sources destroyed: the constructor deep-copied them
+        ModelPluginSpec m;
+        m.user_metallib_path = g_plugin;
+        std::string e;
+        check(d.initialize(m, &e), "deep-copied data should pass validation and init: " + e);
+    }
+    {
+        Fixture f;
+        MPPIDriver d(f.config);
+        ModelPluginSpec m;
+        m.user_metallib_path = "/nonexistent/path.hip";
+        std::string e;
+        check(!d.initialize(m, &e), "initialize should fail for nonexistent plugin");
+    }
+    {
+        Fixture f;
+        MPPIDriver d(f.config);
+        ModelPluginSpec m;
+        m.user_metallib_path = g_plugin;
+        m.callables.dynamics = "nonexistent_fn";
+        std::string e;
+        check(!d.initialize(m, &e), "initialize should fail for missing dynamics callable");
+        check(contains(e, "nonexistent_fn"), "error should mention the missing callable name");
+    }
+}
+
+void set_params_tests() {
+    {
+        auto d = make_driver();
+        ProblemParams p;
+        std::string e;
+        check(d.set_problem_params(p, &e), "set_problem_params should succeed with zero layout + zero params: " + e);
+    }
+    {
+        ParamLayout L;
+        L.model_params_size = 16;
+        auto d = make_driver(L);
+        uint8_t buf[8];
+        ProblemParams p;
+        p.model_params = {buf, 8};
+        check(!d.set_problem_params(p, nullptr), "set_problem_params should fail for model size mismatch");
+    }
+    {
+        ParamLayout L;
+        L.cost_params_size = 32;
+        auto d = make_driver(L);
+        uint8_t buf[16];
+        ProblemParams p;
+        p.cost_params = {buf, 16};
+        check(!d.set_problem_params(p, nullptr), "set_problem_params should fail for cost size mismatch");
+    }
+    {
+        ParamLayout L;
+        L.model_params_size = 16;
+        auto d = make_driver(L);
+        ProblemParams p;
+        p.model_params = {nullptr, 16};
+        check(!d.set_problem_params(p, nullptr),
+              "set_problem_params should fail for null data with nonzero bytes");
+    }
+    {
+        ParamLayout L;
+        L.model_params_size = 16;
+        L.cost_params_size = 8;
+        auto d = make_driver(L);
+        std::vector<uint8_t> mb(16, 0), cb(8, 0);
+        ProblemParams p;
+        p.model_params = {mb.data(), 16};
+        p.cost_params = {cb.data(), 8};
+        std::string e;
+        check(d.set_problem_params(p, &e), "set_problem_params should succeed with matching sizes: " + e);
+    }
+    {
+        ParamLayout L;
+        L.model_params_size = 16;
+        auto d = make_driver(L);
+        ProblemParams p;
+        check(!d.set_problem_params(p, nullptr),
+              "set_problem_params should fail when layout requires params but none provided");
+    }
+}
+
+void step_validation_tests() {
+    std::vector<float> x0(4, 0.0f), u(20, 0.0f);
+    {
+        auto d = make_driver();
+        StepInputs in;
+        in.x0 = {nullptr, 4};
+        check(!d.step(in, MutableControlSequenceView{u.data(), 10, 2}, nullptr, nullptr),
+              "step should fail for null x0");
+    }
+    {
+        auto d = make_driver();
+        StepInputs in;
+        in.x0 = {x0.data(), 99};
+        check(!d.step(in, MutableControlSequenceView{u.data(), 10, 2}, nullptr, nullptr),
+              "step should fail for x0 state_dim mismatch");
+    }
+    {
+        auto d = make_driver();
+        StepInputs in;
+        in.x0 = {x0.data(), 4};
+        check(!d.step(in, MutableControlSequenceView{nullptr, 10, 2}, nullptr, nullptr),
+              "step should fail for null output data");
+    }
+    {
+        auto d = make_driver();
+        std::vector<float> ones(4, 1.0f), out(20, -999.0f);
+        StepInputs in;
+        in.x0 = {ones.data(), 4};
+        StepDiagnostics diag;
+        std::string e;
+        check(d.step(in, MutableControlSequenceView{out.data(), 10, 2}, &diag, &e),
+              "step should succeed with valid inputs: " + e);
+    }
+    {
+        auto d = make_driver();
+        std::vector<float> ovr(100, 0.0f);
+        StepInputs in;
+        in.x0 = {x0.data(), 4};
+        in.u_nominal_override = {ovr.data(), 99, 99};
+        check(!d.step(in, MutableControlSequenceView{u.data(), 10, 2}, nullptr, nullptr),
+              "step should fail for u_nominal_override shape mismatch");
+    }
+}
+
+void step_behaviour_tests() {
+    {
+        auto d = make_driver({}, "mppi_stage_cost");
+        std::vector<float> x0{1.0f, 1.0f, 0.0f, 0.0f}, u(20, 0.0f);
+        StepInputs in;
+        in.x0 = {x0.data(), 4};
+        StepDiagnostics diag;
+        std::string e;
+        check(d.step(in, MutableControlSequenceView{u.data(), 10, 2}, &diag, &e),
+              "step with stage cost should succeed: " + e);
+        check(u[0] < 0.0f, "ax should be negative to move toward origin (got " + std::to_string(u[0]) + ")");
+        check(u[1] < 0.0f, "ay should be negative to move toward origin (got " + std::to_string(u[1]) + ")");
+        check(diag.best_cost > 0.0f, "best_cost should be > 0 (got " + std::to_string(diag.best_cost) + ")");
+        check(diag.mean_cost >= diag.best_cost, "mean_cost should be >= best_cost");
+        check(diag.effective_sample_size > 0.0f, "ESS should be > 0");
+    }
+    {
+        auto d1 = make_driver({}, "mppi_stage_cost");
+        auto d2 = make_driver({}, "mppi_stage_cost");
+        std::vector<float> x0{1.0f, 1.0f, 0.0f, 0.0f}, u1(20, 0.0f), u2(20, 0.0f);
+        StepInputs in;
+        in.x0 = {x0.data(), 4};
+        d1.step(in, MutableControlSequenceView{u1.data(), 10, 2}, nullptr, nullptr);
+        d2.step(in, MutableControlSequenceView{u2.data(), 10, 2}, nullptr, nullptr);
+        check(std::memcmp(u1.data(), u2.data(), sizeof(float) * 20) == 0,
+              "Same seed should produce identical outputs (Philox determinism)");
+    }
+}
+
+bool close_all(const std::vector<float>& a, const std::vector<float>& b, float tol) {
+    if (a.size() != b.size()) return false;
+    for (size_t i = 0; i < a.size(); ++i)
+        if (!(std::fabs(a[i] - b[i]) <= tol)) return false;
+    return true;
+}
+
+void simulate_tests() {
+    const uint32_t H = 10, C = 2, D = 4;
+    {
+        // simulate() vs a loop of step() with the same plant on the host.
+        auto d1 = make_driver({}, "mppi_stage_cost");
+        auto d2 = make_driver({}, "mppi_stage_cost");
+        const uint32_t N = 5;
+        std::vector<float> x{1.0f, 1.0f, 0.0f, 0.0f}, u(H * C, 0.0f);
+        std::vector<float> hs(N * D), hc(N * C), hk(N);
+        for (uint32_t i = 0; i < N; ++i) {
+            StepInputs in;
+            in.x0 = {x.data(), D};
+            StepDiagnostics diag;
+            d1.step(in, MutableControlSequenceView{u.data(), H, C}, &diag, nullptr);
+            for (uint32_t k = 0; k < D; ++k) hs[i * D + k] = x[k];
+            for (uint32_t k = 0; k < C; ++k) hc[i * C + k] = u[k];
+            hk[i] = diag.best_cost;
+            const float dt = 0.1f;
+            x[0] += x[2] * dt;
+            x[1] += x[3] * dt;
+            x[2] += u[0] * dt;
+            x[3] += u[1] * dt;
+        }
+        std::vector<float> x0{1.0f, 1.0f, 0.0f, 0.0f}, ss(N * D), sc(N * C), sk(N);
+        SimulationResults r;
+        r.states_out = ss.data();
+        r.controls_out = sc.data();
+        r.costs_out = sk.data();
+        r.num_steps = N;
+        StepInputs in;
+        in.x0 = {x0.data(), D};
+        std::string e;
+        check(d2.simulate(N, in, r, &e), "simulate() should succeed: " + e);
+        check(close_all(hs, ss, 1e-4f) && close_all(hc, sc, 1e-4f) && close_all(hk, sk, 1e-4f),
+              "Multi-step simulate() should exactly match N iterations of step()");
+    }
+    {
+        // simulate_batch(N=1) vs simulate().
+        auto d1 = make_driver({}, "mppi_stage_cost");
+        auto d2 = make_driver({}, "mppi_stage_cost");
+        const uint32_t T = 5;
+        std::vector<float> x0{1.0f, 1.0f, 0.0f, 0.0f};
+        std::vector<float> s1(T * D), c1(T * C), k1(T), s2(T * D), c2(T * C), k2(T);
+        SimulationResults r1;
+        r1.states_out = s1.data();
+        r1.controls_out = c1.data();
+        r1.costs_out = k1.data();
+        r1.num_steps = T;
+        StepInputs in;
+        in.x0 = {x0.data(), D};
+        std::string e;
+        check(d1.simulate(T, in, r1, &e), "simulate() should succeed: " + e);
+        BatchSimulationResults r2;
+        r2.states_out = s2.data();
+        r2.controls_out = c2.data();
+        r2.costs_out = k2.data();
+        r2.num_agents = 1;
+        r2.num_steps = T;
+        check(d2.simulate_batch(1, T, x0.data(), r2, &e), "simulate_batch(N=1) should succeed: " + e);
+        check(close_all(s1, s2, 1e-4f) && close_all(c1, c2, 1e-4f) && close_all(k1, k2, 1e-4f),
+              "simulate_batch(N=1) should match simulate()");
+    }
+    {
+        auto d = make_driver({}, "mppi_stage_cost");
+        const uint32_t N = 4, T = 5;
+        std::vector<float> x0{1, 1, 0, 0, -1, 1, 0, 0, 1, -1, 0, 0, -1, -1, 0, 0};
+        std::vector<float> st(N * T * D), ct(N * T * C), kt(N * T);
+        BatchSimulationResults r;
+        r.states_out = st.data();
+        r.controls_out = ct.data();
+        r.costs_out = kt.data();
+        r.num_agents = N;
+        r.num_steps = T;
+        std::string e;
+        check(d.simulate_batch(N, T, x0.data(), r, &e), "simulate_batch(N=4) should succeed: " + e);
+        // Same index arithmetic as the reference test (:846-853).
+        bool distinct = true;
+        for (uint32_t a = 1; a < N && distinct; ++a) {
+            bool same = true;
+            for (uint32_t k = 0; k < D; ++k)
+                if (st[a * T * D + k] != st[k]) same = false;
+            if (same) distinct = false;
+        }
+        check(distinct, "Agents should have distinct initial states");
+        bool pos = true;
+        for (float c : kt) pos = pos && c > 0.0f;
+        check(pos, "All batch costs should be > 0");
+    }
+    {
+        auto dt = make_driver({}, "mppi_stage_cost");
+        auto dx = make_driver({}, "mppi_stage_cost");
+        const uint32_t N = 3, T = 6;
+        std::vector<float> x0{1.0f, 1.0f, 0, 0, -1.5f, 0.5f, 0, 0, 0.5f, -1.0f, 0, 0};
+        std::vector<float> term(N * D, 0.0f);
+        BatchSimulationResults rt;
+        rt.terminal_states_out = term.data();
+        rt.num_agents = N;
+        rt.num_steps = T;
+        std::string e;
+        check(dt.simulate_batch(N, T, x0.data(), rt, &e),
+              "simulate_batch with terminal_states_out should succeed: " + e);
+        std::vector<float> ext(N * (T + 1) * D, 0.0f);
+        BatchSimulationResults rx;
+        rx.states_out = ext.data();
+        rx.num_agents = N;
+        rx.num_steps = T + 1;
+        check(dx.simulate_batch(N, T + 1, x0.data(), rx, &e),
+              "simulate_batch(T+1) for terminal-state reference should succeed: " + e);
+        std::vector<float> want(ext.begin() + size_t(T) * N * D, ext.end());
+        check(close_all(want, term, 1e-4f),
+              "terminal_states_out should match post-propagation state after final step");
+    }
+    {
+        auto dr = make_driver({}, "mppi_stage_cost");
+        auto dt = make_driver({}, "mppi_stage_cost");
+        const uint32_t N = 4, T = 5;
+        std::vector<float> x0{1, 1, 0, 0, -1, 1, 0, 0, 1, -1, 0, 0, -1, -1, 0, 0};
+        std::vector<float> st(N * T * D), tr(N * D), to(N * D);
+        BatchSimulationResults rr;
+        rr.states_out = st.data();
+        rr.terminal_states_out = tr.data();
+        rr.num_agents = N;
+        rr.num_steps = T;
+        std::string e;
+        check(dr.simulate_batch(N, T, x0.data(), rr, &e), "reference simulate_batch should succeed: " + e);
+        BatchSimulationResults ro;
+        ro.terminal_states_out = to.data();
+        ro.num_agents = N;
+        ro.num_steps = T;
+        check(dt.simulate_batch(N, T, x0.data(), ro, &e),
+              "simulate_batch with terminal_states_out only should succeed: " + e);
+        check(close_all(tr, to, 1e-4f),
+              "terminal-only output should match reference full-output terminal states");
+    }
+    {
+        auto d = make_driver({}, "mppi_stage_cost");
+        std::vector<float> x0{1, 0, 0, 0, -1, 0, 0, 0};
+        BatchSimulationResults r;
+        r.num_agents = 2;
+        r.num_steps = 4;
+        std::string e;
+        check(d.simulate_batch(2, 4, x0.data(), r, &e),
+              "simulate_batch should succeed with all optional outputs set to null: " + e);
+    }
+    {
+        auto d = make_driver({}, "mppi_stage_cost");
+        const uint32_t N = 3, T = 5;
+        std::vector<float> x0{1, 1, 0, 0, -1, 1, 0, 0, 0, -1, 0, 0};
+        std::vector<float> ct(N * T * C, 0.0f), kt(N * T, 0.0f);
+        BatchSimulationResults r;
+        r.controls_out = ct.data();
+        r.costs_out = kt.data();
+        r.num_agents = N;
+        r.num_steps = T;
+        std::string e;
+        check(d.simulate_batch(N, T, x0.data(), r, &e),
+              "simulate_batch with controls_out + costs_out only should succeed: " + e);
+        bool in_bounds = true;
+        for (float u : ct) in_bounds = in_bounds && u >= -1.0001f && u <= 1.0001f;
+        check(in_bounds, "controls_out should respect configured bounds [-1, 1]");
+        bool pos = true;
+        for (float c : kt) pos = pos && c > 0.0f;
+        check(pos, "costs_out should contain positive costs");
+    }
+}
+
+std::string default_plugin_path(const char* argv0) {
+    std::string self = argv0;
+    char buf[4096];
+    ssize_t n = readlink("/proc/self/exe", buf, sizeof(buf) - 1);
+    if (n > 0) {
+        buf[n] = 0;
+        self = buf;
+    }
+    std::string dir = self.substr(0, self.rfind('/'));
+    return dir + "/../../mppi_metal_amd/plugins/double_integrator.hip";
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    bool cpu_only = argc > 1 && std::strcmp(argv[1], "--cpu-only") == 0;
+    const char* env = std::getenv("MPPI_TEST_PLUGIN");
+    g_plugin = env && *env ? env : default_plugin_path(argv[0]);
+
+    dims_and_noise_and_bounds();
+    null_error_pointer();
+    empty_plugin_path();
+    reset_tests();
+    pre_init_calls();
+    if (!cpu_only) {
+        init_tests();
+        set_params_tests();
+        step_validation_tests();
+        step_behaviour_tests();
+        simulate_tests();
+    }
+    std::cout << "\n" << g_pass << " passed, " << g_fail << " failed.\n";
+    return g_fail > 0 ? 1 : 0;
+}
