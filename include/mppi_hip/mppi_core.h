@@ -18,6 +18,7 @@
 //   terminal cost (optional; reference MppiTerminalCostFn, mppi_core.h:24-26):
 //     MPPI_CALLABLE float f(const float* state, const uint8_t* model_params,
 //                           const uint8_t* cost_params);
+//   Either cost may add a trailing `mppi_time` argument (new; see below).
 //
 // The state argument is passed through mppi_state_ref / mppi_cstate_ref and
 // the parameter blobs through mppi_blob_ref, which convert implicitly to
@@ -72,6 +73,16 @@ MPPI_HD mppi_f32x4 mppi_rng_normal4(const mppi_rng& r, uint32_t draw) {
     mppi_box_muller(w.z, w.w, &z.z, &z.w);
     return z;
 }
+
+// Time of a cost evaluation: timestep t within the horizon (0-based; the
+// terminal cost gets t = H) and the MPPI step index.  A stage or terminal
+// cost may take it as a trailing argument (e.g. to follow a moving target);
+// the library detects which form a plugin uses, so plugins without it are
+// called as in the reference.
+struct mppi_time {
+    uint32_t t;
+    uint32_t step;
+};
 
 // Argument proxies (see the header comment).
 struct mppi_state_ref {

@@ -33,7 +33,14 @@
 #define MPPI_HD inline
 #endif
 
-#if !defined(__HIPCC_RTC__)
+#if defined(__HIPCC_RTC__)
+// hipRTC has no <stdint.h>; the compiler's builtin type macros give the same types.
+typedef __UINT8_TYPE__ uint8_t;
+typedef __UINT32_TYPE__ uint32_t;
+typedef __UINT64_TYPE__ uint64_t;
+typedef __INT32_TYPE__ int32_t;
+typedef __SIZE_TYPE__ size_t;
+#else
 #include <stdint.h>
 #endif
 

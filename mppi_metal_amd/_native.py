@@ -1,0 +1,91 @@
+"""ctypes binding of the C ABI declared in include/mppi_hip.h.
+
+The shared library is built in-tree by mppi_metal_amd/build.py.  Loading
+fails loudly (ImportError) when it is missing: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libmppi_hip.so"
+
+MPPI_HIP_OK = 0
+STATUS_NAMES = {
+    0: "OK", -1: "E_INVALID", -2: "E_NOT_INITIALIZED", -3: "E_PLUGIN",
+    -4: "E_COMPILE", -5: "E_DEVICE", -6: "E_NO_MEMORY",
+}
+COV_DIAGONAL, COV_FULL = 0, 1
+FP32, FP16, MIXED = 0, 1, 2
+
+
+class MppiHipConfig(ctypes.Structure):
+    """Mirror of struct mppi_hip_config (include/mppi_hip.h)."""
+
+    _fields_ = [
+        ("state_dim", ctypes.c_uint32),
+        ("control_dim", ctypes.c_uint32),
+        ("horizon", ctypes.c_uint32),
+        ("sample_count", ctypes.c_uint32),
+        ("lambda_", ctypes.c_float),
+        ("covariance_form", ctypes.c_uint32),
+        ("sigma", ctypes.POINTER(ctypes.c_float)),
+        ("u_min", ctypes.POINTER(ctypes.c_float)),
+        ("u_max", ctypes.POINTER(ctypes.c_float)),
+        ("numeric_mode", ctypes.c_uint32),
+        ("device", ctypes.c_int32),
+    ]
+
+
+_F = ctypes.POINTER(ctypes.c_float)
+_VP = ctypes.c_void_p
+_CTX = ctypes.c_void_p
+
+# (name, restype, argtypes) for every entry point of include/mppi_hip.h.
+SIGNATURES = [
+    ("mppi_hip_create", ctypes.c_int, [ctypes.POINTER(MppiHipConfig), ctypes.POINTER(_CTX)]),
+    ("mppi_hip_destroy", None, [_CTX]),
+    ("mppi_hip_last_error", ctypes.c_char_p, [_CTX]),
+    ("mppi_hip_link_plugin", ctypes.c_int,
+     [_CTX, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+      ctypes.c_size_t, ctypes.c_size_t, ctypes.c_uint64]),
+    ("mppi_hip_link_plugin_source", ctypes.c_int,
+     [_CTX, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+      ctypes.c_size_t, ctypes.c_size_t, ctypes.c_uint64]),
+    ("mppi_hip_bind_params", ctypes.c_int, [_CTX, _VP, ctypes.c_size_t, _VP, ctypes.c_size_t]),
+    ("mppi_hip_step", ctypes.c_int,
+     [_CTX, ctypes.c_uint64, _F, _F, _VP, ctypes.c_size_t, _VP, ctypes.c_size_t, _F, _F]),
+    ("mppi_hip_simulate", ctypes.c_int,
+     [_CTX, ctypes.c_uint32, ctypes.c_uint64, _F, _F, _VP, ctypes.c_size_t, _VP, ctypes.c_size_t,
+      _F, _F, _F]),
+    ("mppi_hip_simulate_batch", ctypes.c_int,
+     [_CTX, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, _F, _F, _F, _F, _F, _F]),
+    ("mppi_hip_read_costs", ctypes.c_int, [_CTX, ctypes.c_uint32, _F, ctypes.c_size_t]),
+    ("mppi_hip_read_weights", ctypes.c_int, [_CTX, ctypes.c_uint32, _F, ctypes.c_size_t]),
+    ("mppi_hip_set_reduce_samples_per_thread", ctypes.c_int, [_CTX, ctypes.c_uint32]),
+    ("mppi_hip_profile_enable", ctypes.c_int, [_CTX, ctypes.c_int]),
+    ("mppi_hip_profile_read", ctypes.c_int,
+     [_CTX, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+      ctypes.POINTER(ctypes.c_uint64)]),
+    ("mppi_hip_version", ctypes.c_char_p, []),
+]
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libmppi_hip.so (once) and declare the entry points."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m mppi_metal_amd.build`")
+    lib = ctypes.CDLL(os.fspath(LIB_PATH))
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

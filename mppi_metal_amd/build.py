@@ -32,6 +32,7 @@ NO_PACKED_FP32 = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 BUNDLED_PLUGINS = [
     ("double_integrator.hip", "mppi_dynamics", "mppi_stage_cost", "mppi_terminal_cost", 4, 2),
     ("unicycle.hip", "mppi_dynamics", "unicycle_stage_cost_obstacles", "unicycle_terminal_cost", 4, 2),
+    ("bicycle.hip", "bicycle_dynamics", "bicycle_pursuit_cost", "bicycle_terminal_cost", 6, 2),
 ]
 
 EMBEDDED = [

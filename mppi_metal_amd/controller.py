@@ -1,0 +1,182 @@
+"""Python host binding of the HIP MPPI backend (one C-ABI context).
+
+`HipMPPI` wraps one `mppi_hip_ctx`: it owns a HIP stream on one device and a
+linked plugin.  Its methods are the C ABI's (include/mppi_hip.h), with numpy
+arrays in place of pointers; every call raises `MppiHipError` with the
+library's message on failure.  The C++ `mppi_metal::MPPIDriver` facade is the
+drop-in for the reference's driver; this class is what tests and the
+benchmark use from Python.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+from . import _native
+
+PLUGIN_DIR = Path(__file__).resolve().parent / "plugins"
+
+
+class MppiHipError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{_native.STATUS_NAMES.get(code, code)}] {message}")
+        self.code = code
+        self.message = message
+
+
+def _fptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _f32(a, shape=None):
+    if a is None:
+        return None
+    out = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+    if shape is not None and out.size != int(np.prod(shape)):
+        raise ValueError(f"expected {int(np.prod(shape))} values, got {out.size}")
+    return out
+
+
+def _blob(b):
+    """bytes / numpy array / None -> (c_void_p, nbytes, keepalive)."""
+    if b is None:
+        return None, 0, None
+    if isinstance(b, np.ndarray):
+        arr = np.ascontiguousarray(b)
+        return ctypes.c_void_p(arr.ctypes.data), arr.nbytes, arr
+    buf = ctypes.create_string_buffer(bytes(b), len(b))
+    return ctypes.cast(buf, ctypes.c_void_p), len(b), buf
+
+
+class HipMPPI:
+    """One MPPI controller context on one GPU."""
+
+    def __init__(self, state_dim, control_dim, horizon, sample_count, lam, sigma, u_min, u_max,
+                 covariance_full=False, device=-1):
+        self.lib = _native.load()
+        self.D, self.C, self.H, self.K = int(state_dim), int(control_dim), int(horizon), int(sample_count)
+        self.lam = float(lam)
+        n_sigma = self.C * self.C if covariance_full else self.C
+        self._sigma = _f32(sigma, (n_sigma,))
+        self._umin = _f32(u_min, (self.C,))
+        self._umax = _f32(u_max, (self.C,))
+        cfg = _native.MppiHipConfig(self.D, self.C, self.H, self.K, self.lam,
+                                    _native.COV_FULL if covariance_full else _native.COV_DIAGONAL,
+                                    _fptr(self._sigma), _fptr(self._umin), _fptr(self._umax),
+                                    _native.FP32, int(device))
+        ctx = ctypes.c_void_p()
+        rc = self.lib.mppi_hip_create(ctypes.byref(cfg), ctypes.byref(ctx))
+        if rc != 0:
+            raise MppiHipError(rc, self.lib.mppi_hip_last_error(None).decode())
+        self.ctx = ctx
+        self.model_bytes = 0
+        self.cost_bytes = 0
+
+    # -- lifecycle ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.mppi_hip_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise MppiHipError(rc, self.lib.mppi_hip_last_error(self.ctx).decode())
+
+    # -- plugin and parameters --------------------------------------------
+    def link(self, plugin, dynamics="mppi_dynamics", stage_cost="", terminal_cost="",
+             model_bytes=0, cost_bytes=0, seed=0):
+        """Compile and link a plugin file (a name like "unicycle.hip" is looked
+        up among the bundled plugins)."""
+        p = Path(plugin)
+        if not p.exists() and (PLUGIN_DIR / plugin).exists():
+            p = PLUGIN_DIR / plugin
+        self._check(self.lib.mppi_hip_link_plugin(
+            self.ctx, str(p).encode(), dynamics.encode(), (stage_cost or "").encode(),
+            (terminal_cost or "").encode(), model_bytes, cost_bytes, int(seed) & (2**64 - 1)))
+        self.model_bytes, self.cost_bytes = model_bytes, cost_bytes
+
+    def link_source(self, source, dynamics="mppi_dynamics", stage_cost="", terminal_cost="",
+                    model_bytes=0, cost_bytes=0, seed=0, name="<source>"):
+        self._check(self.lib.mppi_hip_link_plugin_source(
+            self.ctx, source.encode(), name.encode(), dynamics.encode(),
+            (stage_cost or "").encode(), (terminal_cost or "").encode(), model_bytes, cost_bytes,
+            int(seed) & (2**64 - 1)))
+        self.model_bytes, self.cost_bytes = model_bytes, cost_bytes
+
+    def bind_params(self, model=None, cost=None):
+        mp, mn, mk = _blob(model)
+        cp, cn, ck = _blob(cost)
+        self._check(self.lib.mppi_hip_bind_params(self.ctx, mp, mn, cp, cn))
+
+    # -- MPPI --------------------------------------------------------------
+    def step(self, step_index, x0, u_nominal, model_override=None, cost_override=None):
+        """One blocking iteration; returns (u_next [H, C], diag [3])."""
+        x0 = _f32(x0, (self.D,))
+        un = _f32(u_nominal, (self.H * self.C,))
+        mp, mn, mk = _blob(model_override)
+        cp, cn, ck = _blob(cost_override)
+        u_out = np.empty(self.H * self.C, dtype=np.float32)
+        diag = np.empty(3, dtype=np.float32)
+        self._check(self.lib.mppi_hip_step(self.ctx, int(step_index), _fptr(x0), _fptr(un), mp, mn,
+                                           cp, cn, _fptr(u_out), _fptr(diag)))
+        return u_out.reshape(self.H, self.C), diag
+
+    def simulate(self, num_steps, start_step, x0, u_nominal, model_override=None,
+                 cost_override=None, outputs=("states", "controls", "costs")):
+        x0 = _f32(x0, (self.D,))
+        un = _f32(u_nominal, (self.H * self.C,))
+        mp, mn, mk = _blob(model_override)
+        cp, cn, ck = _blob(cost_override)
+        T = int(num_steps)
+        st = np.empty((T, self.D), np.float32) if "states" in outputs else None
+        ct = np.empty((T, self.C), np.float32) if "controls" in outputs else None
+        kt = np.empty(T, np.float32) if "costs" in outputs else None
+        self._check(self.lib.mppi_hip_simulate(self.ctx, T, int(start_step), _fptr(x0), _fptr(un),
+                                               mp, mn, cp, cn, _fptr(st), _fptr(ct), _fptr(kt)))
+        return st, ct, kt
+
+    def simulate_batch(self, num_agents, num_steps, start_step, x0_packed, u_nominal_packed=None,
+                       outputs=("states", "controls", "costs", "terminal")):
+        N, T = int(num_agents), int(num_steps)
+        x0 = _f32(x0_packed, (N * self.D,))
+        un = _f32(u_nominal_packed, (N * self.H * self.C,))
+        st = np.empty((T, N, self.D), np.float32) if "states" in outputs else None
+        ct = np.empty((T, N, self.C), np.float32) if "controls" in outputs else None
+        kt = np.empty((T, N), np.float32) if "costs" in outputs else None
+        tt = np.empty((N, self.D), np.float32) if "terminal" in outputs else None
+        self._check(self.lib.mppi_hip_simulate_batch(self.ctx, N, T, int(start_step), _fptr(x0),
+                                                     _fptr(un), _fptr(st), _fptr(ct), _fptr(kt),
+                                                     _fptr(tt)))
+        return st, ct, kt, tt
+
+    def read_costs(self, agent=0):
+        out = np.empty(self.K, np.float32)
+        self._check(self.lib.mppi_hip_read_costs(self.ctx, agent, _fptr(out), self.K))
+        return out
+
+    def read_weights(self, agent=0):
+        out = np.empty(self.K, np.float32)
+        self._check(self.lib.mppi_hip_read_weights(self.ctx, agent, _fptr(out), self.K))
+        return out
+
+    def set_reduce_samples_per_thread(self, spt):
+        self._check(self.lib.mppi_hip_set_reduce_samples_per_thread(self.ctx, int(spt)))
+
+    def profile(self, enable=True):
+        self._check(self.lib.mppi_hip_profile_enable(self.ctx, 1 if enable else 0))
+
+    def profile_read(self):
+        r, d, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_uint64()
+        self._check(self.lib.mppi_hip_profile_read(self.ctx, ctypes.byref(r), ctypes.byref(d),
+                                                   ctypes.byref(n)))
+        return r.value, d.value, n.value

@@ -60,6 +60,23 @@ constexpr int MPPI_G = 4 / mppi_gcd(CDIM, 4);
 constexpr int MPPI_NG = MPPI_G * CDIM;     // normals per group (multiple of 4)
 constexpr int MPPI_BG = MPPI_NG / 4;       // Philox blocks per group
 
+// Cost calls with or without the optional trailing mppi_time argument: the
+// first overload is viable only when the plugin's function takes it.
+struct mppi_form_plain {};
+struct mppi_form_timed : mppi_form_plain {};
+template <class X, class U, class M, class C>
+__device__ inline auto mppi_stage(mppi_form_timed, X x, U u, M m, C c, mppi_time tm)
+    -> decltype(MPPI_STAGE_COST(x, u, m, c, tm)) { return MPPI_STAGE_COST(x, u, m, c, tm); }
+template <class X, class U, class M, class C>
+__device__ inline auto mppi_stage(mppi_form_plain, X x, U u, M m, C c, mppi_time)
+    -> decltype(MPPI_STAGE_COST(x, u, m, c)) { return MPPI_STAGE_COST(x, u, m, c); }
+template <class X, class M, class C>
+__device__ inline auto mppi_terminal(mppi_form_timed, X x, M m, C c, mppi_time tm)
+    -> decltype(MPPI_TERMINAL_COST(x, m, c, tm)) { return MPPI_TERMINAL_COST(x, m, c, tm); }
+template <class X, class M, class C>
+__device__ inline auto mppi_terminal(mppi_form_plain, X x, M m, C c, mppi_time)
+    -> decltype(MPPI_TERMINAL_COST(x, m, c)) { return MPPI_TERMINAL_COST(x, m, c); }
+
 __device__ inline float mppi_wave_min(float v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -193,13 +210,14 @@ mppi_rollout(mppi_rollout_args a) {
                                       a.u_min, a.u_max, u);
                     mppi_rng rng{t << 8, k, step, dyn_tag, a.key0, a.key1};
                     MPPI_DYNAMICS(mppi_state_ref{x}, u, mppi_blob_ref{a.model}, rng);
-                    total += MPPI_STAGE_COST(mppi_cstate_ref{x}, u, mppi_blob_ref{a.model},
-                                             mppi_blob_ref{a.cost});
+                    total += mppi_stage(mppi_form_timed{}, mppi_cstate_ref{x}, (const float*)u,
+                                        mppi_blob_ref{a.model}, mppi_blob_ref{a.cost},
+                                        mppi_time{t, step});
                 }
             }
         }
-        total += MPPI_TERMINAL_COST(mppi_cstate_ref{x}, mppi_blob_ref{a.model},
-                                    mppi_blob_ref{a.cost});
+        total += mppi_terminal(mppi_form_timed{}, mppi_cstate_ref{x}, mppi_blob_ref{a.model},
+                               mppi_blob_ref{a.cost}, mppi_time{H, step});
         a.costs[size_t(agent) * K + k] = total;
     }
 

@@ -1,0 +1,114 @@
+"""Synthetic benchmark scenes and parameter blobs for the bundled plugins.
+
+Scenes are drawn on the host, once, from Philox4x32-10 with the run's seed and
+the scene tag in counter word 3 (block i, sample 0, step 0, tag 3 << 28), so
+the CPU oracle and the GPU see byte-identical parameter blobs.
+
+Unicycle workloads (the model of the reference's
+examples/nonholonomic_vehicle/nonholonomic_vehicle.metal:24-91, weights from
+examples/nonholonomic_vehicle/main.cpp:71-80):
+  dt = 0.05, v_max = 2, sigma = [0.5, 0.3], lambda = 1, bounds +-[2, 1],
+  goal ~ U[-10, 10]^2, n obstacles with centres ~ U[-10, 10]^2 (rejected when
+  within r + 0.5 of x0 or the goal) and radius ~ U[0.5, 1.5];
+  stage cost 10 |p - g|^2 + 0.1 |u|^2 + 1000 sum_i max(0, r_i - |p - c_i|),
+  terminal cost 100 |p - g|^2.
+"""
+from __future__ import annotations
+
+import struct
+from dataclasses import dataclass, field
+
+import numpy as np
+
+TAG_SCENE = 3
+UNICYCLE_MAX_OBSTACLES = 32
+UNICYCLE_MODEL_BYTES = 8
+UNICYCLE_COST_BYTES = 8 * 4 + UNICYCLE_MAX_OBSTACLES * 12
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 on Python integers (Random123 constants)."""
+    c = [int(v) & 0xFFFFFFFF for v in ctr]
+    k0, k1 = int(key[0]) & 0xFFFFFFFF, int(key[1]) & 0xFFFFFFFF
+    for _ in range(10):
+        p0 = 0xD2511F53 * c[0]
+        p1 = 0xCD9E8D57 * c[2]
+        c = [((p1 >> 32) ^ c[1] ^ k0) & 0xFFFFFFFF, p1 & 0xFFFFFFFF,
+             ((p0 >> 32) ^ c[3] ^ k1) & 0xFFFFFFFF, p0 & 0xFFFFFFFF]
+        k0 = (k0 + 0x9E3779B9) & 0xFFFFFFFF
+        k1 = (k1 + 0xBB67AE85) & 0xFFFFFFFF
+    return c
+
+
+def u01(word):
+    """(0, 1) uniform of a 32-bit word, as the library converts it."""
+    return float(np.float32(((int(word) >> 8) | 1) * 2.0 ** -24))
+
+
+class SceneStream:
+    """Sequential uniforms of the scene stream of one seed."""
+
+    def __init__(self, seed: int, agent: int = 0):
+        self.key = (seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF)
+        self.agent = agent
+        self.block = 0
+        self.buf = []
+
+    def uniform(self, lo=0.0, hi=1.0):
+        if not self.buf:
+            w = philox4x32_10((self.block, 0, 0, (TAG_SCENE << 28) | self.agent), self.key)
+            self.buf = [u01(x) for x in w]
+            self.block += 1
+        u = self.buf.pop(0)
+        return lo + (hi - lo) * u
+
+
+@dataclass
+class UnicycleScene:
+    goal: tuple
+    obstacles: list                 # [(cx, cy, r)]
+    v_max: float = 2.0
+    dt: float = 0.05
+    pos_weight: float = 10.0
+    control_weight: float = 0.1
+    obstacle_weight: float = 1000.0
+    goal_radius: float = 0.2
+    x0: tuple = (0.0, 0.0, 0.0, 0.0)
+    sigma: tuple = (0.5, 0.3)
+    u_min: tuple = (-2.0, -1.0)
+    u_max: tuple = (2.0, 1.0)
+    lam: float = 1.0
+    extra: dict = field(default_factory=dict)
+
+    def model_blob(self) -> bytes:
+        return struct.pack("<2f", self.v_max, self.dt)
+
+    def cost_blob(self) -> bytes:
+        obs = list(self.obstacles)[:UNICYCLE_MAX_OBSTACLES]
+        head = struct.pack("<7fI", self.goal[0], self.goal[1], self.goal_radius, self.pos_weight,
+                           0.0, self.control_weight, self.obstacle_weight, len(obs))
+        flat = []
+        for i in range(UNICYCLE_MAX_OBSTACLES):
+            flat.extend(obs[i] if i < len(obs) else (0.0, 0.0, 0.0))
+        return head + struct.pack(f"<{3 * UNICYCLE_MAX_OBSTACLES}f", *flat)
+
+
+def unicycle_scene(n_obstacles: int, seed: int = 42) -> UnicycleScene:
+    s = SceneStream(seed)
+    goal = (s.uniform(-10, 10), s.uniform(-10, 10))
+    x0 = (0.0, 0.0)
+    obstacles = []
+    tries = 0
+    while len(obstacles) < n_obstacles and tries < 10000:
+        tries += 1
+        cx, cy, r = s.uniform(-10, 10), s.uniform(-10, 10), s.uniform(0.5, 1.5)
+        if np.hypot(cx - x0[0], cy - x0[1]) < r + 0.5 or np.hypot(cx - goal[0], cy - goal[1]) < r + 0.5:
+            continue
+        obstacles.append((float(np.float32(cx)), float(np.float32(cy)), float(np.float32(r))))
+    goal = (float(np.float32(goal[0])), float(np.float32(goal[1])))
+    return UnicycleScene(goal=goal, obstacles=obstacles)
+
+
+# BASELINE workloads: (name, n_obstacles, H, K, closed-loop steps)
+UNICYCLE_C1 = dict(name="unicycle_k1024_h32_obs8", n_obstacles=8, H=32, K=1024, steps=200)
+UNICYCLE_C2 = dict(name="unicycle_k65536_h64_obs16", n_obstacles=16, H=64, K=65536, steps=500)

@@ -46,6 +46,9 @@ namespace plug_di {
 namespace plug_uni {
 #include "../mppi_metal_amd/plugins/unicycle.hip"
 }
+namespace plug_bic {
+#include "../mppi_metal_amd/plugins/bicycle.hip"
+}
 
 namespace {
 
@@ -104,17 +107,20 @@ struct RngCtx {
 template <typename T>
 using DynFn = void (*)(T* x, const T* u, const uint8_t* model, const RngCtx& rng);
 template <typename T>
-using StageFn = T (*)(const T* x, const T* u, const uint8_t* model, const uint8_t* cost);
+using StageFn = T (*)(const T* x, const T* u, const uint8_t* model, const uint8_t* cost,
+                      const mppi_time& tm);
 template <typename T>
-using TermFn = T (*)(const T* x, const uint8_t* model, const uint8_t* cost);
+using TermFn = T (*)(const T* x, const uint8_t* model, const uint8_t* cost, const mppi_time& tm);
 
 mppi_rng to_rng(const RngCtx& r) {
     return mppi_rng{r.t << 8, r.sample, r.step, (r.tag << 28) | (r.agent & 0x0fffffffu),
                     uint32_t(r.seed), uint32_t(r.seed >> 32)};
 }
 
-template <typename T> T zero_stage(const T*, const T*, const uint8_t*, const uint8_t*) { return T(0); }
-template <typename T> T zero_term(const T*, const uint8_t*, const uint8_t*) { return T(0); }
+template <typename T>
+T zero_stage(const T*, const T*, const uint8_t*, const uint8_t*, const mppi_time&) { return T(0); }
+template <typename T>
+T zero_term(const T*, const uint8_t*, const uint8_t*, const mppi_time&) { return T(0); }
 
 // ---- double integrator (shaders/test_user_dynamics.metal:8-43) ----
 void di_dyn_d(double* s, const double* u, const uint8_t*, const RngCtx&) {
@@ -124,10 +130,10 @@ void di_dyn_d(double* s, const double* u, const uint8_t*, const RngCtx&) {
     s[2] += u[0] * dt;
     s[3] += u[1] * dt;
 }
-double di_cost_d(const double* s, const double*, const uint8_t*, const uint8_t*) {
+double di_cost_d(const double* s, const double*, const uint8_t*, const uint8_t*, const mppi_time&) {
     return s[0] * s[0] + s[1] * s[1];
 }
-double di_term_d(const double* s, const uint8_t*, const uint8_t*) { return s[0] * s[0] + s[1] * s[1]; }
+double di_term_d(const double* s, const uint8_t*, const uint8_t*, const mppi_time&) { return s[0] * s[0] + s[1] * s[1]; }
 
 // ---- unicycle (examples/nonholonomic_vehicle/nonholonomic_vehicle.metal:24-91) ----
 struct UniModel { float v_max, dt; };
@@ -151,14 +157,16 @@ void uni_dyn_d(double* s, const double* u, const uint8_t* mp, const RngCtx&) {
     if (s[3] > p->v_max) s[3] = p->v_max;
     if (s[3] < -p->v_max) s[3] = -p->v_max;
 }
-double uni_stage_d(const double* s, const double* u, const uint8_t*, const uint8_t* cp) {
+double uni_stage_d(const double* s, const double* u, const uint8_t*, const uint8_t* cp,
+                   const mppi_time&) {
     const UniCost* p = reinterpret_cast<const UniCost*>(cp);
     double dx = s[0] - p->x_goal, dy = s[1] - p->y_goal;
     return p->pos_weight * (dx * dx + dy * dy) + p->control_weight * (u[0] * u[0] + u[1] * u[1]);
 }
-double uni_stage_obs_d(const double* s, const double* u, const uint8_t* mp, const uint8_t* cp) {
+double uni_stage_obs_d(const double* s, const double* u, const uint8_t* mp, const uint8_t* cp,
+                       const mppi_time& tm) {
     const UniCost* p = reinterpret_cast<const UniCost*>(cp);
-    double c = uni_stage_d(s, u, mp, cp), pen = 0.0;
+    double c = uni_stage_d(s, u, mp, cp, tm), pen = 0.0;
     uint32_t n = std::min<uint32_t>(p->n_obstacles, 32);
     for (uint32_t i = 0; i < n; ++i) {
         double ox = s[0] - p->obstacles[i][0], oy = s[1] - p->obstacles[i][1];
@@ -166,10 +174,59 @@ double uni_stage_obs_d(const double* s, const double* u, const uint8_t* mp, cons
     }
     return c + p->obstacle_weight * pen;
 }
-double uni_term_d(const double* s, const uint8_t*, const uint8_t* cp) {
+double uni_term_d(const double* s, const uint8_t*, const uint8_t* cp, const mppi_time&) {
     const UniCost* p = reinterpret_cast<const UniCost*>(cp);
     double dx = s[0] - p->x_goal, dy = s[1] - p->y_goal;
     return p->pos_weight * (dx * dx + dy * dy) * 10.0;
+}
+
+// ---- dynamic bicycle (mppi_metal_amd/plugins/bicycle.hip) ----
+struct BicModel {
+    float mass, yaw_inertia, lf, lr, cf, cr, drag, v_min, v_max, dt;
+    uint32_t substeps;
+    float pad;
+};
+struct BicCost {
+    float target_x, target_y, target_vx, target_vy, w_pos, w_control, w_zone, w_terminal, dt;
+    uint32_t n_zones;
+    float pad[2];
+    float zones[32][3];
+};
+void bic_dyn_d(double* s, const double* u, const uint8_t* mp, const RngCtx&) {
+    const BicModel* p = reinterpret_cast<const BicModel*>(mp);
+    const uint32_t n = p->substeps > 0 ? p->substeps : 1u;
+    const double h = double(p->dt) / n;
+    for (uint32_t i = 0; i < n; ++i) {
+        const double vx = s[3], vy = s[4], r = s[5], sp = std::sin(s[2]), cp = std::cos(s[2]);
+        const double uu = std::max(vx, double(p->v_min));
+        const double fyf = p->cf * (u[0] - (vy + p->lf * r) / uu);
+        const double fyr = p->cr * (-(vy - p->lr * r) / uu);
+        const double d[6] = {vx * cp - vy * sp, vx * sp + vy * cp, r, u[1] - p->drag * vx + r * vy,
+                             (fyf + fyr) / p->mass - r * vx, (p->lf * fyf - p->lr * fyr) / p->yaw_inertia};
+        for (int j = 0; j < 6; ++j) s[j] += h * d[j];
+        s[3] = std::min(std::max(s[3], 0.0), double(p->v_max));
+    }
+}
+double bic_target_err2(const double* s, const BicCost* c, double tau) {
+    const double ex = s[0] - (c->target_x + c->target_vx * tau);
+    const double ey = s[1] - (c->target_y + c->target_vy * tau);
+    return ex * ex + ey * ey;
+}
+double bic_stage_d(const double* s, const double* u, const uint8_t*, const uint8_t* cp,
+                   const mppi_time& tm) {
+    const BicCost* c = reinterpret_cast<const BicCost*>(cp);
+    double cost = c->w_pos * bic_target_err2(s, c, double(tm.step + tm.t + 1) * c->dt) +
+                  c->w_control * (u[0] * u[0] + u[1] * u[1]);
+    double pen = 0.0;
+    for (uint32_t i = 0; i < std::min<uint32_t>(c->n_zones, 32); ++i) {
+        const double ox = s[0] - c->zones[i][0], oy = s[1] - c->zones[i][1];
+        pen += std::max(0.0, double(c->zones[i][2]) - std::sqrt(ox * ox + oy * oy));
+    }
+    return cost + c->w_zone * pen;
+}
+double bic_term_d(const double* s, const uint8_t*, const uint8_t* cp, const mppi_time& tm) {
+    const BicCost* c = reinterpret_cast<const BicCost*>(cp);
+    return c->w_terminal * bic_target_err2(s, c, double(tm.step + tm.t) * c->dt);
 }
 
 struct Callable {
@@ -188,12 +245,12 @@ const Callable kCallables[] = {
      }),
      FP(&di_dyn_d)},
     {"double_integrator", "mppi_stage_cost", 1,
-     FP(+[](const float* x, const float* u, const uint8_t* m, const uint8_t* c) {
+     FP(+[](const float* x, const float* u, const uint8_t* m, const uint8_t* c, const mppi_time&) {
          return plug_di::mppi_stage_cost(x, u, m, c);
      }),
      FP(&di_cost_d)},
     {"double_integrator", "mppi_terminal_cost", 2,
-     FP(+[](const float* x, const uint8_t* m, const uint8_t* c) {
+     FP(+[](const float* x, const uint8_t* m, const uint8_t* c, const mppi_time&) {
          return plug_di::mppi_terminal_cost(x, m, c);
      }),
      FP(&di_term_d)},
@@ -203,25 +260,40 @@ const Callable kCallables[] = {
      }),
      FP(&uni_dyn_d)},
     {"unicycle", "mppi_stage_cost", 1,
-     FP(+[](const float* x, const float* u, const uint8_t* m, const uint8_t* c) {
+     FP(+[](const float* x, const float* u, const uint8_t* m, const uint8_t* c, const mppi_time&) {
          return plug_uni::mppi_stage_cost(x, u, m, c);
      }),
      FP(&uni_stage_d)},
     {"unicycle", "unicycle_stage_cost_obstacles", 1,
-     FP(+[](const float* x, const float* u, const uint8_t* m, const uint8_t* c) {
+     FP(+[](const float* x, const float* u, const uint8_t* m, const uint8_t* c, const mppi_time&) {
          return plug_uni::unicycle_stage_cost_obstacles(x, u, m, c);
      }),
      FP(&uni_stage_obs_d)},
     {"unicycle", "mppi_terminal_cost", 2,
-     FP(+[](const float* x, const uint8_t* m, const uint8_t* c) {
+     FP(+[](const float* x, const uint8_t* m, const uint8_t* c, const mppi_time&) {
          return plug_uni::mppi_terminal_cost(x, m, c);
      }),
      FP(&uni_term_d)},
     {"unicycle", "unicycle_terminal_cost", 2,
-     FP(+[](const float* x, const uint8_t* m, const uint8_t* c) {
+     FP(+[](const float* x, const uint8_t* m, const uint8_t* c, const mppi_time&) {
          return plug_uni::unicycle_terminal_cost(x, m, c);
      }),
      FP(&uni_term_d)},
+    {"bicycle", "bicycle_dynamics", 0,
+     FP(+[](float* x, const float* u, const uint8_t* m, const RngCtx& r) {
+         plug_bic::bicycle_dynamics(x, u, m, to_rng(r));
+     }),
+     FP(&bic_dyn_d)},
+    {"bicycle", "bicycle_pursuit_cost", 1,
+     FP(+[](const float* x, const float* u, const uint8_t* m, const uint8_t* c, const mppi_time& tm) {
+         return plug_bic::bicycle_pursuit_cost(x, u, m, c, tm);
+     }),
+     FP(&bic_stage_d)},
+    {"bicycle", "bicycle_terminal_cost", 2,
+     FP(+[](const float* x, const uint8_t* m, const uint8_t* c, const mppi_time& tm) {
+         return plug_bic::bicycle_terminal_cost(x, m, c, tm);
+     }),
+     FP(&bic_term_d)},
 };
 #undef FP
 
@@ -370,9 +442,9 @@ void rollout_range(const oracle_problem* p, const Resolved<T>& r, uint32_t step,
                 u[i] = clampv<T>(u_nom[t * C + i] + eps[i], r.lo[i], r.hi[i]);
             RngCtx rc{t, k, step, MPPI_TAG_DYNAMICS, agent, p->seed};
             r.dyn(x.data(), u.data(), r.model, rc);
-            total += r.stage(x.data(), u.data(), r.model, r.cost);
+            total += r.stage(x.data(), u.data(), r.model, r.cost, mppi_time{t, step});
         }
-        total += r.term(x.data(), r.model, r.cost);
+        total += r.term(x.data(), r.model, r.cost, mppi_time{H, step});
         costs[k] = total;
     }
 }
@@ -428,18 +500,41 @@ void weights(const oracle_problem* p, const T* costs, T* w, T diag[3]) {
 }
 
 // Weighted-noise average (mppi_reduce.metal:130-168): for each (t, d),
-// u = clamp(u_nom + sum_{k=0..K-1} w_k * eps_k(t, d)), k ascending.
+// u = clamp(u_nom + sum_{k=0..K-1} w_k * eps_k(t, d)), k ascending.  With
+// threads > 1 the samples are split into contiguous ranges whose partial sums
+// are added in range order (used by the CPU baseline timing).
 template <typename T>
-void reduce(const oracle_problem* p, const Resolved<T>& r, uint32_t step, uint32_t agent,
-            const T* w, const T* u_nom, T* u_out) {
+void reduce_range(const oracle_problem* p, const Resolved<T>& r, uint32_t step, uint32_t agent,
+                  const T* w, uint32_t k0, uint32_t k1, T* acc) {
     const uint32_t C = p->C, H = p->H, HC = H * C;
-    std::vector<T> acc(HC, T(0)), z(HC), eps(C);
-    for (uint32_t k = 0; k < p->K; ++k) {
+    std::vector<T> z(HC), eps(C);
+    for (uint32_t k = k0; k < k1; ++k) {
         sample_normals<T>(p, k, step, agent, z.data());
         for (uint32_t t = 0; t < H; ++t) {
             noise_eps<T>(p, r, z.data(), t, eps.data());
             for (uint32_t i = 0; i < C; ++i) acc[t * C + i] += w[k] * eps[i];
         }
+    }
+}
+
+template <typename T>
+void reduce(const oracle_problem* p, const Resolved<T>& r, uint32_t step, uint32_t agent,
+            const T* w, const T* u_nom, T* u_out, int threads = 1) {
+    const uint32_t C = p->C, H = p->H, HC = H * C, K = p->K;
+    std::vector<T> acc(HC, T(0));
+    if (threads <= 1 || K < 1024) {
+        reduce_range<T>(p, r, step, agent, w, 0, K, acc.data());
+    } else {
+        std::vector<std::vector<T>> part(threads, std::vector<T>(HC, T(0)));
+        std::vector<std::thread> pool;
+        const uint32_t per = (K + threads - 1) / threads;
+        for (int i = 0; i < threads; ++i) {
+            uint32_t a = std::min<uint32_t>(K, i * per), b = std::min<uint32_t>(K, a + per);
+            pool.emplace_back([=, &r, &part] { reduce_range<T>(p, r, step, agent, w, a, b, part[i].data()); });
+        }
+        for (auto& th : pool) th.join();
+        for (int i = 0; i < threads; ++i)
+            for (uint32_t n = 0; n < HC; ++n) acc[n] += part[i][n];
     }
     for (uint32_t n = 0; n < HC; ++n) {
         const uint32_t i = n % C;
@@ -456,7 +551,7 @@ int step_impl(const oracle_problem* p, uint32_t step, uint32_t agent, const floa
     T diag[3];
     rollout<T>(p, r, step, agent, x0, u_nom, costs.data(), threads);
     weights<T>(p, costs.data(), w.data(), diag);
-    reduce<T>(p, r, step, agent, w.data(), u_nom, u_out);
+    reduce<T>(p, r, step, agent, w.data(), u_nom, u_out, threads);
     if (diag3) std::copy(diag, diag + 3, diag3);
     if (costs_out) std::copy(costs.begin(), costs.end(), costs_out);
     if (weights_out) std::copy(w.begin(), w.end(), weights_out);
@@ -483,7 +578,7 @@ int simulate_impl(const oracle_problem* p, uint32_t T_steps, uint32_t start_step
             T diag[3];
             rollout<T>(p, r, step, a, x.data(), un.data(), costs.data(), threads);
             weights<T>(p, costs.data(), w.data(), diag);
-            reduce<T>(p, r, step, a, w.data(), un.data(), uo.data());
+            reduce<T>(p, r, step, a, w.data(), un.data(), uo.data(), threads);
             const size_t row = size_t(s) * n_agents + a;
             if (states_out) for (uint32_t i = 0; i < D; ++i) states_out[row * D + i] = x[i];
             if (controls_out) for (uint32_t i = 0; i < C; ++i) controls_out[row * C + i] = float(uo[i]);
@@ -541,6 +636,22 @@ void oracle_mathf(int fn, const float* x, float* y, size_t n) {
 }
 void oracle_fmodf(const float* x, const float* y, float* out, size_t n) {
     for (size_t i = 0; i < n; ++i) out[i] = mppi_fmodf(x[i], y[i]);
+}
+
+// The library header's Philox4x32-10 (include/mppi_hip/mppi_philox.h)
+// compiled for the host, so tests can compare it with this oracle's own.
+void oracle_library_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    mppi_u32x4 r = mppi_philox4x32_10(mppi_u32x4{ctr[0], ctr[1], ctr[2], ctr[3]}, key[0], key[1]);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
+}
+
+// Costs of samples [k0, k1) only (costs is indexed by absolute sample).
+int oracle_rollout_range(const oracle_problem* p, uint32_t step, uint32_t agent, const float* x0,
+                         const float* u_nom, uint32_t k0, uint32_t k1, float* costs) {
+    Resolved<float> r = resolve<float>(p);
+    if (!r.ok || k1 > p->K || k0 > k1) return -1;
+    rollout_range<float>(p, r, step, agent, x0, u_nom, k0, k1, costs);
+    return 0;
 }
 
 int oracle_rollout(const oracle_problem* p, uint32_t step, uint32_t agent, const float* x0,

@@ -1,0 +1,57 @@
+"""Shared problem builders: the same workload for the HIP backend and the
+oracle (test infrastructure)."""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle_lib as ol
+from mppi_metal_amd import HipMPPI, scenes
+
+SEED = 42
+
+
+def unicycle(K, H, n_obs, seed=SEED, stage="unicycle_stage_cost_obstacles",
+             terminal="unicycle_terminal_cost", lam=1.0, cov=None):
+    sc = scenes.unicycle_scene(n_obs, seed)
+    mb, cb = sc.model_blob(), sc.cost_blob()
+    sigma = np.array(sc.sigma, np.float32) if cov is None else np.asarray(cov, np.float32).ravel()
+    hip = HipMPPI(4, 2, H, K, lam, sigma, sc.u_min, sc.u_max, covariance_full=cov is not None)
+    hip.link("unicycle.hip", "mppi_dynamics", stage, terminal, len(mb), len(cb), seed)
+    hip.bind_params(mb, cb)
+    ora = ol.Problem("unicycle", 4, 2, H, K, lam, sigma, sc.u_min, sc.u_max, seed,
+                     "mppi_dynamics", stage, terminal, mb, cb, cov_full=cov is not None)
+    return hip, ora, sc
+
+
+def unicycle_oracle(K, H, n_obs, seed=SEED, stage="unicycle_stage_cost_obstacles",
+                    terminal="unicycle_terminal_cost", lam=1.0):
+    """The oracle side only (no GPU needed)."""
+    sc = scenes.unicycle_scene(n_obs, seed)
+    mb, cb = sc.model_blob(), sc.cost_blob()
+    ora = ol.Problem("unicycle", 4, 2, H, K, lam, np.array(sc.sigma, np.float32), sc.u_min,
+                     sc.u_max, seed, "mppi_dynamics", stage, terminal, mb, cb)
+    return ora, sc
+
+
+def double_integrator(K=64, H=10, seed=0, stage="mppi_stage_cost", terminal="", sigma=(1.0, 1.0),
+                      bound=1.0, lam=1.0):
+    lo, hi = [-bound, -bound], [bound, bound]
+    hip = HipMPPI(4, 2, H, K, lam, sigma, lo, hi)
+    hip.link("double_integrator.hip", "mppi_dynamics", stage, terminal, 0, 0, seed)
+    ora = ol.Problem("double_integrator", 4, 2, H, K, lam, sigma, lo, hi, seed, "mppi_dynamics",
+                     stage, terminal)
+    return hip, ora
+
+
+def rel_err(a, b, floor=0.0):
+    """max |a - b| / max(|b|, floor), elementwise."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
+
+
+def seq_rel_err(a, b):
+    """Error of a control sequence relative to its largest magnitude."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))

@@ -1,0 +1,173 @@
+"""GPU parity of the MPPI step against the CPU oracle, through the C ABI.
+
+Tolerances (FP32):
+  sample costs      bit-exact (same operations in the same order on both
+                    sides, shared elementary functions, contraction off)
+  best cost (min)   bit-exact
+  weights           1e-5 relative, elementwise
+  u_next            1e-5 of the sequence's largest magnitude (the GPU sums
+                    w_k z_k in a tree, the oracle sequentially in k)
+  mean cost, ESS    1e-5 relative
+Every result's error against the FP64 oracle is printed (pytest -s) and
+bounded.
+"""
+import numpy as np
+import pytest
+
+from problems import double_integrator, rel_err, seq_rel_err, unicycle
+
+pytestmark = pytest.mark.gpu
+
+
+def _report(name, **errs):
+    print(f"[{name}] " + " ".join(f"{k}={v:.3e}" for k, v in errs.items()), flush=True)
+
+
+def _check_step(hip, ora, step, x0, u_nom, name, check_fp64=True):
+    u_g, d_g = hip.step(step, x0, u_nom)
+    c_g, w_g = hip.read_costs(), hip.read_weights()
+    u_o, d_o, c_o, w_o = ora.step(step, x0, u_nom, threads=8)
+    ulps = np.abs(c_g.view(np.int32).astype(np.int64) - c_o.view(np.int32).astype(np.int64))
+    assert np.array_equal(c_g, c_o), f"costs differ: {np.count_nonzero(ulps)} samples, max {ulps.max()} ulp"
+    assert d_g[0] == d_o[0]
+    assert rel_err(w_g, w_o, 1e-37) <= 1e-5
+    assert abs(w_g.sum() - 1.0) <= 1e-5
+    assert seq_rel_err(u_g, u_o) <= 1e-5
+    assert rel_err(d_g[1:], d_o[1:]) <= 1e-5
+    errs = dict(u_vs_fp32=seq_rel_err(u_g, u_o), w_vs_fp32=rel_err(w_g, w_o, 1e-37),
+                ess_vs_fp32=rel_err(d_g[2:], d_o[2:]))
+    if check_fp64:
+        u_d, d_d, c_d, w_d = ora.step(step, x0, np.asarray(u_nom, np.float64), threads=8, fp64=True)
+        errs.update(cost_vs_fp64=rel_err(c_g, c_d), u_vs_fp64=seq_rel_err(u_g, u_d),
+                    oracle32_u_vs_fp64=seq_rel_err(u_o, u_d))
+        assert errs["cost_vs_fp64"] < 1e-3
+    _report(name, **errs)
+    return u_g, d_g
+
+
+def test_unicycle_k1024_h32_obs8_parity(require_gpu):
+    hip, ora, sc = unicycle(1024, 32, 8)
+    x0 = np.zeros(4, np.float32)
+    rng = np.random.default_rng(5)
+    u_nom = (rng.standard_normal((32, 2)) * 0.3).astype(np.float32)
+    for step in (0, 1, 7):
+        _check_step(hip, ora, step, x0, u_nom, f"unicycle C1 step {step}")
+
+
+def test_double_integrator_reference_fixture(require_gpu):
+    hip, ora = double_integrator(K=64, H=10, seed=0)
+    x0 = np.array([1.0, 1.0, 0.0, 0.0], np.float32)
+    u, d = _check_step(hip, ora, 0, x0, np.zeros((10, 2), np.float32), "double integrator")
+    assert u[0, 0] < 0 and u[0, 1] < 0 and d[0] > 0 and d[1] >= d[0] and d[2] > 0
+
+
+def test_full_covariance_parity(require_gpu):
+    cov = np.array([[0.30, 0.12], [0.12, 0.20]], np.float32)
+    hip, ora, _ = unicycle(2048, 24, 8, cov=cov)
+    _check_step(hip, ora, 3, np.array([0.5, -0.5, 0.3, 0.4], np.float32),
+                np.zeros((24, 2), np.float32), "full covariance")
+
+
+def test_determinism_same_seed(require_gpu):
+    h1, _, _ = unicycle(4096, 32, 8)
+    h2, _, _ = unicycle(4096, 32, 8)
+    x0 = np.zeros(4, np.float32)
+    un = np.zeros((32, 2), np.float32)
+    u1, d1 = h1.step(11, x0, un)
+    u2, d2 = h2.step(11, x0, un)
+    assert np.array_equal(u1, u2) and np.array_equal(d1, d2)
+    u3, _ = h1.step(11, x0, un)
+    assert np.array_equal(u1, u3)
+
+
+def test_simulate_matches_oracle(require_gpu):
+    hip, ora, sc = unicycle(1024, 32, 8)
+    x0 = np.zeros(4, np.float32)
+    st, ct, kt = hip.simulate(6, 0, x0, np.zeros((32, 2), np.float32))
+    so, co, ko, _ = ora.simulate(6, 0, x0, threads=8)
+    # Costs are bit-exact at step 0; afterwards the plant states differ by the
+    # reduce's rounding, which the closed loop carries.
+    assert kt[0] == ko[0, 0]
+    assert np.max(np.abs(st - so[:, 0])) <= 1e-4
+    assert np.max(np.abs(ct - co[:, 0])) <= 1e-4
+    assert rel_err(kt, ko[:, 0]) <= 1e-4
+
+
+def test_simulate_batch_matches_oracle(require_gpu):
+    hip, ora, sc = unicycle(512, 16, 8)
+    x0 = np.array([[0, 0, 0, 0], [1, -1, 0.5, 0.2], [-2, 1, -0.3, 0.0]], np.float32)
+    st, ct, kt, tt = hip.simulate_batch(3, 4, 10, x0)
+    so, co, ko, to = ora.simulate(4, 10, x0, n_agents=3, threads=8)
+    assert np.array_equal(kt[0], ko[0])
+    assert np.max(np.abs(st - so)) <= 1e-4
+    assert np.max(np.abs(ct - co)) <= 1e-4
+    assert np.max(np.abs(tt - to)) <= 1e-4
+
+
+def test_unicycle_k65536_h64_obs16_full_size(require_gpu):
+    hip, ora, sc = unicycle(65536, 64, 16)
+    x0 = np.zeros(4, np.float32)
+    u_nom = np.zeros((64, 2), np.float32)
+    u_g, d_g = hip.step(0, x0, u_nom)
+    c_g, w_g = hip.read_costs(), hip.read_weights()
+    for k0, k1 in ((0, 512), (32768, 33024), (65024, 65536)):
+        assert np.array_equal(c_g[k0:k1], ora.rollout_range(0, x0, u_nom, k0, k1))
+    assert abs(float(w_g.astype(np.float64).sum()) - 1.0) <= 1e-5
+    assert np.all(w_g >= 0) and np.all(np.isfinite(w_g))
+    assert int(np.argmax(w_g)) == int(np.argmin(c_g))
+    assert np.all(u_g >= np.array(sc.u_min) - 1e-7) and np.all(u_g <= np.array(sc.u_max) + 1e-7)
+    u_o, d_o, c_o, w_o = ora.step(0, x0, u_nom, threads=8)
+    assert np.array_equal(c_g, c_o)
+    assert seq_rel_err(u_g, u_o) <= 1e-5
+    assert rel_err(w_g, w_o, 1e-37) <= 1e-5
+    _report("unicycle C2 full size", u_vs_fp32=seq_rel_err(u_g, u_o), w_vs_fp32=rel_err(w_g, w_o, 1e-37))
+
+
+def test_k_2_18_uses_fp64_combine(require_gpu):
+    # K = 2^18 takes the FP64 cross-block path; compare with the FP64 oracle.
+    hip, ora, sc = unicycle(1 << 18, 8, 4, lam=50.0)
+    x0 = np.array([1.0, 2.0, 0.1, 0.5], np.float32)
+    u_nom = np.zeros((8, 2), np.float32)
+    u_g, d_g = hip.step(2, x0, u_nom)
+    w_g = hip.read_weights()
+    u_d, d_d, c_d, w_d = ora.step(2, x0, np.zeros((8, 2)), threads=8, fp64=True)
+    u_o, d_o, c_o, w_o = ora.step(2, x0, u_nom, threads=8)
+    e64, e32 = seq_rel_err(u_g, u_d), seq_rel_err(u_o, u_d)
+    _report("K=2^18", gpu_u_vs_fp64=e64, oracle32_u_vs_fp64=e32, ess_vs_fp64=rel_err(d_g[2:], d_d[2:]))
+    assert e64 <= 1e-5
+    assert rel_err(d_g[2:], d_d[2:]) <= 1e-5
+    assert abs(float(w_g.astype(np.float64).sum()) - 1.0) <= 1e-5
+
+
+NAN_PLUGIN = r'''
+#include "mppi_core.h"
+MPPI_CALLABLE void dyn(float* x, const float* u, const uint8_t* m, mppi_rng r) {
+    x[0] += u[0] * 0.1f; x[1] += u[1] * 0.1f;
+}
+// Samples whose first control noise drives x0 negative get a NaN or infinite cost.
+MPPI_CALLABLE float stage(const float* x, const float* u, const uint8_t* m, const uint8_t* c) {
+    const float* mode = reinterpret_cast<const float*>(c);
+    if (mode[0] > 0.5f) return __builtin_nanf("");
+    if (x[0] < -0.05f) return mode[1] > 0.5f ? __builtin_inff() : __builtin_nanf("");
+    return x[0] * x[0] + x[1] * x[1];
+}
+'''
+
+
+def test_nonfinite_costs_get_zero_weight(require_gpu):
+    from mppi_metal_amd import HipMPPI
+    hip = HipMPPI(2, 2, 4, 1024, 1.0, [1.0, 1.0], [-1, -1], [1, 1])
+    hip.link_source(NAN_PLUGIN, "dyn", "stage", "", 0, 8, 9)
+    for inf in (0.0, 1.0):
+        hip.bind_params(None, np.array([0.0, inf], np.float32))
+        u, d = hip.step(0, np.zeros(2, np.float32), np.zeros(8, np.float32))
+        c, w = hip.read_costs(), hip.read_weights()
+        bad = ~np.isfinite(c)
+        assert bad.any() and (~bad).any()
+        assert np.all(np.isfinite(w)) and np.all(w[bad] == 0.0) and np.all(np.isfinite(u))
+        assert abs(float(w.sum()) - 1.0) <= 1e-5
+    # Every cost NaN: uniform weights, ESS = K.
+    hip.bind_params(None, np.array([1.0, 0.0], np.float32))
+    u, d = hip.step(0, np.zeros(2, np.float32), np.zeros(8, np.float32))
+    w = hip.read_weights()
+    assert np.allclose(w, 1.0 / 1024) and abs(d[2] - 1024) <= 1e-2 and np.all(np.isfinite(u))

@@ -1,0 +1,81 @@
+"""Accuracy of the shared single-precision elementary functions
+(include/mppi_hip/mppi_math.h) against FP64 numpy, and exactness of fmod.
+These functions are what make CPU and GPU costs bit-identical; their error
+against the true function is bounded here."""
+import numpy as np
+
+import oracle_lib as ol
+
+
+def ulp_err(y32, exact):
+    exact = np.asarray(exact, np.float64)
+    ulp = np.spacing(np.abs(exact).astype(np.float32)).astype(np.float64)
+    ulp = np.maximum(ulp, np.float64(np.finfo(np.float32).smallest_subnormal))
+    return np.abs(y32.astype(np.float64) - exact) / ulp
+
+
+def test_exp_ulp():
+    x = np.linspace(-87.0, 88.5, 400001, dtype=np.float32)
+    assert ulp_err(ol.mathf(0, x), np.exp(x.astype(np.float64))).max() <= 2.0
+    assert ol.mathf(0, np.array([0.0], np.float32))[0] == 1.0
+    assert np.isinf(ol.mathf(0, np.array([89.0], np.float32))[0])
+    assert ol.mathf(0, np.array([-120.0], np.float32))[0] == 0.0
+
+
+def test_exp_subnormal_range():
+    x = np.linspace(-103.9, -87.4, 20001, dtype=np.float32)
+    y = ol.mathf(0, x).astype(np.float64)
+    e = np.exp(x.astype(np.float64))
+    # subnormal results: absolute error within 2 subnormal steps
+    assert np.max(np.abs(y - e)) <= 2 * np.finfo(np.float32).smallest_subnormal * 1.0001
+
+
+def test_log_ulp():
+    x = np.concatenate([np.geomspace(1e-37, 1e37, 400001), np.linspace(0.5, 2.0, 100001)]).astype(np.float32)
+    assert ulp_err(ol.mathf(1, x), np.log(x.astype(np.float64))).max() <= 2.0
+    # Box-Muller range: u in [2^-24, 1)
+    u = ((np.arange(0, 2**23, 37, dtype=np.uint64) * 2 + 1) * 2.0**-24).astype(np.float32)
+    assert ulp_err(ol.mathf(1, u), np.log(u.astype(np.float64))).max() <= 2.0
+
+
+def test_sin_cos_abs():
+    x = np.linspace(-200.0, 200.0, 400001, dtype=np.float32)
+    xd = x.astype(np.float64)
+    eps = np.finfo(np.float32).eps
+    assert np.max(np.abs(ol.mathf(2, x) - np.sin(xd))) <= 1.5 * eps
+    assert np.max(np.abs(ol.mathf(3, x) - np.cos(xd))) <= 1.5 * eps
+    small = np.linspace(-0.7, 0.7, 100001, dtype=np.float32)
+    assert ulp_err(ol.mathf(2, small[small != 0]), np.sin(small[small != 0].astype(np.float64))).max() <= 2.0
+
+
+def test_sincospi():
+    x = np.linspace(0.0, 2.0, 400001, dtype=np.float32)
+    xd = x.astype(np.float64)
+    eps = np.finfo(np.float32).eps
+    assert np.max(np.abs(ol.mathf(4, x) - np.sin(np.pi * xd))) <= 1.5 * eps
+    assert np.max(np.abs(ol.mathf(5, x) - np.cos(np.pi * xd))) <= 1.5 * eps
+    # exact special points
+    assert ol.mathf(4, np.array([0.5], np.float32))[0] == 1.0
+    assert ol.mathf(5, np.array([1.0], np.float32))[0] == -1.0
+
+
+def test_tanh():
+    x = np.linspace(-12.0, 12.0, 400001, dtype=np.float32)
+    y = ol.mathf(6, x).astype(np.float64)
+    t = np.tanh(x.astype(np.float64))
+    assert np.max(np.abs(y - t) / np.maximum(np.abs(t), 1e-30)) <= 4 * np.finfo(np.float32).eps
+
+
+def test_sqrt_is_correctly_rounded():
+    x = np.random.default_rng(0).uniform(0, 1e6, 100000).astype(np.float32)
+    assert np.array_equal(ol.mathf(7, x), np.sqrt(x))
+
+
+def test_fmod_exact():
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1e4, 1e4, 200000).astype(np.float32)
+    y = rng.uniform(0.1, 50, 200000).astype(np.float32)
+    assert np.array_equal(ol.fmodf(x, y), np.fmod(x, y))
+    two_pi = np.float32(2.0 * np.float32(3.1415926535))
+    th = np.linspace(-10, 10, 100001, dtype=np.float32)
+    assert np.array_equal(ol.fmodf(th, np.full_like(th, two_pi)), np.fmod(th, two_pi))

@@ -1,0 +1,211 @@
+"""Pins the CPU oracle's MPPI step against facts fixed by the mathematics,
+so that a dropped term, a wrong sign or index, or a transposed operand fails
+here before the oracle is used to judge the GPU.
+
+  weights  softmax(-c / lambda) from scipy (independent implementation);
+           invariants sum w = 1, argmax w = argmin c, 1 <= ESS <= K,
+           mean >= min; special cases (equal costs, one dominant sample,
+           lambda -> infinity).
+  reduce   uniform weights give the sample mean of the noise; one-hot
+           weights give that sample's noise; clamping to the bounds.
+  rollout  the double integrator is linear, so its costs have a closed form
+           given the noise: x_{t+1} = A x_t + B u_t in matrix form.
+  noise    normal n = t*C + d of a sample comes from Philox block n // 4,
+           lane n % 4; full covariance noise has covariance L L^T = Sigma.
+  closed loop  the reference tests' relations: simulate == loop of step +
+           plant; batch N=1 == single; terminal(T) == state(T) of T+1.
+"""
+import numpy as np
+import pytest
+from scipy.special import softmax
+
+import oracle_lib as ol
+from problems import unicycle_oracle
+
+SIG = np.array([0.7, 0.4], np.float32)
+LO, HI = np.array([-1.5, -1.0], np.float32), np.array([1.5, 1.0], np.float32)
+
+
+def di(K=16, H=6, lam=1.0, seed=3, stage="mppi_stage_cost", terminal="mppi_terminal_cost",
+       sigma=SIG, lo=LO, hi=HI, cov_full=False):
+    return ol.Problem("double_integrator", 4, 2, H, K, lam, sigma, lo, hi, seed, "mppi_dynamics",
+                      stage, terminal, cov_full=cov_full)
+
+
+def noise64(p, k, step, agent=0):
+    """All H*C normals of sample k, FP64, from the documented stream layout."""
+    HC = p.H * p.C
+    blocks = [ol.normals4(b, k, step, 0, agent, p.s.seed, fp64=True) for b in range((HC + 3) // 4)]
+    return np.concatenate(blocks)[:HC].reshape(p.H, p.C)
+
+
+# ---------------------------------------------------------------- weights --
+@pytest.mark.parametrize("fp64", [False, True])
+def test_weights_match_softmax_and_invariants(fp64):
+    rng = np.random.default_rng(0)
+    for K, lam in ((16, 1.0), (257, 3.0), (4096, 0.2)):
+        p = di(K=K, lam=lam)
+        c = rng.uniform(0.0, 20.0, K).astype(np.float32)
+        w, d = p.weights(c, fp64=fp64)
+        ref = softmax(-c.astype(np.float64) / np.float64(np.float32(lam)))
+        # FP32: the exponent's argument a = (c - min)/lambda carries ~1 ulp of
+        # rounding, so exp(-a) is good to about |a| * 2^-23 relative.
+        arg = (c.astype(np.float64) - c.min()) / np.float32(lam)
+        tol = 1e-12 if fp64 else 2e-6 + 2.4e-7 * arg
+        # plus two subnormal steps where the result falls below FLT_MIN
+        floor = 0.0 if fp64 else 2 * float(np.finfo(np.float32).smallest_subnormal)
+        assert np.all(np.abs(w - ref) <= ref * tol + floor)
+        assert abs(w.sum() - 1.0) < (1e-12 if fp64 else 1e-5)
+        assert np.argmax(w) == np.argmin(c)
+        assert d[0] == c.min()
+        assert abs(d[1] - c.astype(np.float64).mean()) <= 1e-5 * c.mean()
+        assert 1.0 <= d[2] <= K
+        assert abs(d[2] - 1.0 / np.sum(ref ** 2)) <= (1e-9 if fp64 else 1e-4) * d[2]
+
+
+def test_weights_special_cases():
+    p = di(K=1000)
+    w, d = p.weights(np.full(1000, 3.25, np.float32))
+    assert np.all(w == np.float32(1.0 / 1000)) and abs(d[2] - 1000) < 1e-2
+    c = np.full(1000, 50.0, np.float32)
+    c[123] = 1.0                                   # lower by 49 lambda
+    w, d = p.weights(c)
+    assert w[123] > 0.999999 and abs(d[2] - 1.0) < 1e-5
+    p_inf = di(K=1000, lam=1e30)                   # lambda -> infinity: uniform
+    w, d = p_inf.weights(np.random.default_rng(1).uniform(0, 100, 1000).astype(np.float32))
+    assert np.allclose(w, 1e-3, rtol=1e-6)
+
+
+def test_weights_nonfinite_costs():
+    p = di(K=8)
+    c = np.array([1, np.nan, 2, np.inf, 0.5, 3, np.nan, 1], np.float32)
+    w, d = p.weights(c)
+    assert np.all(np.isfinite(w)) and w[1] == 0 and w[3] == 0 and w[6] == 0
+    assert abs(w.sum() - 1) < 1e-6 and d[0] == 0.5
+    w, d = p.weights(np.full(8, np.nan, np.float32))
+    assert np.all(w == np.float32(0.125))
+
+
+# ----------------------------------------------------------------- reduce --
+@pytest.mark.parametrize("fp64", [False, True])
+def test_reduce_uniform_weights_is_mean_noise(fp64):
+    K, H = 64, 5
+    p = di(K=K, H=H, lo=np.array([-50, -50], np.float32), hi=np.array([50, 50], np.float32))
+    u_nom = np.random.default_rng(2).uniform(-0.5, 0.5, (H, 2))
+    u = p.reduce(9, np.full(K, 1.0 / K), u_nom, fp64=fp64)
+    eps = np.mean([noise64(p, k, 9) for k in range(K)], axis=0) * SIG.astype(np.float64)
+    assert np.max(np.abs(u - (u_nom + eps))) < (1e-12 if fp64 else 2e-6)
+
+
+def test_reduce_one_hot_weights_and_clamp():
+    K, H = 32, 4
+    p = di(K=K, H=H, lo=np.array([-0.1, -0.2], np.float32), hi=np.array([0.1, 0.2], np.float32))
+    for j in (0, 17, 31):
+        w = np.zeros(K)
+        w[j] = 1.0
+        u = p.reduce(4, w, np.zeros((H, 2)), fp64=True)
+        want = np.clip(noise64(p, j, 4) * SIG.astype(np.float64), p.u_min.astype(np.float64),
+                       p.u_max.astype(np.float64))
+        assert np.max(np.abs(u - want)) < 1e-12
+
+
+# ---------------------------------------------------------------- rollout --
+def closed_form_di_costs(p, x0, u_nom, step, with_terminal=True):
+    """Costs of the double integrator in matrix form: x_{t+1} = A x_t + B u_t."""
+    dt = np.float64(np.float32(0.1))
+    A = np.array([[1, 0, dt, 0], [0, 1, 0, dt], [0, 0, 1, 0], [0, 0, 0, 1]])
+    B = np.array([[0, 0], [0, 0], [dt, 0], [0, dt]])
+    out = []
+    for k in range(p.K):
+        u = np.clip(u_nom + noise64(p, k, step) * SIG.astype(np.float64), LO, HI)
+        x = x0.astype(np.float64)
+        c = 0.0
+        for t in range(p.H):
+            x = A @ x + B @ u[t]
+            c += x[0] ** 2 + x[1] ** 2
+        if with_terminal:
+            c += x[0] ** 2 + x[1] ** 2
+        out.append(c)
+    return np.array(out)
+
+
+def test_rollout_double_integrator_closed_form():
+    p = di(K=24, H=12)
+    x0 = np.array([1.0, -2.0, 0.3, 0.1], np.float32)
+    u_nom = np.random.default_rng(4).uniform(-0.5, 0.5, (12, 2))
+    want = closed_form_di_costs(p, x0, u_nom, 5)
+    got64 = p.rollout(5, x0, u_nom, fp64=True)
+    assert np.max(np.abs(got64 - want) / want) < 1e-12
+    got32 = p.rollout(5, x0, u_nom.astype(np.float32))
+    assert np.max(np.abs(got32 - want) / want) < 1e-5
+    # no terminal cost callable: the terminal term disappears
+    p2 = di(K=24, H=12, terminal="")
+    assert np.max(np.abs(p2.rollout(5, x0, u_nom, fp64=True) -
+                         closed_form_di_costs(p2, x0, u_nom, 5, with_terminal=False)) / want) < 1e-12
+
+
+def test_rollout_depends_on_step_agent_and_seed():
+    p = di(K=16)
+    x0 = np.array([1, 1, 0, 0], np.float32)
+    u = np.zeros((6, 2), np.float32)
+    base = p.rollout(0, x0, u)
+    assert not np.array_equal(base, p.rollout(1, x0, u))
+    assert not np.array_equal(base, p.rollout(0, x0, u, agent=1))
+    assert not np.array_equal(base, di(K=16, seed=4).rollout(0, x0, u))
+    assert np.array_equal(base, p.rollout(0, x0, u, threads=4))
+
+
+def test_full_covariance_noise_has_sigma_covariance():
+    cov = np.array([[0.5, 0.2], [0.2, 0.3]], np.float32)
+    p = di(K=4096, H=1, sigma=cov, cov_full=True, lo=np.array([-1e3, -1e3], np.float32),
+           hi=np.array([1e3, 1e3], np.float32))
+    # one-hot reduce at t=0 returns eps_k = L z_k exactly
+    L = np.linalg.cholesky(cov.astype(np.float64))
+    eps = []
+    for k in range(0, 4096, 4):
+        w = np.zeros(4096)
+        w[k] = 1.0
+        e = p.reduce(1, w, np.zeros((1, 2)), fp64=True)[0]
+        assert np.max(np.abs(e - L @ noise64(p, k, 1)[0])) < 1e-6
+        eps.append(e)
+    assert np.max(np.abs(np.cov(np.array(eps).T) - cov)) < 0.05
+
+
+def test_unicycle_fp32_vs_fp64_costs():
+    ora, _ = unicycle_oracle(1024, 32, 8)
+    x0 = np.zeros(4, np.float32)
+    c32 = ora.rollout(0, x0, np.zeros((32, 2), np.float32), threads=8)
+    c64 = ora.rollout(0, x0, np.zeros((32, 2)), threads=8, fp64=True)
+    err = np.max(np.abs(c32 - c64) / c64)
+    print(f"[unicycle C1] FP32 oracle cost error vs FP64: {err:.3e}")
+    assert err < 1e-4
+
+
+# ------------------------------------------------------------- closed loop --
+def test_simulate_equals_step_loop_with_plant():
+    p = di(K=64, H=10, sigma=np.array([1, 1], np.float32), lo=np.array([-1, -1], np.float32),
+           hi=np.array([1, 1], np.float32), terminal="")
+    x = np.array([1, 1, 0, 0], np.float32)
+    un = np.zeros((10, 2), np.float32)
+    st, ct, kt, tt = p.simulate(5, 0, x)
+    dt = np.float32(0.1)
+    for s in range(5):
+        u, d, _, _ = p.step(s, x, un)
+        assert np.array_equal(st[s, 0], x) and np.array_equal(ct[s, 0], u[0]) and kt[s, 0] == d[0]
+        x = np.array([x[0] + x[2] * dt, x[1] + x[3] * dt, x[2] + u[0, 0] * dt, x[3] + u[0, 1] * dt],
+                     np.float32)
+        un = np.concatenate([u[1:], u[-1:]])
+    assert np.array_equal(tt[0], x)
+
+
+def test_batch_agent0_equals_single_and_terminal_relation():
+    p = di(K=64, H=10)
+    x0 = np.array([[1, 1, 0, 0], [-1, 0.5, 0, 0]], np.float32)
+    s1 = p.simulate(4, 0, x0[0])
+    sb = p.simulate(4, 0, x0, n_agents=2)
+    st1, ct1, kt1, tt1 = s1
+    stb, ctb, ktb, ttb = sb
+    assert np.array_equal(st1[:, 0], stb[:, 0]) and np.array_equal(ct1[:, 0], ctb[:, 0])
+    assert np.array_equal(kt1[:, 0], ktb[:, 0]) and np.array_equal(tt1[0], ttb[0])
+    ext = p.simulate(5, 0, x0, n_agents=2)
+    assert np.array_equal(sb[3], ext[0][4])
