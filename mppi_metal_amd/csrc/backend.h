@@ -1,0 +1,119 @@
+// backend.h — internal declarations shared by the C-ABI implementation files
+// (hip_backend.cpp: contexts, plugin linking, single-agent and uniform-batch
+// steps; batch_backend.cpp: the uneven multi-agent step).  Not installed.
+#pragma once
+
+#include "mppi_hip.h"
+#include "mppi_launch.h"
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace mppi_hip_impl {
+
+constexpr uint32_t kBlock = 256;      // workgroup size of the grid-mapped kernels
+constexpr uint32_t kReduceBPG = 8;    // noise blocks per reduce tile
+constexpr uint32_t kMaxHC = 4096;     // horizon * control_dim limit
+
+// One loaded code object and the kernels looked up in it.
+struct LoadedModule {
+    hipModule_t module = nullptr;
+    hipFunction_t rollout = nullptr;  // grid-mapped kernels
+    hipFunction_t reduce = nullptr;
+    // batch kernels, indexed [block 64/128/256 -> 0/1/2][table 0 / persistent 1]
+    hipFunction_t rollout_b[3][2] = {};
+    hipFunction_t reduce_b[3][2] = {};
+};
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void release();
+};
+
+struct PinnedBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void release();
+};
+
+size_t align_up(size_t x, size_t a);
+
+}  // namespace mppi_hip_impl
+
+struct mppi_hip_ctx {
+    // configuration
+    uint32_t D = 0, C = 0, H = 0, K = 0;
+    float lambda = 1.0f;
+    uint32_t cov_form = MPPI_HIP_COV_DIAGONAL;
+    std::vector<float> noise_scale;  // sigma (diag) or lower Cholesky factor (full)
+    std::vector<float> u_min, u_max;
+    uint32_t numeric_mode = MPPI_HIP_FP32;
+    int device = 0;
+    std::string arch;
+    int num_cus = 256;
+
+    // plugin (kept so the batch kernels can be compiled later)
+    bool linked = false;
+    std::shared_ptr<mppi_hip_impl::LoadedModule> mod;        // plain kernels
+    std::shared_ptr<mppi_hip_impl::LoadedModule> batch_mod;  // + batch kernels, lazily
+    std::string plugin_src, plugin_name, plugin_dir;
+    std::string dyn_name, stage_name, term_name;
+    size_t model_bytes = 0, cost_bytes = 0;
+    uint32_t key0 = 0, key1 = 0;
+
+    // runtime
+    hipStream_t stream = nullptr;
+    std::string error;
+    uint32_t spt_override = 0;
+
+    // device buffers of the single-agent / uniform-batch path
+    mppi_hip_impl::DeviceBuffer consts;  // noise_scale | u_min | u_max | zero blob
+    mppi_hip_impl::DeviceBuffer model_bound, cost_bound;
+    bool model_is_bound = false, cost_is_bound = false;
+    mppi_hip_impl::DeviceBuffer up;      // desc | agents | x | u_nom | model ovr | cost ovr
+    mppi_hip_impl::DeviceBuffer down;    // u_out | diag
+    mppi_hip_impl::DeviceBuffer costs, blk_stats, weights, partials;
+    mppi_hip_impl::DeviceBuffer hist_states, hist_controls, hist_costs, terminal;
+    mppi_hip_impl::PinnedBuffer up_host, down_host;
+    uint32_t last_agents = 0;
+    // where read_costs / read_weights find the last iteration's data
+    const float* last_costs = nullptr;
+    const float* last_weights = nullptr;
+    std::vector<mppi_agent_desc> last_desc;
+
+    // profiling
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending_rollout, pending_reduce;
+    double prof_rollout_ms = 0.0, prof_reduce_ms = 0.0;
+    uint64_t prof_launches = 0;
+};
+
+namespace mppi_hip_impl {
+
+int fail(mppi_hip_ctx* ctx, int code, const std::string& msg);
+int ensure_device_buffer(mppi_hip_ctx* ctx, DeviceBuffer* b, size_t bytes);
+int ensure_pinned_buffer(mppi_hip_ctx* ctx, PinnedBuffer* b, size_t bytes);
+int check_ctx(mppi_hip_ctx* ctx);
+// Compile (or fetch from the caches) and load the plugin module; batch = 1
+// adds the tile-table and persistent kernels.
+int load_module(mppi_hip_ctx* ctx, bool batch, std::shared_ptr<LoadedModule>* out);
+hipEvent_t take_event(mppi_hip_ctx* ctx);
+void collect_profile(mppi_hip_ctx* ctx);
+// Common kernel arguments (constants, bound blobs, work buffers) of a step.
+void fill_common_args(mppi_hip_ctx* ctx, mppi_step_args* a);
+
+}  // namespace mppi_hip_impl
+
+#define MPPI_HIP_TRY(ctx, expr)                                                          \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return ::mppi_hip_impl::fail(ctx, MPPI_HIP_E_DEVICE,                          \
+                                         std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
+    } while (0)

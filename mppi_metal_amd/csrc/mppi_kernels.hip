@@ -7,28 +7,34 @@
 //   MPPI_DYNAMICS                   name of the dynamics callable
 //   MPPI_STAGE_COST, MPPI_TERMINAL_COST  cost callables (or the defaults)
 //   MPPI_COV_FULL                   0: diagonal sigma, 1: lower Cholesky factor
+//   MPPI_BATCH_KERNELS              1: also emit the tile-table and persistent
+//                                   kernels of the uneven multi-agent step
 // so the plugin's callables are direct, inlined calls (the reference reaches
 // them through a visible function table at every timestep,
 // shaders/mppi_rollout.metal:90-91).
 //
 // One MPPI step is two launches:
-//   mppi_rollout  one sample per lane; control noise from Philox4x32-10 in
-//                 registers (never stored); clamp; dynamics; stage cost on the
-//                 post-dynamics state; terminal cost; FP32 cost accumulation.
-//                 Each workgroup also reduces its 256 costs to
-//                 (min, sum exp, sum cost, sum exp^2) with wave shuffles and
-//                 LDS, and the last workgroup of an agent combines those
-//                 partials into the softmax normaliser (log-sum-exp form).
-//                 Reference: mppi_rollout (mppi_rollout.metal:36-96) and the
-//                 min / sum passes of mppi_compute_weights
-//                 (mppi_reduce.metal:14-121).
-//   mppi_reduce   weighted-noise average, parallel over samples: every lane
-//                 regenerates its noise blocks from the same counters and
-//                 accumulates w_k * z_k; workgroup partials are summed by the
-//                 last workgroup, which writes u_next, the diagnostics and,
-//                 in closed-loop mode, steps the plant and shifts the warm
-//                 start.  Reference: mppi_reduce (mppi_reduce.metal:130-168)
-//                 and mppi_propagate_and_shift (mppi_reduce.metal:175-231).
+//   rollout  one sample per lane; control noise from Philox4x32-10 in
+//            registers (never stored); clamp; dynamics; stage cost on the
+//            post-dynamics state; terminal cost; FP32 cost accumulation.
+//            Each workgroup also reduces its costs to (min, sum exp, sum cost,
+//            sum exp^2) with wave shuffles and LDS, and the last workgroup of
+//            an agent combines those partials into the softmax normaliser
+//            (log-sum-exp form).  Reference: mppi_rollout
+//            (mppi_rollout.metal:36-96, batch :103-170) and the min / sum
+//            passes of mppi_compute_weights (mppi_reduce.metal:14-121, :242-356).
+//   reduce   weighted-noise average, parallel over samples: every lane
+//            regenerates its noise blocks from the same counters and
+//            accumulates w~_k * z_k; workgroup partials are summed by the
+//            last workgroup of the agent, which writes u_next, the
+//            diagnostics and, in closed-loop mode, steps the plant and shifts
+//            the warm start.  Reference: mppi_reduce (mppi_reduce.metal:130-168,
+//            :363-406) and mppi_propagate_and_shift (:175-231, :413-488).
+// Work is cut into tiles: rollout (agent, sample block), reduce (agent,
+// sample chunk, noise-block group).  Three dispatch modes share the tile
+// bodies: grid-mapped (one workgroup per tile in grid order), tile table
+// (workgroup i runs tile i of a host-built, longest-first table) and
+// persistent (a fixed grid pulls tiles from an atomic queue).
 // Cross-block sums (normaliser and weighted noise) run in FP64 when K >= 2^18.
 #if !defined(__HIPCC_RTC__)
 #include <hip/hip_runtime.h>   // hipRTC provides these builtins itself
@@ -46,13 +52,16 @@
 #ifndef MPPI_COV_FULL
 #define MPPI_COV_FULL 0
 #endif
+#ifndef MPPI_BATCH_KERNELS
+#define MPPI_BATCH_KERNELS 0
+#endif
 
 #define SDIM MPPI_SDIM
 #define CDIM MPPI_CDIM
-#define MPPI_BLOCK 256
-#define MPPI_WAVES (MPPI_BLOCK / 64)
-#define MPPI_REDUCE_BPG 8                 // noise blocks per reduce workgroup
+#define MPPI_REDUCE_BPG 8                 // noise blocks per reduce tile
+#define MPPI_NV (4 * MPPI_REDUCE_BPG)     // accumulators per reduce thread
 #define MPPI_FP64_MIN_K (1u << 18)
+#define MPPI_MAX_HC 4096
 
 constexpr int mppi_gcd(int a, int b) { return b == 0 ? a : mppi_gcd(b, a % b); }
 // Timesteps whose C*G noise values fill whole Philox blocks.
@@ -86,7 +95,8 @@ __device__ inline float mppi_wave_min(float v) {
     return v;
 }
 
-__device__ inline float mppi_wave_sum(float v) {
+template <typename T>
+__device__ inline T mppi_wave_sum(T v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
@@ -96,7 +106,7 @@ __device__ inline bool mppi_isfinite(float x) {
     return (mppi_f2u(x) & 0x7f800000u) != 0x7f800000u;
 }
 
-// Control for timestep t from the normals z[0..CDIM): u = clamp(u_nom + eps).
+// Control for one timestep from the normals z[0..CDIM): u = clamp(u_nom + eps).
 __device__ inline void mppi_make_control(const float* __restrict__ un, const float* z,
                                          const float* __restrict__ scale,
                                          const float* __restrict__ lo,
@@ -118,19 +128,22 @@ __device__ inline void mppi_make_control(const float* __restrict__ un, const flo
 #endif
 }
 
-
-// Combine the per-workgroup (min, sum e, sum c, sum e^2) partials of one agent
-// into the softmax normaliser: m = min m_g, S = sum_g S_g exp(-(m_g - m)/lambda),
+// ---------------------------------------------------------------------------
+// Softmax normaliser: combine the per-workgroup (min, sum e, sum c, sum e^2)
+// partials of one agent: m = min m_g, S = sum_g S_g exp(-(m_g - m)/lambda),
 // Q likewise with the squared factor, ESS = S^2 / Q (= 1 / sum w^2).
 // T is the accumulation type: float, or double when K >= 2^18.
-template <typename T>
-__device__ void mppi_combine_stats(const mppi_rollout_args& a, uint32_t agent) {
-    __shared__ float s_min[MPPI_WAVES];
-    __shared__ T s_sum[3][MPPI_WAVES];
+// ---------------------------------------------------------------------------
+template <int BLOCK, typename T>
+__device__ void mppi_combine_stats(const mppi_step_args& a, uint32_t agent,
+                                   const mppi_agent_desc& d) {
+    constexpr int WAVES = BLOCK / 64;
+    __shared__ float s_min[WAVES];
+    __shared__ T s_sum[3][WAVES];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const float* st = a.blk_stats + size_t(agent) * a.n_blk * 4;
+    const float* st = a.blk_stats + d.blk_off * 4;
     float mg = __builtin_inff();
-    for (uint32_t i = threadIdx.x; i < a.n_blk; i += MPPI_BLOCK) {
+    for (uint32_t i = threadIdx.x; i < d.n_blk; i += BLOCK) {
         float v = st[i * 4];
         mg = v < mg ? v : mg;
     }
@@ -139,65 +152,67 @@ __device__ void mppi_combine_stats(const mppi_rollout_args& a, uint32_t agent) {
     __syncthreads();
     mg = s_min[0];
 #pragma unroll
-    for (int w = 1; w < MPPI_WAVES; ++w) mg = s_min[w] < mg ? s_min[w] : mg;
+    for (int w = 1; w < WAVES; ++w) mg = s_min[w] < mg ? s_min[w] : mg;
     T S = T(0), Cs = T(0), Q = T(0);
-    for (uint32_t i = threadIdx.x; i < a.n_blk; i += MPPI_BLOCK) {
+    for (uint32_t i = threadIdx.x; i < d.n_blk; i += BLOCK) {
         float bm = st[i * 4 + 0];
         float f = (mppi_isfinite(bm) && mppi_isfinite(mg)) ? mppi_expf(-(bm - mg) / a.lambda) : 0.0f;
         S += T(st[i * 4 + 1]) * T(f);
         Cs += T(st[i * 4 + 2]);
         Q += T(st[i * 4 + 3]) * (T(f) * T(f));
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        S += __shfl_xor(S, o, 64);
-        Cs += __shfl_xor(Cs, o, 64);
-        Q += __shfl_xor(Q, o, 64);
-    }
+    S = mppi_wave_sum(S);
+    Cs = mppi_wave_sum(Cs);
+    Q = mppi_wave_sum(Q);
     if (lane == 0) { s_sum[0][wave] = S; s_sum[1][wave] = Cs; s_sum[2][wave] = Q; }
     __syncthreads();
     if (threadIdx.x == 0) {
         S = s_sum[0][0]; Cs = s_sum[1][0]; Q = s_sum[2][0];
 #pragma unroll
-        for (int w = 1; w < MPPI_WAVES; ++w) { S += s_sum[0][w]; Cs += s_sum[1][w]; Q += s_sum[2][w]; }
+        for (int w = 1; w < WAVES; ++w) { S += s_sum[0][w]; Cs += s_sum[1][w]; Q += s_sum[2][w]; }
         if (!mppi_isfinite(mg)) {
             // Every cost is NaN or infinite: fall back to uniform weights.
-            S = T(a.K);
-            Q = T(a.K);
+            S = T(d.K);
+            Q = T(d.K);
         }
         mppi_agent_state* ag = &a.agents[agent];
         ag->min_cost = mg;
         ag->weight_sum = double(S);
-        ag->mean_cost = double(Cs / T(a.K));
+        ag->mean_cost = double(Cs / T(d.K));
         ag->ess = double((S * S) / Q);
         ag->rollout_arrivals = 0u;
         __threadfence();
     }
 }
 
-extern "C" __global__ void __launch_bounds__(MPPI_BLOCK)
-mppi_rollout(mppi_rollout_args a) {
-    const uint32_t agent = blockIdx.y;
-    const uint32_t k = blockIdx.x * MPPI_BLOCK + threadIdx.x;
-    const uint32_t K = a.K, H = a.H;
-    const bool active = k < K;
+// ---------------------------------------------------------------------------
+// Rollout tile: samples [blk*BLOCK, blk*BLOCK + BLOCK) of one agent.
+// ---------------------------------------------------------------------------
+template <int BLOCK>
+__device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint32_t blk) {
+    constexpr int WAVES = BLOCK / 64;
+    const mppi_agent_desc d = a.desc[agent];
+    const uint32_t k = blk * BLOCK + threadIdx.x;
+    const uint32_t H = d.H;
+    const bool active = k < d.K;
     const uint32_t step = a.agents[agent].step;
     const uint32_t tag_agent = (MPPI_TAG_CONTROL_NOISE << 28) | (agent & MPPI_AGENT_MASK);
     const uint32_t dyn_tag = (MPPI_TAG_DYNAMICS << 28) | (agent & MPPI_AGENT_MASK);
-    const float* __restrict__ un = a.u_nom + size_t(agent) * H * CDIM;
+    const float* __restrict__ un = a.u_nom + d.seq_off;
+    const uint8_t* model = a.model + agent * a.model_stride;
+    const uint8_t* cost = a.cost + agent * a.cost_stride;
 
     float total = 0.0f;
     if (active) {
         float x[SDIM];
 #pragma unroll
-        for (int i = 0; i < SDIM; ++i) x[i] = a.x0[size_t(agent) * SDIM + i];
+        for (int i = 0; i < SDIM; ++i) x[i] = a.x[size_t(agent) * SDIM + i];
         for (uint32_t t0 = 0; t0 < H; t0 += MPPI_G) {
             float z[MPPI_NG];
             const uint32_t b0 = (t0 * CDIM) >> 2;
 #pragma unroll
             for (int q = 0; q < MPPI_BG; ++q) {
-                mppi_f32x4 v = mppi_normal4(
-                    mppi_u32x4{b0 + q, k, step, tag_agent}, a.key0, a.key1);
+                mppi_f32x4 v = mppi_normal4(mppi_u32x4{b0 + q, k, step, tag_agent}, a.key0, a.key1);
                 z[4 * q + 0] = v.x; z[4 * q + 1] = v.y;
                 z[4 * q + 2] = v.z; z[4 * q + 3] = v.w;
             }
@@ -206,23 +221,23 @@ mppi_rollout(mppi_rollout_args a) {
                 const uint32_t t = t0 + g;
                 if (t < H) {
                     float u[CDIM];
-                    mppi_make_control(un + t * CDIM, z + g * CDIM, a.noise_scale,
-                                      a.u_min, a.u_max, u);
+                    mppi_make_control(un + t * CDIM, z + g * CDIM, a.noise_scale, a.u_min,
+                                      a.u_max, u);
                     mppi_rng rng{t << 8, k, step, dyn_tag, a.key0, a.key1};
-                    MPPI_DYNAMICS(mppi_state_ref{x}, u, mppi_blob_ref{a.model}, rng);
+                    MPPI_DYNAMICS(mppi_state_ref{x}, u, mppi_blob_ref{model}, rng);
                     total += mppi_stage(mppi_form_timed{}, mppi_cstate_ref{x}, (const float*)u,
-                                        mppi_blob_ref{a.model}, mppi_blob_ref{a.cost},
+                                        mppi_blob_ref{model}, mppi_blob_ref{cost},
                                         mppi_time{t, step});
                 }
             }
         }
-        total += mppi_terminal(mppi_form_timed{}, mppi_cstate_ref{x}, mppi_blob_ref{a.model},
-                               mppi_blob_ref{a.cost}, mppi_time{H, step});
-        a.costs[size_t(agent) * K + k] = total;
+        total += mppi_terminal(mppi_form_timed{}, mppi_cstate_ref{x}, mppi_blob_ref{model},
+                               mppi_blob_ref{cost}, mppi_time{H, step});
+        a.costs[d.cost_off + k] = total;
     }
 
-    // ---- workgroup softmax partials over this block's costs ----
-    __shared__ float s_red[4][MPPI_WAVES];
+    // ---- workgroup softmax partials over this tile's costs ----
+    __shared__ float s_red[4][WAVES];
     __shared__ int s_last;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     float c = (active && mppi_isfinite(total)) ? total : __builtin_inff();
@@ -231,165 +246,88 @@ mppi_rollout(mppi_rollout_args a) {
     __syncthreads();
     m = s_red[0][0];
 #pragma unroll
-    for (int w = 1; w < MPPI_WAVES; ++w) m = s_red[0][w] < m ? s_red[0][w] : m;
+    for (int w = 1; w < WAVES; ++w) m = s_red[0][w] < m ? s_red[0][w] : m;
     float e = (mppi_isfinite(c) && mppi_isfinite(m)) ? mppi_expf(-(c - m) / a.lambda) : 0.0f;
     float se = mppi_wave_sum(e);
     float sq = mppi_wave_sum(e * e);
     float sc = mppi_wave_sum(active ? total : 0.0f);
     if (lane == 0) { s_red[1][wave] = se; s_red[2][wave] = sc; s_red[3][wave] = sq; }
     __syncthreads();
-    float* bs = a.blk_stats + (size_t(agent) * a.n_blk + blockIdx.x) * 4;
     if (threadIdx.x == 0) {
         float S = 0.0f, Cs = 0.0f, Q = 0.0f;
 #pragma unroll
-        for (int w = 0; w < MPPI_WAVES; ++w) { S += s_red[1][w]; Cs += s_red[2][w]; Q += s_red[3][w]; }
+        for (int w = 0; w < WAVES; ++w) { S += s_red[1][w]; Cs += s_red[2][w]; Q += s_red[3][w]; }
+        float* bs = a.blk_stats + (d.blk_off + blk) * 4;
         bs[0] = m; bs[1] = S; bs[2] = Cs; bs[3] = Q;
         __threadfence();
         uint32_t prev = atomicAdd(&a.agents[agent].rollout_arrivals, 1u);
-        s_last = (prev == a.n_blk - 1);
+        s_last = (prev == d.n_blk - 1);
     }
     __syncthreads();
     if (!s_last) return;
-
-    // ---- last workgroup of this agent: combine the partials ----
     __threadfence();
-    if (K >= MPPI_FP64_MIN_K) mppi_combine_stats<double>(a, agent);
-    else mppi_combine_stats<float>(a, agent);
+    if (d.K >= MPPI_FP64_MIN_K) mppi_combine_stats<BLOCK, double>(a, agent, d);
+    else mppi_combine_stats<BLOCK, float>(a, agent, d);
 }
 
-
-// Sum the per-workgroup weighted-noise partials of one agent over all sample
-// chunks and divide by the normaliser: zs[n] = (sum_k w~_k z_k(n)) / S for the
-// NB*4 noise slots n.  Columns are spread over all threads in row groups so
-// the loads stay coalesced and several are in flight per thread.
-template <typename T>
-__device__ void mppi_sum_partials(const mppi_reduce_args& a, uint32_t agent, float* zs) {
-    __shared__ T s_col[MPPI_BLOCK];
-    const uint32_t W = a.NB * 4;
-    const T S = T(a.agents[agent].weight_sum);
-    const float* pb = a.partials + size_t(agent) * a.n_chunks * W;
-    if (W <= MPPI_BLOCK) {
-        const uint32_t R = MPPI_BLOCK / W;
+// ---------------------------------------------------------------------------
+// Finalize (last reduce workgroup of an agent): sum the weighted-noise
+// partials over sample chunks, divide by the normaliser, write u_next and the
+// diagnostics; in closed loop also log history, step the plant and shift.
+// ---------------------------------------------------------------------------
+template <int BLOCK, typename T>
+__device__ void mppi_sum_partials(const mppi_step_args& a, const mppi_agent_desc& d,
+                                  double weight_sum, float* zs) {
+    __shared__ T s_col[BLOCK];
+    const uint32_t W = d.NB * 4;
+    const T S = T(weight_sum);
+    const float* pb = a.partials + d.part_off;
+    if (W <= BLOCK) {
+        const uint32_t R = BLOCK / W;
         const uint32_t c = threadIdx.x % W, r = threadIdx.x / W;
         if (r < R) {
             T s = T(0);
-            for (uint32_t ch = r; ch < a.n_chunks; ch += R) s += T(pb[size_t(ch) * W + c]);
+            for (uint32_t ch = r; ch < d.n_chunks; ch += R) s += T(pb[size_t(ch) * W + c]);
             s_col[r * W + c] = s;
         }
         __syncthreads();
-        for (uint32_t n = threadIdx.x; n < W; n += MPPI_BLOCK) {
+        for (uint32_t n = threadIdx.x; n < W; n += BLOCK) {
             T s = T(0);
             for (uint32_t q = 0; q < R; ++q) s += s_col[q * W + n];
             zs[n] = float(s / S);
         }
     } else {
-        for (uint32_t n = threadIdx.x; n < W; n += MPPI_BLOCK) {
+        for (uint32_t n = threadIdx.x; n < W; n += BLOCK) {
             T s = T(0);
-            for (uint32_t ch = 0; ch < a.n_chunks; ++ch) s += T(pb[size_t(ch) * W + n]);
+            for (uint32_t ch = 0; ch < d.n_chunks; ++ch) s += T(pb[size_t(ch) * W + n]);
             zs[n] = float(s / S);
         }
     }
 }
 
-extern "C" __global__ void __launch_bounds__(MPPI_BLOCK)
-mppi_reduce(mppi_reduce_args a) {
-    const uint32_t agent = blockIdx.z;
-    const uint32_t chunk = blockIdx.x, grp = blockIdx.y;
-    const uint32_t K = a.K, H = a.H, NB = a.NB;
-    const uint32_t step = a.agents[agent].step;
-    const double mind = a.agents[agent].min_cost;
-    const float m = float(mind);
-    const float Sf = float(a.agents[agent].weight_sum);
-    const bool uniform = !mppi_isfinite(m);
-    const uint32_t tag_agent = (MPPI_TAG_CONTROL_NOISE << 28) | (agent & MPPI_AGENT_MASK);
-    const uint32_t b_begin = grp * MPPI_REDUCE_BPG;
-
-    float acc[MPPI_REDUCE_BPG][4];
-#pragma unroll
-    for (int q = 0; q < MPPI_REDUCE_BPG; ++q)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[q][j] = 0.0f;
-
-    const uint32_t k_base = chunk * MPPI_BLOCK * a.spt + threadIdx.x;
-    for (uint32_t s = 0; s < a.spt; ++s) {
-        const uint32_t k = k_base + s * MPPI_BLOCK;
-        if (k >= K) break;
-        const float c = a.costs[size_t(agent) * K + k];
-        float w;
-        if (uniform) w = 1.0f;
-        else w = mppi_isfinite(c) ? mppi_expf(-(c - m) / a.lambda) : 0.0f;
-        if (grp == 0 && a.weights) a.weights[size_t(agent) * K + k] = w / Sf;
-#pragma unroll
-        for (int q = 0; q < MPPI_REDUCE_BPG; ++q) {
-            if (b_begin + q < NB) {
-                mppi_f32x4 z = mppi_normal4(mppi_u32x4{b_begin + q, k, step, tag_agent},
-                                            a.key0, a.key1);
-                acc[q][0] = mppi_fma(w, z.x, acc[q][0]);
-                acc[q][1] = mppi_fma(w, z.y, acc[q][1]);
-                acc[q][2] = mppi_fma(w, z.z, acc[q][2]);
-                acc[q][3] = mppi_fma(w, z.w, acc[q][3]);
-            }
-        }
-    }
-
-    // ---- workgroup sum of the 4*BPG accumulators (transpose through LDS) ----
-    constexpr int NV = 4 * MPPI_REDUCE_BPG;        // 32 values per thread
-    __shared__ float s_acc[MPPI_BLOCK * (NV + 1)];
-    __shared__ int s_last;
-#pragma unroll
-    for (int q = 0; q < MPPI_REDUCE_BPG; ++q)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) s_acc[threadIdx.x * (NV + 1) + q * 4 + j] = acc[q][j];
+template <int BLOCK>
+__device__ void mppi_finalize(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
+                              float* zs) {
+    const uint32_t H = d.H, HC = H * CDIM;
+    mppi_agent_state* ag = &a.agents[agent];
+    const uint32_t step = ag->step;
+    if (d.K >= MPPI_FP64_MIN_K) mppi_sum_partials<BLOCK, double>(a, d, ag->weight_sum, zs);
+    else mppi_sum_partials<BLOCK, float>(a, d, ag->weight_sum, zs);
     __syncthreads();
-    {
-        // 256 threads: value v = tid % NV, segment = tid / NV sums 256*NV/256 rows
-        const int v = threadIdx.x % NV, seg = threadIdx.x / NV;
-        constexpr int NSEG = MPPI_BLOCK / NV;      // 8
-        constexpr int ROWS = MPPI_BLOCK / NSEG;    // 32
-        float sum = 0.0f;
-        for (int r = 0; r < ROWS; ++r) sum += s_acc[(seg * ROWS + r) * (NV + 1) + v];
-        __syncthreads();
-        s_acc[seg * NV + v] = sum;
-        __syncthreads();
-        if (threadIdx.x < NV) {
-            float tot = 0.0f;
-#pragma unroll
-            for (int g = 0; g < NSEG; ++g) tot += s_acc[g * NV + threadIdx.x];
-            const uint32_t b = b_begin + threadIdx.x / 4;
-            if (b < NB)
-                a.partials[((size_t(agent) * a.n_chunks + chunk) * NB + b) * 4 + (threadIdx.x & 3)] = tot;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        uint32_t prev = atomicAdd(&a.agents[agent].reduce_arrivals, 1u);
-        s_last = (prev == a.n_chunks * a.n_groups - 1);
-    }
-    __syncthreads();
-    if (!s_last) return;
-
-    // ---- last workgroup of this agent: finalize u_next ----
-    __threadfence();
-    const uint32_t HC = H * CDIM;
-    float* zs = s_acc;                                  // reuse: HC <= 4096 floats
-    if (K >= MPPI_FP64_MIN_K) mppi_sum_partials<double>(a, agent, zs);
-    else mppi_sum_partials<float>(a, agent, zs);
-    __syncthreads();
-    float* uo = a.u_out + size_t(agent) * HC;
-    float* un = a.u_nom + size_t(agent) * HC;
-    for (uint32_t n = threadIdx.x; n < HC; n += MPPI_BLOCK) {
+    float* uo = a.u_out + d.seq_off;
+    float* un = a.u_nom + d.seq_off;
+    for (uint32_t n = threadIdx.x; n < HC; n += BLOCK) {
         const uint32_t t = n / CDIM, i = n % CDIM;
 #if MPPI_COV_FULL
         float e = 0.0f;
         for (uint32_t j = 0; j <= i; ++j) e = mppi_fma(a.noise_scale[i * CDIM + j], zs[t * CDIM + j], e);
 #else
+        (void)t;
         float e = a.noise_scale[i] * zs[n];
 #endif
         uo[n] = mppi_clampf(un[n] + e, a.u_min[i], a.u_max[i]);
     }
     __syncthreads();
-    mppi_agent_state* ag = &a.agents[agent];
     if (threadIdx.x == 0 && a.diag) {
         a.diag[agent * 3 + 0] = float(ag->min_cost);
         a.diag[agent * 3 + 1] = float(ag->mean_cost);
@@ -398,7 +336,7 @@ mppi_reduce(mppi_reduce_args a) {
     if (a.closed_loop) {
         if (threadIdx.x == 0) {
             const uint32_t row = ag->hist;
-            float* xs = a.x_state + size_t(agent) * SDIM;
+            float* xs = a.x + size_t(agent) * SDIM;
             float x[SDIM];
             float u[CDIM];
 #pragma unroll
@@ -414,7 +352,7 @@ mppi_reduce(mppi_reduce_args a) {
             if (a.hist_costs) a.hist_costs[size_t(row) * a.N + agent] = float(ag->min_cost);
             mppi_rng rng{0u, 0xffffffffu, step,
                          (MPPI_TAG_PLANT << 28) | (agent & MPPI_AGENT_MASK), a.key0, a.key1};
-            MPPI_DYNAMICS(mppi_state_ref{x}, u, mppi_blob_ref{a.model}, rng);
+            MPPI_DYNAMICS(mppi_state_ref{x}, u, mppi_blob_ref{a.model + agent * a.model_stride}, rng);
 #pragma unroll
             for (int i = 0; i < SDIM; ++i) xs[i] = x[i];
             if (a.terminal)
@@ -422,10 +360,9 @@ mppi_reduce(mppi_reduce_args a) {
             ag->hist = row + 1;
             ag->step = step + 1;
         }
-        for (uint32_t n = threadIdx.x; n < HC; n += MPPI_BLOCK) {
+        for (uint32_t n = threadIdx.x; n < HC; n += BLOCK) {
             const uint32_t t = n / CDIM, i = n % CDIM;
-            const uint32_t src = (t + 1 < H ? t + 1 : H - 1) * CDIM + i;
-            un[n] = uo[src];
+            un[n] = uo[(t + 1 < H ? t + 1 : H - 1) * CDIM + i];
         }
     }
     if (threadIdx.x == 0) {
@@ -433,3 +370,154 @@ mppi_reduce(mppi_reduce_args a) {
         __threadfence();
     }
 }
+
+// ---------------------------------------------------------------------------
+// Reduce tile: samples of chunk `chunk` times noise blocks of group `grp`.
+// ---------------------------------------------------------------------------
+template <int BLOCK>
+__device__ void mppi_reduce_tile(const mppi_step_args& a, uint32_t agent, uint32_t chunk,
+                                 uint32_t grp) {
+    const mppi_agent_desc d = a.desc[agent];
+    const uint32_t K = d.K, NB = d.NB;
+    const mppi_agent_state* ag = &a.agents[agent];
+    const uint32_t step = ag->step;
+    const float m = float(ag->min_cost);
+    const float Sf = float(ag->weight_sum);
+    const bool uniform = !mppi_isfinite(m);
+    const uint32_t tag_agent = (MPPI_TAG_CONTROL_NOISE << 28) | (agent & MPPI_AGENT_MASK);
+    const uint32_t b_begin = grp * MPPI_REDUCE_BPG;
+    const float* costs = a.costs + d.cost_off;
+
+    float acc[MPPI_REDUCE_BPG][4];
+#pragma unroll
+    for (int q = 0; q < MPPI_REDUCE_BPG; ++q)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[q][j] = 0.0f;
+
+    const uint32_t k_base = chunk * BLOCK * d.spt + threadIdx.x;
+    for (uint32_t s = 0; s < d.spt; ++s) {
+        const uint32_t k = k_base + s * BLOCK;
+        if (k >= K) break;
+        const float c = costs[k];
+        float w;
+        if (uniform) w = 1.0f;
+        else w = mppi_isfinite(c) ? mppi_expf(-(c - m) / a.lambda) : 0.0f;
+        if (grp == 0 && a.weights) a.weights[d.cost_off + k] = w / Sf;
+#pragma unroll
+        for (int q = 0; q < MPPI_REDUCE_BPG; ++q) {
+            if (b_begin + q < NB) {
+                mppi_f32x4 z = mppi_normal4(mppi_u32x4{b_begin + q, k, step, tag_agent},
+                                            a.key0, a.key1);
+                acc[q][0] = mppi_fma(w, z.x, acc[q][0]);
+                acc[q][1] = mppi_fma(w, z.y, acc[q][1]);
+                acc[q][2] = mppi_fma(w, z.z, acc[q][2]);
+                acc[q][3] = mppi_fma(w, z.w, acc[q][3]);
+            }
+        }
+    }
+
+    // ---- workgroup sum of the accumulators, transposed through LDS ----
+    constexpr int LDS = (BLOCK * (MPPI_NV + 1) > MPPI_MAX_HC + 4) ? BLOCK * (MPPI_NV + 1)
+                                                                  : MPPI_MAX_HC + 4;
+    __shared__ float s_acc[LDS];
+    __shared__ int s_last;
+#pragma unroll
+    for (int q = 0; q < MPPI_REDUCE_BPG; ++q)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) s_acc[threadIdx.x * (MPPI_NV + 1) + q * 4 + j] = acc[q][j];
+    __syncthreads();
+    {
+        constexpr int NSEG = BLOCK / MPPI_NV;     // row segments
+        constexpr int ROWS = BLOCK / NSEG;        // rows per segment
+        const int v = threadIdx.x % MPPI_NV, seg = threadIdx.x / MPPI_NV;
+        float sum = 0.0f;
+        for (int r = 0; r < ROWS; ++r) sum += s_acc[(seg * ROWS + r) * (MPPI_NV + 1) + v];
+        __syncthreads();
+        s_acc[seg * MPPI_NV + v] = sum;
+        __syncthreads();
+        if (threadIdx.x < MPPI_NV) {
+            float tot = 0.0f;
+#pragma unroll
+            for (int g = 0; g < NSEG; ++g) tot += s_acc[g * MPPI_NV + threadIdx.x];
+            const uint32_t b = b_begin + threadIdx.x / 4;
+            if (b < NB) a.partials[d.part_off + (size_t(chunk) * NB + b) * 4 + (threadIdx.x & 3)] = tot;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        uint32_t prev = atomicAdd(&a.agents[agent].reduce_arrivals, 1u);
+        s_last = (prev == d.n_chunks * d.n_groups - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    mppi_finalize<BLOCK>(a, agent, d, s_acc);
+}
+
+// ---------------------------------------------------------------------------
+// Entry points.  Grid-mapped (the single-agent and uniform batch path):
+//   rollout grid (max n_blk, N), reduce grid (max n_chunks, max n_groups, N).
+// ---------------------------------------------------------------------------
+extern "C" __global__ void __launch_bounds__(256) mppi_rollout(mppi_step_args a) {
+    const uint32_t agent = blockIdx.y;
+    if (blockIdx.x >= a.desc[agent].n_blk) return;
+    mppi_rollout_tile<256>(a, agent, blockIdx.x);
+}
+
+extern "C" __global__ void __launch_bounds__(256) mppi_reduce(mppi_step_args a) {
+    const uint32_t agent = blockIdx.z;
+    if (blockIdx.x >= a.desc[agent].n_chunks || blockIdx.y >= a.desc[agent].n_groups) return;
+    mppi_reduce_tile<256>(a, agent, blockIdx.x, blockIdx.y);
+}
+
+#if MPPI_BATCH_KERNELS
+// Tile-table and persistent kernels of the uneven multi-agent step.  Tiles
+// are (agent, x, y) triples: rollout (agent, block, 0), reduce (agent, chunk,
+// group).  The persistent form loops until the queue is empty; the last
+// workgroup to leave resets the queue for the next launch.
+template <int BLOCK, bool ROLLOUT>
+__device__ inline void mppi_run_tile(const mppi_step_args& a, uint32_t i) {
+    const uint32_t* t = a.tiles + size_t(i) * 3;
+    if (ROLLOUT) mppi_rollout_tile<BLOCK>(a, t[0], t[1]);
+    else mppi_reduce_tile<BLOCK>(a, t[0], t[1], t[2]);
+}
+
+template <int BLOCK, bool ROLLOUT>
+__device__ inline void mppi_persistent(const mppi_step_args& a) {
+    __shared__ uint32_t s_tile;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(&a.queue[0], 1u);
+        __syncthreads();
+        const uint32_t i = s_tile;
+        __syncthreads();
+        if (i >= a.n_tiles) break;
+        mppi_run_tile<BLOCK, ROLLOUT>(a, i);
+    }
+    if (threadIdx.x == 0) {
+        if (atomicAdd(&a.queue[1], 1u) == gridDim.x - 1) {
+            a.queue[0] = 0u;
+            a.queue[1] = 0u;
+            __threadfence();
+        }
+    }
+}
+
+#define MPPI_BATCH_ENTRIES(B)                                                                   \
+    extern "C" __global__ void __launch_bounds__(B) mppi_rollout_table_b##B(mppi_step_args a) {   \
+        mppi_run_tile<B, true>(a, blockIdx.x);                                                  \
+    }                                                                                           \
+    extern "C" __global__ void __launch_bounds__(B) mppi_reduce_table_b##B(mppi_step_args a) {    \
+        mppi_run_tile<B, false>(a, blockIdx.x);                                                 \
+    }                                                                                           \
+    extern "C" __global__ void __launch_bounds__(B) mppi_rollout_persist_b##B(mppi_step_args a) { \
+        mppi_persistent<B, true>(a);                                                            \
+    }                                                                                           \
+    extern "C" __global__ void __launch_bounds__(B) mppi_reduce_persist_b##B(mppi_step_args a) {  \
+        mppi_persistent<B, false>(a);                                                           \
+    }
+
+MPPI_BATCH_ENTRIES(64)
+MPPI_BATCH_ENTRIES(128)
+MPPI_BATCH_ENTRIES(256)
+#endif

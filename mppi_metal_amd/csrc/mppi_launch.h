@@ -1,6 +1,7 @@
 // mppi_launch.h — argument blocks of the MPPI kernels, shared by the kernel
 // source (compiled by hipRTC) and the host backend that fills them, so both
-// sides agree on the layout.  Plain C layout: pointers, 32-bit words, doubles.
+// sides agree on the layout.  Plain C layout: pointers, 32/64-bit words,
+// doubles.
 #ifndef MPPI_HIP_LAUNCH_H
 #define MPPI_HIP_LAUNCH_H
 
@@ -8,6 +9,7 @@
 #include <stdint.h>
 #endif
 
+// Per-agent mutable state on the device.
 struct mppi_agent_state {
     uint32_t step;               // low 32 bits of the MPPI step index
     uint32_t hist;               // history row (closed loop)
@@ -19,45 +21,52 @@ struct mppi_agent_state {
     double ess;
 };
 
-struct mppi_rollout_args {
-    const float* x0;             // [N][D]
-    const float* u_nom;          // [N][H][C]
-    const uint8_t* model;
-    const uint8_t* cost;
+// Per-agent shape and buffer offsets (immutable for a batch shape).  All
+// offsets are in elements of the buffer they index.
+struct mppi_agent_desc {
+    uint32_t K;                  // samples
+    uint32_t H;                  // horizon
+    uint32_t n_blk;              // rollout workgroups: ceil(K / block)
+    uint32_t NB;                 // noise blocks per sample: ceil(H*C / 4)
+    uint32_t n_chunks;           // reduce sample chunks: ceil(K / (block * spt))
+    uint32_t n_groups;           // reduce noise-block groups: ceil(NB / 8)
+    uint32_t spt;                // reduce samples per thread
+    uint32_t pad;
+    uint64_t cost_off;           // costs / weights
+    uint64_t seq_off;            // u_nom / u_out (floats, H*C per agent)
+    uint64_t blk_off;            // blk_stats (4 floats per rollout workgroup)
+    uint64_t part_off;           // partials (floats)
+};
+
+struct mppi_step_args {
+    const mppi_agent_desc* desc; // [N]
+    mppi_agent_state* agents;    // [N]
+    const uint32_t* tiles;       // table / persistent modes: [n_tiles][3]
+    uint32_t* queue;             // persistent mode: [2] head, exits (self-resetting)
+    uint32_t n_tiles;
+    uint32_t N;
+    float* x;                    // [N][D] state (x0; plant state in closed loop)
+    float* u_nom;                // packed by desc.seq_off
+    float* u_out;                // packed by desc.seq_off
+    const uint8_t* model;        // model blob of agent 0
+    const uint8_t* cost;         // cost blob of agent 0
+    uint64_t model_stride;       // bytes between agents' model blobs (0: shared)
+    uint64_t cost_stride;        // bytes between agents' cost blobs (0: shared)
     const float* noise_scale;    // [C] sigma or [C][C] lower factor
     const float* u_min;          // [C]
     const float* u_max;          // [C]
-    float* costs;                // [N][K]
-    float* blk_stats;            // [N][n_blk][4]
-    mppi_agent_state* agents;    // [N]
-    uint32_t K, H, N, n_blk;
-    uint32_t key0, key1;
-    float lambda;
-    uint32_t pad;
-};
-
-struct mppi_reduce_args {
-    const float* costs;          // [N][K]
-    mppi_agent_state* agents;    // [N]
-    float* u_nom;                // [N][H][C]; shifted in place in closed loop
-    float* u_out;                // [N][H][C]
-    const float* noise_scale;
-    const float* u_min;
-    const float* u_max;
-    float* weights;              // [N][K] or null
-    float* partials;             // [N][n_chunks][NB][4]
+    float* costs;                // packed by desc.cost_off
+    float* weights;              // packed by desc.cost_off, or null
+    float* blk_stats;            // packed by desc.blk_off
+    float* partials;             // packed by desc.part_off
     float* diag;                 // [N][3]
-    float* x_state;              // [N][D] plant state (closed loop)
     float* hist_states;          // [T][N][D] or null
     float* hist_controls;        // [T][N][C] or null
     float* hist_costs;           // [T][N] or null
     float* terminal;             // [N][D] or null
-    const uint8_t* model;
-    uint32_t K, H, N, NB;
-    uint32_t n_chunks, n_groups, spt, closed_loop;
+    uint32_t closed_loop;        // 1: plant step + warm-start shift after the reduce
     uint32_t key0, key1;
     float lambda;
-    uint32_t pad;
 };
 
 #endif  // MPPI_HIP_LAUNCH_H
