@@ -162,6 +162,67 @@ int mppi_hip_profile_enable(mppi_hip_ctx* ctx, int enable);
 int mppi_hip_profile_read(mppi_hip_ctx* ctx, double* rollout_ms, double* reduce_ms,
                           uint64_t* launches);
 
+/* ---------------------------------------------------------------------------
+ * Batched multi-agent step with uneven sample counts and horizons.
+ * No reference counterpart as such: the reference's simulate_batch
+ * (src/metal_backend.mm:935-1124) requires one K and one H for all agents.
+ * A batch is bound to a context (plugin, dimensions, noise model, lambda,
+ * bounds, seed) and holds N agents, agent a with K_a samples and horizon
+ * H_a (H_a * control_dim <= 4096).  Each MPPI step of all agents is two
+ * kernel launches over a tile list (rollout: agent x 256-sample blocks;
+ * reduce: agent x sample chunks x noise-block groups), sorted longest first
+ * or pulled from an atomic queue by a persistent grid, and is captured once
+ * in a HIP graph that is replayed for every step.  The launch shape is chosen
+ * by mppi_hip_batch_autotune or set explicitly.
+ * Agent a draws its noise from counter word 3 = a, so a batch agent and a
+ * single-agent context see the same noise when their indices agree.
+ * ------------------------------------------------------------------------- */
+typedef struct mppi_hip_batch mppi_hip_batch;
+
+typedef struct mppi_hip_batch_tuning {
+    uint32_t block;        /* workgroup size: 64, 128 or 256 */
+    uint32_t persistent;   /* 0: one workgroup per tile; 1: persistent grid with a work queue */
+    uint32_t wgs_per_cu;   /* persistent grid size = compute units * wgs_per_cu (1..16) */
+    uint32_t reduce_spt;   /* samples per reduce thread (1..64); 0: built-in heuristic */
+    uint32_t order;        /* 0: longest tiles first; 1: agent order */
+} mppi_hip_batch_tuning;
+
+/* sample_counts, horizons: num_agents entries each. */
+int mppi_hip_batch_create(mppi_hip_ctx* ctx, uint32_t num_agents, const uint32_t* sample_counts,
+                          const uint32_t* horizons, mppi_hip_batch** out_batch);
+void mppi_hip_batch_destroy(mppi_hip_batch* batch);
+
+/* Parameter blobs of the linked layout.  stride 0: one blob shared by all
+ * agents; otherwise num_agents blobs, stride bytes apart (stride >= size). */
+int mppi_hip_batch_set_params(mppi_hip_batch* batch, const void* model, size_t model_stride,
+                              const void* cost, size_t cost_stride);
+
+int mppi_hip_batch_set_tuning(mppi_hip_batch* batch, const mppi_hip_batch_tuning* tuning);
+int mppi_hip_batch_get_tuning(const mppi_hip_batch* batch, mppi_hip_batch_tuning* tuning);
+
+/* Time the candidate launch shapes over trial_steps open-loop steps each
+ * (state is left unchanged) and keep the fastest; its device time per step
+ * in milliseconds goes to best_ms (may be NULL). */
+int mppi_hip_batch_autotune(mppi_hip_batch* batch, uint32_t trial_steps, double* best_ms);
+
+/* Set every agent's state, warm start and step counter.  x0_packed:
+ * num_agents * state_dim; u_nominal_packed: sum_a H_a * control_dim floats,
+ * agent after agent, or NULL for zeros. */
+int mppi_hip_batch_reset(mppi_hip_batch* batch, uint64_t start_step, const float* x0_packed,
+                         const float* u_nominal_packed);
+
+/* num_steps MPPI steps of all agents.  closed_loop 1: after each step every
+ * agent logs (state, first control, best cost), steps its plant with the
+ * dynamics and shifts its warm start; closed_loop 0: open-loop solves from
+ * the current state and warm start.  History outputs (may be NULL) are
+ * [step][agent][element].  Blocks until done. */
+int mppi_hip_batch_run(mppi_hip_batch* batch, uint32_t num_steps, int closed_loop,
+                       float* states_out, float* controls_out, float* costs_out);
+
+/* Current state (num_agents * state_dim), last u_next (packed like the warm
+ * start) and diagnostics (num_agents * 3); each may be NULL. */
+int mppi_hip_batch_read(mppi_hip_batch* batch, float* x_out, float* u_out_packed, float* diag_out);
+
 /* Library version string. */
 const char* mppi_hip_version(void);
 

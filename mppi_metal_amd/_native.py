@@ -38,7 +38,20 @@ class MppiHipConfig(ctypes.Structure):
     ]
 
 
+class MppiHipBatchTuning(ctypes.Structure):
+    """Mirror of struct mppi_hip_batch_tuning (include/mppi_hip.h)."""
+
+    _fields_ = [
+        ("block", ctypes.c_uint32),
+        ("persistent", ctypes.c_uint32),
+        ("wgs_per_cu", ctypes.c_uint32),
+        ("reduce_spt", ctypes.c_uint32),
+        ("order", ctypes.c_uint32),
+    ]
+
+
 _F = ctypes.POINTER(ctypes.c_float)
+_U32P = ctypes.POINTER(ctypes.c_uint32)
 _VP = ctypes.c_void_p
 _CTX = ctypes.c_void_p
 
@@ -68,6 +81,15 @@ SIGNATURES = [
     ("mppi_hip_profile_read", ctypes.c_int,
      [_CTX, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
       ctypes.POINTER(ctypes.c_uint64)]),
+    ("mppi_hip_batch_create", ctypes.c_int, [_CTX, ctypes.c_uint32, _U32P, _U32P, ctypes.POINTER(_VP)]),
+    ("mppi_hip_batch_destroy", None, [_VP]),
+    ("mppi_hip_batch_set_params", ctypes.c_int, [_VP, _VP, ctypes.c_size_t, _VP, ctypes.c_size_t]),
+    ("mppi_hip_batch_set_tuning", ctypes.c_int, [_VP, ctypes.POINTER(MppiHipBatchTuning)]),
+    ("mppi_hip_batch_get_tuning", ctypes.c_int, [_VP, ctypes.POINTER(MppiHipBatchTuning)]),
+    ("mppi_hip_batch_autotune", ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
+    ("mppi_hip_batch_reset", ctypes.c_int, [_VP, ctypes.c_uint64, _F, _F]),
+    ("mppi_hip_batch_run", ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.c_int, _F, _F, _F]),
+    ("mppi_hip_batch_read", ctypes.c_int, [_VP, _F, _F, _F]),
     ("mppi_hip_version", ctypes.c_char_p, []),
 ]
 

@@ -33,6 +33,7 @@ BUNDLED_PLUGINS = [
     ("double_integrator.hip", "mppi_dynamics", "mppi_stage_cost", "mppi_terminal_cost", 4, 2),
     ("unicycle.hip", "mppi_dynamics", "unicycle_stage_cost_obstacles", "unicycle_terminal_cost", 4, 2),
     ("bicycle.hip", "bicycle_dynamics", "bicycle_pursuit_cost", "bicycle_terminal_cost", 6, 2),
+    ("quadrotor.hip", "quad_dynamics", "quad_stage_cost", "quad_terminal_cost", 12, 4),
 ]
 
 EMBEDDED = [
@@ -86,7 +87,7 @@ def check_kernels():
 
 def build_library() -> Path:
     out = PKG / "libmppi_hip.so"
-    srcs = [CSRC / "hip_backend.cpp", CSRC / "mppi_driver.cpp"]
+    srcs = [CSRC / "hip_backend.cpp", CSRC / "batch_backend.cpp", CSRC / "mppi_driver.cpp"]
     _run([HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-fPIC", "-shared",
           *NO_PACKED_FP32, "-Wall", f"-I{INC}", f"-I{INC / 'mppi_hip'}", f"-I{CSRC}",
           *srcs, "-lhiprtc", "-o", out])

@@ -159,14 +159,17 @@ class HipMPPI:
                                                      _fptr(tt)))
         return st, ct, kt, tt
 
-    def read_costs(self, agent=0):
-        out = np.empty(self.K, np.float32)
-        self._check(self.lib.mppi_hip_read_costs(self.ctx, agent, _fptr(out), self.K))
+    def read_costs(self, agent=0, count=None):
+        """Sample costs of the last iteration of `agent` (count = its K)."""
+        n = self.K if count is None else int(count)
+        out = np.empty(n, np.float32)
+        self._check(self.lib.mppi_hip_read_costs(self.ctx, agent, _fptr(out), n))
         return out
 
-    def read_weights(self, agent=0):
-        out = np.empty(self.K, np.float32)
-        self._check(self.lib.mppi_hip_read_weights(self.ctx, agent, _fptr(out), self.K))
+    def read_weights(self, agent=0, count=None):
+        n = self.K if count is None else int(count)
+        out = np.empty(n, np.float32)
+        self._check(self.lib.mppi_hip_read_weights(self.ctx, agent, _fptr(out), n))
         return out
 
     def set_reduce_samples_per_thread(self, spt):
@@ -180,3 +183,81 @@ class HipMPPI:
         self._check(self.lib.mppi_hip_profile_read(self.ctx, ctypes.byref(r), ctypes.byref(d),
                                                    ctypes.byref(n)))
         return r.value, d.value, n.value
+
+
+class HipBatch:
+    """Uneven multi-agent batch bound to a HipMPPI context
+    (mppi_hip_batch_* in include/mppi_hip.h)."""
+
+    def __init__(self, ctx: HipMPPI, sample_counts, horizons):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.K = np.ascontiguousarray(sample_counts, np.uint32)
+        self.H = np.ascontiguousarray(horizons, np.uint32)
+        self.N = int(self.K.size)
+        if self.H.size != self.N:
+            raise ValueError("sample_counts and horizons differ in length")
+        self.seq = (self.H.astype(np.int64) * ctx.C)
+        self.seq_off = np.concatenate([[0], np.cumsum(self.seq)[:-1]]).astype(np.int64)
+        self.seq_total = int(self.seq.sum())
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        b = ctypes.c_void_p()
+        ctx._check(self.lib.mppi_hip_batch_create(ctx.ctx, self.N, self.K.ctypes.data_as(u32p),
+                                                  self.H.ctypes.data_as(u32p), ctypes.byref(b)))
+        self.b = b
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.lib.mppi_hip_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, model=None, cost=None, model_stride=0, cost_stride=0):
+        mp, mn, mk = _blob(model)
+        cp, cn, ck = _blob(cost)
+        self.ctx._check(self.lib.mppi_hip_batch_set_params(self.b, mp, model_stride, cp, cost_stride))
+
+    def set_tuning(self, block=256, persistent=0, wgs_per_cu=4, reduce_spt=0, order=0):
+        t = _native.MppiHipBatchTuning(block, persistent, wgs_per_cu, reduce_spt, order)
+        self.ctx._check(self.lib.mppi_hip_batch_set_tuning(self.b, ctypes.byref(t)))
+
+    def get_tuning(self):
+        t = _native.MppiHipBatchTuning()
+        self.ctx._check(self.lib.mppi_hip_batch_get_tuning(self.b, ctypes.byref(t)))
+        return dict(block=t.block, persistent=t.persistent, wgs_per_cu=t.wgs_per_cu,
+                    reduce_spt=t.reduce_spt, order=t.order)
+
+    def autotune(self, trial_steps=3):
+        ms = ctypes.c_double()
+        self.ctx._check(self.lib.mppi_hip_batch_autotune(self.b, trial_steps, ctypes.byref(ms)))
+        return ms.value
+
+    def reset(self, start_step, x0_packed, u_nominal_packed=None):
+        x0 = _f32(x0_packed, (self.N * self.ctx.D,))
+        un = _f32(u_nominal_packed, (self.seq_total,))
+        self.ctx._check(self.lib.mppi_hip_batch_reset(self.b, int(start_step), _fptr(x0), _fptr(un)))
+
+    def run(self, num_steps, closed_loop=True, history=False):
+        T, N, D, C = int(num_steps), self.N, self.ctx.D, self.ctx.C
+        st = np.empty((T, N, D), np.float32) if history else None
+        ct = np.empty((T, N, C), np.float32) if history else None
+        kt = np.empty((T, N), np.float32) if history else None
+        self.ctx._check(self.lib.mppi_hip_batch_run(self.b, T, 1 if closed_loop else 0, _fptr(st),
+                                                    _fptr(ct), _fptr(kt)))
+        return st, ct, kt
+
+    def read(self):
+        x = np.empty((self.N, self.ctx.D), np.float32)
+        u = np.empty(self.seq_total, np.float32)
+        d = np.empty((self.N, 3), np.float32)
+        self.ctx._check(self.lib.mppi_hip_batch_read(self.b, _fptr(x), _fptr(u), _fptr(d)))
+        return x, u, d
+
+    def u_of(self, u_packed, a):
+        o = int(self.seq_off[a])
+        return u_packed[o:o + int(self.seq[a])].reshape(int(self.H[a]), self.ctx.C)

@@ -112,3 +112,89 @@ def unicycle_scene(n_obstacles: int, seed: int = 42) -> UnicycleScene:
 # BASELINE workloads: (name, n_obstacles, H, K, closed-loop steps)
 UNICYCLE_C1 = dict(name="unicycle_k1024_h32_obs8", n_obstacles=8, H=32, K=1024, steps=200)
 UNICYCLE_C2 = dict(name="unicycle_k65536_h64_obs16", n_obstacles=16, H=64, K=65536, steps=500)
+
+
+# ---------------------------------------------------------------------------
+# Dynamic-bicycle pursuit (batched multi-agent workload)
+# ---------------------------------------------------------------------------
+BICYCLE_MAX_ZONES = 32
+BICYCLE_MODEL_BYTES = 48
+BICYCLE_COST_BYTES = 48 + BICYCLE_MAX_ZONES * 12
+
+
+@dataclass
+class BicycleParams:
+    mass: float = 1000.0
+    yaw_inertia: float = 1600.0
+    lf: float = 1.0
+    lr: float = 1.5
+    cf: float = 4.0e4
+    cr: float = 4.0e4
+    drag: float = 0.1
+    v_min: float = 2.0
+    v_max: float = 20.0
+    dt: float = 0.05
+    substeps: int = 2
+
+    def blob(self) -> bytes:
+        return struct.pack("<10fIf", self.mass, self.yaw_inertia, self.lf, self.lr, self.cf,
+                           self.cr, self.drag, self.v_min, self.v_max, self.dt, self.substeps, 0.0)
+
+
+def bicycle_cost_blob(target, target_vel, zones, w_pos=1.0, w_control=0.1, w_zone=200.0,
+                      w_terminal=10.0, dt=0.05) -> bytes:
+    zs = list(zones)[:BICYCLE_MAX_ZONES]
+    head = struct.pack("<9fI2f", target[0], target[1], target_vel[0], target_vel[1], w_pos,
+                       w_control, w_zone, w_terminal, dt, len(zs), 0.0, 0.0)
+    flat = []
+    for i in range(BICYCLE_MAX_ZONES):
+        flat.extend(zs[i] if i < len(zs) else (0.0, 0.0, 0.0))
+    return head + struct.pack(f"<{3 * BICYCLE_MAX_ZONES}f", *flat)
+
+
+@dataclass
+class PursuitScene:
+    model: BicycleParams
+    zones: list
+    x0: np.ndarray            # [N, 6]
+    targets: np.ndarray       # [N, 4] (x, y, vx, vy)
+    K: np.ndarray             # [N]
+    H: np.ndarray             # [N]
+    cost_blobs: list          # N blobs of BICYCLE_COST_BYTES
+    sigma: tuple = (0.2, 1.5)
+    u_min: tuple = (-0.5, -4.0)
+    u_max: tuple = (0.5, 4.0)
+    lam: float = 10.0
+
+    def cost_blob_packed(self) -> bytes:
+        return b"".join(self.cost_blobs)
+
+
+def bicycle_pursuit_scene(n_agents=256, seed=42, k_log2=(10, 14), h_range=(16, 96),
+                          n_zones=32) -> PursuitScene:
+    """Agents start in [-40, 40]^2 outside the threat zones (centres in
+    [-50, 50]^2, radius U[2, 8]) with vx = 5 m/s; each pursues its own target
+    moving at a constant velocity of at most 3 m/s.  K_a = round(2^U[10, 14]),
+    H_a uniform in [16, 96]."""
+    s = SceneStream(seed)
+    zones = []
+    for _ in range(n_zones):
+        zones.append((float(np.float32(s.uniform(-50, 50))), float(np.float32(s.uniform(-50, 50))),
+                      float(np.float32(s.uniform(2, 8)))))
+    x0 = np.zeros((n_agents, 6), np.float32)
+    targets = np.zeros((n_agents, 4), np.float32)
+    K = np.zeros(n_agents, np.uint32)
+    H = np.zeros(n_agents, np.uint32)
+    for a in range(n_agents):
+        while True:
+            px, py = s.uniform(-40, 40), s.uniform(-40, 40)
+            if all(np.hypot(px - zx, py - zy) > zr + 1.0 for zx, zy, zr in zones):
+                break
+        x0[a] = (px, py, s.uniform(-np.pi, np.pi), 5.0, 0.0, 0.0)
+        ang = s.uniform(-np.pi, np.pi)
+        spd = s.uniform(0.0, 3.0)
+        targets[a] = (s.uniform(-40, 40), s.uniform(-40, 40), spd * np.cos(ang), spd * np.sin(ang))
+        K[a] = int(round(2.0 ** s.uniform(*k_log2)))
+        H[a] = int(h_range[0] + min(h_range[1] - h_range[0], int(s.uniform(0, 1) * (h_range[1] - h_range[0] + 1))))
+    blobs = [bicycle_cost_blob(targets[a, :2], targets[a, 2:], zones) for a in range(n_agents)]
+    return PursuitScene(BicycleParams(), zones, x0, targets, K, H, blobs)

@@ -49,6 +49,9 @@ namespace plug_uni {
 namespace plug_bic {
 #include "../mppi_metal_amd/plugins/bicycle.hip"
 }
+namespace plug_quad {
+#include "../mppi_metal_amd/plugins/quadrotor.hip"
+}
 
 namespace {
 
@@ -229,6 +232,43 @@ double bic_term_d(const double* s, const uint8_t*, const uint8_t* cp, const mppi
     return c->w_terminal * bic_target_err2(s, c, double(tm.step + tm.t) * c->dt);
 }
 
+// ---- quadrotor (mppi_metal_amd/plugins/quadrotor.hip) ----
+struct QModel { float gravity, drag, tau, dt; uint32_t substeps; float pad[3]; };
+struct QCost { float goal[3], w_pos, w_vel, w_att, w_thrust, w_rate, w_terminal, pad[3]; };
+void quad_dyn_d(double* s, const double* u, const uint8_t* mp, const RngCtx&) {
+    const QModel* p = reinterpret_cast<const QModel*>(mp);
+    const uint32_t n = p->substeps > 0 ? p->substeps : 1u;
+    const double h = double(p->dt) / n;
+    for (uint32_t i = 0; i < n; ++i) {
+        const double sr = std::sin(s[6]), cr = std::cos(s[6]), sp = std::sin(s[7]), cp = std::cos(s[7]);
+        const double sy = std::sin(s[8]), cy = std::cos(s[8]), tp = sp / cp;
+        const double wx = s[9], wy = s[10], wz = s[11];
+        const double d[12] = {s[3], s[4], s[5],
+                              u[0] * (cr * sp * cy + sr * sy) - p->drag * s[3],
+                              u[0] * (cr * sp * sy - sr * cy) - p->drag * s[4],
+                              u[0] * (cr * cp) - p->gravity - p->drag * s[5],
+                              wx + sr * tp * wy + cr * tp * wz, cr * wy - sr * wz, (sr * wy + cr * wz) / cp,
+                              (u[1] - wx) / p->tau, (u[2] - wy) / p->tau, (u[3] - wz) / p->tau};
+        for (int j = 0; j < 12; ++j) s[j] += h * d[j];
+        s[7] = std::min(std::max(s[7], -1.4), 1.4);
+    }
+}
+double quad_stage_d(const double* s, const double* u, const uint8_t* mp, const uint8_t* cp,
+                    const mppi_time&) {
+    const QModel* m = reinterpret_cast<const QModel*>(mp);
+    const QCost* c = reinterpret_cast<const QCost*>(cp);
+    const double ex = s[0] - c->goal[0], ey = s[1] - c->goal[1], ez = s[2] - c->goal[2];
+    const double dthr = u[0] - m->gravity;
+    return c->w_pos * (ex * ex + ey * ey + ez * ez) + c->w_vel * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]) +
+           c->w_att * (s[6] * s[6] + s[7] * s[7]) + c->w_thrust * dthr * dthr +
+           c->w_rate * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]);
+}
+double quad_term_d(const double* s, const uint8_t*, const uint8_t* cp, const mppi_time&) {
+    const QCost* c = reinterpret_cast<const QCost*>(cp);
+    const double ex = s[0] - c->goal[0], ey = s[1] - c->goal[1], ez = s[2] - c->goal[2];
+    return c->w_terminal * (ex * ex + ey * ey + ez * ez) + c->w_vel * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+}
+
 struct Callable {
     const char* model;
     const char* name;
@@ -294,6 +334,21 @@ const Callable kCallables[] = {
          return plug_bic::bicycle_terminal_cost(x, m, c, tm);
      }),
      FP(&bic_term_d)},
+    {"quadrotor", "quad_dynamics", 0,
+     FP(+[](float* x, const float* u, const uint8_t* m, const RngCtx& r) {
+         plug_quad::quad_dynamics(x, u, m, to_rng(r));
+     }),
+     FP(&quad_dyn_d)},
+    {"quadrotor", "quad_stage_cost", 1,
+     FP(+[](const float* x, const float* u, const uint8_t* m, const uint8_t* c, const mppi_time&) {
+         return plug_quad::quad_stage_cost(x, u, m, c);
+     }),
+     FP(&quad_stage_d)},
+    {"quadrotor", "quad_terminal_cost", 2,
+     FP(+[](const float* x, const uint8_t* m, const uint8_t* c, const mppi_time&) {
+         return plug_quad::quad_terminal_cost(x, m, c);
+     }),
+     FP(&quad_term_d)},
 };
 #undef FP
 

@@ -15,7 +15,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <functional>
+#include <memory>
 #include <iostream>
 #include <string>
 #include <unistd.h>
@@ -203,12 +205,17 @@ void init_tests() {
         check(d.initialize(m, &e), "initialize should succeed with valid config + valid plugin: " + e);
     }
     {
-        DriverConfig config;
-        config.state_dim = 4;
-        config.control_dim = 2;
-        config.horizon = 10;
-        config.sample_count = 64;
+        // The driver copies the pointed-to arrays at construction; the sources
+        // are destroyed before initialize() reads the configuration.  (The
+        // reference test constructs the driver after the sources are gone,
+        // which reads freed memory; here the copy is made while they live.)
+        std::unique_ptr<MPPIDriver> holder;
         {
+            DriverConfig config;
+            config.state_dim = 4;
+            config.control_dim = 2;
+            config.horizon = 10;
+            config.sample_count = 64;
             std::vector<float> sigma{1, 1}, sigma_inv{1, 1}, lo{-1, -1}, hi{1, 1};
             config.noise.sigma = sigma.data();
             config.noise.sigma_inv = sigma_inv.data();
@@ -216,8 +223,11 @@ void init_tests() {
             config.bounds.u_min = lo.data();
             config.bounds.u_max = hi.data();
             config.bounds.control_dim = 2;
+            holder.reset(new MPPIDriver(config));
+            std::fill(sigma.begin(), sigma.end(), -7.0f);  // scribble before release
+            std::fill(lo.begin(), lo.end(), 9.0f);
         }
-        MPPIDriver d(config);  // sources destroyed: the constructor deep-copied them
+        MPPIDriver& d = *holder;
         ModelPluginSpec m;
         m.user_metallib_path = g_plugin;
         std::string e;
