@@ -52,10 +52,11 @@ def reduce_bytes(K: int, H: int, C: int, n_chunks: int) -> int:
     return 8 * K + 8 * H * C + 2 * 16 * n_chunks * nb
 
 
-def reduce_chunks(K: int, H: int, C: int, N: int = 1) -> int:
+def reduce_chunks(K: int, H: int, C: int, N: int = 1, spt: int = 0) -> int:
     nb = (H * C + 3) // 4
     groups = (nb + 7) // 8
-    spt = max(1, min(64, -(-(K * groups * N) // (256 * 2048))))
+    if not spt:
+        spt = max(1, min(64, -(-(K * groups * N) // (256 * 2048))))
     return -(-K // (256 * spt))
 
 
@@ -81,6 +82,7 @@ def main():
     ap.add_argument("--K", type=int, default=65536)
     ap.add_argument("--H", type=int, default=64)
     ap.add_argument("--obstacles", type=int, default=16)
+    ap.add_argument("--spt", type=int, default=0, help="reduce samples per thread (0: built-in)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -97,6 +99,8 @@ def main():
     hip.link("unicycle.hip", "mppi_dynamics", "unicycle_stage_cost_obstacles",
              "unicycle_terminal_cost", len(mb), len(cb), 42)
     hip.bind_params(mb, cb)
+    if args.spt:
+        hip.set_reduce_samples_per_thread(args.spt)
     x0 = np.zeros(4, np.float32)
     u0 = np.zeros((H, C), np.float32)
     print(f"[bench] workload unicycle K={K} H={H} obstacles={n_obs}; warmup {args.warmup}",
@@ -122,7 +126,7 @@ def main():
     red_s = red_ms / launches / 1e3
     w_ss = valu_ops_per_sample_step(n_obs)
     achieved_ops = K * H * w_ss / roll_s
-    n_chunks = reduce_chunks(K, H, C)
+    n_chunks = reduce_chunks(K, H, C, spt=args.spt)
     rb = reduce_bytes(K, H, C, n_chunks)
 
     # Blocking step latency (host round trip per step).

@@ -171,7 +171,8 @@ int mppi_hip_profile_read(mppi_hip_ctx* ctx, double* rollout_ms, double* reduce_
  * H_a (H_a * control_dim <= 4096).  Each MPPI step of all agents is two
  * kernel launches over a tile list (rollout: agent x 256-sample blocks;
  * reduce: agent x sample chunks x noise-block groups), sorted longest first
- * or pulled from an atomic queue by a persistent grid, and is captured once
+ * (a persistent grid pulling tiles from an atomic queue is built but switched
+ * off, see mppi_hip_batch_tuning.persistent), and is captured once
  * in a HIP graph that is replayed for every step.  The launch shape is chosen
  * by mppi_hip_batch_autotune or set explicitly.
  * Agent a draws its noise from counter word 3 = a, so a batch agent and a
@@ -181,7 +182,8 @@ typedef struct mppi_hip_batch mppi_hip_batch;
 
 typedef struct mppi_hip_batch_tuning {
     uint32_t block;        /* workgroup size: 64, 128 or 256 */
-    uint32_t persistent;   /* 0: one workgroup per tile; 1: persistent grid with a work queue */
+    uint32_t persistent;   /* 0: one workgroup per tile; 1: persistent grid with a work queue
+                              (switched off: rejected with MPPI_HIP_E_INVALID) */
     uint32_t wgs_per_cu;   /* persistent grid size = compute units * wgs_per_cu (1..16) */
     uint32_t reduce_spt;   /* samples per reduce thread (1..64); 0: built-in heuristic */
     uint32_t order;        /* 0: longest tiles first; 1: agent order */
@@ -223,8 +225,85 @@ int mppi_hip_batch_run(mppi_hip_batch* batch, uint32_t num_steps, int closed_loo
  * start) and diagnostics (num_agents * 3); each may be NULL. */
 int mppi_hip_batch_read(mppi_hip_batch* batch, float* x_out, float* u_out_packed, float* diag_out);
 
+/* ---------------------------------------------------------------------------
+ * Learned-dynamics surrogate.  No reference counterpart.  An MLP
+ * (state_dim + control_dim) -> 128 -> 128 -> state_dim with tanh hidden
+ * layers, BF16 weights on the matrix cores and FP32 accumulation, trained on
+ * the GPU on transitions that the linked plugin's own dynamics generates
+ * (states drawn uniformly from [state_lo, state_hi], controls from the
+ * control bounds), predicting the state change of one step.  When enabled,
+ * the rollout of mppi_hip_step / mppi_hip_simulate uses the network in place
+ * of the plugin dynamics; the plugin stays the plant of the closed loop and
+ * the judge of the network's error.  Requires state_dim + control_dim <= 16.
+ * ------------------------------------------------------------------------- */
+/* How the network sees a state component. */
+#define MPPI_HIP_SURROGATE_INPUT_BOX 0       /* normalised over [lo, hi] */
+#define MPPI_HIP_SURROGATE_INPUT_PERIODIC 1  /* the dynamics is periodic in it with period
+                                                hi - lo: wrapped into [lo, hi] first */
+#define MPPI_HIP_SURROGATE_INPUT_IGNORED 2   /* the dynamics is invariant to it: not an input */
+
+/* Floats in the normalisation block of export / import: input centre, input
+ * 1 / half-range (0 for an ignored input), input period (0 if not periodic),
+ * output mean, output standard deviation; 16 each. */
+#define MPPI_HIP_SURROGATE_NORM_FLOATS 80
+
+typedef struct mppi_hip_surrogate_config {
+    const float* state_lo;      /* state_dim: sampling box of the training states */
+    const float* state_hi;
+    const uint32_t* state_input; /* state_dim MPPI_HIP_SURROGATE_INPUT_* (NULL: all BOX) */
+    uint32_t num_transitions;   /* generated transitions (rounded up to whole batches) */
+    uint32_t batch;             /* samples per optimiser step, a multiple of 128 */
+    uint32_t steps;             /* Adam steps */
+    float learning_rate;        /* peak step size (cosine decay to 10%) */
+    uint64_t seed;              /* data stream and initialisation */
+} mppi_hip_surrogate_config;
+
+/* Generate the data and train; loss_history (steps floats, may be NULL)
+ * receives the batch MSE (normalised units) of every step. */
+int mppi_hip_surrogate_train(mppi_hip_ctx* ctx, const mppi_hip_surrogate_config* config,
+                             float* loss_history);
+
+/* Error of one predicted step on num_samples fresh transitions (a data stream
+ * the training never saw): per state component, the RMSE of the predicted
+ * state change and that RMSE divided by the standard deviation of the true
+ * change (each array state_dim floats, either may be NULL). */
+int mppi_hip_surrogate_evaluate(mppi_hip_ctx* ctx, uint32_t num_samples, uint64_t seed,
+                                float* rmse, float* relative_rmse);
+
+/* Next states predicted by the network for n (state, control) pairs. */
+int mppi_hip_surrogate_predict(mppi_hip_ctx* ctx, uint32_t n, const float* states,
+                               const float* controls, float* next_states);
+
+/* 1: rollouts use the network; 0: the plugin dynamics (the default). */
+int mppi_hip_surrogate_use(mppi_hip_ctx* ctx, int enable);
+
+/* Checkpoint: FP32 parameters ([W1 | b1 | W2 | b2 | W3 | b3], row-major
+ * [out][in]) and the normalisation block (MPPI_HIP_SURROGATE_NORM_FLOATS). */
+int mppi_hip_surrogate_export(mppi_hip_ctx* ctx, float* params, size_t capacity, size_t* n_params,
+                              float* norm);
+int mppi_hip_surrogate_import(mppi_hip_ctx* ctx, const float* params, size_t n_params,
+                              const float* norm);
+
+/* One forward and backward pass on n samples (a multiple of 128) of
+ * normalised inputs x (n * (state_dim + control_dim)) and raw state-change
+ * targets (n * state_dim), without updating: the summed gradients of the
+ * batch MSE in grads (n_params floats) and the loss. */
+int mppi_hip_surrogate_gradients(mppi_hip_ctx* ctx, uint32_t n, const float* x, const float* targets,
+                                 float* grads, float* loss);
+
 /* Library version string. */
 const char* mppi_hip_version(void);
+
+/* Compile a plugin with the library kernels for gfx950 without a device
+ * (build hosts, CI): the same sources and options as linking.  No reference
+ * counterpart (the reference compiles its shaders ahead of time with xcrun,
+ * CMakeLists.txt:21-45).  kernel_sets: bit 0 adds the uneven-batch kernels, bit 1
+ * the surrogate kernels.  On MPPI_HIP_E_COMPILE the compiler log is copied
+ * into log (log_capacity bytes, may be NULL). */
+int mppi_hip_compile_check(const char* plugin_source, const char* include_dir, const char* dynamics_fn,
+                           const char* stage_cost_fn, const char* terminal_cost_fn, uint32_t state_dim,
+                           uint32_t control_dim, int covariance_full, uint32_t kernel_sets, char* log,
+                           size_t log_capacity);
 
 #ifdef __cplusplus
 }

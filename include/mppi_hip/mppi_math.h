@@ -161,14 +161,15 @@ MPPI_HD float mppi_cos_poly(float r) {
     return mppi_fma(r2, p, 1.0f);
 }
 
-// Select (sin, cos) of r + q*pi/2 from the kernels' values.
+// Select (sin, cos) of r + q*pi/2 from the kernels' values, without
+// branches: odd quadrants swap sin and cos, quadrants 2 and 3 negate sin,
+// quadrants 1 and 2 negate cos.
 MPPI_HD void mppi_quadrant(int q, float s, float c, float* so, float* co) {
-    switch (q & 3) {
-        case 0: *so = s;  *co = c;  break;
-        case 1: *so = c;  *co = -s; break;
-        case 2: *so = -s; *co = -c; break;
-        default: *so = -c; *co = s; break;
-    }
+    const bool swap = (q & 1) != 0;
+    const float sa = swap ? c : s;
+    const float ca = swap ? s : c;
+    *so = (q & 2) ? -sa : sa;
+    *co = ((q + 1) & 2) ? -ca : ca;
 }
 
 // sin(x), cos(x).  Accurate for |x| <= 1e5 (the three-part pi/2 keeps the
@@ -204,6 +205,8 @@ MPPI_HD float mppi_fmodf(float x, float y) {
     if (!(ax == ax) || !(ay == ay) || ay == 0.0f || ax == mppi_u2f(0x7f800000u))
         return mppi_u2f(0x7fc00000u);
     if (ax < ay) return x;
+    // One subtraction suffices when ay <= ax < 2 ay (exact by Sterbenz).
+    if (ax < ay * 2.0f) return x < 0.0f ? -(ax - ay) : ax - ay;
     // Binary long division: yj runs over ay*2^i from the largest value with
     // ax < 2*yj down to ay.  With yj <= r < 2*yj each r - yj is exact
     // (Sterbenz), and halving yj is exact, so the remainder is exact.

@@ -50,6 +50,24 @@ class MppiHipBatchTuning(ctypes.Structure):
     ]
 
 
+class MppiHipSurrogateConfig(ctypes.Structure):
+    """Mirror of struct mppi_hip_surrogate_config (include/mppi_hip.h)."""
+
+    _fields_ = [
+        ("state_lo", ctypes.POINTER(ctypes.c_float)),
+        ("state_hi", ctypes.POINTER(ctypes.c_float)),
+        ("state_input", ctypes.POINTER(ctypes.c_uint32)),
+        ("num_transitions", ctypes.c_uint32),
+        ("batch", ctypes.c_uint32),
+        ("steps", ctypes.c_uint32),
+        ("learning_rate", ctypes.c_float),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+SURROGATE_INPUT_BOX, SURROGATE_INPUT_PERIODIC, SURROGATE_INPUT_IGNORED = 0, 1, 2
+SURROGATE_NORM_FLOATS = 80
+
 _F = ctypes.POINTER(ctypes.c_float)
 _U32P = ctypes.POINTER(ctypes.c_uint32)
 _VP = ctypes.c_void_p
@@ -90,7 +108,18 @@ SIGNATURES = [
     ("mppi_hip_batch_reset", ctypes.c_int, [_VP, ctypes.c_uint64, _F, _F]),
     ("mppi_hip_batch_run", ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.c_int, _F, _F, _F]),
     ("mppi_hip_batch_read", ctypes.c_int, [_VP, _F, _F, _F]),
+    ("mppi_hip_surrogate_train", ctypes.c_int, [_CTX, ctypes.POINTER(MppiHipSurrogateConfig), _F]),
+    ("mppi_hip_surrogate_evaluate", ctypes.c_int, [_CTX, ctypes.c_uint32, ctypes.c_uint64, _F, _F]),
+    ("mppi_hip_surrogate_predict", ctypes.c_int, [_CTX, ctypes.c_uint32, _F, _F, _F]),
+    ("mppi_hip_surrogate_use", ctypes.c_int, [_CTX, ctypes.c_int]),
+    ("mppi_hip_surrogate_export", ctypes.c_int,
+     [_CTX, _F, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _F]),
+    ("mppi_hip_surrogate_import", ctypes.c_int, [_CTX, _F, ctypes.c_size_t, _F]),
+    ("mppi_hip_surrogate_gradients", ctypes.c_int, [_CTX, ctypes.c_uint32, _F, _F, _F, _F]),
     ("mppi_hip_version", ctypes.c_char_p, []),
+    ("mppi_hip_compile_check", ctypes.c_int,
+     [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_size_t]),
 ]
 
 _lib = None

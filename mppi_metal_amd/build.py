@@ -42,6 +42,7 @@ EMBEDDED = [
     ("kCoreHeader", INC / "mppi_hip" / "mppi_core.h"),
     ("kMathHeader", INC / "mppi_hip" / "mppi_math.h"),
     ("kPhiloxHeader", INC / "mppi_hip" / "mppi_philox.h"),
+    ("kMlpHeader", CSRC / "mppi_mlp.h"),
 ]
 
 
@@ -67,27 +68,33 @@ def embed_sources() -> Path:
     return out
 
 
-def check_kernels():
-    """Ahead-of-time device compile of the kernels with each bundled plugin."""
-    tmp = ROOT / "_build" / "kernel_check"
-    tmp.mkdir(parents=True, exist_ok=True)
+def check_kernels(lib_path: Path):
+    """Compile the library kernels with each bundled plugin through hipRTC,
+    exactly as linking does at run time (mppi_hip_compile_check: no device
+    needed), in every kernel set: plain, full covariance with the batch
+    kernels, and the surrogate kernels where state + control <= 16."""
+    import ctypes
+    lib = ctypes.CDLL(str(lib_path))
+    fn = lib.mppi_hip_compile_check
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_char_p] * 5 + [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32,
+                                           ctypes.c_char_p, ctypes.c_size_t]
+    log = ctypes.create_string_buffer(1 << 16)
     for plugin, dyn, stage, term, sdim, cdim in BUNDLED_PLUGINS:
-        pdir = tmp / plugin.replace(".hip", "")
-        pdir.mkdir(exist_ok=True)
-        (pdir / "mppi_user_plugin.hip").write_text((PKG / "plugins" / plugin).read_text())
-        for cov in (0, 1):
-            _run([HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-ffp-contract=off",
-                  *NO_PACKED_FP32, "--cuda-device-only", "-c",
-                  f"-I{pdir}", f"-I{INC / 'mppi_hip'}", f"-I{CSRC}",
-                  f"-DMPPI_SDIM={sdim}", f"-DMPPI_CDIM={cdim}", f"-DMPPI_DYNAMICS={dyn}",
-                  f"-DMPPI_STAGE_COST={stage}", f"-DMPPI_TERMINAL_COST={term}",
-                  f"-DMPPI_COV_FULL={cov}",
-                  CSRC / "mppi_kernels.hip", "-o", pdir / f"kernels_cov{cov}.o"])
+        src = (PKG / "plugins" / plugin).read_bytes()
+        variants = [(0, 0), (1, 1)] + ([(0, 2)] if sdim + cdim <= 16 else [])
+        for cov, sets in variants:
+            rc = fn(src, str(PKG / "plugins").encode(), dyn.encode(), stage.encode(), term.encode(),
+                    sdim, cdim, cov, sets, log, len(log))
+            if rc != 0:
+                raise RuntimeError(f"{plugin} (covariance_full={cov}, kernel_sets={sets}) failed to "
+                                   f"compile:\n{log.value.decode(errors='replace')}")
 
 
 def build_library() -> Path:
     out = PKG / "libmppi_hip.so"
-    srcs = [CSRC / "hip_backend.cpp", CSRC / "batch_backend.cpp", CSRC / "mppi_driver.cpp"]
+    srcs = [CSRC / "hip_backend.cpp", CSRC / "batch_backend.cpp", CSRC / "surrogate.hip",
+            CSRC / "mppi_driver.cpp"]
     _run([HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-fPIC", "-shared",
           *NO_PACKED_FP32, "-Wall", f"-I{INC}", f"-I{INC / 'mppi_hip'}", f"-I{CSRC}",
           *srcs, "-lhiprtc", "-o", out])
@@ -109,8 +116,8 @@ def build_oracle() -> Path:
 
 def build_all():
     embed_sources()
-    check_kernels()
     lib = build_library()
+    check_kernels(lib)
     build_cpp_tests()
     build_oracle()
     return lib

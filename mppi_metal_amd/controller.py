@@ -175,6 +175,65 @@ class HipMPPI:
     def set_reduce_samples_per_thread(self, spt):
         self._check(self.lib.mppi_hip_set_reduce_samples_per_thread(self.ctx, int(spt)))
 
+    # -- learned-dynamics surrogate (mppi_hip_surrogate_* in include/mppi_hip.h) --
+    def surrogate_train(self, state_lo, state_hi, state_input=None, num_transitions=1 << 20,
+                        batch=4096, steps=2000, learning_rate=2e-3, seed=1):
+        """Generate transitions with the plugin dynamics and train the MLP on
+        the GPU; returns the per-step batch MSE (normalised units)."""
+        lo, hi = _f32(state_lo, (self.D,)), _f32(state_hi, (self.D,))
+        mode = None if state_input is None else np.ascontiguousarray(state_input, np.uint32)
+        cfg = _native.MppiHipSurrogateConfig(
+            _fptr(lo), _fptr(hi),
+            None if mode is None else mode.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+            int(num_transitions), int(batch), int(steps), float(learning_rate), int(seed))
+        hist = np.empty(int(steps), np.float32)
+        self._check(self.lib.mppi_hip_surrogate_train(self.ctx, ctypes.byref(cfg), _fptr(hist)))
+        return hist
+
+    def surrogate_evaluate(self, num_samples=65536, seed=7):
+        """(rmse, relative rmse) per state component of one predicted step."""
+        rmse, rel = np.empty(self.D, np.float32), np.empty(self.D, np.float32)
+        self._check(self.lib.mppi_hip_surrogate_evaluate(self.ctx, int(num_samples), int(seed),
+                                                         _fptr(rmse), _fptr(rel)))
+        return rmse, rel
+
+    def surrogate_predict(self, states, controls):
+        s = _f32(states).reshape(-1, self.D)
+        u = _f32(controls, (s.shape[0] * self.C,))
+        out = np.empty_like(s)
+        self._check(self.lib.mppi_hip_surrogate_predict(self.ctx, s.shape[0], _fptr(s), _fptr(u), _fptr(out)))
+        return out
+
+    def surrogate_use(self, enable=True):
+        self._check(self.lib.mppi_hip_surrogate_use(self.ctx, 1 if enable else 0))
+
+    def surrogate_export(self):
+        n = ctypes.c_size_t()
+        self._check(self.lib.mppi_hip_surrogate_export(self.ctx, None, 0, ctypes.byref(n), None))
+        params = np.empty(n.value, np.float32)
+        norm = np.empty(_native.SURROGATE_NORM_FLOATS, np.float32)
+        self._check(self.lib.mppi_hip_surrogate_export(self.ctx, _fptr(params), params.size,
+                                                       ctypes.byref(n), _fptr(norm)))
+        return params, norm
+
+    def surrogate_import(self, params, norm):
+        p = _f32(params)
+        nb = _f32(norm, (_native.SURROGATE_NORM_FLOATS,))
+        self._check(self.lib.mppi_hip_surrogate_import(self.ctx, _fptr(p), p.size, _fptr(nb)))
+
+    def surrogate_gradients(self, x, targets):
+        """Summed gradients of the batch MSE (no update) and the loss."""
+        xs = _f32(x)
+        n = xs.shape[0]
+        ts = _f32(targets, (n * self.D,))
+        n_params = ctypes.c_size_t()
+        self._check(self.lib.mppi_hip_surrogate_export(self.ctx, None, 0, ctypes.byref(n_params), None))
+        g = np.empty(n_params.value, np.float32)
+        loss = np.empty(1, np.float32)
+        self._check(self.lib.mppi_hip_surrogate_gradients(self.ctx, n, _fptr(xs), _fptr(ts), _fptr(g),
+                                                          _fptr(loss)))
+        return g, float(loss[0])
+
     def profile(self, enable=True):
         self._check(self.lib.mppi_hip_profile_enable(self.ctx, 1 if enable else 0))
 

@@ -27,7 +27,12 @@ struct LoadedModule {
     // batch kernels, indexed [block 64/128/256 -> 0/1/2][table 0 / persistent 1]
     hipFunction_t rollout_b[3][2] = {};
     hipFunction_t reduce_b[3][2] = {};
+    // surrogate kernels (MPPI_SURROGATE_KERNELS)
+    hipFunction_t surrogate_data = nullptr;
+    hipFunction_t rollout_surrogate = nullptr;
 };
+
+struct SurrogateState;  // surrogate.hip
 
 struct DeviceBuffer {
     void* ptr = nullptr;
@@ -86,6 +91,9 @@ struct mppi_hip_ctx {
     const float* last_weights = nullptr;
     std::vector<mppi_agent_desc> last_desc;
 
+    // learned-dynamics surrogate (surrogate.hip)
+    std::shared_ptr<mppi_hip_impl::SurrogateState> sur;
+
     // profiling
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
@@ -100,9 +108,14 @@ int fail(mppi_hip_ctx* ctx, int code, const std::string& msg);
 int ensure_device_buffer(mppi_hip_ctx* ctx, DeviceBuffer* b, size_t bytes);
 int ensure_pinned_buffer(mppi_hip_ctx* ctx, PinnedBuffer* b, size_t bytes);
 int check_ctx(mppi_hip_ctx* ctx);
-// Compile (or fetch from the caches) and load the plugin module; batch = 1
-// adds the tile-table and persistent kernels.
-int load_module(mppi_hip_ctx* ctx, bool batch, std::shared_ptr<LoadedModule>* out);
+// Compile (or fetch from the caches) and load the plugin module; `batch`
+// adds the tile-table and persistent kernels, `surrogate` the learned-dynamics
+// data and rollout kernels.
+int load_module(mppi_hip_ctx* ctx, bool batch, bool surrogate, std::shared_ptr<LoadedModule>* out);
+// Rollout kernel and argument block of the surrogate when it is enabled on
+// ctx (else returns false).
+bool surrogate_rollout(mppi_hip_ctx* ctx, const mppi_step_args& a, hipFunction_t* fn,
+                       void* args_storage, size_t storage_bytes);
 hipEvent_t take_event(mppi_hip_ctx* ctx);
 void collect_profile(mppi_hip_ctx* ctx);
 // Common kernel arguments (constants, bound blobs, work buffers) of a step.

@@ -183,28 +183,74 @@ bool cholesky(const std::vector<float>& a, uint32_t n, std::vector<float>* l) {
     return true;
 }
 
-}  // namespace
-
-int load_module(mppi_hip_ctx* ctx, bool batch, std::shared_ptr<LoadedModule>* out) {
+// hipRTC options of the library kernels for one plugin configuration.
+std::vector<std::string> jit_options(const std::string& arch, uint32_t D, uint32_t C, bool cov_full,
+                                     bool batch, bool surrogate, const std::string& dyn,
+                                     const std::string& stage, const std::string& term,
+                                     const std::string& include_dir) {
     std::vector<std::string> opts = {
-        "--offload-arch=" + ctx->arch,
+        "--offload-arch=" + arch,
         "-O3",
         "-std=c++17",
         "-ffp-contract=off",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        "-DMPPI_SDIM=" + std::to_string(ctx->D),
-        "-DMPPI_CDIM=" + std::to_string(ctx->C),
-        "-DMPPI_DYNAMICS=" + ctx->dyn_name,
-        "-DMPPI_COV_FULL=" + std::to_string(ctx->cov_form == MPPI_HIP_COV_FULL ? 1 : 0),
+        "-DMPPI_SDIM=" + std::to_string(D),
+        "-DMPPI_CDIM=" + std::to_string(C),
+        "-DMPPI_DYNAMICS=" + dyn,
+        "-DMPPI_COV_FULL=" + std::to_string(cov_full ? 1 : 0),
         "-DMPPI_BATCH_KERNELS=" + std::to_string(batch ? 1 : 0),
+        "-DMPPI_SURROGATE_KERNELS=" + std::to_string(surrogate ? 1 : 0),
     };
-    if (!ctx->stage_name.empty()) opts.push_back("-DMPPI_STAGE_COST=" + ctx->stage_name);
-    if (!ctx->term_name.empty()) opts.push_back("-DMPPI_TERMINAL_COST=" + ctx->term_name);
-    if (!ctx->plugin_dir.empty()) opts.push_back("-I" + ctx->plugin_dir);
+    if (!stage.empty()) opts.push_back("-DMPPI_STAGE_COST=" + stage);
+    if (!term.empty()) opts.push_back("-DMPPI_TERMINAL_COST=" + term);
+    if (!include_dir.empty()) opts.push_back("-I" + include_dir);
+    return opts;
+}
+
+// Compile the library kernels with the plugin source (no device needed).
+// Returns false with the compiler log on failure.
+bool jit_compile(const std::string& plugin_src, const std::vector<std::string>& opts,
+                 std::vector<char>* code, std::string* log) {
+    const char* hdr_src[] = {kCoreHeader, kMathHeader, kPhiloxHeader, kLaunchHeader,
+                             plugin_src.c_str(), kCoreHeader, kMlpHeader};
+    const char* hdr_names[] = {"mppi_core.h", "mppi_math.h", "mppi_philox.h",
+                               "mppi_launch.h", "mppi_user_plugin.hip", "mppi_hip/mppi_core.h",
+                               "mppi_mlp.h"};
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, kKernelSource, "mppi_kernels.hip", 7, hdr_src, hdr_names) !=
+        HIPRTC_SUCCESS) {
+        *log = "hiprtcCreateProgram failed.";
+        return false;
+    }
+    std::vector<const char*> copts;
+    for (auto& o : opts) copts.push_back(o.c_str());
+    hiprtcResult r = hiprtcCompileProgram(prog, int(copts.size()), copts.data());
+    size_t log_size = 0;
+    hiprtcGetProgramLogSize(prog, &log_size);
+    log->assign(log_size, '\0');
+    if (log_size) hiprtcGetProgramLog(prog, &(*log)[0]);
+    if (r != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return false;
+    }
+    size_t code_size = 0;
+    hiprtcGetCodeSize(prog, &code_size);
+    code->resize(code_size);
+    hiprtcGetCode(prog, code->data());
+    hiprtcDestroyProgram(&prog);
+    return true;
+}
+
+}  // namespace
+
+int load_module(mppi_hip_ctx* ctx, bool batch, bool surrogate, std::shared_ptr<LoadedModule>* out) {
+    const std::vector<std::string> opts =
+        jit_options(ctx->arch, ctx->D, ctx->C, ctx->cov_form == MPPI_HIP_COV_FULL, batch, surrogate,
+                    ctx->dyn_name, ctx->stage_name, ctx->term_name, ctx->plugin_dir);
 
     std::string keysrc = std::string(kKernelSource) + "\x01" + kLaunchHeader + "\x01" +
                          kCoreHeader + "\x01" + kMathHeader + "\x01" + kPhiloxHeader + "\x01" +
-                         ctx->plugin_src + "\x01" + mppi_hip_version();
+                         kMlpHeader + "\x01" + ctx->plugin_src + "\x01" + mppi_hip_version();
     for (auto& o : opts) keysrc += "\x02" + o;
     char keybuf[40];
     std::snprintf(keybuf, sizeof(keybuf), "%016llx%016llx",
@@ -233,31 +279,9 @@ int load_module(mppi_hip_ctx* ctx, bool batch, std::shared_ptr<LoadedModule>* ou
         if (read_file(cpath, &bytes)) code.assign(bytes.begin(), bytes.end());
     }
     if (code.empty()) {
-        const char* hdr_src[] = {kCoreHeader, kMathHeader, kPhiloxHeader, kLaunchHeader,
-                                 ctx->plugin_src.c_str(), kCoreHeader};
-        const char* hdr_names[] = {"mppi_core.h", "mppi_math.h", "mppi_philox.h",
-                                   "mppi_launch.h", "mppi_user_plugin.hip", "mppi_hip/mppi_core.h"};
-        hiprtcProgram prog;
-        if (hiprtcCreateProgram(&prog, kKernelSource, "mppi_kernels.hip", 6, hdr_src, hdr_names) !=
-            HIPRTC_SUCCESS)
-            return fail(ctx, MPPI_HIP_E_COMPILE, "hiprtcCreateProgram failed.");
-        std::vector<const char*> copts;
-        for (auto& o : opts) copts.push_back(o.c_str());
-        hiprtcResult r = hiprtcCompileProgram(prog, int(copts.size()), copts.data());
-        size_t log_size = 0;
-        hiprtcGetProgramLogSize(prog, &log_size);
-        std::string log(log_size, '\0');
-        if (log_size) hiprtcGetProgramLog(prog, &log[0]);
-        if (r != HIPRTC_SUCCESS) {
-            hiprtcDestroyProgram(&prog);
-            return fail(ctx, MPPI_HIP_E_COMPILE,
-                        "Plugin '" + ctx->plugin_name + "' failed to compile: " + log);
-        }
-        size_t code_size = 0;
-        hiprtcGetCodeSize(prog, &code_size);
-        code.resize(code_size);
-        hiprtcGetCode(prog, code.data());
-        hiprtcDestroyProgram(&prog);
+        std::string log;
+        if (!jit_compile(ctx->plugin_src, opts, &code, &log))
+            return fail(ctx, MPPI_HIP_E_COMPILE, "Plugin '" + ctx->plugin_name + "' failed to compile: " + log);
         if (!cpath.empty()) {
             mkdirs(dir);
             std::string tmp = cpath + ".tmp" + std::to_string(uint64_t(std::rand()));
@@ -290,6 +314,9 @@ int load_module(mppi_hip_ctx* ctx, bool batch, std::shared_ptr<LoadedModule>* ou
                      hipModuleGetFunction(&m->reduce_b[b][md], m->module, d.c_str()) == hipSuccess;
             }
     }
+    if (ok && surrogate)
+        ok = hipModuleGetFunction(&m->surrogate_data, m->module, "mppi_surrogate_data") == hipSuccess &&
+             hipModuleGetFunction(&m->rollout_surrogate, m->module, "mppi_rollout_surrogate") == hipSuccess;
     if (!ok) {
         (void)hipModuleUnload(m->module);
         return fail(ctx, MPPI_HIP_E_COMPILE, "Required library kernels missing from code object.");
@@ -472,8 +499,16 @@ int launch_iteration(mppi_hip_ctx* ctx, const std::vector<mppi_agent_desc>& desc
         e0 = take_event(ctx); e1 = take_event(ctx); e2 = take_event(ctx); e3 = take_event(ctx);
         if (e0) (void)hipEventRecord(e0, ctx->stream);
     }
-    MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(ctx->mod->rollout, d.n_blk, N, 1, kBlock, 1, 1, 0,
-                                            ctx->stream, params, nullptr));
+    alignas(16) unsigned char sur_args[1024];
+    hipFunction_t sur_fn = nullptr;
+    if (surrogate_rollout(ctx, args, &sur_fn, sur_args, sizeof(sur_args))) {
+        void* sparams[] = {sur_args};
+        MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(sur_fn, d.n_blk, N, 1, kBlock, 1, 1, 0, ctx->stream,
+                                                sparams, nullptr));
+    } else {
+        MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(ctx->mod->rollout, d.n_blk, N, 1, kBlock, 1, 1, 0,
+                                                ctx->stream, params, nullptr));
+    }
     if (ctx->profiling) {
         if (e1) (void)hipEventRecord(e1, ctx->stream);
         if (e2) (void)hipEventRecord(e2, ctx->stream);
@@ -569,6 +604,35 @@ using namespace mppi_hip_impl;
 extern "C" {
 
 const char* mppi_hip_version(void) { return "mppi_hip 0.2.0"; }
+
+int mppi_hip_compile_check(const char* plugin_source, const char* include_dir, const char* dynamics_fn,
+                           const char* stage_cost_fn, const char* terminal_cost_fn, uint32_t state_dim,
+                           uint32_t control_dim, int covariance_full, uint32_t kernel_sets, char* log,
+                           size_t log_capacity) {
+    if (log && log_capacity) log[0] = '\0';
+    auto put_log = [&](const std::string& s) {
+        if (log && log_capacity) {
+            const size_t n = std::min(s.size(), log_capacity - 1);
+            std::memcpy(log, s.data(), n);
+            log[n] = '\0';
+        }
+    };
+    if (!plugin_source || !dynamics_fn || !*dynamics_fn || state_dim == 0 || control_dim == 0) {
+        put_log("plugin_source and dynamics_fn are required; dimensions must be > 0.");
+        return MPPI_HIP_E_INVALID;
+    }
+    const std::vector<std::string> opts =
+        jit_options("gfx950", state_dim, control_dim, covariance_full != 0, (kernel_sets & 1u) != 0,
+                    (kernel_sets & 2u) != 0, dynamics_fn, stage_cost_fn ? stage_cost_fn : "",
+                    terminal_cost_fn ? terminal_cost_fn : "", include_dir ? include_dir : "");
+    std::vector<char> code;
+    std::string clog;
+    if (!jit_compile(plugin_source, opts, &code, &clog)) {
+        put_log(clog);
+        return MPPI_HIP_E_COMPILE;
+    }
+    return code.empty() ? MPPI_HIP_E_COMPILE : MPPI_HIP_OK;
+}
 
 const char* mppi_hip_last_error(const mppi_hip_ctx* ctx) {
     return ctx ? ctx->error.c_str() : g_thread_error.c_str();
@@ -703,7 +767,8 @@ int link_common(mppi_hip_ctx* ctx, const std::string& src, const std::string& sr
     ctx->linked = false;
     ctx->batch_mod.reset();
     std::shared_ptr<LoadedModule> mod;
-    int rc = load_module(ctx, false, &mod);
+    ctx->sur.reset();
+    int rc = load_module(ctx, false, false, &mod);
     if (rc) return rc;
     ctx->mod = mod;
     ctx->model_bytes = model_bytes;

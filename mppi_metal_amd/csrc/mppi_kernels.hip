@@ -9,6 +9,9 @@
 //   MPPI_COV_FULL                   0: diagonal sigma, 1: lower Cholesky factor
 //   MPPI_BATCH_KERNELS              1: also emit the tile-table and persistent
 //                                   kernels of the uneven multi-agent step
+//   MPPI_SURROGATE_KERNELS          1: also emit the learned-surrogate kernels
+//                                   (training data from the plugin dynamics,
+//                                   rollout with the MLP as dynamics)
 // so the plugin's callables are direct, inlined calls (the reference reaches
 // them through a visible function table at every timestep,
 // shaders/mppi_rollout.metal:90-91).
@@ -54,6 +57,9 @@
 #endif
 #ifndef MPPI_BATCH_KERNELS
 #define MPPI_BATCH_KERNELS 0
+#endif
+#ifndef MPPI_SURROGATE_KERNELS
+#define MPPI_SURROGATE_KERNELS 0
 #endif
 
 #define SDIM MPPI_SDIM
@@ -107,10 +113,8 @@ __device__ inline bool mppi_isfinite(float x) {
 }
 
 // Control for one timestep from the normals z[0..CDIM): u = clamp(u_nom + eps).
-__device__ inline void mppi_make_control(const float* __restrict__ un, const float* z,
-                                         const float* __restrict__ scale,
-                                         const float* __restrict__ lo,
-                                         const float* __restrict__ hi, float* u) {
+__device__ inline void mppi_make_control(const float* un, const float* z, const float* scale,
+                                         const float* lo, const float* hi, float* u) {
 #if MPPI_COV_FULL
 #pragma unroll
     for (int i = 0; i < CDIM; ++i) {
@@ -185,58 +189,12 @@ __device__ void mppi_combine_stats(const mppi_step_args& a, uint32_t agent,
     }
 }
 
-// ---------------------------------------------------------------------------
-// Rollout tile: samples [blk*BLOCK, blk*BLOCK + BLOCK) of one agent.
-// ---------------------------------------------------------------------------
+// Workgroup softmax partials over one rollout tile's costs (min, sum e,
+// sum c, sum e^2); the last tile of an agent combines all partials.
 template <int BLOCK>
-__device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint32_t blk) {
+__device__ void mppi_block_stats(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
+                                 uint32_t blk, bool active, float total) {
     constexpr int WAVES = BLOCK / 64;
-    const mppi_agent_desc d = a.desc[agent];
-    const uint32_t k = blk * BLOCK + threadIdx.x;
-    const uint32_t H = d.H;
-    const bool active = k < d.K;
-    const uint32_t step = a.agents[agent].step;
-    const uint32_t tag_agent = (MPPI_TAG_CONTROL_NOISE << 28) | (agent & MPPI_AGENT_MASK);
-    const uint32_t dyn_tag = (MPPI_TAG_DYNAMICS << 28) | (agent & MPPI_AGENT_MASK);
-    const float* __restrict__ un = a.u_nom + d.seq_off;
-    const uint8_t* model = a.model + agent * a.model_stride;
-    const uint8_t* cost = a.cost + agent * a.cost_stride;
-
-    float total = 0.0f;
-    if (active) {
-        float x[SDIM];
-#pragma unroll
-        for (int i = 0; i < SDIM; ++i) x[i] = a.x[size_t(agent) * SDIM + i];
-        for (uint32_t t0 = 0; t0 < H; t0 += MPPI_G) {
-            float z[MPPI_NG];
-            const uint32_t b0 = (t0 * CDIM) >> 2;
-#pragma unroll
-            for (int q = 0; q < MPPI_BG; ++q) {
-                mppi_f32x4 v = mppi_normal4(mppi_u32x4{b0 + q, k, step, tag_agent}, a.key0, a.key1);
-                z[4 * q + 0] = v.x; z[4 * q + 1] = v.y;
-                z[4 * q + 2] = v.z; z[4 * q + 3] = v.w;
-            }
-#pragma unroll
-            for (int g = 0; g < MPPI_G; ++g) {
-                const uint32_t t = t0 + g;
-                if (t < H) {
-                    float u[CDIM];
-                    mppi_make_control(un + t * CDIM, z + g * CDIM, a.noise_scale, a.u_min,
-                                      a.u_max, u);
-                    mppi_rng rng{t << 8, k, step, dyn_tag, a.key0, a.key1};
-                    MPPI_DYNAMICS(mppi_state_ref{x}, u, mppi_blob_ref{model}, rng);
-                    total += mppi_stage(mppi_form_timed{}, mppi_cstate_ref{x}, (const float*)u,
-                                        mppi_blob_ref{model}, mppi_blob_ref{cost},
-                                        mppi_time{t, step});
-                }
-            }
-        }
-        total += mppi_terminal(mppi_form_timed{}, mppi_cstate_ref{x}, mppi_blob_ref{model},
-                               mppi_blob_ref{cost}, mppi_time{H, step});
-        a.costs[d.cost_off + k] = total;
-    }
-
-    // ---- workgroup softmax partials over this tile's costs ----
     __shared__ float s_red[4][WAVES];
     __shared__ int s_last;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -271,6 +229,72 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
 }
 
 // ---------------------------------------------------------------------------
+// Rollout tile: samples [blk*BLOCK, blk*BLOCK + BLOCK) of one agent.
+// ---------------------------------------------------------------------------
+template <int BLOCK>
+__device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint32_t blk) {
+    constexpr int WAVES = BLOCK / 64;
+    const mppi_agent_desc d = a.desc[agent];
+    const uint32_t k = blk * BLOCK + threadIdx.x;
+    const uint32_t H = d.H;
+    const bool active = k < d.K;
+    const uint32_t step = a.agents[agent].step;
+    const uint32_t tag_agent = (MPPI_TAG_CONTROL_NOISE << 28) | (agent & MPPI_AGENT_MASK);
+    const uint32_t dyn_tag = (MPPI_TAG_DYNAMICS << 28) | (agent & MPPI_AGENT_MASK);
+    const uint8_t* model = a.model + agent * a.model_stride;
+    const uint8_t* cost = a.cost + agent * a.cost_stride;
+
+    // The nominal sequence goes to LDS and the noise scale and bounds to
+    // registers, so the time loop issues no memory waits of its own.
+    __shared__ float s_un[MPPI_MAX_HC];
+    for (uint32_t i = threadIdx.x; i < H * CDIM; i += BLOCK) s_un[i] = a.u_nom[d.seq_off + i];
+    float scale[MPPI_COV_FULL ? CDIM * CDIM : CDIM], lo[CDIM], hi[CDIM];
+#pragma unroll
+    for (int i = 0; i < (MPPI_COV_FULL ? CDIM * CDIM : CDIM); ++i) scale[i] = a.noise_scale[i];
+#pragma unroll
+    for (int i = 0; i < CDIM; ++i) {
+        lo[i] = a.u_min[i];
+        hi[i] = a.u_max[i];
+    }
+    __syncthreads();
+
+    float total = 0.0f;
+    if (active) {
+        float x[SDIM];
+#pragma unroll
+        for (int i = 0; i < SDIM; ++i) x[i] = a.x[size_t(agent) * SDIM + i];
+        for (uint32_t t0 = 0; t0 < H; t0 += MPPI_G) {
+            float z[MPPI_NG];
+            const uint32_t b0 = (t0 * CDIM) >> 2;
+#pragma unroll
+            for (int q = 0; q < MPPI_BG; ++q) {
+                mppi_f32x4 v = mppi_normal4(mppi_u32x4{b0 + q, k, step, tag_agent}, a.key0, a.key1);
+                z[4 * q + 0] = v.x; z[4 * q + 1] = v.y;
+                z[4 * q + 2] = v.z; z[4 * q + 3] = v.w;
+            }
+#pragma unroll
+            for (int g = 0; g < MPPI_G; ++g) {
+                const uint32_t t = t0 + g;
+                if (t < H) {
+                    float u[CDIM];
+                    mppi_make_control(s_un + t * CDIM, z + g * CDIM, scale, lo, hi, u);
+                    mppi_rng rng{t << 8, k, step, dyn_tag, a.key0, a.key1};
+                    MPPI_DYNAMICS(mppi_state_ref{x}, u, mppi_blob_ref{model}, rng);
+                    total += mppi_stage(mppi_form_timed{}, mppi_cstate_ref{x}, (const float*)u,
+                                        mppi_blob_ref{model}, mppi_blob_ref{cost},
+                                        mppi_time{t, step});
+                }
+            }
+        }
+        total += mppi_terminal(mppi_form_timed{}, mppi_cstate_ref{x}, mppi_blob_ref{model},
+                               mppi_blob_ref{cost}, mppi_time{H, step});
+        a.costs[d.cost_off + k] = total;
+    }
+
+    mppi_block_stats<BLOCK>(a, agent, d, blk, active, total);
+}
+
+// ---------------------------------------------------------------------------
 // Finalize (last reduce workgroup of an agent): sum the weighted-noise
 // partials over sample chunks, divide by the normaliser, write u_next and the
 // diagnostics; in closed loop also log history, step the plant and shift.
@@ -282,26 +306,33 @@ __device__ void mppi_sum_partials(const mppi_step_args& a, const mppi_agent_desc
     const uint32_t W = d.NB * 4;
     const T S = T(weight_sum);
     const float* pb = a.partials + d.part_off;
-    if (W <= BLOCK) {
-        const uint32_t R = BLOCK / W;
-        const uint32_t c = threadIdx.x % W, r = threadIdx.x / W;
-        if (r < R) {
-            T s = T(0);
-            for (uint32_t ch = r; ch < d.n_chunks; ch += R) s += T(pb[size_t(ch) * W + c]);
-            s_col[r * W + c] = s;
+    // P columns per pass, R threads per column; each thread keeps eight
+    // independent accumulators (combined in a fixed order) so that eight
+    // loads are in flight at a time.
+    const uint32_t P = W < BLOCK ? W : BLOCK;
+    const uint32_t R = BLOCK / P;
+    const uint32_t c = threadIdx.x % P, r = threadIdx.x / P;
+    for (uint32_t base = 0; base < W; base += P) {
+        const uint32_t col = base + c;
+        if (r < R && col < W) {
+            T acc[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc[u] = T(0);
+            uint32_t ch = r;
+            for (; ch + 7 * R < d.n_chunks; ch += 8 * R) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) acc[u] += T(pb[size_t(ch + u * R) * W + col]);
+            }
+            for (; ch < d.n_chunks; ch += R) acc[0] += T(pb[size_t(ch) * W + col]);
+            s_col[r * P + c] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
         }
         __syncthreads();
-        for (uint32_t n = threadIdx.x; n < W; n += BLOCK) {
-            T s = T(0);
-            for (uint32_t q = 0; q < R; ++q) s += s_col[q * W + n];
-            zs[n] = float(s / S);
+        if (threadIdx.x < P && base + threadIdx.x < W) {
+            T sum = T(0);
+            for (uint32_t q = 0; q < R; ++q) sum += s_col[q * P + threadIdx.x];
+            zs[base + threadIdx.x] = float(sum / S);
         }
-    } else {
-        for (uint32_t n = threadIdx.x; n < W; n += BLOCK) {
-            T s = T(0);
-            for (uint32_t ch = 0; ch < d.n_chunks; ++ch) s += T(pb[size_t(ch) * W + n]);
-            zs[n] = float(s / S);
-        }
+        __syncthreads();
     }
 }
 
@@ -520,4 +551,135 @@ __device__ inline void mppi_persistent(const mppi_step_args& a) {
 MPPI_BATCH_ENTRIES(64)
 MPPI_BATCH_ENTRIES(128)
 MPPI_BATCH_ENTRIES(256)
+#endif
+
+#if MPPI_SURROGATE_KERNELS
+#include "mppi_mlp.h"
+
+// Training transitions: (state, control) drawn uniformly from the sampling
+// box with Philox (tag SURROGATE, counter (block, i, stream, tag)), labelled
+// with one step of the plugin's dynamics: t = f(x, u) - x.
+extern "C" __global__ void __launch_bounds__(256) mppi_surrogate_data(mppi_surrogate_data_args a) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.M) return;
+    constexpr int IN = SDIM + CDIM;
+    float v[IN];
+#pragma unroll
+    for (int q = 0; q < (IN + 3) / 4; ++q) {
+        mppi_u32x4 w = mppi_philox4x32_10(
+            mppi_u32x4{uint32_t(q), i, a.stream, MPPI_TAG_SURROGATE << 28}, a.key0, a.key1);
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = 4 * q + j;
+            if (k < IN) v[k] = a.lo[k] + (a.hi[k] - a.lo[k]) * mppi_u01(ws[j]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < IN; ++k) a.x[size_t(i) * IN + k] = mppi_mlp_input(v[k], a.center[k], a.inv_half[k], a.period[k]);
+    float xs[SDIM];
+#pragma unroll
+    for (int k = 0; k < SDIM; ++k) xs[k] = v[k];
+    mppi_rng rng{0u, i, a.stream, (MPPI_TAG_SURROGATE << 28) | 1u, a.key0, a.key1};
+    MPPI_DYNAMICS(mppi_state_ref{xs}, v + SDIM, mppi_blob_ref{a.model}, rng);
+#pragma unroll
+    for (int k = 0; k < SDIM; ++k) a.t[size_t(i) * SDIM + k] = xs[k] - v[k];
+}
+
+// Rollout with the MLP as dynamics (grid-mapped like mppi_rollout; the
+// reduce kernel is shared).  Per timestep each wave stages its 64 samples'
+// normalised (state, control) in LDS, evaluates the network on four
+// 16-sample blocks with the matrix cores (weight fragments in LDS), and adds
+// the denormalised delta to each lane's state; costs are the plugin's.
+extern "C" __global__ void __launch_bounds__(256) mppi_rollout_surrogate(mppi_surrogate_rollout_args sa) {
+    constexpr int IN = SDIM + CDIM;
+    constexpr int NFRAG = MPPI_MLP_FRAG_W2T;   // W1, W2, W3 fragments
+    __shared__ short s_frag[NFRAG * MPPI_MLP_FRAG_ELEMS];
+    __shared__ float s_b[2 * MPPI_MLP_HID + 16];
+    __shared__ float s_io[4][64][IN + 1];
+    __shared__ float s_dx[4][64][SDIM + 1];
+    const mppi_step_args& a = sa.s;
+    const uint32_t agent = blockIdx.y;
+    const mppi_agent_desc d = a.desc[agent];
+    if (blockIdx.x >= d.n_blk) return;
+    for (uint32_t e = threadIdx.x; e < NFRAG * MPPI_MLP_FRAG_ELEMS / 4; e += 256)
+        reinterpret_cast<uint2*>(s_frag)[e] = reinterpret_cast<const uint2*>(sa.frags)[e];
+    for (uint32_t e = threadIdx.x; e < MPPI_MLP_HID; e += 256) {
+        s_b[e] = sa.params[sa.b1 + e];
+        s_b[MPPI_MLP_HID + e] = sa.params[sa.b2 + e];
+    }
+    if (threadIdx.x < SDIM) s_b[2 * MPPI_MLP_HID + threadIdx.x] = sa.params[sa.b3 + threadIdx.x];
+    __syncthreads();
+
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
+    const uint32_t blk = blockIdx.x;
+    const uint32_t k = blk * 256 + threadIdx.x;
+    const uint32_t H = d.H;
+    const bool active = k < d.K;
+    const uint32_t step = a.agents[agent].step;
+    const uint32_t tag_agent = (MPPI_TAG_CONTROL_NOISE << 28) | (agent & MPPI_AGENT_MASK);
+    const float* __restrict__ un = a.u_nom + d.seq_off;
+    const uint8_t* model = a.model + agent * a.model_stride;
+    const uint8_t* cost = a.cost + agent * a.cost_stride;
+
+    float x[SDIM];
+#pragma unroll
+    for (int i = 0; i < SDIM; ++i) x[i] = a.x[size_t(agent) * SDIM + i];
+    float total = 0.0f;
+    for (uint32_t t0 = 0; t0 < H; t0 += MPPI_G) {
+        float z[MPPI_NG];
+        const uint32_t b0 = (t0 * CDIM) >> 2;
+#pragma unroll
+        for (int q = 0; q < MPPI_BG; ++q) {
+            mppi_f32x4 v = mppi_normal4(mppi_u32x4{b0 + q, k, step, tag_agent}, a.key0, a.key1);
+            z[4 * q + 0] = v.x; z[4 * q + 1] = v.y;
+            z[4 * q + 2] = v.z; z[4 * q + 3] = v.w;
+        }
+#pragma unroll
+        for (int gi = 0; gi < MPPI_G; ++gi) {
+            const uint32_t t = t0 + gi;
+            if (t < H) {
+                float u[CDIM];
+                mppi_make_control(un + t * CDIM, z + gi * CDIM, a.noise_scale, a.u_min, a.u_max, u);
+#pragma unroll
+                for (int i = 0; i < SDIM; ++i)
+                    s_io[wave][lane][i] = mppi_mlp_input(x[i], sa.in_center[i], sa.in_inv_half[i], sa.in_period[i]);
+#pragma unroll
+                for (int i = 0; i < CDIM; ++i)
+                    s_io[wave][lane][SDIM + i] = mppi_mlp_input(u[i], sa.in_center[SDIM + i], sa.in_inv_half[SDIM + i],
+                                                                sa.in_period[SDIM + i]);
+                __syncthreads();
+                // The four 16-sample blocks of the wave one after the other
+                // (unrolling them would only multiply the live registers).
+#pragma unroll 1
+                for (int nb = 0; nb < 4; ++nb) {
+                    mppi_v4bf xb;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int kk = 4 * g + j;
+                        xb[j] = mppi_f2bf(kk < IN ? s_io[wave][16 * nb + c][kk < IN ? kk : 0] : 0.0f);
+                    }
+                    mppi_v4f y = mppi_mlp_block(s_frag, s_b, s_b + MPPI_MLP_HID, xb, lane, nullptr, nullptr);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int r = 4 * g + j;
+                        if (r < SDIM) s_dx[wave][16 * nb + c][r] = y[j] + s_b[2 * MPPI_MLP_HID + r];
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < SDIM; ++i) x[i] += s_dx[wave][lane][i] * sa.out_sd[i] + sa.out_mu[i];
+                if (active)
+                    total += mppi_stage(mppi_form_timed{}, mppi_cstate_ref{x}, (const float*)u,
+                                        mppi_blob_ref{model}, mppi_blob_ref{cost}, mppi_time{t, step});
+            }
+        }
+    }
+    if (active) {
+        total += mppi_terminal(mppi_form_timed{}, mppi_cstate_ref{x}, mppi_blob_ref{model},
+                               mppi_blob_ref{cost}, mppi_time{H, step});
+        a.costs[d.cost_off + k] = total;
+    }
+    mppi_block_stats<256>(a, agent, d, blk, active, total);
+}
 #endif

@@ -69,4 +69,25 @@ struct mppi_step_args {
     float lambda;
 };
 
+// Learned-dynamics surrogate (mppi_mlp.h): training data and rollout.
+struct mppi_surrogate_data_args {
+    float* x;                    // [M][in]  normalised (state, control) inputs
+    float* t;                    // [M][out] state deltas of the plugin dynamics
+    const uint8_t* model;        // model blob of the plugin dynamics
+    uint32_t M;
+    uint32_t key0, key1;
+    uint32_t stream;             // counter word 2: data stream id
+    float lo[16], hi[16];        // sampling box of (state, control)
+    float center[16], inv_half[16], period[16];   // input normalisation (mppi_mlp.h)
+};
+
+struct mppi_surrogate_rollout_args {
+    mppi_step_args s;
+    const short* frags;          // bf16 A fragments (mppi_mlp.h)
+    const float* params;         // FP32 parameters (biases are read from here)
+    uint32_t b1, b2, b3;         // bias offsets into params
+    uint32_t pad;
+    float in_center[16], in_inv_half[16], in_period[16], out_mu[16], out_sd[16];
+};
+
 #endif  // MPPI_HIP_LAUNCH_H

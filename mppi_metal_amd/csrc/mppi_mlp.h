@@ -1,0 +1,130 @@
+// mppi_mlp.h — the learned-dynamics surrogate's network on the device: an MLP
+// in -> 128 -> 128 -> out with tanh hidden layers (in = state_dim +
+// control_dim <= 16, out = state_dim <= 16), BF16 operands on the matrix
+// cores with FP32 accumulation.
+//
+// Layout: every layer is computed transposed, Z^T = W X^T, so the samples are
+// the N dimension of v_mfma_f32_16x16x16_bf16 and the hidden units are M.  In
+// that instruction lane l holds
+//   A (16x16, M x K):  A[l % 16][4 * (l / 16) + j]          j = 0..3
+//   B (16x16, K x N):  B[4 * (l / 16) + j][l % 16]
+//   D (16x16, M x N):  D[4 * (l / 16) + j][l % 16]
+// so the result tile of hidden block m is, register for register, the B
+// operand of K block m of the next layer: layers chain without moving data
+// between lanes.  Weights are kept as ready-made A fragments (64 lanes x 4
+// bf16 each, "fragment-major"), built from the FP32 master copy by the pack
+// kernel after every optimiser step.
+//
+// Fragment table (indices into the fragment buffer, 256 bf16 per fragment):
+//   W1   : 8 fragments, m-block mb:            A[i][k] = W1[16mb + i][k]       (k < in)
+//   W2   : 64 fragments, mb * 8 + kb:          A[i][k] = W2[16mb + i][16kb + k]
+//   W3   : 8 fragments, k-block kb:            A[i][k] = W3[i][16kb + k]       (i < out)
+//   W2T  : 64 fragments, mb * 8 + kb:          A[i][k] = W2[16kb + k][16mb + i]
+//   W3T  : 8 fragments, m-block mb:            A[i][k] = W3[k][16mb + i]       (k < out)
+#ifndef MPPI_HIP_MLP_H
+#define MPPI_HIP_MLP_H
+
+#if !defined(__HIPCC_RTC__)
+#include <stdint.h>
+#endif
+
+#define MPPI_MLP_HID 128
+#define MPPI_MLP_FRAG_W1 0
+#define MPPI_MLP_FRAG_W2 8
+#define MPPI_MLP_FRAG_W3 72
+#define MPPI_MLP_FRAG_W2T 80
+#define MPPI_MLP_FRAG_W3T 144
+#define MPPI_MLP_NUM_FRAGS 152
+#define MPPI_MLP_FRAG_ELEMS 256
+
+// Offsets of the FP32 parameter vector [W1 | b1 | W2 | b2 | W3 | b3] (row
+// major [out][in]) for in / out dimensions.
+struct mppi_mlp_dims {
+    uint32_t in, out;
+    uint32_t w1, b1, w2, b2, w3, b3, n_params;
+    uint32_t pad;
+};
+
+// Input normalisation x_n = wrap(x - center) * inv_half, where wrap reduces
+// into [-period/2, period/2] when period > 0 (inputs the dynamics is periodic
+// in) and inv_half = 0 drops an input (one the dynamics is invariant to);
+// output (state delta) delta = y * out_sd + out_mu.
+struct mppi_mlp_norm {
+    float in_center[16];
+    float in_inv_half[16];
+    float in_period[16];
+    float out_mu[16];
+    float out_sd[16];
+};
+
+#if defined(__HIPCC_RTC__) || defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC__)
+typedef short mppi_v4bf __attribute__((ext_vector_type(4)));
+typedef float mppi_v4f __attribute__((ext_vector_type(4)));
+
+__device__ inline float mppi_mlp_input(float v, float center, float inv_half, float period) {
+    float d = v - center;
+    if (period > 0.0f) d -= period * __builtin_rintf(d / period);
+    return d * inv_half;
+}
+
+// float -> bfloat16, round to nearest even.
+__device__ inline short mppi_f2bf(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return short(u >> 16);
+}
+__device__ inline float mppi_bf2f(short h) {
+    return __builtin_bit_cast(float, uint32_t((unsigned short)h) << 16);
+}
+
+// tanh(x) = 1 - 2 / (exp(2x) + 1) with the hardware exp2 and reciprocal
+// (about 2 ulp); |x| > 9 saturates.
+__device__ inline float mppi_tanh_fast(float x) {
+    x = x > 9.0f ? 9.0f : (x < -9.0f ? -9.0f : x);
+    const float e = __builtin_amdgcn_exp2f(x * 2.88539008177792681472f);  // 2 / ln 2
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+
+__device__ inline mppi_v4f mppi_mfma(mppi_v4bf a, mppi_v4bf b, mppi_v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ inline mppi_v4bf mppi_frag(const short* frags, uint32_t f, uint32_t lane) {
+    return *reinterpret_cast<const mppi_v4bf*>(frags + size_t(f) * MPPI_MLP_FRAG_ELEMS + lane * 4);
+}
+
+// Forward pass of one 16-sample block.  xb: the B operand of the inputs
+// (rows 4*(lane/16)+j, zero beyond `in`).  Returns the output tile (rows
+// 4*(lane/16)+j < out valid, without the output bias); h1b / h2b receive the
+// bf16 hidden activations when non-null (for training).
+__device__ inline mppi_v4f mppi_mlp_block(const short* frags, const float* b1, const float* b2,
+                                            mppi_v4bf xb, uint32_t lane, mppi_v4bf* h1b_out,
+                                            mppi_v4bf* h2b_out) {
+    const uint32_t g = lane >> 4;
+    mppi_v4bf h1b[8], h2b[8];
+    const mppi_v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int mb = 0; mb < 8; ++mb) {
+        mppi_v4f acc = mppi_mfma(mppi_frag(frags, MPPI_MLP_FRAG_W1 + mb, lane), xb, zero);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) h1b[mb][j] = mppi_f2bf(mppi_tanh_fast(acc[j] + b1[16 * mb + 4 * g + j]));
+    }
+#pragma unroll
+    for (int mb = 0; mb < 8; ++mb) {
+        mppi_v4f acc = zero;
+#pragma unroll
+        for (int kb = 0; kb < 8; ++kb)
+            acc = mppi_mfma(mppi_frag(frags, MPPI_MLP_FRAG_W2 + mb * 8 + kb, lane), h1b[kb], acc);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) h2b[mb][j] = mppi_f2bf(mppi_tanh_fast(acc[j] + b2[16 * mb + 4 * g + j]));
+    }
+    mppi_v4f out = zero;
+#pragma unroll
+    for (int kb = 0; kb < 8; ++kb) out = mppi_mfma(mppi_frag(frags, MPPI_MLP_FRAG_W3 + kb, lane), h2b[kb], out);
+    if (h1b_out) for (int i = 0; i < 8; ++i) h1b_out[i] = h1b[i];
+    if (h2b_out) for (int i = 0; i < 8; ++i) h2b_out[i] = h2b[i];
+    return out;
+}
+#endif
+
+#endif  // MPPI_HIP_MLP_H

@@ -380,6 +380,9 @@ struct oracle_problem {
     const void* cost_params;
 };
 
+// mlp_oracle.cpp
+void oracle_mlp_forward1(const float* P, uint32_t in, uint32_t out, const float* x, float* y);
+
 }  // extern "C"
 
 namespace {
@@ -481,10 +484,37 @@ void sample_normals(const oracle_problem* p, uint32_t k, uint32_t step, uint32_t
     }
 }
 
-// Rollout of samples [k0, k1) (mppi_rollout.metal:36-96).
+// Learned-dynamics surrogate in place of the plugin dynamics (no reference
+// counterpart): parameters and the 80-float normalisation block (input
+// centre, input 1/half-range, input period, output mean, output sd; 16 each).
+struct Surrogate {
+    const float* params;
+    const float* norm;
+};
+
+// x <- x + delta(x, u) as the GPU's surrogate rollout computes it.
+void surrogate_dynamics(const Surrogate& s, uint32_t D, uint32_t C, float* x, const float* u) {
+    const float* center = s.norm;
+    const float* inv_half = s.norm + 16;
+    const float* period = s.norm + 32;
+    const float* mu = s.norm + 48;
+    const float* sd = s.norm + 64;
+    float xin[16], y[16];
+    for (uint32_t k = 0; k < D + C; ++k) {
+        float d = (k < D ? x[k] : u[k - D]) - center[k];
+        if (period[k] > 0.0f) d -= period[k] * std::nearbyint(d / period[k]);
+        xin[k] = d * inv_half[k];
+    }
+    oracle_mlp_forward1(s.params, D + C, D, xin, y);
+    for (uint32_t k = 0; k < D; ++k) x[k] += y[k] * sd[k] + mu[k];
+}
+
+// Rollout of samples [k0, k1) (mppi_rollout.metal:36-96); with `sur` the
+// network replaces the plugin dynamics (FP32 only).
 template <typename T>
 void rollout_range(const oracle_problem* p, const Resolved<T>& r, uint32_t step, uint32_t agent,
-                   const float* x0, const T* u_nom, uint32_t k0, uint32_t k1, T* costs) {
+                   const float* x0, const T* u_nom, uint32_t k0, uint32_t k1, T* costs,
+                   const Surrogate* sur = nullptr) {
     const uint32_t D = p->D, C = p->C, H = p->H;
     std::vector<T> z(size_t(H) * C), x(D), u(C), eps(C);
     for (uint32_t k = k0; k < k1; ++k) {
@@ -495,8 +525,12 @@ void rollout_range(const oracle_problem* p, const Resolved<T>& r, uint32_t step,
             noise_eps<T>(p, r, z.data(), t, eps.data());
             for (uint32_t i = 0; i < C; ++i)
                 u[i] = clampv<T>(u_nom[t * C + i] + eps[i], r.lo[i], r.hi[i]);
-            RngCtx rc{t, k, step, MPPI_TAG_DYNAMICS, agent, p->seed};
-            r.dyn(x.data(), u.data(), r.model, rc);
+            if (sur) {
+                if constexpr (sizeof(T) == 4) surrogate_dynamics(*sur, D, C, x.data(), u.data());
+            } else {
+                RngCtx rc{t, k, step, MPPI_TAG_DYNAMICS, agent, p->seed};
+                r.dyn(x.data(), u.data(), r.model, rc);
+            }
             total += r.stage(x.data(), u.data(), r.model, r.cost, mppi_time{t, step});
         }
         total += r.term(x.data(), r.model, r.cost, mppi_time{H, step});
@@ -706,6 +740,36 @@ int oracle_rollout_range(const oracle_problem* p, uint32_t step, uint32_t agent,
     Resolved<float> r = resolve<float>(p);
     if (!r.ok || k1 > p->K || k0 > k1) return -1;
     rollout_range<float>(p, r, step, agent, x0, u_nom, k0, k1, costs);
+    return 0;
+}
+
+// Realised cost of a closed loop: out[i] = stage(x[i], u[i], time {0,
+// steps[i]}) for n (post-step state, applied control) pairs, the cost the
+// rollout's first timestep assigns to what the plant actually did.
+int oracle_stage_costs(const oracle_problem* p, uint32_t n, const float* states, const float* controls,
+                       const uint32_t* steps, float* out) {
+    Resolved<float> r = resolve<float>(p);
+    if (!r.ok) return -1;
+    for (uint32_t i = 0; i < n; ++i)
+        out[i] = r.stage(states + size_t(i) * p->D, controls + size_t(i) * p->C, r.model, r.cost,
+                         mppi_time{0u, steps[i]});
+    return 0;
+}
+
+// One step x <- x + delta(x, u) of the surrogate network.
+void oracle_surrogate_dynamics(uint32_t D, uint32_t C, float* x, const float* u, const float* params,
+                               const float* norm80) {
+    surrogate_dynamics(Surrogate{params, norm80}, D, C, x, u);
+}
+
+// Costs of samples [k0, k1) with the surrogate network as dynamics.
+int oracle_rollout_surrogate_range(const oracle_problem* p, uint32_t step, uint32_t agent,
+                                   const float* x0, const float* u_nom, uint32_t k0, uint32_t k1,
+                                   const float* params, const float* norm80, float* costs) {
+    Resolved<float> r = resolve<float>(p);
+    if (!r.ok || k1 > p->K || k0 > k1 || p->D + p->C > 16) return -1;
+    const Surrogate s{params, norm80};
+    rollout_range<float>(p, r, step, agent, x0, u_nom, k0, k1, costs, &s);
     return 0;
 }
 

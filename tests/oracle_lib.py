@@ -121,6 +121,40 @@ class Problem:
         assert lib().oracle_rollout_range(self.ref, step, agent, _p(x0), _p(un), k0, k1, _p(out)) == 0
         return out[k0:k1]
 
+    def rollout_surrogate_range(self, step, x0, u_nom, k0, k1, params, norm80, agent=0):
+        """Costs of samples [k0, k1) with the surrogate network as dynamics."""
+        x0 = np.ascontiguousarray(x0, np.float32)
+        un = np.ascontiguousarray(u_nom, np.float32).ravel()
+        P = np.ascontiguousarray(params, np.float32)
+        nb = np.ascontiguousarray(norm80, np.float32)
+        out = np.full(self.K, np.nan, np.float32)
+        L = lib()
+        L.oracle_rollout_surrogate_range.restype = ctypes.c_int
+        assert L.oracle_rollout_surrogate_range(self.ref, ctypes.c_uint32(step), ctypes.c_uint32(agent), _p(x0),
+                                                _p(un), ctypes.c_uint32(k0), ctypes.c_uint32(k1), _p(P),
+                                                _p(nb), _p(out)) == 0
+        return out[k0:k1]
+
+    def stage_costs(self, states, controls, steps):
+        """stage(x_i, u_i, time {0, steps_i}) for each row."""
+        s = np.ascontiguousarray(states, np.float32).reshape(-1, self.D)
+        u = np.ascontiguousarray(controls, np.float32).reshape(-1, self.C)
+        st = np.ascontiguousarray(steps, np.uint32)
+        out = np.empty(s.shape[0], np.float32)
+        L = lib()
+        L.oracle_stage_costs.restype = ctypes.c_int
+        assert L.oracle_stage_costs(self.ref, ctypes.c_uint32(s.shape[0]), _p(s), _p(u),
+                                    st.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _p(out)) == 0
+        return out
+
+    def closed_loop_cost(self, states, controls, terminal, start_step=0):
+        """Realised cost of a closed loop from simulate(): each step's stage
+        cost on the state the plant reached with the applied control."""
+        st = np.asarray(states, np.float32).reshape(-1, self.D)
+        nxt = np.concatenate([st[1:], np.asarray(terminal, np.float32).reshape(1, self.D)])
+        steps = np.arange(start_step, start_step + st.shape[0], dtype=np.uint32)
+        return self.stage_costs(nxt, controls, steps)
+
     def weights(self, costs, fp64=False):
         if fp64:
             c = np.ascontiguousarray(costs, np.float64)

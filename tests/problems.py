@@ -43,6 +43,30 @@ def double_integrator(K=64, H=10, seed=0, stage="mppi_stage_cost", terminal="", 
     return hip, ora
 
 
+def bicycle(K, H, seed=SEED, agent=0, substeps=2, lam=None):
+    """Single-agent dynamic-bicycle pursuit: agent `agent` of the pursuit
+    scene (start state, moving target, 32 threat zones)."""
+    sc = scenes.bicycle_pursuit_scene(max(agent + 1, 1), seed)
+    sc.model.substeps = substeps
+    mb, cb = sc.model.blob(), sc.cost_blobs[agent]
+    lam = sc.lam if lam is None else lam
+    sigma = np.array(sc.sigma, np.float32)
+    hip = HipMPPI(6, 2, H, K, lam, sigma, sc.u_min, sc.u_max)
+    hip.link("bicycle.hip", "bicycle_dynamics", "bicycle_pursuit_cost", "bicycle_terminal_cost",
+             len(mb), len(cb), seed)
+    hip.bind_params(mb, cb)
+    ora = ol.Problem("bicycle", 6, 2, H, K, lam, sigma, sc.u_min, sc.u_max, seed, "bicycle_dynamics",
+                     "bicycle_pursuit_cost", "bicycle_terminal_cost", mb, cb)
+    return hip, ora, sc
+
+
+# Operating envelope of the bicycle surrogate: (x, y) are not inputs (the
+# dynamics is translation invariant), the heading is periodic.
+BICYCLE_ENVELOPE = dict(state_lo=[-50, -50, -np.pi, 0.0, -4.0, -4.0],
+                        state_hi=[50, 50, np.pi, 20.0, 4.0, 4.0],
+                        state_input=[2, 2, 1, 0, 0, 0])
+
+
 def rel_err(a, b, floor=0.0):
     """max |a - b| / max(|b|, floor), elementwise."""
     a = np.asarray(a, np.float64)

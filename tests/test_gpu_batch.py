@@ -19,6 +19,15 @@ pytestmark = pytest.mark.gpu
 TUNINGS = [
     dict(block=256, persistent=0, reduce_spt=0),
     dict(block=64, persistent=0, reduce_spt=1),
+    dict(block=128, persistent=0, reduce_spt=2, order=1),
+    dict(block=256, persistent=0, reduce_spt=8),
+]
+
+# The persistent work-queue mode is switched off in the library (set_tuning
+# rejects it): one of its launches did not complete on gfx950 and the cause
+# has not been found.  Its parity cases stay here, skipped, for when it is.
+PERSISTENT_OFF = "persistent work-queue mode switched off: a launch did not complete on gfx950, cause not found"
+PERSISTENT_TUNINGS = [
     dict(block=128, persistent=1, wgs_per_cu=2, reduce_spt=2),
     dict(block=256, persistent=1, wgs_per_cu=4, reduce_spt=8),
 ]
@@ -41,7 +50,9 @@ def oracle_for(sc, mb, a, seed=7):
                       "bicycle_terminal_cost", mb, sc.cost_blobs[a])
 
 
-@pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: "-".join(f"{k}{v}" for k, v in t.items()))
+@pytest.mark.parametrize("tuning", TUNINGS + [pytest.param(t, marks=pytest.mark.skip(reason=PERSISTENT_OFF))
+                                              for t in PERSISTENT_TUNINGS],
+                         ids=lambda t: "-".join(f"{k}{v}" for k, v in t.items()))
 def test_uneven_batch_matches_oracle(require_gpu, tuning):
     hip, b, sc, mb = pursuit(12)
     b.set_tuning(**tuning)
@@ -77,6 +88,14 @@ def test_uniform_batch_equals_simulate_batch(require_gpu):
         xb, _, _ = b.read()
         assert np.max(np.abs(xb - tt)) <= 1e-4
         b.close()
+
+
+def test_persistent_mode_is_rejected(require_gpu):
+    from mppi_metal_amd import MppiHipError
+    hip, b, sc, mb = pursuit(4)
+    with pytest.raises(MppiHipError, match="switched off"):
+        b.set_tuning(block=128, persistent=1, wgs_per_cu=2)
+    assert b.get_tuning()["persistent"] == 0
 
 
 def test_graph_replay_is_stable(require_gpu):

@@ -80,17 +80,25 @@ def test_determinism_same_seed(require_gpu):
     assert np.array_equal(u1, u3)
 
 
+# Closed loop: step 0 is compared at the single-step bar (costs bit-exact,
+# controls 1e-5).  From step 1 on, the GPU's and the oracle's plant states
+# differ by the reduce's last-bit rounding (tree versus sequential sums), and
+# with lambda = 1 and an effective sample size near 1 the softmax amplifies a
+# relative cost change dc by dc / lambda: a 1e-7 state difference moves the
+# controls by up to ~1e-3.  The later steps are therefore held to 2e-3.
+CLOSED_LOOP_TOL = 2e-3
+
+
 def test_simulate_matches_oracle(require_gpu):
     hip, ora, sc = unicycle(1024, 32, 8)
     x0 = np.zeros(4, np.float32)
     st, ct, kt = hip.simulate(6, 0, x0, np.zeros((32, 2), np.float32))
     so, co, ko, _ = ora.simulate(6, 0, x0, threads=8)
-    # Costs are bit-exact at step 0; afterwards the plant states differ by the
-    # reduce's rounding, which the closed loop carries.
     assert kt[0] == ko[0, 0]
-    assert np.max(np.abs(st - so[:, 0])) <= 1e-4
-    assert np.max(np.abs(ct - co[:, 0])) <= 1e-4
-    assert rel_err(kt, ko[:, 0]) <= 1e-4
+    assert np.max(np.abs(ct[0] - co[0, 0])) <= 1e-5
+    assert np.max(np.abs(st - so[:, 0])) <= CLOSED_LOOP_TOL
+    assert np.max(np.abs(ct - co[:, 0])) <= CLOSED_LOOP_TOL
+    assert rel_err(kt, ko[:, 0]) <= CLOSED_LOOP_TOL
 
 
 def test_simulate_batch_matches_oracle(require_gpu):
@@ -99,9 +107,10 @@ def test_simulate_batch_matches_oracle(require_gpu):
     st, ct, kt, tt = hip.simulate_batch(3, 4, 10, x0)
     so, co, ko, to = ora.simulate(4, 10, x0, n_agents=3, threads=8)
     assert np.array_equal(kt[0], ko[0])
-    assert np.max(np.abs(st - so)) <= 1e-4
-    assert np.max(np.abs(ct - co)) <= 1e-4
-    assert np.max(np.abs(tt - to)) <= 1e-4
+    assert np.max(np.abs(ct[0] - co[0])) <= 1e-5
+    assert np.max(np.abs(st - so)) <= CLOSED_LOOP_TOL
+    assert np.max(np.abs(ct - co)) <= CLOSED_LOOP_TOL
+    assert np.max(np.abs(tt - to)) <= CLOSED_LOOP_TOL
 
 
 def test_unicycle_k65536_h64_obs16_full_size(require_gpu):

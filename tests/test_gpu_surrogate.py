@@ -1,0 +1,203 @@
+"""GPU tests of the learned-dynamics surrogate through the C ABI
+(mppi_hip_surrogate_*), against the CPU oracle.
+
+Tolerances (bf16 operands, FP32 accumulation on the matrix cores):
+  forward pass      |y_gpu - y_oracle| <= 2e-3 per output, mean <= 1e-4; the
+                    oracle rounds the same operands to bf16, so what remains is
+                    FP32-versus-FP64 accumulation and the GPU's tanh (about
+                    2 ulp), which move a bf16 activation by one step in rare ties
+  gradients         relative L2 error per parameter tensor <= 1e-2 (same
+                    roundings, including the bf16 backward operands)
+  surrogate rollout sample costs within 1e-3 relative for 99% of samples
+                    over a 16-step horizon (the same rare ties, compounded)
+  training          the loss falls by 50x and the one-step relative RMSE of
+                    every state component on unseen transitions is < 0.1
+  closed loop       mean realised cost within 5% of plain MPPI (bench.py
+                    reports the gap at the full workload)
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from problems import BICYCLE_ENVELOPE, bicycle
+
+pytestmark = pytest.mark.gpu
+
+IN, HID, OUT = 8, 128, 6
+N_PARAMS = HID * IN + HID + HID * HID + HID + OUT * HID + OUT
+
+
+def _random_params(seed):
+    rng = np.random.default_rng(seed)
+    parts = []
+    for (o, i) in ((HID, IN), (HID, HID), (OUT, HID)):
+        a = np.sqrt(6.0 / (o + i))
+        parts += [rng.uniform(-a, a, (o, i)).astype(np.float32), rng.normal(0, 0.1, o).astype(np.float32)]
+    return parts
+
+
+def _identity_norm():
+    n = np.zeros(80, np.float32)
+    n[16:32] = 1.0     # inv_half
+    n[64:80] = 1.0     # out sd
+    return n
+
+
+def _flat(parts):
+    return np.concatenate([p.ravel() for p in parts]).astype(np.float32)
+
+
+def _oracle_forward1(P, x):
+    L = ol.lib()
+    L.oracle_mlp_forward1.restype = None
+    fp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    y = np.empty((x.shape[0], OUT), np.float32)
+    for s in range(x.shape[0]):
+        row = np.ascontiguousarray(x[s], np.float32)
+        out = np.empty(OUT, np.float32)
+        L.oracle_mlp_forward1(fp(P), ctypes.c_uint32(IN), ctypes.c_uint32(OUT), fp(row), fp(out))
+        y[s] = out
+    return y
+
+
+def _report(name, **v):
+    print(f"[{name}] " + " ".join(f"{k}={x:.3e}" if isinstance(x, float) else f"{k}={x}"
+                                  for k, x in v.items()), flush=True)
+
+
+def test_forward_matches_oracle(require_gpu):
+    hip, _, _ = bicycle(256, 8)
+    parts = _random_params(1)
+    P = _flat(parts)
+    hip.surrogate_import(P, _identity_norm())
+    rng = np.random.default_rng(2)
+    n = 200                                         # not a multiple of 16: padded block
+    states = rng.uniform(-1, 1, (n, 6)).astype(np.float32)
+    controls = rng.uniform(-1, 1, (n, 2)).astype(np.float32)
+    nxt = hip.surrogate_predict(states, controls)
+    y_gpu = (nxt.astype(np.float64) - states)
+    y_ora = _oracle_forward1(P, np.concatenate([states, controls], 1))
+    err = np.abs(y_gpu - y_ora)
+    _report("forward", max_abs=float(err.max()), mean_abs=float(err.mean()),
+            max_y=float(np.abs(y_ora).max()))
+    assert err.max() <= 2e-3 and err.mean() <= 1e-4
+
+
+def test_gradients_match_oracle(require_gpu):
+    import test_oracle_mlp as tm
+    hip, _, _ = bicycle(256, 8)
+    parts = _random_params(3)
+    hip.surrogate_import(_flat(parts), _identity_norm())
+    rng = np.random.default_rng(4)
+    n = 512
+    x = rng.uniform(-1, 1, (n, IN)).astype(np.float32)
+    t = rng.normal(0, 1, (n, OUT)).astype(np.float32)
+    g_gpu, loss_gpu = hip.surrogate_gradients(x, t)
+    loss_o, gs = tm._grad(parts, x, t, bf16=1)
+    off = 0
+    errs = {}
+    for name, g_o in zip(("w1", "b1", "w2", "b2", "w3", "b3"), gs):
+        g = g_gpu[off:off + g_o.size].reshape(g_o.shape)
+        off += g_o.size
+        errs[name] = float(np.linalg.norm(g - g_o) / np.linalg.norm(g_o))
+    errs["loss"] = abs(loss_gpu - loss_o) / loss_o
+    _report("gradients", **errs)
+    assert off == N_PARAMS
+    for k, e in errs.items():
+        assert e <= (1e-3 if k == "loss" else 1e-2), k
+
+
+@pytest.fixture(scope="module")
+def trained():
+    hip, ora, sc = bicycle(1024, 16)
+    hist = hip.surrogate_train(**BICYCLE_ENVELOPE, num_transitions=1 << 20, batch=4096, steps=3000,
+                               learning_rate=2e-3, seed=11)
+    return hip, ora, sc, hist
+
+
+def test_training_reduces_loss_and_generalises(require_gpu, trained):
+    hip, _, _, hist = trained
+    rmse, rel = hip.surrogate_evaluate(65536, seed=99)
+    _report("training", loss_first=float(hist[:5].mean()), loss_last=float(hist[-50:].mean()),
+            rel_rmse=" ".join(f"{v:.4f}" for v in rel))
+    assert np.all(np.isfinite(hist))
+    assert hist[-50:].mean() < hist[:5].mean() / 50
+    assert np.all(rel < 0.1)
+
+
+def test_export_import_round_trip(require_gpu, trained):
+    hip, _, _, _ = trained
+    P, norm = hip.surrogate_export()
+    assert P.size == N_PARAMS and np.all(np.isfinite(P))
+    other, _, _ = bicycle(256, 8)
+    other.surrogate_import(P, norm)
+    rng = np.random.default_rng(5)
+    s = np.stack([rng.uniform(-40, 40, 64), rng.uniform(-40, 40, 64), rng.uniform(-3, 3, 64),
+                  rng.uniform(0, 20, 64), rng.uniform(-2, 2, 64), rng.uniform(-2, 2, 64)], 1).astype(np.float32)
+    u = np.stack([rng.uniform(-0.5, 0.5, 64), rng.uniform(-4, 4, 64)], 1).astype(np.float32)
+    assert np.array_equal(hip.surrogate_predict(s, u), other.surrogate_predict(s, u))
+    P2, norm2 = other.surrogate_export()
+    assert np.array_equal(P, P2) and np.array_equal(norm, norm2)
+
+
+def test_surrogate_rollout_matches_oracle(require_gpu, trained):
+    hip, ora, sc, _ = trained
+    P, norm = hip.surrogate_export()
+    x0 = sc.x0[0]
+    rng = np.random.default_rng(6)
+    u_nom = (rng.standard_normal((16, 2)) * [0.1, 1.0]).astype(np.float32)
+    hip.surrogate_use(True)
+    try:
+        u_g, d_g = hip.step(3, x0, u_nom)
+        c_g = hip.read_costs()
+    finally:
+        hip.surrogate_use(False)
+    c_o = ora.rollout_surrogate_range(3, x0, u_nom, 0, 1024, P, norm)
+    rel = np.abs(c_g.astype(np.float64) - c_o) / np.maximum(np.abs(c_o), 1e-30)
+    _report("surrogate rollout", q50=float(np.quantile(rel, 0.5)), q99=float(np.quantile(rel, 0.99)),
+            max=float(rel.max()))
+    assert np.all(np.isfinite(c_g))
+    assert np.quantile(rel, 0.99) <= 1e-3
+    # The weights and update follow from these costs exactly as in the plain
+    # path (shared kernels): recompute them with the oracle from the GPU costs.
+    w_o, d_o = ora.weights(c_g)
+    assert d_g[0] == d_o[0]
+    assert np.max(np.abs(hip.read_weights() - w_o)) <= 1e-5 * max(1e-30, float(w_o.max())) + 1e-12
+    # Plugin dynamics again once the surrogate is switched off.
+    hip.step(3, x0, u_nom)
+    assert np.array_equal(hip.read_costs(), ora.rollout_range(3, x0, u_nom, 0, 1024))
+
+
+def test_closed_loop_gap(require_gpu, trained):
+    hip_t, _, _, _ = trained
+    P, norm = hip_t.surrogate_export()
+    hip, ora, sc = bicycle(16384, 64)
+    x0 = sc.x0[0]
+    un = np.zeros((64, 2), np.float32)
+    T = 200
+    st, ct, _, tt = hip.simulate_batch(1, T, 0, x0, un)
+    plain = ora.closed_loop_cost(st[:, 0], ct[:, 0], tt[0])
+    hip.surrogate_import(P, norm)
+    hip.surrogate_use(True)
+    st2, ct2, _, tt2 = hip.simulate_batch(1, T, 0, x0, un)
+    hip.surrogate_use(False)
+    sur = ora.closed_loop_cost(st2[:, 0], ct2[:, 0], tt2[0])
+    gap = abs(float(sur.mean()) - float(plain.mean())) / float(plain.mean())
+    _report("closed loop", plain=float(plain.mean()), surrogate=float(sur.mean()), gap=gap)
+    assert np.all(np.isfinite(sur))
+    assert gap <= 0.05
+
+
+def test_rejects_bad_calls(require_gpu):
+    from mppi_metal_amd.controller import MppiHipError
+    hip, _, _ = bicycle(256, 8)
+    with pytest.raises(MppiHipError):
+        hip.surrogate_use(True)                      # nothing trained yet
+    with pytest.raises(MppiHipError):
+        hip.surrogate_evaluate(16)
+    with pytest.raises(MppiHipError):
+        hip.surrogate_train(**BICYCLE_ENVELOPE, batch=100, steps=1)
+    with pytest.raises(MppiHipError):
+        hip.surrogate_import(np.zeros(10, np.float32), _identity_norm())
