@@ -17,6 +17,22 @@ Printed as one JSON line:
                    the reduce kernel against HBM bandwidth
   cpu_baseline     the CPU oracle (scalar C++, same algorithm and noise
                    stream) on the host's cores over a bounded sample of steps
+
+Two more workloads are reported in the same line (skip with --no-batch,
+--no-surrogate):
+  batch            256-agent dynamic-bicycle pursuit with uneven K (log-uniform
+                   1024..16384) and H (16..96), 32 threat zones, one
+                   graph-replayed step of all agents with the autotuned launch
+                   shape; against the plain GPU path driven one agent at a time
+                   (one context per agent, blocking step() each, as a user of
+                   the reference's single-agent driver would) and against a
+                   uniform batch padded to the largest K and H (the
+                   reference's batch form needs one K and H for all agents).
+  surrogate        the learned bicycle dynamics (MLP 8-128-128-6, trained on
+                   the GPU on 4M plugin transitions) inside the rollout at
+                   K = 262144, H = 64, 500 closed-loop steps, against plugin
+                   dynamics from the same start and seed: sample-steps/s of
+                   both and the gap of the mean realised closed-loop cost.
 """
 from __future__ import annotations
 
@@ -52,11 +68,12 @@ def reduce_bytes(K: int, H: int, C: int, n_chunks: int) -> int:
     return 8 * K + 8 * H * C + 2 * 16 * n_chunks * nb
 
 
-def reduce_chunks(K: int, H: int, C: int, N: int = 1, spt: int = 0) -> int:
+def reduce_chunks(K: int, H: int, C: int, N: int = 1, spt: int = 0, cus: int = 256) -> int:
+    """Sample chunks of the reduce launch (the library's default_spt rule)."""
     nb = (H * C + 3) // 4
     groups = (nb + 7) // 8
     if not spt:
-        spt = max(1, min(64, -(-(K * groups * N) // (256 * 2048))))
+        spt = max(1, min(64, -(-(K * groups * N) // (256 * 2 * cus))))
     return -(-K // (256 * spt))
 
 
@@ -71,18 +88,156 @@ def cpu_baseline(n_obs, H, K, steps, threads):
     return K * H * steps / dt, dt
 
 
+def bench_batch(args):
+    """Uneven 256-agent pursuit: batched step vs per-agent plain path."""
+    from mppi_metal_amd import HipBatch, HipMPPI, scenes
+    sc = scenes.bicycle_pursuit_scene(256, 42)
+    mb = sc.model.blob()
+    N = len(sc.K)
+    work = int(np.sum(sc.K.astype(np.int64) * sc.H.astype(np.int64)))   # sample-steps per step
+    hip = HipMPPI(6, 2, 8, 256, sc.lam, sc.sigma, sc.u_min, sc.u_max)
+    hip.link("bicycle.hip", "bicycle_dynamics", "bicycle_pursuit_cost", "bicycle_terminal_cost",
+             len(mb), scenes.BICYCLE_COST_BYTES, 42)
+    b = HipBatch(hip, sc.K, sc.H)
+    b.set_params(mb, sc.cost_blob_packed(), 0, scenes.BICYCLE_COST_BYTES)
+    b.reset(0, sc.x0)
+    tune_ms = b.autotune(3)
+    tuning = b.get_tuning()
+    b.reset(0, sc.x0)
+    b.run(5, closed_loop=True)
+    T = args.batch_steps
+    t0 = time.perf_counter()
+    b.run(T, closed_loop=True)
+    el = time.perf_counter() - t0
+    batch_rate = work * T / el
+    print(f"[bench] batch: {T} steps in {el:.3f}s, tuning {tuning}", file=sys.stderr, flush=True)
+
+    # Plain GPU path, one agent at a time (blocking step per agent).
+    ctxs = []
+    for a in range(N):
+        h = HipMPPI(6, 2, int(sc.H[a]), int(sc.K[a]), sc.lam, sc.sigma, sc.u_min, sc.u_max)
+        h.link("bicycle.hip", "bicycle_dynamics", "bicycle_pursuit_cost", "bicycle_terminal_cost",
+               len(mb), scenes.BICYCLE_COST_BYTES, 42)
+        h.bind_params(mb, sc.cost_blobs[a])
+        ctxs.append(h)
+    noms = [np.zeros((int(sc.H[a]), 2), np.float32) for a in range(N)]
+    for a in range(N):
+        ctxs[a].step(0, sc.x0[a], noms[a])
+    Ts = args.serial_steps
+    t0 = time.perf_counter()
+    for s in range(Ts):
+        for a in range(N):
+            u, _ = ctxs[a].step(s, sc.x0[a], noms[a])
+    el_s = time.perf_counter() - t0
+    serial_rate = work * Ts / el_s
+    for h in ctxs:
+        h.close()
+
+    # Reference-style uniform batch padded to the largest K and H.
+    Kmax, Hmax = int(sc.K.max()), int(sc.H.max())
+    pad = HipMPPI(6, 2, Hmax, Kmax, sc.lam, sc.sigma, sc.u_min, sc.u_max)
+    pad.link("bicycle.hip", "bicycle_dynamics", "bicycle_pursuit_cost", "bicycle_terminal_cost",
+             len(mb), scenes.BICYCLE_COST_BYTES, 42)
+    pad.bind_params(mb, sc.cost_blobs[0])
+    pad.simulate_batch(N, 2, 0, sc.x0, outputs=())
+    Tp = args.serial_steps
+    t0 = time.perf_counter()
+    pad.simulate_batch(N, Tp, 0, sc.x0, outputs=())
+    el_p = time.perf_counter() - t0
+    padded_rate = work * Tp / el_p
+    pad.close()
+    hip.close()
+    return {
+        "workload": "256-agent dynamic-bicycle pursuit, uneven K (2^U[10,14]) and H (U[16,96]), "
+                    "32 threat zones, closed loop",
+        "agents": N, "sample_steps_per_step": work, "steps": T,
+        "value": batch_rate, "unit": "sample-steps/s", "ms_per_step": el / T * 1e3,
+        "tuning": tuning, "autotune_best_ms": tune_ms,
+        "plain_per_agent": {"value": serial_rate, "ms_per_step": el_s / Ts * 1e3, "steps": Ts,
+                            "how": "one context per agent, blocking step() per agent"},
+        "padded_uniform_batch": {"value_useful": padded_rate, "ms_per_step": el_p / Tp * 1e3,
+                                 "K": Kmax, "H": Hmax, "steps": Tp,
+                                 "how": "simulate_batch with every agent at the largest K and H"},
+        "speedup_vs_plain_per_agent": batch_rate / serial_rate,
+        "speedup_vs_padded_batch": batch_rate / padded_rate,
+    }
+
+
+def bench_surrogate(args):
+    """Learned bicycle dynamics in the rollout vs plugin dynamics."""
+    from mppi_metal_amd import HipMPPI, scenes
+    sc = scenes.bicycle_pursuit_scene(1, 42)
+    mb, cb = sc.model.blob(), sc.cost_blobs[0]
+    K, H, T = 262144, 64, args.surrogate_steps
+    hip = HipMPPI(6, 2, H, K, sc.lam, sc.sigma, sc.u_min, sc.u_max)
+    hip.link("bicycle.hip", "bicycle_dynamics", "bicycle_pursuit_cost", "bicycle_terminal_cost",
+             len(mb), len(cb), 42)
+    hip.bind_params(mb, cb)
+    t0 = time.perf_counter()
+    hist = hip.surrogate_train(**scenes.BICYCLE_SURROGATE_ENVELOPE, num_transitions=4 << 20, batch=4096,
+                               steps=args.train_steps, learning_rate=2e-3, seed=7)
+    train_s = time.perf_counter() - t0
+    _, rel = hip.surrogate_evaluate(1 << 16, seed=99)
+    print(f"[bench] surrogate trained in {train_s:.2f}s, loss {hist[0]:.3e} -> {hist[-20:].mean():.3e}",
+          file=sys.stderr, flush=True)
+    x0 = sc.x0[0]
+    un = np.zeros((H, 2), np.float32)
+    res = {}
+    for name, use in (("plain", False), ("surrogate", True)):
+        hip.surrogate_use(use)
+        hip.simulate_batch(1, 3, 0, x0, un, outputs=())
+        hip.profile(True)
+        t0 = time.perf_counter()
+        st, ct, _, tt = hip.simulate_batch(1, T, 0, x0, un)
+        el = time.perf_counter() - t0
+        roll_ms, red_ms, launches = hip.profile_read()
+        hip.profile(False)
+        nxt = np.concatenate([st[1:, 0], tt])
+        cost = scenes.pursuit_realised_cost(sc, 0, nxt, ct[:, 0], np.arange(T))
+        res[name] = {"value": K * H * T / el, "ms_per_step": el / T * 1e3,
+                     "rollout_us": roll_ms / launches * 1e3, "reduce_us": red_ms / launches * 1e3,
+                     "mean_realised_cost": float(cost.mean())}
+        print(f"[bench] {name}: {T} steps in {el:.3f}s, mean cost {cost.mean():.3f}", file=sys.stderr,
+              flush=True)
+    hip.surrogate_use(False)
+    hip.close()
+    # Matrix-core flops of the network per sample-step (input and output
+    # padded to the 16-wide tiles): 2 * (16*128 + 128*128 + 128*16).
+    mfma_flops = 2 * (16 * 128 + 128 * 128 + 128 * 16)
+    sur = res["surrogate"]
+    return {
+        "workload": "learned dynamic-bicycle surrogate (MLP 8-128-128-6 tanh, bf16 MFMA, fp32 accumulate), "
+                    "pursuit, closed loop",
+        "K": K, "H": H, "closed_loop_steps": T,
+        "train": {"transitions": 4 << 20, "batch": 4096, "steps": args.train_steps, "seconds": train_s,
+                  "loss_first": float(hist[0]), "loss_last": float(hist[-20:].mean()),
+                  "one_step_relative_rmse": [float(v) for v in rel]},
+        "plain": res["plain"], "surrogate": sur,
+        "speed_ratio_surrogate_vs_plain": sur["value"] / res["plain"]["value"],
+        "closed_loop_cost_gap": (sur["mean_realised_cost"] - res["plain"]["mean_realised_cost"])
+                                / res["plain"]["mean_realised_cost"],
+        "surrogate_rollout_mfma_tflops": K * H * mfma_flops / (sur["rollout_us"] * 1e-6) / 1e12,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--latency-steps", type=int, default=200)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--K", type=int, default=65536)
     ap.add_argument("--H", type=int, default=64)
     ap.add_argument("--obstacles", type=int, default=16)
     ap.add_argument("--spt", type=int, default=0, help="reduce samples per thread (0: built-in)")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--serial-steps", type=int, default=5)
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--surrogate-steps", type=int, default=500)
+    ap.add_argument("--train-steps", type=int, default=4000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -149,6 +304,9 @@ def main():
                "sample": f"CPU oracle (scalar C++, FP32, same noise stream), first {args.cpu_steps} "
                          f"closed-loop steps of the same workload, {threads} threads, {cdt:.1f}s"}
 
+    batch = None if args.no_batch else bench_batch(args)
+    surrogate = None if args.no_surrogate else bench_surrogate(args)
+
     out = {
         "metric": "MPPI sample-steps/s (+ step p50/p99 ms)",
         "value": value,
@@ -177,6 +335,8 @@ def main():
                             "bytes_per_launch": rb, "avg_launch_us": red_s * 1e6},
         "cpu_baseline": cpu,
         "final_closed_loop_cost": float(kt[-1]),
+        "batch": batch,
+        "surrogate": surrogate,
     }
     print(json.dumps(out), flush=True)
 

@@ -37,6 +37,24 @@
 
 #define MPPI_CALLABLE MPPI_HD
 
+// True when `c` holds for this sample or, on the GPU, for any sample of the
+// same wavefront.  A callable can guard work that only some samples need so
+// that a wavefront in which no sample needs it skips it (a uniform branch
+// instead of computing and discarding the work on every lane):
+//   bool need = false;
+//   for (i...) need |= cheap_test(i);
+//   if (mppi_any(need)) for (i...) if (cheap_test(i)) expensive(i);
+// Guard a loop or a large block: the optimiser evaluates a small guarded
+// block unconditionally and selects its result.  Results are the same either
+// way; on the host it is the identity.
+MPPI_HD bool mppi_any(bool c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ballot_w64(c) != 0;
+#else
+    return c;
+#endif
+}
+
 // Random-number handle given to the dynamics callable.  Draw j of timestep t
 // of sample k uses counter (t << 8 | j, k, step, tag << 28 | agent); a
 // callable may make up to 256 draws of four words each per step.

@@ -81,12 +81,24 @@ MPPI_CALLABLE float bicycle_pursuit_cost(const float* s, const float* u, const u
     cost += c->w_control * (u[0] * u[0] + u[1] * u[1]);
     float pen = 0.0f;
     const uint32_t nz = c->n_zones < BICYCLE_MAX_ZONES ? c->n_zones : BICYCLE_MAX_ZONES;
+    // The square root is only needed inside a zone: a first pass finds
+    // whether this sample is inside any; the penalty pass runs only in
+    // wavefronts where some sample is.
+    bool inside_any = false;
     for (uint32_t i = 0; i < nz; ++i) {
         const float ox = s[0] - c->zones[i][0];
         const float oy = s[1] - c->zones[i][1];
         const float rz = c->zones[i][2];
-        const float d2 = ox * ox + oy * oy;
-        if (d2 < rz * rz) pen += rz - mppi_sqrtf(d2);
+        inside_any |= ox * ox + oy * oy < rz * rz;
+    }
+    if (mppi_any(inside_any)) {
+        for (uint32_t i = 0; i < nz; ++i) {
+            const float ox = s[0] - c->zones[i][0];
+            const float oy = s[1] - c->zones[i][1];
+            const float rz = c->zones[i][2];
+            const float d2 = ox * ox + oy * oy;
+            if (d2 < rz * rz) pen += rz - mppi_sqrtf(d2);
+        }
     }
     return cost + c->w_zone * pen;
 }

@@ -75,17 +75,28 @@ MPPI_CALLABLE float unicycle_stage_cost_obstacles(const float* state, const floa
     float cost = mppi_stage_cost(state, control, model_params, cost_params_raw);
     float pen = 0.0f;
     uint32_t n = p->n_obstacles < UNICYCLE_MAX_OBSTACLES ? p->n_obstacles : UNICYCLE_MAX_OBSTACLES;
-    // Four obstacles per iteration so their parameter loads are issued
-    // together; slots past n have radius 0 and never add a penalty.
+    // max(0, r - |p - c|) summed over the obstacles.  The square root is
+    // only needed inside an obstacle: a first pass finds whether this sample
+    // is inside any, and the penalty pass runs only in wavefronts where some
+    // sample is.  Four obstacles per iteration so their parameter loads are
+    // issued together; slots past n have radius 0 and never add a penalty.
     const uint32_t n4 = (n + 3u) & ~3u;
+    bool inside_any = false;
     for (uint32_t i = 0; i < n4; i += 4) {
         MPPI_UNROLL
         for (uint32_t j = 0; j < 4; ++j) {
             float ox = state[0] - p->obstacles[i + j][0];
             float oy = state[1] - p->obstacles[i + j][1];
             float r = p->obstacles[i + j][2];
+            inside_any |= ox * ox + oy * oy < r * r;
+        }
+    }
+    if (mppi_any(inside_any)) {
+        for (uint32_t i = 0; i < n4; ++i) {
+            float ox = state[0] - p->obstacles[i][0];
+            float oy = state[1] - p->obstacles[i][1];
+            float r = p->obstacles[i][2];
             float d2 = ox * ox + oy * oy;
-            // max(0, r - |p - c|); the square root is only needed inside.
             if (d2 < r * r) pen += r - mppi_sqrtf(d2);
         }
     }

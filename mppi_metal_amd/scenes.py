@@ -198,3 +198,29 @@ def bicycle_pursuit_scene(n_agents=256, seed=42, k_log2=(10, 14), h_range=(16, 9
         H[a] = int(h_range[0] + min(h_range[1] - h_range[0], int(s.uniform(0, 1) * (h_range[1] - h_range[0] + 1))))
     blobs = [bicycle_cost_blob(targets[a, :2], targets[a, 2:], zones) for a in range(n_agents)]
     return PursuitScene(BicycleParams(), zones, x0, targets, K, H, blobs)
+
+
+# Operating envelope of the learned bicycle surrogate's training data: states
+# uniform over this box (positions are not network inputs: the dynamics is
+# translation invariant; the heading is periodic), controls over the bounds.
+BICYCLE_SURROGATE_ENVELOPE = dict(state_lo=[-50.0, -50.0, -np.pi, 0.0, -4.0, -4.0],
+                                  state_hi=[50.0, 50.0, np.pi, 20.0, 4.0, 4.0],
+                                  state_input=[2, 2, 1, 0, 0, 0])
+
+
+def pursuit_realised_cost(scene: PursuitScene, agent: int, next_states, controls, steps,
+                          w_pos=1.0, w_control=0.1, w_zone=200.0, dt=0.05) -> np.ndarray:
+    """The pursuit stage cost (plugins/bicycle.hip, bicycle_pursuit_cost) of
+    what a closed loop did: for each closed-loop step s, the state the plant
+    reached with the applied control, at time tau = (s + 1) dt; FP64."""
+    x = np.asarray(next_states, np.float64).reshape(-1, 6)
+    u = np.asarray(controls, np.float64).reshape(-1, 2)
+    tau = (np.asarray(steps, np.float64) + 1.0) * dt
+    tx, ty, tvx, tvy = (float(v) for v in scene.targets[agent])
+    ex = x[:, 0] - (tx + tvx * tau)
+    ey = x[:, 1] - (ty + tvy * tau)
+    cost = w_pos * (ex * ex + ey * ey) + w_control * (u[:, 0] ** 2 + u[:, 1] ** 2)
+    for cx, cy, r in scene.zones:
+        d = np.hypot(x[:, 0] - cx, x[:, 1] - cy)
+        cost += w_zone * np.where(d < r, r - d, 0.0)
+    return cost

@@ -60,11 +60,7 @@ def bicycle(K, H, seed=SEED, agent=0, substeps=2, lam=None):
     return hip, ora, sc
 
 
-# Operating envelope of the bicycle surrogate: (x, y) are not inputs (the
-# dynamics is translation invariant), the heading is periodic.
-BICYCLE_ENVELOPE = dict(state_lo=[-50, -50, -np.pi, 0.0, -4.0, -4.0],
-                        state_hi=[50, 50, np.pi, 20.0, 4.0, 4.0],
-                        state_input=[2, 2, 1, 0, 0, 0])
+BICYCLE_ENVELOPE = scenes.BICYCLE_SURROGATE_ENVELOPE
 
 
 def rel_err(a, b, floor=0.0):

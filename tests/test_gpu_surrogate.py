@@ -2,14 +2,16 @@
 (mppi_hip_surrogate_*), against the CPU oracle.
 
 Tolerances (bf16 operands, FP32 accumulation on the matrix cores):
-  forward pass      |y_gpu - y_oracle| <= 2e-3 per output, mean <= 1e-4; the
+  forward pass      |y_gpu - y_oracle| <= 1e-3 per output, mean <= 1e-4; the
                     oracle rounds the same operands to bf16, so what remains is
                     FP32-versus-FP64 accumulation and the GPU's tanh (about
                     2 ulp), which move a bf16 activation by one step in rare ties
-  gradients         relative L2 error per parameter tensor <= 1e-2 (same
-                    roundings, including the bf16 backward operands)
-  surrogate rollout sample costs within 1e-3 relative for 99% of samples
-                    over a 16-step horizon (the same rare ties, compounded)
+  gradients         relative L2 error per parameter tensor <= 1e-3, loss
+                    <= 1e-4 (same roundings, including the bf16 backward
+                    operands; measured about 1e-4 and 2e-6)
+  surrogate rollout sample costs within 2e-4 relative for 99% of samples and
+                    1e-3 for all, over a 16-step horizon (the same rare ties,
+                    compounded; measured 4e-5 and 1.2e-4, median 0)
   training          the loss falls by 50x and the one-step relative RMSE of
                     every state component on unseen transitions is < 0.1
   closed loop       mean realised cost within 5% of plain MPPI (bench.py
@@ -82,7 +84,7 @@ def test_forward_matches_oracle(require_gpu):
     err = np.abs(y_gpu - y_ora)
     _report("forward", max_abs=float(err.max()), mean_abs=float(err.mean()),
             max_y=float(np.abs(y_ora).max()))
-    assert err.max() <= 2e-3 and err.mean() <= 1e-4
+    assert err.max() <= 1e-3 and err.mean() <= 1e-4
 
 
 def test_gradients_match_oracle(require_gpu):
@@ -106,7 +108,7 @@ def test_gradients_match_oracle(require_gpu):
     _report("gradients", **errs)
     assert off == N_PARAMS
     for k, e in errs.items():
-        assert e <= (1e-3 if k == "loss" else 1e-2), k
+        assert e <= (1e-4 if k == "loss" else 1e-3), k
 
 
 @pytest.fixture(scope="module")
@@ -159,7 +161,7 @@ def test_surrogate_rollout_matches_oracle(require_gpu, trained):
     _report("surrogate rollout", q50=float(np.quantile(rel, 0.5)), q99=float(np.quantile(rel, 0.99)),
             max=float(rel.max()))
     assert np.all(np.isfinite(c_g))
-    assert np.quantile(rel, 0.99) <= 1e-3
+    assert np.quantile(rel, 0.99) <= 2e-4 and rel.max() <= 1e-3
     # The weights and update follow from these costs exactly as in the plain
     # path (shared kernels): recompute them with the oracle from the GPU costs.
     w_o, d_o = ora.weights(c_g)

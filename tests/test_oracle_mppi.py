@@ -209,3 +209,28 @@ def test_batch_agent0_equals_single_and_terminal_relation():
     assert np.array_equal(kt1[:, 0], ktb[:, 0]) and np.array_equal(tt1[0], ttb[0])
     ext = p.simulate(5, 0, x0, n_agents=2)
     assert np.array_equal(sb[3], ext[0][4])
+
+
+def test_pursuit_realised_cost_matches_oracle_stage_cost():
+    """scenes.pursuit_realised_cost (numpy, FP64) restates the bicycle
+    pursuit stage cost that the oracle evaluates in FP32 from the plugin."""
+    from mppi_metal_amd import scenes
+    sc = scenes.bicycle_pursuit_scene(3, 42)
+    a = 2
+    mb, cb = sc.model.blob(), sc.cost_blobs[a]
+    ora = ol.Problem("bicycle", 6, 2, 8, 64, sc.lam, np.array(sc.sigma, np.float32), sc.u_min, sc.u_max, 42,
+                     "bicycle_dynamics", "bicycle_pursuit_cost", "bicycle_terminal_cost", mb, cb)
+    rng = np.random.default_rng(1)
+    n = 200
+    z = np.array(sc.zones, np.float32)
+    # half the states inside a threat zone, so the hinge term is exercised
+    states = np.zeros((n, 6), np.float32)
+    states[:, :2] = rng.uniform(-50, 50, (n, 2))
+    states[::2, :2] = z[rng.integers(0, len(z), n // 2), :2] + rng.uniform(-1, 1, (n // 2, 2))
+    states[:, 2:] = rng.uniform(-2, 2, (n, 4))
+    controls = rng.uniform(-1, 1, (n, 2)).astype(np.float32)
+    steps = rng.integers(0, 500, n).astype(np.uint32)
+    c32 = ora.stage_costs(states, controls, steps)
+    c64 = scenes.pursuit_realised_cost(sc, a, states, controls, steps)
+    assert np.max(np.abs(c32 - c64) / np.maximum(1.0, np.abs(c64))) <= 1e-4
+    assert np.count_nonzero(c64 > scenes.pursuit_realised_cost(sc, a, states, controls, steps, w_zone=0.0)) > 10
