@@ -60,7 +60,14 @@ typedef struct mppi_hip_config {
     const float* sigma;
     const float* u_min;         /* control_dim */
     const float* u_max;         /* control_dim */
-    uint32_t numeric_mode;      /* mppi_hip_numeric_mode; only FP32 is implemented */
+    uint32_t numeric_mode;      /* mppi_hip_numeric_mode (the reference declares the modes,
+                                   DriverConfig::NumericMode, and uses none):
+                                   FP32  every operation exactly rounded as written; sample
+                                         costs bit-identical to the CPU oracle
+                                   MIXED FP32 data, fast hardware transcendentals, fused
+                                         multiply-adds and approximate division in the
+                                         library's noise path and in plugin mppi_* math
+                                   FP16  MIXED with the control noise rounded to binary16 */
     int32_t device;             /* HIP device ordinal; -1 = current device */
 } mppi_hip_config;
 
@@ -302,8 +309,8 @@ const char* mppi_hip_version(void);
  * into log (log_capacity bytes, may be NULL). */
 int mppi_hip_compile_check(const char* plugin_source, const char* include_dir, const char* dynamics_fn,
                            const char* stage_cost_fn, const char* terminal_cost_fn, uint32_t state_dim,
-                           uint32_t control_dim, int covariance_full, uint32_t kernel_sets, char* log,
-                           size_t log_capacity);
+                           uint32_t control_dim, int covariance_full, uint32_t numeric_mode,
+                           uint32_t kernel_sets, char* log, size_t log_capacity);
 
 #ifdef __cplusplus
 }

@@ -24,6 +24,13 @@
 //           is exact by Sterbenz's lemma), as IEEE fmod requires.
 //   tanh  : via exp for |x| >= 0.55 and an odd Taylor series below.
 // Accuracy against FP64 libm is measured in tests/test_oracle_math.py.
+//
+// Fast mode (MPPI_FAST_MATH = 1, set by the library for the MIXED and FP16
+// numeric modes; GPU code only): exp, log, sqrt, sin, cos, sincospi and tanh
+// map to the hardware's single-precision instructions (v_exp_f32, v_log_f32,
+// v_sqrt_f32, v_sin_f32 / v_cos_f32 on revolutions, v_rcp_f32), about 1-2 ulp
+// and a few instructions each instead of dozens.  Results are then close to,
+// not equal to, the CPU oracle's.  fmod stays exact.
 #ifndef MPPI_HIP_MATH_H
 #define MPPI_HIP_MATH_H
 
@@ -44,6 +51,12 @@ typedef __SIZE_TYPE__ size_t;
 #include <stdint.h>
 #endif
 
+#if defined(__HIP_DEVICE_COMPILE__) && defined(MPPI_FAST_MATH) && MPPI_FAST_MATH
+#define MPPI_FAST 1
+#else
+#define MPPI_FAST 0
+#endif
+
 #if defined(__clang__)
 #define MPPI_UNROLL _Pragma("unroll")
 #else
@@ -54,7 +67,13 @@ MPPI_HD uint32_t mppi_f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
 MPPI_HD float mppi_u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
 
 MPPI_HD float mppi_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-MPPI_HD float mppi_sqrtf(float x) { return __builtin_sqrtf(x); }
+MPPI_HD float mppi_sqrtf(float x) {
+#if MPPI_FAST
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return __builtin_sqrtf(x);
+#endif
+}
 MPPI_HD float mppi_fabsf(float x) { return mppi_u2f(mppi_f2u(x) & 0x7fffffffu); }
 
 // Clamp with explicit comparisons (NaN passes through unchanged on both
@@ -77,6 +96,9 @@ MPPI_HD float mppi_pow2i(int k) { return mppi_u2f(uint32_t(k + 127) << 23); }
 // exp
 // ---------------------------------------------------------------------------
 MPPI_HD float mppi_expf(float x) {
+#if MPPI_FAST
+    return __builtin_amdgcn_exp2f(x * 1.44269504088896340736f);
+#endif
     if (!(x == x)) return x;                       // NaN
     if (x > 88.72283935546875f) return mppi_u2f(0x7f800000u);  // +inf
     if (x < -103.972084045410156f) return 0.0f;    // below the smallest subnormal
@@ -105,6 +127,9 @@ MPPI_HD float mppi_expf(float x) {
 // log
 // ---------------------------------------------------------------------------
 MPPI_HD float mppi_logf(float x) {
+#if MPPI_FAST
+    return __builtin_amdgcn_logf(x) * 0.693147180559945309417f;
+#endif
     if (!(x == x)) return x;
     if (x < 0.0f) return mppi_u2f(0x7fc00000u);    // NaN
     if (x == 0.0f) return mppi_u2f(0xff800000u);   // -inf
@@ -175,6 +200,17 @@ MPPI_HD void mppi_quadrant(int q, float s, float c, float* so, float* co) {
 // sin(x), cos(x).  Accurate for |x| <= 1e5 (the three-part pi/2 keeps the
 // reduced argument exact enough there); deterministic everywhere finite.
 MPPI_HD void mppi_sincosf(float x, float* s, float* c) {
+#if MPPI_FAST
+    {
+        // v_sin/v_cos take revolutions and are accurate for |r| < 256.
+        const float r = x * 0.159154943091895335769f;   // 1 / (2 pi)
+        if (mppi_fabsf(r) < 256.0f) {
+            *s = __builtin_amdgcn_sinf(r);
+            *c = __builtin_amdgcn_cosf(r);
+            return;
+        }
+    }
+#endif
     const float two_over_pi = 0.636619772367581343076f;
     const float pio2_1 = 1.57079625129699707031e+00f;  // 0x3fc90fda
     const float pio2_2 = 7.54978941586159635335e-08f;  // 0x33a22168
@@ -191,6 +227,13 @@ MPPI_HD float mppi_cosf(float x) { float s, c; mppi_sincosf(x, &s, &c); return c
 
 // sin(pi*x), cos(pi*x) for |x| < 2^21.
 MPPI_HD void mppi_sincospif(float x, float* s, float* c) {
+#if MPPI_FAST
+    if (mppi_fabsf(x) < 512.0f) {
+        *s = __builtin_amdgcn_sinf(x * 0.5f);            // exact scaling to revolutions
+        *c = __builtin_amdgcn_cosf(x * 0.5f);
+        return;
+    }
+#endif
     float n = mppi_rintf(x * 2.0f);                // exact scaling
     float f = x - n * 0.5f;                        // exact: f in [-1/4, 1/4]
     const float pi_f = 3.14159274101257324219f;
@@ -225,6 +268,12 @@ MPPI_HD float mppi_fmodf(float x, float y) {
 // tanh
 // ---------------------------------------------------------------------------
 MPPI_HD float mppi_tanhf(float x) {
+#if MPPI_FAST
+    {
+        const float t = x > 9.0f ? 9.0f : (x < -9.0f ? -9.0f : x);
+        return 1.0f - 2.0f * __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(t * 2.88539008177792681472f) + 1.0f);
+    }
+#endif
     float ax = mppi_fabsf(x);
     float r;
     if (ax < 0.55f) {

@@ -75,6 +75,23 @@ MPPI_HD float mppi_u01(uint32_t x) {
     return float((x >> 8) | 1u) * 5.9604644775390625e-8f;  // 2^-24
 }
 
+// x rounded to the nearest binary16 value (ties to even), as a float: the
+// FP16 numeric mode draws its control noise at half precision.  Integer and
+// exact float operations only, so the host oracle reproduces it.
+MPPI_HD float mppi_round_f16(float x) {
+    uint32_t u = mppi_f2u(x);
+    const uint32_t sign = u & 0x80000000u;
+    uint32_t a = u & 0x7fffffffu;
+    if (a > 0x7f800000u) return x;                       // NaN
+    if (a >= 0x477ff000u) return mppi_u2f(sign | 0x7f800000u);  // rounds past 65504
+    if (a < 0x38800000u) {                               // half subnormal: quantum 2^-24
+        const float q = mppi_rintf(mppi_u2f(a) * 16777216.0f) * 5.9604644775390625e-8f;
+        return mppi_u2f(sign | mppi_f2u(q));
+    }
+    a = (a + 0x0fffu + ((a >> 13) & 1u)) & 0xffffe000u;  // keep 10 fraction bits
+    return mppi_u2f(sign | a);
+}
+
 // Box-Muller: one pair of uniform words -> two standard normals.
 MPPI_HD void mppi_box_muller(uint32_t a, uint32_t b, float* z0, float* z1) {
     float u1 = mppi_u01(a);
@@ -92,6 +109,12 @@ MPPI_HD mppi_f32x4 mppi_normal4(mppi_u32x4 ctr, uint32_t k0, uint32_t k1) {
     mppi_f32x4 z;
     mppi_box_muller(w.x, w.y, &z.x, &z.y);
     mppi_box_muller(w.z, w.w, &z.z, &z.w);
+#if defined(MPPI_NOISE_F16) && MPPI_NOISE_F16
+    z.x = mppi_round_f16(z.x);
+    z.y = mppi_round_f16(z.y);
+    z.z = mppi_round_f16(z.z);
+    z.w = mppi_round_f16(z.w);
+#endif
     return z;
 }
 

@@ -119,7 +119,8 @@ SIGNATURES = [
     ("mppi_hip_version", ctypes.c_char_p, []),
     ("mppi_hip_compile_check", ctypes.c_int,
      [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
-      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_size_t]),
+      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_char_p,
+      ctypes.c_size_t]),
 ]
 
 _lib = None

@@ -78,17 +78,18 @@ def check_kernels(lib_path: Path):
     fn = lib.mppi_hip_compile_check
     fn.restype = ctypes.c_int
     fn.argtypes = [ctypes.c_char_p] * 5 + [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32,
-                                           ctypes.c_char_p, ctypes.c_size_t]
+                                           ctypes.c_uint32, ctypes.c_char_p, ctypes.c_size_t]
     log = ctypes.create_string_buffer(1 << 16)
     for plugin, dyn, stage, term, sdim, cdim in BUNDLED_PLUGINS:
         src = (PKG / "plugins" / plugin).read_bytes()
-        variants = [(0, 0), (1, 1)] + ([(0, 2)] if sdim + cdim <= 16 else [])
-        for cov, sets in variants:
+        # (covariance_full, numeric_mode, kernel sets)
+        variants = [(0, 0, 0), (1, 0, 1), (1, 2, 1), (0, 1, 0)] + ([(0, 0, 2)] if sdim + cdim <= 16 else [])
+        for cov, mode, sets in variants:
             rc = fn(src, str(PKG / "plugins").encode(), dyn.encode(), stage.encode(), term.encode(),
-                    sdim, cdim, cov, sets, log, len(log))
+                    sdim, cdim, cov, mode, sets, log, len(log))
             if rc != 0:
-                raise RuntimeError(f"{plugin} (covariance_full={cov}, kernel_sets={sets}) failed to "
-                                   f"compile:\n{log.value.decode(errors='replace')}")
+                raise RuntimeError(f"{plugin} (covariance_full={cov}, numeric_mode={mode}, kernel_sets={sets}) "
+                                   f"failed to compile:\n{log.value.decode(errors='replace')}")
 
 
 def build_library() -> Path:

@@ -56,7 +56,8 @@ class HipMPPI:
     """One MPPI controller context on one GPU."""
 
     def __init__(self, state_dim, control_dim, horizon, sample_count, lam, sigma, u_min, u_max,
-                 covariance_full=False, device=-1):
+                 covariance_full=False, device=-1, numeric_mode=0):
+        """numeric_mode: _native.FP32 (0), FP16 (1) or MIXED (2)."""
         self.lib = _native.load()
         self.D, self.C, self.H, self.K = int(state_dim), int(control_dim), int(horizon), int(sample_count)
         self.lam = float(lam)
@@ -67,7 +68,7 @@ class HipMPPI:
         cfg = _native.MppiHipConfig(self.D, self.C, self.H, self.K, self.lam,
                                     _native.COV_FULL if covariance_full else _native.COV_DIAGONAL,
                                     _fptr(self._sigma), _fptr(self._umin), _fptr(self._umax),
-                                    _native.FP32, int(device))
+                                    int(numeric_mode), int(device))
         ctx = ctypes.c_void_p()
         rc = self.lib.mppi_hip_create(ctypes.byref(cfg), ctypes.byref(ctx))
         if rc != 0:

@@ -88,8 +88,11 @@ int build_layout(mppi_hip_batch* b) {
         }
         d.spt = spt;
         d.n_chunks = (d.K + B * spt - 1) / (B * spt);
+        d.slot = a;
         d.cost_off = cost_off;
+        d.w_off = cost_off;
         d.seq_off = seq_off;
+        d.out_off = seq_off;
         d.blk_off = blk_off;
         d.part_off = part_off;
         cost_off += d.K;

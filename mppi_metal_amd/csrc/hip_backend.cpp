@@ -184,15 +184,20 @@ bool cholesky(const std::vector<float>& a, uint32_t n, std::vector<float>* l) {
 }
 
 // hipRTC options of the library kernels for one plugin configuration.
+// FP32 keeps every operation exactly rounded as written (bit-exact with the
+// CPU oracle); MIXED and FP16 compile the fast hardware transcendentals
+// (MPPI_FAST_MATH), fused multiply-adds and approximate division; FP16 also
+// rounds the control noise to binary16 (MPPI_NOISE_F16).
 std::vector<std::string> jit_options(const std::string& arch, uint32_t D, uint32_t C, bool cov_full,
                                      bool batch, bool surrogate, const std::string& dyn,
                                      const std::string& stage, const std::string& term,
-                                     const std::string& include_dir) {
+                                     const std::string& include_dir, uint32_t numeric_mode) {
+    const bool fast = numeric_mode != MPPI_HIP_FP32;
     std::vector<std::string> opts = {
         "--offload-arch=" + arch,
         "-O3",
         "-std=c++17",
-        "-ffp-contract=off",
+        fast ? "-ffp-contract=fast" : "-ffp-contract=off",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-DMPPI_SDIM=" + std::to_string(D),
         "-DMPPI_CDIM=" + std::to_string(C),
@@ -204,6 +209,11 @@ std::vector<std::string> jit_options(const std::string& arch, uint32_t D, uint32
     if (!stage.empty()) opts.push_back("-DMPPI_STAGE_COST=" + stage);
     if (!term.empty()) opts.push_back("-DMPPI_TERMINAL_COST=" + term);
     if (!include_dir.empty()) opts.push_back("-I" + include_dir);
+    if (fast) {
+        opts.push_back("-fno-hip-fp32-correctly-rounded-divide-sqrt");
+        opts.push_back("-DMPPI_FAST_MATH=1");
+    }
+    if (numeric_mode == MPPI_HIP_FP16) opts.push_back("-DMPPI_NOISE_F16=1");
     return opts;
 }
 
@@ -246,7 +256,7 @@ bool jit_compile(const std::string& plugin_src, const std::vector<std::string>& 
 int load_module(mppi_hip_ctx* ctx, bool batch, bool surrogate, std::shared_ptr<LoadedModule>* out) {
     const std::vector<std::string> opts =
         jit_options(ctx->arch, ctx->D, ctx->C, ctx->cov_form == MPPI_HIP_COV_FULL, batch, surrogate,
-                    ctx->dyn_name, ctx->stage_name, ctx->term_name, ctx->plugin_dir);
+                    ctx->dyn_name, ctx->stage_name, ctx->term_name, ctx->plugin_dir, ctx->numeric_mode);
 
     std::string keysrc = std::string(kKernelSource) + "\x01" + kLaunchHeader + "\x01" +
                          kCoreHeader + "\x01" + kMathHeader + "\x01" + kPhiloxHeader + "\x01" +
@@ -395,8 +405,11 @@ std::vector<mppi_agent_desc> uniform_descs(const mppi_hip_ctx* ctx, uint32_t N) 
     d.n_chunks = (ctx->K + kBlock * d.spt - 1) / (kBlock * d.spt);
     std::vector<mppi_agent_desc> v(N, d);
     for (uint32_t a = 0; a < N; ++a) {
+        v[a].slot = a;
         v[a].cost_off = uint64_t(a) * ctx->K;
+        v[a].w_off = v[a].cost_off;
         v[a].seq_off = uint64_t(a) * ctx->H * ctx->C;
+        v[a].out_off = v[a].seq_off;
         v[a].blk_off = uint64_t(a) * d.n_blk;
         v[a].part_off = uint64_t(a) * d.n_chunks * d.NB * 4;
     }
@@ -609,8 +622,8 @@ const char* mppi_hip_version(void) { return "mppi_hip 0.2.0"; }
 
 int mppi_hip_compile_check(const char* plugin_source, const char* include_dir, const char* dynamics_fn,
                            const char* stage_cost_fn, const char* terminal_cost_fn, uint32_t state_dim,
-                           uint32_t control_dim, int covariance_full, uint32_t kernel_sets, char* log,
-                           size_t log_capacity) {
+                           uint32_t control_dim, int covariance_full, uint32_t numeric_mode,
+                           uint32_t kernel_sets, char* log, size_t log_capacity) {
     if (log && log_capacity) log[0] = '\0';
     auto put_log = [&](const std::string& s) {
         if (log && log_capacity) {
@@ -619,14 +632,17 @@ int mppi_hip_compile_check(const char* plugin_source, const char* include_dir, c
             log[n] = '\0';
         }
     };
-    if (!plugin_source || !dynamics_fn || !*dynamics_fn || state_dim == 0 || control_dim == 0) {
-        put_log("plugin_source and dynamics_fn are required; dimensions must be > 0.");
+    if (!plugin_source || !dynamics_fn || !*dynamics_fn || state_dim == 0 || control_dim == 0 ||
+        numeric_mode > MPPI_HIP_MIXED) {
+        put_log("plugin_source and dynamics_fn are required; dimensions must be > 0; numeric_mode "
+                "must be FP32, FP16 or MIXED.");
         return MPPI_HIP_E_INVALID;
     }
     const std::vector<std::string> opts =
         jit_options("gfx950", state_dim, control_dim, covariance_full != 0, (kernel_sets & 1u) != 0,
                     (kernel_sets & 2u) != 0, dynamics_fn, stage_cost_fn ? stage_cost_fn : "",
-                    terminal_cost_fn ? terminal_cost_fn : "", include_dir ? include_dir : "");
+                    terminal_cost_fn ? terminal_cost_fn : "", include_dir ? include_dir : "",
+                    numeric_mode);
     std::vector<char> code;
     std::string clog;
     if (!jit_compile(plugin_source, opts, &code, &clog)) {
@@ -658,9 +674,9 @@ int mppi_hip_create(const mppi_hip_config* cfg, mppi_hip_ctx** out) {
     if (!cfg->u_max) return fail(nullptr, MPPI_HIP_E_INVALID, "u_max must not be null.");
     if (cfg->covariance_form != MPPI_HIP_COV_DIAGONAL && cfg->covariance_form != MPPI_HIP_COV_FULL)
         return fail(nullptr, MPPI_HIP_E_INVALID, "covariance_form is not DIAGONAL or FULL.");
-    if (cfg->numeric_mode != MPPI_HIP_FP32)
-        return fail(nullptr, MPPI_HIP_E_INVALID,
-                    "numeric_mode: only FP32 is implemented by this backend.");
+    if (cfg->numeric_mode != MPPI_HIP_FP32 && cfg->numeric_mode != MPPI_HIP_FP16 &&
+        cfg->numeric_mode != MPPI_HIP_MIXED)
+        return fail(nullptr, MPPI_HIP_E_INVALID, "numeric_mode is not FP32, FP16 or MIXED.");
     for (uint32_t i = 0; i < cfg->control_dim; ++i)
         if (cfg->u_min[i] > cfg->u_max[i])
             return fail(nullptr, MPPI_HIP_E_INVALID, "u_min[" + std::to_string(i) + "] > u_max.");
@@ -909,7 +925,8 @@ int read_per_sample(mppi_hip_ctx* ctx, const float* base, uint32_t agent, float*
     const mppi_agent_desc& d = ctx->last_desc[agent];
     if (!out || count != d.K)
         return fail(ctx, MPPI_HIP_E_INVALID, std::string(what) + ": out must hold the agent's sample count.");
-    MPPI_HIP_TRY(ctx, hipMemcpyAsync(out, base + d.cost_off, sizeof(float) * count,
+    const uint64_t off = std::strcmp(what, "read_weights") == 0 ? d.w_off : d.cost_off;
+    MPPI_HIP_TRY(ctx, hipMemcpyAsync(out, base + off, sizeof(float) * count,
                                      hipMemcpyDeviceToHost, ctx->stream));
     MPPI_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return MPPI_HIP_OK;

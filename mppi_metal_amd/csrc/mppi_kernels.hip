@@ -179,7 +179,7 @@ __device__ void mppi_combine_stats(const mppi_step_args& a, uint32_t agent,
             S = T(d.K);
             Q = T(d.K);
         }
-        mppi_agent_state* ag = &a.agents[agent];
+        mppi_agent_state* ag = &a.agents[d.slot];
         ag->min_cost = mg;
         ag->weight_sum = double(S);
         ag->mean_cost = double(Cs / T(d.K));
@@ -218,7 +218,7 @@ __device__ void mppi_block_stats(const mppi_step_args& a, uint32_t agent, const 
         float* bs = a.blk_stats + (d.blk_off + blk) * 4;
         bs[0] = m; bs[1] = S; bs[2] = Cs; bs[3] = Q;
         __threadfence();
-        uint32_t prev = atomicAdd(&a.agents[agent].rollout_arrivals, 1u);
+        uint32_t prev = atomicAdd(&a.agents[d.slot].rollout_arrivals, 1u);
         s_last = (prev == d.n_blk - 1);
     }
     __syncthreads();
@@ -238,11 +238,12 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
     const uint32_t k = blk * BLOCK + threadIdx.x;
     const uint32_t H = d.H;
     const bool active = k < d.K;
-    const uint32_t step = a.agents[agent].step;
-    const uint32_t tag_agent = (MPPI_TAG_CONTROL_NOISE << 28) | (agent & MPPI_AGENT_MASK);
-    const uint32_t dyn_tag = (MPPI_TAG_DYNAMICS << 28) | (agent & MPPI_AGENT_MASK);
-    const uint8_t* model = a.model + agent * a.model_stride;
-    const uint8_t* cost = a.cost + agent * a.cost_stride;
+    const uint32_t slot = d.slot;
+    const uint32_t step = a.agents[slot].step;
+    const uint32_t tag_agent = (MPPI_TAG_CONTROL_NOISE << 28) | (slot & MPPI_AGENT_MASK);
+    const uint32_t dyn_tag = (MPPI_TAG_DYNAMICS << 28) | (slot & MPPI_AGENT_MASK);
+    const uint8_t* model = a.model + slot * a.model_stride;
+    const uint8_t* cost = a.cost + slot * a.cost_stride;
 
     // The nominal sequence goes to LDS and the noise scale and bounds to
     // registers, so the time loop issues no memory waits of its own.
@@ -262,7 +263,7 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
     if (active) {
         float x[SDIM];
 #pragma unroll
-        for (int i = 0; i < SDIM; ++i) x[i] = a.x[size_t(agent) * SDIM + i];
+        for (int i = 0; i < SDIM; ++i) x[i] = a.x[size_t(slot) * SDIM + i];
         for (uint32_t t0 = 0; t0 < H; t0 += MPPI_G) {
             float z[MPPI_NG];
             const uint32_t b0 = (t0 * CDIM) >> 2;
@@ -340,12 +341,13 @@ template <int BLOCK>
 __device__ void mppi_finalize(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
                               float* zs) {
     const uint32_t H = d.H, HC = H * CDIM;
-    mppi_agent_state* ag = &a.agents[agent];
+    const uint32_t slot = d.slot;
+    mppi_agent_state* ag = &a.agents[slot];
     const uint32_t step = ag->step;
     if (d.K >= MPPI_FP64_MIN_K) mppi_sum_partials<BLOCK, double>(a, d, ag->weight_sum, zs);
     else mppi_sum_partials<BLOCK, float>(a, d, ag->weight_sum, zs);
     __syncthreads();
-    float* uo = a.u_out + d.seq_off;
+    float* uo = a.u_out + d.out_off;
     float* un = a.u_nom + d.seq_off;
     for (uint32_t n = threadIdx.x; n < HC; n += BLOCK) {
         const uint32_t t = n / CDIM, i = n % CDIM;
@@ -364,10 +366,11 @@ __device__ void mppi_finalize(const mppi_step_args& a, uint32_t agent, const mpp
         a.diag[agent * 3 + 1] = float(ag->mean_cost);
         a.diag[agent * 3 + 2] = float(ag->ess);
     }
+    if (a.first_out && threadIdx.x < CDIM) a.first_out[agent * CDIM + threadIdx.x] = uo[threadIdx.x];
     if (a.closed_loop) {
         if (threadIdx.x == 0) {
             const uint32_t row = ag->hist;
-            float* xs = a.x + size_t(agent) * SDIM;
+            float* xs = a.x + size_t(slot) * SDIM;
             float x[SDIM];
             float u[CDIM];
 #pragma unroll
@@ -382,8 +385,8 @@ __device__ void mppi_finalize(const mppi_step_args& a, uint32_t agent, const mpp
                     a.hist_controls[(size_t(row) * a.N + agent) * CDIM + i] = u[i];
             if (a.hist_costs) a.hist_costs[size_t(row) * a.N + agent] = float(ag->min_cost);
             mppi_rng rng{0u, 0xffffffffu, step,
-                         (MPPI_TAG_PLANT << 28) | (agent & MPPI_AGENT_MASK), a.key0, a.key1};
-            MPPI_DYNAMICS(mppi_state_ref{x}, u, mppi_blob_ref{a.model + agent * a.model_stride}, rng);
+                         (MPPI_TAG_PLANT << 28) | (slot & MPPI_AGENT_MASK), a.key0, a.key1};
+            MPPI_DYNAMICS(mppi_state_ref{x}, u, mppi_blob_ref{a.model + slot * a.model_stride}, rng);
 #pragma unroll
             for (int i = 0; i < SDIM; ++i) xs[i] = x[i];
             if (a.terminal)
@@ -410,12 +413,12 @@ __device__ void mppi_reduce_tile(const mppi_step_args& a, uint32_t agent, uint32
                                  uint32_t grp) {
     const mppi_agent_desc d = a.desc[agent];
     const uint32_t K = d.K, NB = d.NB;
-    const mppi_agent_state* ag = &a.agents[agent];
+    const mppi_agent_state* ag = &a.agents[d.slot];
     const uint32_t step = ag->step;
     const float m = float(ag->min_cost);
     const float Sf = float(ag->weight_sum);
     const bool uniform = !mppi_isfinite(m);
-    const uint32_t tag_agent = (MPPI_TAG_CONTROL_NOISE << 28) | (agent & MPPI_AGENT_MASK);
+    const uint32_t tag_agent = (MPPI_TAG_CONTROL_NOISE << 28) | (d.slot & MPPI_AGENT_MASK);
     const uint32_t b_begin = grp * MPPI_REDUCE_BPG;
     const float* costs = a.costs + d.cost_off;
 
@@ -433,7 +436,7 @@ __device__ void mppi_reduce_tile(const mppi_step_args& a, uint32_t agent, uint32
         float w;
         if (uniform) w = 1.0f;
         else w = mppi_isfinite(c) ? mppi_expf(-(c - m) / a.lambda) : 0.0f;
-        if (grp == 0 && a.weights) a.weights[d.cost_off + k] = w / Sf;
+        if (grp == 0 && a.weights) a.weights[d.w_off + k] = w / Sf;
 #pragma unroll
         for (int q = 0; q < MPPI_REDUCE_BPG; ++q) {
             if (b_begin + q < NB) {
@@ -477,7 +480,7 @@ __device__ void mppi_reduce_tile(const mppi_step_args& a, uint32_t agent, uint32
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
-        uint32_t prev = atomicAdd(&a.agents[agent].reduce_arrivals, 1u);
+        uint32_t prev = atomicAdd(&a.agents[d.slot].reduce_arrivals, 1u);
         s_last = (prev == d.n_chunks * d.n_groups - 1);
     }
     __syncthreads();
@@ -616,15 +619,16 @@ extern "C" __global__ void __launch_bounds__(256) mppi_rollout_surrogate(mppi_su
     const uint32_t k = blk * 256 + threadIdx.x;
     const uint32_t H = d.H;
     const bool active = k < d.K;
-    const uint32_t step = a.agents[agent].step;
-    const uint32_t tag_agent = (MPPI_TAG_CONTROL_NOISE << 28) | (agent & MPPI_AGENT_MASK);
+    const uint32_t slot = d.slot;
+    const uint32_t step = a.agents[slot].step;
+    const uint32_t tag_agent = (MPPI_TAG_CONTROL_NOISE << 28) | (slot & MPPI_AGENT_MASK);
     const float* __restrict__ un = a.u_nom + d.seq_off;
-    const uint8_t* model = a.model + agent * a.model_stride;
-    const uint8_t* cost = a.cost + agent * a.cost_stride;
+    const uint8_t* model = a.model + slot * a.model_stride;
+    const uint8_t* cost = a.cost + slot * a.cost_stride;
 
     float x[SDIM];
 #pragma unroll
-    for (int i = 0; i < SDIM; ++i) x[i] = a.x[size_t(agent) * SDIM + i];
+    for (int i = 0; i < SDIM; ++i) x[i] = a.x[size_t(slot) * SDIM + i];
     float total = 0.0f;
     for (uint32_t t0 = 0; t0 < H; t0 += MPPI_G) {
         float z[MPPI_NG];

@@ -22,7 +22,11 @@ struct mppi_agent_state {
 };
 
 // Per-agent shape and buffer offsets (immutable for a batch shape).  All
-// offsets are in elements of the buffer they index.
+// offsets are in elements of the buffer they index.  An agent has two
+// indices: its position in the launch (the index of its descriptor; per-call
+// outputs diag, histories, terminal and first_out are indexed by it) and its
+// slot (its persistent data: state x, mppi_agent_state, parameter blobs, and
+// the noise stream's agent word).  The one-call paths use slot = position.
 struct mppi_agent_desc {
     uint32_t K;                  // samples
     uint32_t H;                  // horizon
@@ -31,11 +35,13 @@ struct mppi_agent_desc {
     uint32_t n_chunks;           // reduce sample chunks: ceil(K / (block * spt))
     uint32_t n_groups;           // reduce noise-block groups: ceil(NB / 8)
     uint32_t spt;                // reduce samples per thread
-    uint32_t pad;
-    uint64_t cost_off;           // costs / weights
-    uint64_t seq_off;            // u_nom / u_out (floats, H*C per agent)
+    uint32_t slot;               // persistent agent index (see above)
+    uint64_t cost_off;           // costs (floats)
+    uint64_t seq_off;            // u_nom (floats, H*C per agent)
     uint64_t blk_off;            // blk_stats (4 floats per rollout workgroup)
     uint64_t part_off;           // partials (floats)
+    uint64_t w_off;              // weights (floats)
+    uint64_t out_off;            // u_out (floats, H*C per agent)
 };
 
 struct mppi_step_args {
@@ -64,6 +70,7 @@ struct mppi_step_args {
     float* hist_controls;        // [T][N][C] or null
     float* hist_costs;           // [T][N] or null
     float* terminal;             // [N][D] or null
+    float* first_out;            // [N][C] first control of u_out, or null
     uint32_t closed_loop;        // 1: plant step + warm-start shift after the reduce
     uint32_t key0, key1;
     float lambda;

@@ -224,3 +224,81 @@ def pursuit_realised_cost(scene: PursuitScene, agent: int, next_states, controls
         d = np.hypot(x[:, 0] - cx, x[:, 1] - cy)
         cost += w_zone * np.where(d < r, r - d, 0.0)
     return cost
+
+
+# ---------------------------------------------------------------------------
+# Quadrotor serving workload (plugins/quadrotor.hip)
+# ---------------------------------------------------------------------------
+@dataclass
+class QuadParams:
+    gravity: float = 9.81
+    drag: float = 0.1
+    tau: float = 0.05
+    dt: float = 0.02
+    substeps: int = 2
+
+    def blob(self) -> bytes:
+        return struct.pack("<4fI3f", self.gravity, self.drag, self.tau, self.dt, self.substeps,
+                           0.0, 0.0, 0.0)
+
+
+QUAD_MODEL_BYTES = 32
+QUAD_COST_BYTES = 48
+QUAD_U_MIN = (0.0, -3.0, -3.0, -3.0)      # thrust [m/s^2], body rates [rad/s]
+QUAD_U_MAX = (20.0, 3.0, 3.0, 3.0)
+QUAD_LAMBDA = 10.0
+
+
+def quad_cost_blob(goal, w_pos=4.0, w_vel=0.5, w_att=1.0, w_thrust=0.1, w_rate=0.05,
+                   w_terminal=20.0) -> bytes:
+    return struct.pack("<3f6f3f", goal[0], goal[1], goal[2], w_pos, w_vel, w_att, w_thrust, w_rate,
+                       w_terminal, 0.0, 0.0, 0.0)
+
+
+def quadrotor_covariance(seed: int = 42, C: int = 4, eig=(0.1, 1.0)) -> np.ndarray:
+    """Random SPD control covariance Q diag(lambda) Q^T: lambda ~ U[eig],
+    Q orthogonal from the QR factorisation of a Gaussian matrix (scene
+    stream of `seed`, agent word 1)."""
+    s = SceneStream(seed, agent=1)
+    g = np.array([[np.sqrt(-2 * np.log(s.uniform())) * np.cos(2 * np.pi * s.uniform()) for _ in range(C)]
+                  for _ in range(C)])
+    q, r = np.linalg.qr(g)
+    q = q * np.sign(np.diag(r))
+    lam = np.array([s.uniform(*eig) for _ in range(C)])
+    cov = (q * lam) @ q.T
+    return ((cov + cov.T) / 2).astype(np.float32)
+
+
+@dataclass
+class QuadRequests:
+    arrival_s: np.ndarray     # [n] arrival times (seconds from the start)
+    K: np.ndarray             # [n]
+    H: np.ndarray             # [n]
+    x0: np.ndarray            # [n, 12]
+    goals: np.ndarray         # [n, 3]
+
+    def cost_blob(self, i: int) -> bytes:
+        return quad_cost_blob(self.goals[i])
+
+
+def quadrotor_requests(n: int = 1024, seed: int = 42, rate_hz: float = 2000.0, k_log2=(12, 15),
+                       h_range=(30, 80)) -> QuadRequests:
+    """n agent requests arriving as a Poisson process of `rate_hz`: K =
+    round(2^U[12, 15]) (4096..32768), H uniform in [30, 80], start near hover
+    (position U[-5, 5]^3, velocity U[-1, 1]^3, attitude U[-0.2, 0.2]^3, zero
+    rates), goal U[-5, 5]^3."""
+    s = SceneStream(seed, agent=2)
+    t, arr = 0.0, np.zeros(n)
+    K, H = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+    x0 = np.zeros((n, 12), np.float32)
+    goals = np.zeros((n, 3), np.float32)
+    for i in range(n):
+        t += -np.log(s.uniform()) / rate_hz
+        arr[i] = t
+        K[i] = int(round(2.0 ** s.uniform(*k_log2)))
+        H[i] = int(h_range[0] + min(h_range[1] - h_range[0], int(s.uniform() * (h_range[1] - h_range[0] + 1))))
+        x0[i, 0:3] = [s.uniform(-5, 5) for _ in range(3)]
+        x0[i, 3:6] = [s.uniform(-1, 1) for _ in range(3)]
+        x0[i, 6:9] = [s.uniform(-0.2, 0.2) for _ in range(3)]
+        goals[i] = [s.uniform(-5, 5) for _ in range(3)]
+    return QuadRequests(arr, K, H, x0, goals)

@@ -98,6 +98,20 @@ void normals4(uint32_t block, uint32_t sample, uint32_t step, uint32_t tag, uint
 
 template <typename T> T clampv(T x, T lo, T hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
+// x rounded to the nearest binary16 value, ties to even (IEEE 754 binary16:
+// 11 significant bits, quantum 2^-24 below 2^-14, overflow at 65520), by
+// scaling to an integer multiple of the quantum; independent of the
+// library's bit-level mppi_round_f16.
+double round_f16(double x) {
+    if (!std::isfinite(x) || x == 0.0) return x;
+    const double ax = std::fabs(x);
+    if (ax >= 65520.0) return std::copysign(INFINITY, x);
+    int e;
+    std::frexp(ax, &e);                         // ax in [2^(e-1), 2^e)
+    const double quantum = std::ldexp(1.0, std::max(e - 11, -24));
+    return std::copysign(std::nearbyint(ax / quantum) * quantum, x);
+}
+
 // ---------------------------------------------------------------------------
 // Models.  FP32: the plugin sources.  FP64: hand restatements of the same
 // formulas.
@@ -378,6 +392,7 @@ struct oracle_problem {
     const char* terminal_cost;
     const void* model_params;
     const void* cost_params;
+    uint32_t noise_f16;       // 1: normals rounded to binary16 (the FP16 numeric mode)
 };
 
 // mlp_oracle.cpp
@@ -480,7 +495,7 @@ void sample_normals(const oracle_problem* p, uint32_t k, uint32_t step, uint32_t
     T blk[4];
     for (uint32_t n = 0; n < HC; ++n) {
         if ((n & 3) == 0) normals4<T>(n >> 2, k, step, MPPI_TAG_CONTROL_NOISE, agent, p->seed, blk);
-        z[n] = blk[n & 3];
+        z[n] = p->noise_f16 ? T(round_f16(double(blk[n & 3]))) : blk[n & 3];
     }
 }
 
@@ -754,6 +769,15 @@ int oracle_stage_costs(const oracle_problem* p, uint32_t n, const float* states,
         out[i] = r.stage(states + size_t(i) * p->D, controls + size_t(i) * p->C, r.model, r.cost,
                          mppi_time{0u, steps[i]});
     return 0;
+}
+
+// The binary16 rounding of the FP16 mode: this oracle's restatement and the
+// library header's (include/mppi_hip/mppi_philox.h) compiled for the host.
+void oracle_round_f16(const float* x, float* out, size_t n) {
+    for (size_t i = 0; i < n; ++i) out[i] = float(round_f16(double(x[i])));
+}
+void oracle_library_round_f16(const float* x, float* out, size_t n) {
+    for (size_t i = 0; i < n; ++i) out[i] = mppi_round_f16(x[i]);
 }
 
 // One step x <- x + delta(x, u) of the surrogate network.

@@ -26,6 +26,7 @@ class OracleProblem(ctypes.Structure):
         ("model", ctypes.c_char_p), ("dynamics", ctypes.c_char_p),
         ("stage_cost", ctypes.c_char_p), ("terminal_cost", ctypes.c_char_p),
         ("model_params", ctypes.c_void_p), ("cost_params", ctypes.c_void_p),
+        ("noise_f16", ctypes.c_uint32),
     ]
 
 
@@ -83,7 +84,7 @@ class Problem:
 
     def __init__(self, model, D, C, H, K, lam, sigma, u_min, u_max, seed, dynamics="mppi_dynamics",
                  stage_cost="", terminal_cost="", model_params=None, cost_params=None,
-                 cov_full=False):
+                 cov_full=False, noise_f16=False):
         self.D, self.C, self.H, self.K = D, C, H, K
         self.sigma = np.ascontiguousarray(sigma, np.float32)
         self.u_min = np.ascontiguousarray(u_min, np.float32)
@@ -95,7 +96,7 @@ class Problem:
             D, C, H, K, lam, 1 if cov_full else 0, _p(self.sigma), _p(self.u_min), _p(self.u_max),
             seed, self._names[0], self._names[1], self._names[2], self._names[3],
             ctypes.cast(self._mp, ctypes.c_void_p) if self._mp else None,
-            ctypes.cast(self._cp, ctypes.c_void_p) if self._cp else None)
+            ctypes.cast(self._cp, ctypes.c_void_p) if self._cp else None, 1 if noise_f16 else 0)
 
     @property
     def ref(self):

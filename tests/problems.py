@@ -63,6 +63,22 @@ def bicycle(K, H, seed=SEED, agent=0, substeps=2, lam=None):
 BICYCLE_ENVELOPE = scenes.BICYCLE_SURROGATE_ENVELOPE
 
 
+def quadrotor(K, H, numeric_mode=0, seed=SEED, goal=(2.0, -1.0, 1.5)):
+    """Quadrotor with the random SPD 4x4 control covariance (full-covariance
+    noise), hovering start; the oracle rounds its noise to binary16 when the
+    GPU does (FP16 mode)."""
+    cov = scenes.quadrotor_covariance(seed)
+    mb, cb = scenes.QuadParams().blob(), scenes.quad_cost_blob(goal)
+    hip = HipMPPI(12, 4, H, K, scenes.QUAD_LAMBDA, cov.ravel(), scenes.QUAD_U_MIN, scenes.QUAD_U_MAX,
+                  covariance_full=True, numeric_mode=numeric_mode)
+    hip.link("quadrotor.hip", "quad_dynamics", "quad_stage_cost", "quad_terminal_cost", len(mb), len(cb), seed)
+    hip.bind_params(mb, cb)
+    ora = ol.Problem("quadrotor", 12, 4, H, K, scenes.QUAD_LAMBDA, cov.ravel(), scenes.QUAD_U_MIN,
+                     scenes.QUAD_U_MAX, seed, "quad_dynamics", "quad_stage_cost", "quad_terminal_cost", mb, cb,
+                     cov_full=True, noise_f16=numeric_mode == 1)
+    return hip, ora
+
+
 def rel_err(a, b, floor=0.0):
     """max |a - b| / max(|b|, floor), elementwise."""
     a = np.asarray(a, np.float64)

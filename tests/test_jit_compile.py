@@ -10,11 +10,11 @@ from mppi_metal_amd.build import BUNDLED_PLUGINS
 from mppi_metal_amd.controller import PLUGIN_DIR
 
 
-def _check(src, dyn, stage, term, D, C, cov=0, sets=0):
+def _check(src, dyn, stage, term, D, C, cov=0, sets=0, mode=0):
     lib = _native.load()
     log = ctypes.create_string_buffer(1 << 16)
     rc = lib.mppi_hip_compile_check(src, str(PLUGIN_DIR).encode(), dyn.encode(), stage.encode(), term.encode(),
-                                    D, C, cov, sets, log, len(log))
+                                    D, C, cov, mode, sets, log, len(log))
     return rc, log.value.decode(errors="replace")
 
 
@@ -23,6 +23,13 @@ def _check(src, dyn, stage, term, D, C, cov=0, sets=0):
 def test_bundled_plugin_compiles_with_surrogate_kernels(plugin):
     name, dyn, stage, term, D, C = plugin
     rc, log = _check((PLUGIN_DIR / name).read_bytes(), dyn, stage, term, D, C, sets=2)
+    assert rc == 0, log
+
+
+@pytest.mark.parametrize("mode", [1, 2], ids=["fp16", "mixed"])
+def test_quadrotor_compiles_in_fast_modes(mode):
+    name, dyn, stage, term, D, C = [p for p in BUNDLED_PLUGINS if p[0] == "quadrotor.hip"][0]
+    rc, log = _check((PLUGIN_DIR / name).read_bytes(), dyn, stage, term, D, C, cov=1, sets=1, mode=mode)
     assert rc == 0, log
 
 

@@ -72,3 +72,26 @@ def test_normals_fp32_vs_fp64():
         z64 = ol.normals4(b, 7, 3, 0, 5, 2**40 + 17, fp64=True)
         err = max(err, float(np.max(np.abs(z32 - z64))))
     assert err < 4e-6, err
+
+
+def test_binary16_rounding_matches_numpy():
+    """The FP16 mode's noise rounding: the library's bit-level
+    mppi_round_f16 and the oracle's scaling restatement both equal numpy's
+    float32 -> float16 conversion (IEEE round to nearest even), including
+    ties, half subnormals and overflow."""
+    import ctypes
+    rng = np.random.default_rng(0)
+    x = np.concatenate([
+        rng.standard_normal(200000).astype(np.float32) * 3,
+        rng.uniform(-1e-4, 1e-4, 20000).astype(np.float32),          # half subnormals
+        np.float32(2.0) ** rng.integers(-30, 17, 5000).astype(np.float32) * rng.uniform(1, 2, 5000).astype(np.float32),
+        # exact ties between two binary16 neighbours
+        (np.arange(1, 2049, dtype=np.float32) + np.float32(0.5)) * np.float32(2.0 ** -10),
+        np.array([0.0, -0.0, 65504.0, 65519.0, 65520.0, -70000.0, 5.96e-8, 2.98e-8, 2.99e-8], np.float32),
+    ]).astype(np.float32)
+    want = x.astype(np.float16).astype(np.float32)
+    fp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    for fn in ("oracle_round_f16", "oracle_library_round_f16"):
+        got = np.empty_like(x)
+        getattr(ol.lib(), fn)(fp(x), fp(got), ctypes.c_size_t(x.size))
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), fn
