@@ -28,6 +28,16 @@ Two more workloads are reported in the same line (skip with --no-batch,
                    the reference's single-agent driver would) and against a
                    uniform batch padded to the largest K and H (the
                    reference's batch form needs one K and H for all agents).
+  serving          quadrotor (state 12, control 4, full 4x4 SPD control
+                   covariance), 1024 agent requests arriving as a Poisson
+                   process at 2000 req/s, K log-uniform 4096..32768, H
+                   uniform 30..80; the library's serving loop admits arrived
+                   requests in order against an HBM budget and solves every
+                   admitted agent in one batched closed-loop step (state and
+                   warm start on the device); p50/p99 latency from arrival to
+                   control, per numeric mode (MIXED is the workload's mode),
+                   and a burst of all 1024 requests at once for throughput
+                   under a 32 MB budget (admission in waves).
   surrogate        the learned bicycle dynamics (MLP 8-128-128-6, trained on
                    the GPU on 4M plugin transitions) inside the rollout at
                    K = 262144, H = 64, 500 closed-loop steps, against plugin
@@ -163,6 +173,52 @@ def bench_batch(args):
     }
 
 
+def bench_serving(args):
+    """Quadrotor serving loop: Poisson arrivals and a burst."""
+    from mppi_metal_amd import HipMPPI, HipServer, scenes
+    rq = scenes.quadrotor_requests(args.requests, seed=42, rate_hz=2000.0)
+    cov = scenes.quadrotor_covariance(42)
+    mb = scenes.QuadParams().blob()
+    blobs = [rq.cost_blob(i) for i in range(len(rq.K))]
+    hover = []
+    for h in rq.H:
+        u = np.zeros((int(h), 4), np.float32)
+        u[:, 0] = 9.81
+        hover.append(u)
+    work = float(np.sum(rq.K.astype(np.float64) * rq.H))
+    out = {"workload": "quadrotor serving, full 4x4 covariance, Poisson 2000 req/s, K 2^U[12,15], H U[30,80]",
+           "requests": len(rq.K), "sample_steps": work, "modes": {}}
+    for name, mode in (("mixed", 2), ("fp16", 1), ("fp32", 0)):
+        hip = HipMPPI(12, 4, 8, 256, scenes.QUAD_LAMBDA, cov.ravel(), scenes.QUAD_U_MIN, scenes.QUAD_U_MAX,
+                      covariance_full=True, numeric_mode=mode)
+        hip.link("quadrotor.hip", "quad_dynamics", "quad_stage_cost", "quad_terminal_cost", len(mb),
+                 scenes.QUAD_COST_BYTES, 42)
+        hip.bind_params(mb, None)
+        srv = HipServer(hip, 1024, 32 << 20)
+        srv.replay(rq.arrival_s[:32] - rq.arrival_s[0], rq.K[:32], rq.H[:32], rq.x0[:32], blobs[:32],
+                   u_nominal=hover[:32])                                   # warm-up
+        lat, st = srv.replay(rq.arrival_s, rq.K, rq.H, rq.x0, blobs, u_nominal=hover)
+        zero = np.zeros(len(rq.K))
+        blat, bst = srv.replay(zero, rq.K, rq.H, rq.x0, blobs, u_nominal=hover)
+        srv.close()
+        hip.close()
+        out["modes"][name] = {
+            "poisson": {"p50_ms": float(np.median(lat) * 1e3), "p99_ms": float(np.quantile(lat, 0.99) * 1e3),
+                        "throughput": st["sample_steps"] / st["seconds"], "seconds": st["seconds"],
+                        "batches": st["batches"], "max_batch": st["max_batch"]},
+            "burst": {"throughput": bst["sample_steps"] / bst["seconds"], "seconds": bst["seconds"],
+                      "p50_ms": float(np.median(blat) * 1e3), "p99_ms": float(np.quantile(blat, 0.99) * 1e3),
+                      "batches": bst["batches"], "max_batch": bst["max_batch"],
+                      "admission_waits": bst["admission_waits"],
+                      "max_workspace_mb": bst["max_workspace_bytes"] / 2 ** 20, "budget_mb": 32},
+            "unit": "sample-steps/s",
+        }
+        print(f"[bench] serving {name}: p50 {out['modes'][name]['poisson']['p50_ms']:.3f} ms "
+              f"p99 {out['modes'][name]['poisson']['p99_ms']:.3f} ms, burst "
+              f"{out['modes'][name]['burst']['throughput'] / 1e9:.2f} G/s", file=sys.stderr, flush=True)
+    return out
+
+
 def bench_surrogate(args):
     """Learned bicycle dynamics in the rollout vs plugin dynamics."""
     from mppi_metal_amd import HipMPPI, scenes
@@ -236,6 +292,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--serial-steps", type=int, default=5)
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--requests", type=int, default=1024)
     ap.add_argument("--surrogate-steps", type=int, default=500)
     ap.add_argument("--train-steps", type=int, default=4000)
     args = ap.parse_args()
@@ -306,6 +364,7 @@ def main():
 
     batch = None if args.no_batch else bench_batch(args)
     surrogate = None if args.no_surrogate else bench_surrogate(args)
+    serving = None if args.no_serving else bench_serving(args)
 
     out = {
         "metric": "MPPI sample-steps/s (+ step p50/p99 ms)",
@@ -337,6 +396,7 @@ def main():
         "final_closed_loop_cost": float(kt[-1]),
         "batch": batch,
         "surrogate": surrogate,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -298,6 +298,85 @@ int mppi_hip_surrogate_import(mppi_hip_ctx* ctx, const float* params, size_t n_p
 int mppi_hip_surrogate_gradients(mppi_hip_ctx* ctx, uint32_t n, const float* x, const float* targets,
                                  float* grads, float* loss);
 
+/* ---------------------------------------------------------------------------
+ * Serving: agents admitted against an HBM budget and solved in batches of any
+ * composition.  No reference counterpart (the reference serves concurrent
+ * request streams with one blocking driver per host thread,
+ * examples/nonholonomic_vehicle_concurrent/main.cpp:98-177).
+ * A server is bound to a context (plugin, noise model, numeric mode, seed,
+ * bound model blob) and owns one device arena of hbm_budget_bytes.  Admitting
+ * an agent reserves its workspace for its K and H (costs, weights, workgroup
+ * statistics, reduce partials, warm start, update) in the arena and stores its
+ * state, warm start and cost blob on the device; when no free range of the
+ * arena fits, admission fails with MPPI_HIP_E_NO_MEMORY and can be retried
+ * after releases.  A solve runs one closed-loop MPPI step for any set of
+ * admitted agents in one pair of kernel launches: the plant step with the
+ * plugin dynamics and the warm-start shift happen on the device, and only the
+ * first control and the diagnostics come back.  An agent's noise stream is
+ * keyed by its id, so its results do not depend on its batch.
+ * ------------------------------------------------------------------------- */
+typedef struct mppi_hip_server mppi_hip_server;
+
+int mppi_hip_server_create(mppi_hip_ctx* ctx, uint32_t max_agents, size_t hbm_budget_bytes,
+                           mppi_hip_server** out_server);
+void mppi_hip_server_destroy(mppi_hip_server* server);
+
+/* Arena bytes an agent with K samples and horizon H reserves. */
+int mppi_hip_server_workspace_bytes(const mppi_hip_server* server, uint32_t K, uint32_t H,
+                                    size_t* bytes);
+
+/* x0: state_dim floats; u_nominal: H*control_dim floats or NULL (zeros);
+ * cost_params: cost_bytes of the linked layout (its per-agent cost blob; the
+ * model blob is the context's bound one).  Writes the agent id. */
+int mppi_hip_server_admit(mppi_hip_server* server, uint32_t K, uint32_t H, const float* x0,
+                          const float* u_nominal, const void* cost_params, size_t cost_bytes,
+                          uint32_t* agent_id);
+int mppi_hip_server_release(mppi_hip_server* server, uint32_t agent_id);
+
+/* One closed-loop step of n distinct admitted agents.  first_controls:
+ * n*control_dim (the control applied by the plant step), diag: n*3 (best
+ * cost, mean cost, effective sample size); either may be NULL.  Blocking.
+ * Afterwards mppi_hip_read_costs / read_weights(ctx, i, ...) return agent
+ * agent_ids[i]'s sample costs and weights of this solve. */
+int mppi_hip_server_solve(mppi_hip_server* server, uint32_t n, const uint32_t* agent_ids,
+                          float* first_controls, float* diag);
+
+/* An agent's device state (state_dim) and warm start (H*control_dim). */
+int mppi_hip_server_read(mppi_hip_server* server, uint32_t agent_id, float* x_out,
+                         float* u_nominal_out);
+
+int mppi_hip_server_usage(const mppi_hip_server* server, size_t* used_bytes, size_t* budget_bytes,
+                          uint32_t* agents);
+
+/* Serving loop over a request trace in real time: requests arrive at
+ * arrival_s (seconds after the call; non-decreasing); arrived requests are
+ * admitted in arrival order while they fit the budget (the oldest waits when
+ * it does not), every admitted request is in the next batch (at most
+ * max_batch agents, 0 = no limit), and a request completes after `solves`
+ * closed-loop solves, when its agent is released.  latency_s[i] = completion
+ * minus arrival of request i (n doubles). */
+typedef struct mppi_hip_request {
+    double arrival_s;
+    uint32_t K, H;
+    uint32_t solves;
+    uint32_t reserved;
+    const float* x0;            /* state_dim */
+    const float* u_nominal;     /* H*control_dim, or NULL */
+    const void* cost_params;    /* the linked cost layout's size, or NULL when it is 0 */
+} mppi_hip_request;
+
+typedef struct mppi_hip_replay_stats {
+    double seconds;             /* wall time of the whole replay */
+    double sample_steps;        /* sum of K*H over every solve */
+    double max_workspace_bytes; /* largest arena use after a batch */
+    uint64_t batches;
+    uint32_t max_batch;
+    uint32_t admission_waits;   /* loop passes in which the oldest request did not fit */
+} mppi_hip_replay_stats;
+
+int mppi_hip_server_replay(mppi_hip_server* server, uint32_t n, const mppi_hip_request* requests,
+                           uint32_t max_batch, double* latency_s, mppi_hip_replay_stats* stats);
+
 /* Library version string. */
 const char* mppi_hip_version(void);
 

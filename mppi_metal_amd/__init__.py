@@ -6,7 +6,7 @@ The drop-in for the reference's `mppi_metal::MPPIDriver` is the C++ facade
 (`HipMPPI`) and holds the bundled model plugins (plugins/*.hip), the scene
 generator and the build script.
 """
-from .controller import HipBatch, HipMPPI, MppiHipError, PLUGIN_DIR  # noqa: F401
+from .controller import HipBatch, HipMPPI, HipServer, MppiHipError, PLUGIN_DIR  # noqa: F401
 from . import scenes  # noqa: F401
 
-__all__ = ["HipBatch", "HipMPPI", "MppiHipError", "PLUGIN_DIR", "scenes"]
+__all__ = ["HipBatch", "HipMPPI", "HipServer", "MppiHipError", "PLUGIN_DIR", "scenes"]

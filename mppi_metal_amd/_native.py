@@ -68,6 +68,29 @@ class MppiHipSurrogateConfig(ctypes.Structure):
 SURROGATE_INPUT_BOX, SURROGATE_INPUT_PERIODIC, SURROGATE_INPUT_IGNORED = 0, 1, 2
 SURROGATE_NORM_FLOATS = 80
 
+class MppiHipRequest(ctypes.Structure):
+    """Mirror of struct mppi_hip_request (include/mppi_hip.h)."""
+
+    _fields_ = [
+        ("arrival_s", ctypes.c_double),
+        ("K", ctypes.c_uint32), ("H", ctypes.c_uint32),
+        ("solves", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+        ("x0", ctypes.POINTER(ctypes.c_float)),
+        ("u_nominal", ctypes.POINTER(ctypes.c_float)),
+        ("cost_params", ctypes.c_void_p),
+    ]
+
+
+class MppiHipReplayStats(ctypes.Structure):
+    """Mirror of struct mppi_hip_replay_stats (include/mppi_hip.h)."""
+
+    _fields_ = [
+        ("seconds", ctypes.c_double), ("sample_steps", ctypes.c_double),
+        ("max_workspace_bytes", ctypes.c_double), ("batches", ctypes.c_uint64),
+        ("max_batch", ctypes.c_uint32), ("admission_waits", ctypes.c_uint32),
+    ]
+
+
 _F = ctypes.POINTER(ctypes.c_float)
 _U32P = ctypes.POINTER(ctypes.c_uint32)
 _VP = ctypes.c_void_p
@@ -116,6 +139,20 @@ SIGNATURES = [
      [_CTX, _F, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _F]),
     ("mppi_hip_surrogate_import", ctypes.c_int, [_CTX, _F, ctypes.c_size_t, _F]),
     ("mppi_hip_surrogate_gradients", ctypes.c_int, [_CTX, ctypes.c_uint32, _F, _F, _F, _F]),
+    ("mppi_hip_server_create", ctypes.c_int, [_CTX, ctypes.c_uint32, ctypes.c_size_t, ctypes.POINTER(_VP)]),
+    ("mppi_hip_server_destroy", None, [_VP]),
+    ("mppi_hip_server_workspace_bytes", ctypes.c_int,
+     [_VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_size_t)]),
+    ("mppi_hip_server_admit", ctypes.c_int,
+     [_VP, ctypes.c_uint32, ctypes.c_uint32, _F, _F, _VP, ctypes.c_size_t, _U32P]),
+    ("mppi_hip_server_release", ctypes.c_int, [_VP, ctypes.c_uint32]),
+    ("mppi_hip_server_solve", ctypes.c_int, [_VP, ctypes.c_uint32, _U32P, _F, _F]),
+    ("mppi_hip_server_read", ctypes.c_int, [_VP, ctypes.c_uint32, _F, _F]),
+    ("mppi_hip_server_usage", ctypes.c_int,
+     [_VP, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t), _U32P]),
+    ("mppi_hip_server_replay", ctypes.c_int,
+     [_VP, ctypes.c_uint32, ctypes.POINTER(MppiHipRequest), ctypes.c_uint32,
+      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(MppiHipReplayStats)]),
     ("mppi_hip_version", ctypes.c_char_p, []),
     ("mppi_hip_compile_check", ctypes.c_int,
      [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,

@@ -321,3 +321,91 @@ class HipBatch:
     def u_of(self, u_packed, a):
         o = int(self.seq_off[a])
         return u_packed[o:o + int(self.seq[a])].reshape(int(self.H[a]), self.ctx.C)
+
+
+class HipServer:
+    """Serving: agents admitted against an HBM budget, solved in batches
+    (mppi_hip_server_* in include/mppi_hip.h)."""
+
+    def __init__(self, ctx: HipMPPI, max_agents: int, hbm_budget_bytes: int):
+        self.ctx, self.lib = ctx, ctx.lib
+        self.s = ctypes.c_void_p()
+        ctx._check(self.lib.mppi_hip_server_create(ctx.ctx, int(max_agents), int(hbm_budget_bytes),
+                                                   ctypes.byref(self.s)))
+
+    def close(self):
+        if self.s:
+            self.lib.mppi_hip_server_destroy(self.s)
+            self.s = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def workspace_bytes(self, K, H):
+        n = ctypes.c_size_t()
+        self.ctx._check(self.lib.mppi_hip_server_workspace_bytes(self.s, int(K), int(H), ctypes.byref(n)))
+        return n.value
+
+    def admit(self, K, H, x0, u_nominal=None, cost=None):
+        x = _f32(x0, (self.ctx.D,))
+        un = _f32(u_nominal, (int(H) * self.ctx.C,)) if u_nominal is not None else None
+        cp, cn, keep = _blob(cost)
+        aid = ctypes.c_uint32()
+        self.ctx._check(self.lib.mppi_hip_server_admit(self.s, int(K), int(H), _fptr(x), _fptr(un), cp, cn,
+                                                       ctypes.byref(aid)))
+        return aid.value
+
+    def release(self, agent_id):
+        self.ctx._check(self.lib.mppi_hip_server_release(self.s, int(agent_id)))
+
+    def solve(self, agent_ids):
+        ids = np.ascontiguousarray(agent_ids, np.uint32)
+        n = ids.size
+        u0 = np.empty((n, self.ctx.C), np.float32)
+        diag = np.empty((n, 3), np.float32)
+        self.ctx._check(self.lib.mppi_hip_server_solve(self.s, n, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                                       _fptr(u0), _fptr(diag)))
+        return u0, diag
+
+    def read(self, agent_id, H):
+        x = np.empty(self.ctx.D, np.float32)
+        u = np.empty((int(H), self.ctx.C), np.float32)
+        self.ctx._check(self.lib.mppi_hip_server_read(self.s, int(agent_id), _fptr(x), _fptr(u)))
+        return x, u
+
+    def usage(self):
+        used, budget, agents = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_uint32()
+        self.ctx._check(self.lib.mppi_hip_server_usage(self.s, ctypes.byref(used), ctypes.byref(budget),
+                                                       ctypes.byref(agents)))
+        return used.value, budget.value, agents.value
+
+    def replay(self, arrival_s, K, H, x0, cost_blobs, u_nominal=None, solves=1, max_batch=0):
+        """Real-time serving loop over a request trace; returns (latency_s [n],
+        stats dict)."""
+        n = len(arrival_s)
+        x0 = np.ascontiguousarray(x0, np.float32).reshape(n, self.ctx.D)
+        reqs = (_native.MppiHipRequest * n)()
+        keep = []
+        for i in range(n):
+            r = reqs[i]
+            r.arrival_s = float(arrival_s[i])
+            r.K, r.H = int(K[i]), int(H[i])
+            r.solves = int(solves)
+            r.x0 = _fptr(x0[i])
+            if u_nominal is not None:
+                un = np.ascontiguousarray(u_nominal[i], np.float32)
+                keep.append(un)
+                r.u_nominal = _fptr(un)
+            cp, _, k = _blob(cost_blobs[i] if cost_blobs is not None else None)
+            keep.append(k)
+            r.cost_params = cp
+        lat = np.empty(n, np.float64)
+        st = _native.MppiHipReplayStats()
+        self.ctx._check(self.lib.mppi_hip_server_replay(self.s, n, reqs, int(max_batch),
+                                                        lat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                        ctypes.byref(st)))
+        stats = {k: getattr(st, k) for k, _ in _native.MppiHipReplayStats._fields_}
+        return lat, stats
