@@ -87,6 +87,21 @@ def reduce_chunks(K: int, H: int, C: int, N: int = 1, spt: int = 0, cus: int = 2
     return -(-K // (256 * spt))
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` at this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel
+    calibration recorded there), or None."""
+    path = ROOT / "profiles" / "mppi_traffic_unicycle_k65536_h64.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch"):
+            f = line.split()[1:]
+            d = dict(zip(f[0::2], f[1::2]))
+            return int(d[kernel]) if kernel in d else None
+    return None
+
+
 def cpu_baseline(n_obs, H, K, steps, threads):
     sys.path.insert(0, str(ROOT / "tests"))
     from problems import unicycle_oracle
@@ -306,6 +321,7 @@ def main():
     from mppi_metal_amd import HipMPPI, scenes
 
     K, H, C, n_obs = args.K, args.H, 2, args.obstacles
+    default_workload = (K, H, n_obs) == (65536, 64, 16)
     sc = scenes.unicycle_scene(n_obs, 42)
     mb, cb = sc.model_blob(), sc.cost_blob()
     hip = HipMPPI(4, C, H, K, sc.lam, sc.sigma, sc.u_min, sc.u_max)
@@ -387,11 +403,13 @@ def main():
         "roofline": {"bound": "valu", "kernel": "mppi_rollout",
                      "achieved": achieved_ops / 1e12, "peak": VALU_PEAK_LANE_OPS / 1e12,
                      "unit": "T lane-op/s", "frac": achieved_ops / VALU_PEAK_LANE_OPS,
-                     "ops_per_sample_step": w_ss, "avg_launch_us": roll_s * 1e6, "traffic": None},
+                     "ops_per_sample_step": w_ss, "avg_launch_us": roll_s * 1e6,
+                     "traffic": measured_traffic("mppi_rollout") if default_workload else None},
         "reduce_roofline": {"bound": "hbm", "kernel": "mppi_reduce",
                             "achieved": rb / red_s / 1e9, "peak": HBM_PEAK_BPS / 1e9,
                             "unit": "GB/s", "frac": rb / red_s / HBM_PEAK_BPS,
-                            "bytes_per_launch": rb, "avg_launch_us": red_s * 1e6},
+                            "bytes_per_launch": rb, "avg_launch_us": red_s * 1e6,
+                            "traffic": measured_traffic("mppi_reduce") if default_workload else None},
         "cpu_baseline": cpu,
         "final_closed_loop_cost": float(kt[-1]),
         "batch": batch,
