@@ -437,6 +437,11 @@ __device__ void mppi_reduce_tile(const mppi_step_args& a, uint32_t agent, uint32
         if (uniform) w = 1.0f;
         else w = mppi_isfinite(c) ? mppi_expf(-(c - m) / a.lambda) : 0.0f;
         if (grp == 0 && a.weights) a.weights[d.w_off + k] = w / Sf;
+        // A weight that underflowed to exactly 0 adds exactly nothing: a
+        // wavefront whose samples all have zero weight skips regenerating
+        // their noise (most samples, when lambda is small against the
+        // spread of the costs).
+        if (!mppi_any(w != 0.0f)) continue;
 #pragma unroll
         for (int q = 0; q < MPPI_REDUCE_BPG; ++q) {
             if (b_begin + q < NB) {
@@ -591,14 +596,17 @@ extern "C" __global__ void __launch_bounds__(256) mppi_surrogate_data(mppi_surro
 
 // Rollout with the MLP as dynamics (grid-mapped like mppi_rollout; the
 // reduce kernel is shared).  Per timestep each wave stages its 64 samples'
-// normalised (state, control) in LDS, evaluates the network on four
-// 16-sample blocks with the matrix cores (weight fragments in LDS), and adds
-// the denormalised delta to each lane's state; costs are the plugin's.
-extern "C" __global__ void __launch_bounds__(256) mppi_rollout_surrogate(mppi_surrogate_rollout_args sa) {
+// normalised (state, control) in its own LDS rows, evaluates the network on
+// four 16-sample blocks with the matrix cores (weight fragments in LDS), and
+// adds the denormalised delta to each lane's state; costs are the plugin's.
+// The staging rows are private to a wave, whose LDS operations complete in
+// order, so the timestep needs wave-level ordering only (no workgroup
+// barrier).  Two workgroups per CU (two waves per SIMD) bound the registers.
+extern "C" __global__ void __launch_bounds__(256, 2) mppi_rollout_surrogate(mppi_surrogate_rollout_args sa) {
     constexpr int IN = SDIM + CDIM;
     constexpr int NFRAG = MPPI_MLP_FRAG_W2T;   // W1, W2, W3 fragments
     __shared__ short s_frag[NFRAG * MPPI_MLP_FRAG_ELEMS];
-    __shared__ float s_b[2 * MPPI_MLP_HID + 16];
+    __shared__ __attribute__((aligned(16))) float s_b[2 * MPPI_MLP_HID + 16];
     __shared__ float s_io[4][64][IN + 1];
     __shared__ float s_dx[4][64][SDIM + 1];
     const mppi_step_args& a = sa.s;
@@ -652,25 +660,31 @@ extern "C" __global__ void __launch_bounds__(256) mppi_rollout_surrogate(mppi_su
                 for (int i = 0; i < CDIM; ++i)
                     s_io[wave][lane][SDIM + i] = mppi_mlp_input(u[i], sa.in_center[SDIM + i], sa.in_inv_half[SDIM + i],
                                                                 sa.in_period[SDIM + i]);
-                __syncthreads();
+                __builtin_amdgcn_wave_barrier();
                 // The four 16-sample blocks of the wave one after the other
                 // (unrolling them would only multiply the live registers).
 #pragma unroll 1
                 for (int nb = 0; nb < 4; ++nb) {
-                    mppi_v4bf xb;
+                    // An offset the compiler cannot see through keeps the
+                    // fragment reads inside this loop; hoisted out of it, the
+                    // 80 fragments would hold 160 registers per lane.
+                    uint32_t fo = 0;
+                    __asm__ volatile("" : "+s"(fo));
+                    mppi_v4f xv;
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int kk = 4 * g + j;
-                        xb[j] = mppi_f2bf(kk < IN ? s_io[wave][16 * nb + c][kk < IN ? kk : 0] : 0.0f);
+                        xv[j] = kk < IN ? s_io[wave][16 * nb + c][kk < IN ? kk : 0] : 0.0f;
                     }
-                    mppi_v4f y = mppi_mlp_block(s_frag, s_b, s_b + MPPI_MLP_HID, xb, lane, nullptr, nullptr);
+                    mppi_v4f y = mppi_mlp_block(s_frag + fo, s_b, s_b + MPPI_MLP_HID, mppi_pack_bf16(xv), lane,
+                                                nullptr, nullptr);
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int r = 4 * g + j;
                         if (r < SDIM) s_dx[wave][16 * nb + c][r] = y[j] + s_b[2 * MPPI_MLP_HID + r];
                     }
                 }
-                __syncthreads();
+                __builtin_amdgcn_wave_barrier();
 #pragma unroll
                 for (int i = 0; i < SDIM; ++i) x[i] += s_dx[wave][lane][i] * sa.out_sd[i] + sa.out_mu[i];
                 if (active)

@@ -60,6 +60,7 @@ struct mppi_mlp_norm {
 #if defined(__HIPCC_RTC__) || defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC__)
 typedef short mppi_v4bf __attribute__((ext_vector_type(4)));
 typedef float mppi_v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 mppi_v4bf16 __attribute__((ext_vector_type(4)));
 
 __device__ inline float mppi_mlp_input(float v, float center, float inv_half, float period) {
     float d = v - center;
@@ -67,22 +68,31 @@ __device__ inline float mppi_mlp_input(float v, float center, float inv_half, fl
     return d * inv_half;
 }
 
-// float -> bfloat16, round to nearest even.
+// float -> bfloat16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950).
 __device__ inline short mppi_f2bf(float f) {
-    uint32_t u = __builtin_bit_cast(uint32_t, f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return short(u >> 16);
+    return __builtin_bit_cast(short, (__bf16)f);
+}
+// Four floats -> four bfloat16 (two packed conversions).
+__device__ inline mppi_v4bf mppi_pack_bf16(mppi_v4f v) {
+    return __builtin_bit_cast(mppi_v4bf, __builtin_convertvector(v, mppi_v4bf16));
 }
 __device__ inline float mppi_bf2f(short h) {
     return __builtin_bit_cast(float, uint32_t((unsigned short)h) << 16);
 }
 
 // tanh(x) = 1 - 2 / (exp(2x) + 1) with the hardware exp2 and reciprocal
-// (about 2 ulp); |x| > 9 saturates.
+// (about 2 ulp).  Large |x| saturates through IEEE infinities: exp2 gives
+// +inf (reciprocal 0, tanh 1) or 0 (tanh -1).
 __device__ inline float mppi_tanh_fast(float x) {
-    x = x > 9.0f ? 9.0f : (x < -9.0f ? -9.0f : x);
     const float e = __builtin_amdgcn_exp2f(x * 2.88539008177792681472f);  // 2 / ln 2
-    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+    return __builtin_fmaf(-2.0f, __builtin_amdgcn_rcpf(e + 1.0f), 1.0f);
+}
+
+__device__ inline mppi_v4f mppi_tanh4(mppi_v4f v) {
+    mppi_v4f r;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) r[j] = mppi_tanh_fast(v[j]);
+    return r;
 }
 
 __device__ inline mppi_v4f mppi_mfma(mppi_v4bf a, mppi_v4bf b, mppi_v4f c) {
@@ -96,7 +106,9 @@ __device__ inline mppi_v4bf mppi_frag(const short* frags, uint32_t f, uint32_t l
 // Forward pass of one 16-sample block.  xb: the B operand of the inputs
 // (rows 4*(lane/16)+j, zero beyond `in`).  Returns the output tile (rows
 // 4*(lane/16)+j < out valid, without the output bias); h1b / h2b receive the
-// bf16 hidden activations when non-null (for training).
+// bf16 hidden activations when non-null (for training).  b1 and b2 must be
+// 16-byte aligned: the biases initialise the accumulators (four rows per
+// lane), so the bias add costs no instruction.
 __device__ inline mppi_v4f mppi_mlp_block(const short* frags, const float* b1, const float* b2,
                                             mppi_v4bf xb, uint32_t lane, mppi_v4bf* h1b_out,
                                             mppi_v4bf* h2b_out) {
@@ -105,18 +117,16 @@ __device__ inline mppi_v4f mppi_mlp_block(const short* frags, const float* b1, c
     const mppi_v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int mb = 0; mb < 8; ++mb) {
-        mppi_v4f acc = mppi_mfma(mppi_frag(frags, MPPI_MLP_FRAG_W1 + mb, lane), xb, zero);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) h1b[mb][j] = mppi_f2bf(mppi_tanh_fast(acc[j] + b1[16 * mb + 4 * g + j]));
+        const mppi_v4f bias = *reinterpret_cast<const mppi_v4f*>(b1 + 16 * mb + 4 * g);
+        h1b[mb] = mppi_pack_bf16(mppi_tanh4(mppi_mfma(mppi_frag(frags, MPPI_MLP_FRAG_W1 + mb, lane), xb, bias)));
     }
 #pragma unroll
     for (int mb = 0; mb < 8; ++mb) {
-        mppi_v4f acc = zero;
+        mppi_v4f acc = *reinterpret_cast<const mppi_v4f*>(b2 + 16 * mb + 4 * g);
 #pragma unroll
         for (int kb = 0; kb < 8; ++kb)
             acc = mppi_mfma(mppi_frag(frags, MPPI_MLP_FRAG_W2 + mb * 8 + kb, lane), h1b[kb], acc);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) h2b[mb][j] = mppi_f2bf(mppi_tanh_fast(acc[j] + b2[16 * mb + 4 * g + j]));
+        h2b[mb] = mppi_pack_bf16(mppi_tanh4(acc));
     }
     mppi_v4f out = zero;
 #pragma unroll
