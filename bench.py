@@ -346,15 +346,32 @@ def main():
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f}s", file=sys.stderr, flush=True)
 
     # Kernel times from HIP events on the library's stream.
+    from mppi_metal_amd import _native
+    fused = H * C <= 128 and K < 2 ** 18 and -(-K // 256) <= 256
     hip.profile(True)
     prof_steps = min(args.steps, 100)
     hip.simulate(prof_steps, 0, x0, u0)
     roll_ms, red_ms, launches = hip.profile_read()
     hip.profile(False)
     roll_s = roll_ms / launches / 1e3
-    red_s = red_ms / launches / 1e3
-    w_ss = valu_ops_per_sample_step(n_obs)
+    # The one-kernel step also keeps and averages the noise: 2 C more slots
+    # (an LDS store and an FMA per normal).
+    w_ss = valu_ops_per_sample_step(n_obs) + (2 * C if fused else 0)
     achieved_ops = K * H * w_ss / roll_s
+
+    # The two-kernel step (rollout, then the noise-regenerating reduce).
+    hip.set_step_path(_native.PATH_TWO_KERNEL)
+    hip.simulate(max(1, args.warmup), 0, x0, u0)
+    t0 = time.perf_counter()
+    hip.simulate(args.steps, 0, x0, u0)
+    el2 = time.perf_counter() - t0
+    hip.profile(True)
+    hip.simulate(prof_steps, 0, x0, u0)
+    roll2_ms, red2_ms, launches2 = hip.profile_read()
+    hip.profile(False)
+    hip.set_step_path(_native.PATH_AUTO)
+    roll2_s = roll2_ms / launches2 / 1e3
+    red_s = red2_ms / launches2 / 1e3
     n_chunks = reduce_chunks(K, H, C, spt=args.spt)
     rb = reduce_bytes(K, H, C, n_chunks)
 
@@ -400,12 +417,17 @@ def main():
         "config": {"workload": "plain MPPI closed loop, kinematic unicycle", "K": K, "H": H,
                    "obstacles": n_obs, "closed_loop_steps": args.steps, "lambda": 1.0,
                    "sigma": [0.5, 0.3]},
-        "roofline": {"bound": "valu", "kernel": "mppi_rollout",
+        "step_path": "fused (one kernel)" if fused else "two kernels",
+        "roofline": {"bound": "valu", "kernel": "mppi_rollout_fused" if fused else "mppi_rollout",
                      "achieved": achieved_ops / 1e12, "peak": VALU_PEAK_LANE_OPS / 1e12,
                      "unit": "T lane-op/s", "frac": achieved_ops / VALU_PEAK_LANE_OPS,
                      "ops_per_sample_step": w_ss, "avg_launch_us": roll_s * 1e6,
-                     "traffic": measured_traffic("mppi_rollout") if default_workload else None},
-        "reduce_roofline": {"bound": "hbm", "kernel": "mppi_reduce",
+                     "traffic": measured_traffic("mppi_rollout_fused" if fused else "mppi_rollout")
+                     if default_workload else None},
+        "two_kernel_path": {"value": K * H * args.steps / el2, "ms_per_step": el2 / args.steps * 1e3,
+                            "rollout_us": roll2_s * 1e6, "reduce_us": red_s * 1e6,
+                            "rollout_frac": K * H * valu_ops_per_sample_step(n_obs) / roll2_s / VALU_PEAK_LANE_OPS},
+        "reduce_roofline": {"bound": "hbm", "kernel": "mppi_reduce (two-kernel path)",
                             "achieved": rb / red_s / 1e9, "peak": HBM_PEAK_BPS / 1e9,
                             "unit": "GB/s", "frac": rb / red_s / HBM_PEAK_BPS,
                             "bytes_per_launch": rb, "avg_launch_us": red_s * 1e6,

@@ -157,6 +157,21 @@ int mppi_hip_simulate_batch(mppi_hip_ctx* ctx, uint32_t num_agents, uint32_t num
 int mppi_hip_read_costs(mppi_hip_ctx* ctx, uint32_t agent, float* out, size_t count);
 int mppi_hip_read_weights(mppi_hip_ctx* ctx, uint32_t agent, float* out, size_t count);
 
+/* Which kernels run an MPPI iteration of mppi_hip_step / simulate /
+ * simulate_batch.  No reference counterpart.
+ *   TWO_KERNEL  rollout, then a reduce that regenerates the noise from its
+ *               counters and forms the weighted average in parallel over
+ *               samples;
+ *   FUSED       one kernel: the rollout keeps each workgroup's normals in LDS
+ *               and forms its weighted sums, the last workgroup combines them
+ *               (needs horizon * control_dim <= 128 and K < 2^18, else the
+ *               two kernels run); per-sample weights are then formed when
+ *               mppi_hip_read_weights asks for them;
+ *   AUTO        FUSED where it applies and the rollout has at most one
+ *               workgroup per compute unit, else TWO_KERNEL (the default). */
+enum mppi_hip_step_path { MPPI_HIP_PATH_AUTO = 0, MPPI_HIP_PATH_TWO_KERNEL = 1, MPPI_HIP_PATH_FUSED = 2 };
+int mppi_hip_set_step_path(mppi_hip_ctx* ctx, uint32_t path);
+
 /* Launch-shape knobs of the reduce kernel (samples per thread); 0 restores
  * the built-in choice.  No reference counterpart; used by the autotuner. */
 int mppi_hip_set_reduce_samples_per_thread(mppi_hip_ctx* ctx, uint32_t spt);

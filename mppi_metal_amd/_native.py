@@ -18,6 +18,7 @@ STATUS_NAMES = {
 }
 COV_DIAGONAL, COV_FULL = 0, 1
 FP32, FP16, MIXED = 0, 1, 2
+PATH_AUTO, PATH_TWO_KERNEL, PATH_FUSED = 0, 1, 2
 
 
 class MppiHipConfig(ctypes.Structure):
@@ -118,6 +119,7 @@ SIGNATURES = [
     ("mppi_hip_read_costs", ctypes.c_int, [_CTX, ctypes.c_uint32, _F, ctypes.c_size_t]),
     ("mppi_hip_read_weights", ctypes.c_int, [_CTX, ctypes.c_uint32, _F, ctypes.c_size_t]),
     ("mppi_hip_set_reduce_samples_per_thread", ctypes.c_int, [_CTX, ctypes.c_uint32]),
+    ("mppi_hip_set_step_path", ctypes.c_int, [_CTX, ctypes.c_uint32]),
     ("mppi_hip_profile_enable", ctypes.c_int, [_CTX, ctypes.c_int]),
     ("mppi_hip_profile_read", ctypes.c_int,
      [_CTX, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),

@@ -173,6 +173,10 @@ class HipMPPI:
         self._check(self.lib.mppi_hip_read_weights(self.ctx, agent, _fptr(out), n))
         return out
 
+    def set_step_path(self, path):
+        """_native.PATH_AUTO, PATH_TWO_KERNEL or PATH_FUSED."""
+        self._check(self.lib.mppi_hip_set_step_path(self.ctx, int(path)))
+
     def set_reduce_samples_per_thread(self, spt):
         self._check(self.lib.mppi_hip_set_reduce_samples_per_thread(self.ctx, int(spt)))
 

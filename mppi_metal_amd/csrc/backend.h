@@ -18,12 +18,15 @@ namespace mppi_hip_impl {
 constexpr uint32_t kBlock = 256;      // workgroup size of the grid-mapped kernels
 constexpr uint32_t kReduceBPG = 8;    // noise blocks per reduce tile
 constexpr uint32_t kMaxHC = 4096;     // horizon * control_dim limit
+constexpr uint32_t kFusedMaxHC = 128; // horizon * control_dim limit of the one-kernel step
 
 // One loaded code object and the kernels looked up in it.
 struct LoadedModule {
     hipModule_t module = nullptr;
     hipFunction_t rollout = nullptr;  // grid-mapped kernels
     hipFunction_t reduce = nullptr;
+    hipFunction_t rollout_fused = nullptr;  // one-kernel step
+    hipFunction_t weights = nullptr;        // per-sample weights after a one-kernel step
     // batch kernels, indexed [block 64/128/256 -> 0/1/2][table 0 / persistent 1]
     hipFunction_t rollout_b[3][2] = {};
     hipFunction_t reduce_b[3][2] = {};
@@ -90,6 +93,13 @@ struct mppi_hip_ctx {
     const float* last_costs = nullptr;
     const float* last_weights = nullptr;
     std::vector<mppi_agent_desc> last_desc;
+
+    // step path (mppi_hip_set_step_path); a one-kernel step leaves the
+    // per-sample weights to be formed when they are read
+    uint32_t step_path = MPPI_HIP_PATH_AUTO;
+    bool weights_pending = false;
+    mppi_step_args weights_args{};
+    uint32_t weights_grid_x = 0, weights_grid_y = 0;
 
     // learned-dynamics surrogate (surrogate.hip)
     std::shared_ptr<mppi_hip_impl::SurrogateState> sur;

@@ -426,6 +426,7 @@ int mppi_hip_batch_run(mppi_hip_batch* b, uint32_t num_steps, int closed_loop, f
     ctx->last_agents = b->N;
     ctx->last_costs = static_cast<const float*>(b->d_costs.ptr);
     ctx->last_weights = static_cast<const float*>(b->d_weights.ptr);
+    ctx->weights_pending = false;
     ctx->last_desc = b->desc;
     return MPPI_HIP_OK;
 }

@@ -312,7 +312,9 @@ int load_module(mppi_hip_ctx* ctx, bool batch, bool surrogate, std::shared_ptr<L
     if (e != hipSuccess)
         return fail(ctx, MPPI_HIP_E_COMPILE, std::string("hipModuleLoadData failed: ") + hipGetErrorString(e));
     bool ok = hipModuleGetFunction(&m->rollout, m->module, "mppi_rollout") == hipSuccess &&
-              hipModuleGetFunction(&m->reduce, m->module, "mppi_reduce") == hipSuccess;
+              hipModuleGetFunction(&m->reduce, m->module, "mppi_reduce") == hipSuccess &&
+              hipModuleGetFunction(&m->rollout_fused, m->module, "mppi_rollout_fused") == hipSuccess &&
+              hipModuleGetFunction(&m->weights, m->module, "mppi_weights") == hipSuccess;
     if (ok && batch) {
         const char* blocks[3] = {"64", "128", "256"};
         const char* modes[2] = {"table", "persist"};
@@ -411,7 +413,9 @@ std::vector<mppi_agent_desc> uniform_descs(const mppi_hip_ctx* ctx, uint32_t N) 
         v[a].seq_off = uint64_t(a) * ctx->H * ctx->C;
         v[a].out_off = v[a].seq_off;
         v[a].blk_off = uint64_t(a) * d.n_blk;
-        v[a].part_off = uint64_t(a) * d.n_chunks * d.NB * 4;
+        // room for either the reduce's chunk partials or the one-kernel
+        // step's per-workgroup sums
+        v[a].part_off = uint64_t(a) * std::max(d.n_chunks, d.n_blk) * d.NB * 4;
     }
     return v;
 }
@@ -425,7 +429,8 @@ int ensure_work_buffers(mppi_hip_ctx* ctx, const std::vector<mppi_agent_desc>& d
     if ((rc = ensure_device_buffer(ctx, &ctx->weights, sizeof(float) * nk))) return rc;
     if ((rc = ensure_device_buffer(ctx, &ctx->blk_stats, sizeof(float) * 4 * (last.blk_off + last.n_blk)))) return rc;
     if ((rc = ensure_device_buffer(ctx, &ctx->partials,
-                                   sizeof(float) * (last.part_off + size_t(last.n_chunks) * last.NB * 4)))) return rc;
+                                   sizeof(float) * (last.part_off + size_t(std::max(last.n_chunks, last.n_blk)) *
+                                                                        last.NB * 4)))) return rc;
     const size_t down = sizeof(float) * (last.seq_off + size_t(last.H) * ctx->C + 3 * size_t(N));
     if ((rc = ensure_device_buffer(ctx, &ctx->down, down))) return rc;
     if ((rc = ensure_pinned_buffer(ctx, &ctx->down_host, down))) return rc;
@@ -502,7 +507,20 @@ int upload(mppi_hip_ctx* ctx, const std::vector<mppi_agent_desc>& desc, uint64_t
     return MPPI_HIP_OK;
 }
 
-// Enqueue one MPPI iteration (grid-mapped rollout + reduce) for N agents.
+// The one-kernel step applies when the normals of a 256-sample workgroup fit
+// in LDS (H*C <= 128), K is below the FP64 threshold, and the surrogate is
+// not in use.  AUTO takes it when the rollout grid has at most one workgroup
+// per compute unit: the kernel's LDS then costs no occupancy, since each
+// compute unit runs one workgroup anyway.
+bool use_fused(const mppi_hip_ctx* ctx, const mppi_agent_desc& d, uint32_t N, bool surrogate) {
+    if (ctx->step_path == MPPI_HIP_PATH_TWO_KERNEL || surrogate) return false;
+    if (size_t(d.H) * ctx->C > kFusedMaxHC || d.K >= (1u << 18) || d.n_blk > kBlock) return false;
+    if (ctx->step_path == MPPI_HIP_PATH_FUSED) return true;
+    return uint64_t(d.n_blk) * N <= uint64_t(ctx->num_cus);
+}
+
+// Enqueue one MPPI iteration for N agents: grid-mapped rollout + reduce, or
+// the one-kernel step.
 int launch_iteration(mppi_hip_ctx* ctx, const std::vector<mppi_agent_desc>& desc,
                      const mppi_step_args& a) {
     const uint32_t N = uint32_t(desc.size());
@@ -516,7 +534,26 @@ int launch_iteration(mppi_hip_ctx* ctx, const std::vector<mppi_agent_desc>& desc
     }
     alignas(16) unsigned char sur_args[1024];
     hipFunction_t sur_fn = nullptr;
-    if (surrogate_rollout(ctx, args, &sur_fn, sur_args, sizeof(sur_args))) {
+    const bool sur = surrogate_rollout(ctx, args, &sur_fn, sur_args, sizeof(sur_args));
+    ctx->weights_pending = false;
+    if (use_fused(ctx, d, N, sur)) {
+        MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(ctx->mod->rollout_fused, d.n_blk, N, 1, kBlock, 1, 1, 0,
+                                                ctx->stream, params, nullptr));
+        ctx->weights_pending = true;
+        ctx->weights_args = args;
+        ctx->weights_grid_x = (d.K + kBlock - 1) / kBlock;
+        ctx->weights_grid_y = N;
+        if (ctx->profiling) {
+            if (e1) (void)hipEventRecord(e1, ctx->stream);
+            if (e2) (void)hipEventRecord(e2, ctx->stream);
+            if (e3) (void)hipEventRecord(e3, ctx->stream);
+            if (e0 && e1) ctx->pending_rollout.push_back({e0, e1});
+            if (e2 && e3) ctx->pending_reduce.push_back({e2, e3});
+            ctx->prof_launches += 1;
+        }
+        return MPPI_HIP_OK;
+    }
+    if (sur) {
         void* sparams[] = {sur_args};
         MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(sur_fn, d.n_blk, N, 1, kBlock, 1, 1, 0, ctx->stream,
                                                 sparams, nullptr));
@@ -925,7 +962,16 @@ int read_per_sample(mppi_hip_ctx* ctx, const float* base, uint32_t agent, float*
     const mppi_agent_desc& d = ctx->last_desc[agent];
     if (!out || count != d.K)
         return fail(ctx, MPPI_HIP_E_INVALID, std::string(what) + ": out must hold the agent's sample count.");
-    const uint64_t off = std::strcmp(what, "read_weights") == 0 ? d.w_off : d.cost_off;
+    const bool weights = std::strcmp(what, "read_weights") == 0;
+    if (weights && ctx->weights_pending) {
+        // A one-kernel step: form the weights of its iteration now.
+        mppi_step_args wa = ctx->weights_args;
+        void* params[] = {&wa};
+        MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(ctx->mod->weights, ctx->weights_grid_x, ctx->weights_grid_y, 1,
+                                                kBlock, 1, 1, 0, ctx->stream, params, nullptr));
+        ctx->weights_pending = false;
+    }
+    const uint64_t off = weights ? d.w_off : d.cost_off;
     MPPI_HIP_TRY(ctx, hipMemcpyAsync(out, base + off, sizeof(float) * count,
                                      hipMemcpyDeviceToHost, ctx->stream));
     MPPI_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -939,6 +985,13 @@ int mppi_hip_read_costs(mppi_hip_ctx* ctx, uint32_t agent, float* out, size_t co
 
 int mppi_hip_read_weights(mppi_hip_ctx* ctx, uint32_t agent, float* out, size_t count) {
     return read_per_sample(ctx, ctx ? ctx->last_weights : nullptr, agent, out, count, "read_weights");
+}
+
+int mppi_hip_set_step_path(mppi_hip_ctx* ctx, uint32_t path) {
+    if (!ctx) return fail(nullptr, MPPI_HIP_E_INVALID, "ctx is null.");
+    if (path > MPPI_HIP_PATH_FUSED) return fail(ctx, MPPI_HIP_E_INVALID, "step path must be AUTO, TWO_KERNEL or FUSED.");
+    ctx->step_path = path;
+    return MPPI_HIP_OK;
 }
 
 int mppi_hip_set_reduce_samples_per_thread(mppi_hip_ctx* ctx, uint32_t spt) {

@@ -68,6 +68,7 @@
 #define MPPI_NV (4 * MPPI_REDUCE_BPG)     // accumulators per reduce thread
 #define MPPI_FP64_MIN_K (1u << 18)
 #define MPPI_MAX_HC 4096
+#define MPPI_FUSED_MAX_HC 128             // horizon * control_dim limit of the one-kernel step
 
 constexpr int mppi_gcd(int a, int b) { return b == 0 ? a : mppi_gcd(b, a % b); }
 // Timesteps whose C*G noise values fill whole Philox blocks.
@@ -228,10 +229,17 @@ __device__ void mppi_block_stats(const mppi_step_args& a, uint32_t agent, const 
     else mppi_combine_stats<BLOCK, float>(a, agent, d);
 }
 
+template <int BLOCK>
+__device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
+                                 uint32_t blk, bool active, float total, float* s_z);
+
 // ---------------------------------------------------------------------------
 // Rollout tile: samples [blk*BLOCK, blk*BLOCK + BLOCK) of one agent.
+// FUSED (one-kernel step, H*C <= MPPI_FUSED_MAX_HC): every lane also keeps its
+// normals in LDS, and the tile ends with its weighted noise sums
+// (mppi_fused_stats) instead of the plain softmax partials.
 // ---------------------------------------------------------------------------
-template <int BLOCK>
+template <int BLOCK, bool FUSED = false>
 __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint32_t blk) {
     constexpr int WAVES = BLOCK / 64;
     const mppi_agent_desc d = a.desc[agent];
@@ -247,7 +255,11 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
 
     // The nominal sequence goes to LDS and the noise scale and bounds to
     // registers, so the time loop issues no memory waits of its own.
-    __shared__ float s_un[MPPI_MAX_HC];
+    __shared__ float s_un[FUSED ? MPPI_FUSED_MAX_HC : MPPI_MAX_HC];
+    // [lane][H*C + 1]: a lane's writes and a column's reads are both
+    // conflict-free.
+    constexpr int ZS = MPPI_FUSED_MAX_HC + 1;
+    __shared__ float s_z[FUSED ? BLOCK * ZS : 1];
     for (uint32_t i = threadIdx.x; i < H * CDIM; i += BLOCK) s_un[i] = a.u_nom[d.seq_off + i];
     float scale[MPPI_COV_FULL ? CDIM * CDIM : CDIM], lo[CDIM], hi[CDIM];
 #pragma unroll
@@ -273,6 +285,11 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
                 z[4 * q + 0] = v.x; z[4 * q + 1] = v.y;
                 z[4 * q + 2] = v.z; z[4 * q + 3] = v.w;
             }
+            if (FUSED) {
+#pragma unroll
+                for (int i = 0; i < MPPI_NG; ++i)
+                    if (t0 * CDIM + i < H * CDIM) s_z[threadIdx.x * ZS + t0 * CDIM + i] = z[i];
+            }
 #pragma unroll
             for (int g = 0; g < MPPI_G; ++g) {
                 const uint32_t t = t0 + g;
@@ -292,7 +309,8 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
         a.costs[d.cost_off + k] = total;
     }
 
-    mppi_block_stats<BLOCK>(a, agent, d, blk, active, total);
+    if (FUSED) mppi_fused_stats<BLOCK>(a, agent, d, blk, active, total, s_z);
+    else mppi_block_stats<BLOCK>(a, agent, d, blk, active, total);
 }
 
 // ---------------------------------------------------------------------------
@@ -300,6 +318,63 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
 // partials over sample chunks, divide by the normaliser, write u_next and the
 // diagnostics; in closed loop also log history, step the plant and shift.
 // ---------------------------------------------------------------------------
+// zs[col] = (sum_r scale_r * rows[r * W + col]) / S for col < W (W a
+// multiple of 4), summed in a fixed order; scale = null means 1.  Rows were
+// written by other workgroups, possibly on other XCDs: each thread owns a
+// float4 of columns and a stride of rows and keeps 16 row loads in flight,
+// so the sum costs a few memory latencies rather than one per row.
+template <int BLOCK, int U = 16>
+__device__ void mppi_sum_rows(const float* rows, uint32_t n_rows, uint32_t W, const float* scale, float S,
+                              float* zs) {
+    __shared__ float4 s_part[BLOCK];
+    const uint32_t Q = W / 4;                            // column quads
+    const uint32_t T = Q < BLOCK ? BLOCK / Q : 1;        // threads per quad
+    const uint32_t quad0 = threadIdx.x / T, tq = threadIdx.x % T;
+    for (uint32_t qb = 0; qb < Q; qb += BLOCK / T) {
+        const uint32_t q = qb + quad0;
+        float4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (q < Q && quad0 < BLOCK / T) {
+            const float4* base = reinterpret_cast<const float4*>(rows) + q;
+            uint32_t r = tq;
+            for (; r + (U - 1) * T < n_rows; r += U * T) {
+                float4 v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) v[u] = base[size_t(r + u * T) * Q];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const float f = scale ? scale[r + u * T] : 1.0f;
+                    acc.x = mppi_fma(f, v[u].x, acc.x);
+                    acc.y = mppi_fma(f, v[u].y, acc.y);
+                    acc.z = mppi_fma(f, v[u].z, acc.z);
+                    acc.w = mppi_fma(f, v[u].w, acc.w);
+                }
+            }
+            for (; r < n_rows; r += T) {
+                const float4 v = base[size_t(r) * Q];
+                const float f = scale ? scale[r] : 1.0f;
+                acc.x = mppi_fma(f, v.x, acc.x);
+                acc.y = mppi_fma(f, v.y, acc.y);
+                acc.z = mppi_fma(f, v.z, acc.z);
+                acc.w = mppi_fma(f, v.w, acc.w);
+            }
+        }
+        s_part[threadIdx.x] = acc;
+        __syncthreads();
+        if (tq == 0 && q < Q && quad0 < BLOCK / T) {
+            float4 t = s_part[threadIdx.x];
+            for (uint32_t i = 1; i < T; ++i) {
+                const float4 p = s_part[threadIdx.x + i];
+                t.x += p.x; t.y += p.y; t.z += p.z; t.w += p.w;
+            }
+            zs[4 * q + 0] = t.x / S;
+            zs[4 * q + 1] = t.y / S;
+            zs[4 * q + 2] = t.z / S;
+            zs[4 * q + 3] = t.w / S;
+        }
+        __syncthreads();
+    }
+}
+
 template <int BLOCK, typename T>
 __device__ void mppi_sum_partials(const mppi_step_args& a, const mppi_agent_desc& d,
                                   double weight_sum, float* zs) {
@@ -337,16 +412,16 @@ __device__ void mppi_sum_partials(const mppi_step_args& a, const mppi_agent_desc
     }
 }
 
+// Apply the averaged noise zs (H*C values, already divided by the
+// normaliser): u_next, diagnostics, and in closed loop the history, the plant
+// step and the warm-start shift.
 template <int BLOCK>
-__device__ void mppi_finalize(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
-                              float* zs) {
+__device__ void mppi_apply_update(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
+                                  const float* zs) {
     const uint32_t H = d.H, HC = H * CDIM;
     const uint32_t slot = d.slot;
     mppi_agent_state* ag = &a.agents[slot];
     const uint32_t step = ag->step;
-    if (d.K >= MPPI_FP64_MIN_K) mppi_sum_partials<BLOCK, double>(a, d, ag->weight_sum, zs);
-    else mppi_sum_partials<BLOCK, float>(a, d, ag->weight_sum, zs);
-    __syncthreads();
     float* uo = a.u_out + d.out_off;
     float* un = a.u_nom + d.seq_off;
     for (uint32_t n = threadIdx.x; n < HC; n += BLOCK) {
@@ -403,6 +478,146 @@ __device__ void mppi_finalize(const mppi_step_args& a, uint32_t agent, const mpp
         ag->reduce_arrivals = 0u;
         __threadfence();
     }
+}
+
+template <int BLOCK>
+__device__ void mppi_finalize(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
+                              float* zs) {
+    const mppi_agent_state* ag = &a.agents[d.slot];
+    if (d.K >= MPPI_FP64_MIN_K) mppi_sum_partials<BLOCK, double>(a, d, ag->weight_sum, zs);
+    else mppi_sum_rows<BLOCK>(a.partials + d.part_off, d.n_chunks, d.NB * 4, nullptr, float(ag->weight_sum), zs);
+    __syncthreads();
+    mppi_apply_update<BLOCK>(a, agent, d, zs);
+}
+
+// ---------------------------------------------------------------------------
+// One-kernel step (FUSED rollout tiles).  A tile's weights are taken against
+// its own minimum m_g, e_k = exp(-(c_k - m_g)/lambda), and its weighted noise
+// sums V_g = sum_k e_k z_k are formed from the normals kept in LDS; the last
+// tile of the agent scales each tile's sums by exp(-(m_g - m)/lambda) (m the
+// agent's minimum, as in the normaliser), divides by the normaliser and
+// applies the update.  A tile with no finite cost keeps its plain sum of the
+// noise and gets scale 0, which matters only when no cost is finite (uniform
+// weights, as in the two-kernel step).
+// ---------------------------------------------------------------------------
+// The last tile of a one-kernel step (n_blk <= BLOCK tiles, one per thread):
+// the normaliser and diagnostics as mppi_combine_stats forms them, from one
+// read of the tile statistics, then the scaled sum of the tiles' noise sums.
+template <int BLOCK>
+__device__ void mppi_fused_finish(const mppi_step_args& a, const mppi_agent_desc& d, float* zs) {
+    constexpr int WAVES = BLOCK / 64;
+    __shared__ float s_f[BLOCK];
+    __shared__ float s_min[WAVES];
+    __shared__ float s_sum[3][WAVES];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t g = threadIdx.x;
+    float4 st = {__builtin_inff(), 0.0f, 0.0f, 0.0f};
+    if (g < d.n_blk) st = reinterpret_cast<const float4*>(a.blk_stats + d.blk_off * 4)[g];
+    float mg = mppi_wave_min(st.x);
+    if (lane == 0) s_min[wave] = mg;
+    __syncthreads();
+    mg = s_min[0];
+#pragma unroll
+    for (int w = 1; w < WAVES; ++w) mg = s_min[w] < mg ? s_min[w] : mg;
+    const float f = (mppi_isfinite(st.x) && mppi_isfinite(mg)) ? mppi_expf(-(st.x - mg) / a.lambda) : 0.0f;
+    const bool uniform = !mppi_isfinite(mg);
+    s_f[g] = uniform ? 1.0f : f;
+    float S = mppi_wave_sum(st.y * f);
+    float Cs = mppi_wave_sum(st.z);
+    float Q = mppi_wave_sum(st.w * (f * f));
+    if (lane == 0) { s_sum[0][wave] = S; s_sum[1][wave] = Cs; s_sum[2][wave] = Q; }
+    __syncthreads();
+    S = s_sum[0][0]; Cs = s_sum[1][0]; Q = s_sum[2][0];
+#pragma unroll
+    for (int w = 1; w < WAVES; ++w) { S += s_sum[0][w]; Cs += s_sum[1][w]; Q += s_sum[2][w]; }
+    if (uniform) {
+        S = float(d.K);
+        Q = float(d.K);
+    }
+    if (threadIdx.x == 0) {
+        mppi_agent_state* ag = &a.agents[d.slot];
+        ag->min_cost = mg;
+        ag->weight_sum = double(S);
+        ag->mean_cost = double(Cs / float(d.K));
+        ag->ess = double((S * S) / Q);
+        ag->rollout_arrivals = 0u;
+    }
+    __syncthreads();
+    mppi_sum_rows<BLOCK, 32>(a.partials + d.part_off, d.n_blk, d.NB * 4, s_f, S, zs);
+}
+
+template <int BLOCK>
+__device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
+                                 uint32_t blk, bool active, float total, float* s_z) {
+    constexpr int WAVES = BLOCK / 64;
+    constexpr int ZS = MPPI_FUSED_MAX_HC + 1;
+    __shared__ float s_red[4][WAVES];
+    __shared__ float s_e[BLOCK];
+    __shared__ float s_h[BLOCK / 2];
+    __shared__ int s_last;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t HC = d.H * CDIM, W = d.NB * 4;
+    float c = (active && mppi_isfinite(total)) ? total : __builtin_inff();
+    float m = mppi_wave_min(c);
+    if (lane == 0) s_red[0][wave] = m;
+    __syncthreads();
+    m = s_red[0][0];
+#pragma unroll
+    for (int w = 1; w < WAVES; ++w) m = s_red[0][w] < m ? s_red[0][w] : m;
+    const float e = (mppi_isfinite(c) && mppi_isfinite(m)) ? mppi_expf(-(c - m) / a.lambda) : 0.0f;
+    s_e[threadIdx.x] = mppi_isfinite(m) ? e : (active ? 1.0f : 0.0f);
+    const float se = mppi_wave_sum(e);
+    const float sq = mppi_wave_sum(e * e);
+    const float sc = mppi_wave_sum(active ? total : 0.0f);
+    if (lane == 0) { s_red[1][wave] = se; s_red[2][wave] = sc; s_red[3][wave] = sq; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float S = 0.0f, Cs = 0.0f, Q = 0.0f;
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) { S += s_red[1][w]; Cs += s_red[2][w]; Q += s_red[3][w]; }
+        float* bs = a.blk_stats + (d.blk_off + blk) * 4;
+        bs[0] = m; bs[1] = S; bs[2] = Cs; bs[3] = Q;
+    }
+    // V_g[v] = sum over the tile's lanes of e_l z_l[v]: two threads per
+    // column, each over half the lanes in order, then added.  Zero weights
+    // select (lanes past K hold no noise).
+    {
+        // threads [0, BLOCK/2) sum lanes [0, BLOCK/2) of column j, the other
+        // half the other lanes of the same column (distinct banks per wave)
+        const uint32_t half = threadIdx.x / (BLOCK / 2), v = threadIdx.x % (BLOCK / 2);
+        float acc = 0.0f;
+        if (v < HC) {
+            // four independent chains keep the LDS reads in flight
+            float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, c3 = 0.0f;
+            const uint32_t l0 = half * (BLOCK / 2);
+            for (uint32_t l = l0; l < l0 + BLOCK / 2; l += 4) {
+                const float e0 = s_e[l], e1 = s_e[l + 1], e2 = s_e[l + 2], e3 = s_e[l + 3];
+                const float z0 = s_z[l * ZS + v], z1 = s_z[(l + 1) * ZS + v];
+                const float z2 = s_z[(l + 2) * ZS + v], z3 = s_z[(l + 3) * ZS + v];
+                c0 = e0 != 0.0f ? mppi_fma(e0, z0, c0) : c0;
+                c1 = e1 != 0.0f ? mppi_fma(e1, z1, c1) : c1;
+                c2 = e2 != 0.0f ? mppi_fma(e2, z2, c2) : c2;
+                c3 = e3 != 0.0f ? mppi_fma(e3, z3, c3) : c3;
+            }
+            acc = (c0 + c1) + (c2 + c3);
+        }
+        if (half == 1) s_h[v] = acc;
+        __syncthreads();
+        if (half == 0 && v < W) a.partials[d.part_off + size_t(blk) * W + v] = v < HC ? acc + s_h[v] : 0.0f;
+    }
+    // The workgroup barrier completes every thread's stores; thread 0's
+    // device-scope fence then publishes them before the arrival count.
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        uint32_t prev = atomicAdd(&a.agents[d.slot].rollout_arrivals, 1u);
+        s_last = (prev == d.n_blk - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    mppi_fused_finish<BLOCK>(a, d, s_z);
+    mppi_apply_update<BLOCK>(a, agent, d, s_z);
 }
 
 // ---------------------------------------------------------------------------
@@ -508,6 +723,31 @@ extern "C" __global__ void __launch_bounds__(256) mppi_reduce(mppi_step_args a) 
     const uint32_t agent = blockIdx.z;
     if (blockIdx.x >= a.desc[agent].n_chunks || blockIdx.y >= a.desc[agent].n_groups) return;
     mppi_reduce_tile<256>(a, agent, blockIdx.x, blockIdx.y);
+}
+
+// One-kernel step (grid as mppi_rollout; H*C <= MPPI_FUSED_MAX_HC, K < 2^18).
+extern "C" __global__ void __launch_bounds__(256) mppi_rollout_fused(mppi_step_args a) {
+    const uint32_t agent = blockIdx.y;
+    if (blockIdx.x >= a.desc[agent].n_blk) return;
+    mppi_rollout_tile<256, true>(a, agent, blockIdx.x);
+}
+
+// Per-sample weights of the last iteration, w_k = exp(-(c_k - m)/lambda) / S
+// (the reduce kernel's formula), for a one-kernel step, which does not form
+// them; grid (ceil(K / 256), N).
+extern "C" __global__ void __launch_bounds__(256) mppi_weights(mppi_step_args a) {
+    const uint32_t agent = blockIdx.y;
+    const mppi_agent_desc d = a.desc[agent];
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= d.K) return;
+    const mppi_agent_state* ag = &a.agents[d.slot];
+    const float m = float(ag->min_cost);
+    const float Sf = float(ag->weight_sum);
+    const float c = a.costs[d.cost_off + k];
+    float w;
+    if (!mppi_isfinite(m)) w = 1.0f;
+    else w = mppi_isfinite(c) ? mppi_expf(-(c - m) / a.lambda) : 0.0f;
+    a.weights[d.w_off + k] = w / Sf;
 }
 
 #if MPPI_BATCH_KERNELS

@@ -225,6 +225,7 @@ int solve_impl(mppi_hip_server* s, uint32_t n, const uint32_t* ids, float* first
     ctx->last_agents = n;
     ctx->last_costs = arena;
     ctx->last_weights = arena;
+    ctx->weights_pending = false;
     ctx->last_desc = desc;
     return MPPI_HIP_OK;
 }

@@ -9,7 +9,9 @@ Tolerances (FP32):
                     w_k z_k in a tree, the oracle sequentially in k)
   mean cost, ESS    1e-5 relative
 Every result's error against the FP64 oracle is printed (pytest -s) and
-bounded.
+bounded.  The single-step cases run on both step paths (two kernels, and the
+one-kernel step that keeps the noise in LDS); the rest run on the default
+path, which is the one-kernel step for every shape here except K = 2^18.
 """
 import numpy as np
 import pytest
@@ -45,8 +47,13 @@ def _check_step(hip, ora, step, x0, u_nom, name, check_fp64=True):
     return u_g, d_g
 
 
-def test_unicycle_k1024_h32_obs8_parity(require_gpu):
+PATHS = pytest.mark.parametrize("path", [1, 2], ids=["two_kernel", "fused"])
+
+
+@PATHS
+def test_unicycle_k1024_h32_obs8_parity(require_gpu, path):
     hip, ora, sc = unicycle(1024, 32, 8)
+    hip.set_step_path(path)
     x0 = np.zeros(4, np.float32)
     rng = np.random.default_rng(5)
     u_nom = (rng.standard_normal((32, 2)) * 0.3).astype(np.float32)
@@ -54,16 +61,20 @@ def test_unicycle_k1024_h32_obs8_parity(require_gpu):
         _check_step(hip, ora, step, x0, u_nom, f"unicycle C1 step {step}")
 
 
-def test_double_integrator_reference_fixture(require_gpu):
+@PATHS
+def test_double_integrator_reference_fixture(require_gpu, path):
     hip, ora = double_integrator(K=64, H=10, seed=0)
+    hip.set_step_path(path)
     x0 = np.array([1.0, 1.0, 0.0, 0.0], np.float32)
     u, d = _check_step(hip, ora, 0, x0, np.zeros((10, 2), np.float32), "double integrator")
     assert u[0, 0] < 0 and u[0, 1] < 0 and d[0] > 0 and d[1] >= d[0] and d[2] > 0
 
 
-def test_full_covariance_parity(require_gpu):
+@PATHS
+def test_full_covariance_parity(require_gpu, path):
     cov = np.array([[0.30, 0.12], [0.12, 0.20]], np.float32)
     hip, ora, _ = unicycle(2048, 24, 8, cov=cov)
+    hip.set_step_path(path)
     _check_step(hip, ora, 3, np.array([0.5, -0.5, 0.3, 0.4], np.float32),
                 np.zeros((24, 2), np.float32), "full covariance")
 
@@ -89,8 +100,10 @@ def test_determinism_same_seed(require_gpu):
 CLOSED_LOOP_TOL = 2e-3
 
 
-def test_simulate_matches_oracle(require_gpu):
+@PATHS
+def test_simulate_matches_oracle(require_gpu, path):
     hip, ora, sc = unicycle(1024, 32, 8)
+    hip.set_step_path(path)
     x0 = np.zeros(4, np.float32)
     st, ct, kt = hip.simulate(6, 0, x0, np.zeros((32, 2), np.float32))
     so, co, ko, _ = ora.simulate(6, 0, x0, threads=8)
@@ -113,8 +126,10 @@ def test_simulate_batch_matches_oracle(require_gpu):
     assert np.max(np.abs(tt - to)) <= CLOSED_LOOP_TOL
 
 
-def test_unicycle_k65536_h64_obs16_full_size(require_gpu):
+@PATHS
+def test_unicycle_k65536_h64_obs16_full_size(require_gpu, path):
     hip, ora, sc = unicycle(65536, 64, 16)
+    hip.set_step_path(path)
     x0 = np.zeros(4, np.float32)
     u_nom = np.zeros((64, 2), np.float32)
     u_g, d_g = hip.step(0, x0, u_nom)
@@ -163,9 +178,11 @@ MPPI_CALLABLE float stage(const float* x, const float* u, const uint8_t* m, cons
 '''
 
 
-def test_nonfinite_costs_get_zero_weight(require_gpu):
+@PATHS
+def test_nonfinite_costs_get_zero_weight(require_gpu, path):
     from mppi_metal_amd import HipMPPI
     hip = HipMPPI(2, 2, 4, 1024, 1.0, [1.0, 1.0], [-1, -1], [1, 1])
+    hip.set_step_path(path)
     hip.link_source(NAN_PLUGIN, "dyn", "stage", "", 0, 8, 9)
     for inf in (0.0, 1.0):
         hip.bind_params(None, np.array([0.0, inf], np.float32))
