@@ -34,6 +34,7 @@ BUNDLED_PLUGINS = [
     ("unicycle.hip", "mppi_dynamics", "unicycle_stage_cost_obstacles", "unicycle_terminal_cost", 4, 2),
     ("bicycle.hip", "bicycle_dynamics", "bicycle_pursuit_cost", "bicycle_terminal_cost", 6, 2),
     ("quadrotor.hip", "quad_dynamics", "quad_stage_cost", "quad_terminal_cost", 12, 4),
+    ("point_mass3.hip", "pm3_dynamics", "pm3_stage_cost", "pm3_terminal_cost", 6, 3),
 ]
 
 EMBEDDED = [

@@ -52,6 +52,9 @@ namespace plug_bic {
 namespace plug_quad {
 #include "../mppi_metal_amd/plugins/quadrotor.hip"
 }
+namespace plug_pm3 {
+#include "../mppi_metal_amd/plugins/point_mass3.hip"
+}
 
 namespace {
 
@@ -195,6 +198,38 @@ double uni_term_d(const double* s, const uint8_t*, const uint8_t* cp, const mppi
     const UniCost* p = reinterpret_cast<const UniCost*>(cp);
     double dx = s[0] - p->x_goal, dy = s[1] - p->y_goal;
     return p->pos_weight * (dx * dx + dy * dy) * 10.0;
+}
+
+// ---- three-axis point mass (mppi_metal_amd/plugins/point_mass3.hip) ----
+struct Pm3Model { float dt, process_std, pad[2]; };
+struct Pm3Cost { float goal[3], w_pos, w_vel, w_control, w_terminal, pad; };
+void pm3_dyn_d(double* s, const double* u, const uint8_t* mp, const RngCtx& r) {
+    const Pm3Model* p = reinterpret_cast<const Pm3Model*>(mp);
+    double w[4];
+    normals4<double>((r.t << 8) + 0u, r.sample, r.step, r.tag, r.agent, r.seed, w);
+    const double dt = p->dt;
+    for (int i = 0; i < 3; ++i) s[i] += dt * s[3 + i];
+    for (int i = 0; i < 3; ++i) s[3 + i] += dt * u[i] + double(p->process_std) * w[i];
+}
+double pm3_stage_d(const double* s, const double* u, const uint8_t*, const uint8_t* cp, const mppi_time&) {
+    const Pm3Cost* c = reinterpret_cast<const Pm3Cost*>(cp);
+    double ep = 0, ev = 0, eu = 0;
+    for (int i = 0; i < 3; ++i) {
+        const double e = s[i] - c->goal[i];
+        ep += e * e;
+        ev += s[3 + i] * s[3 + i];
+        eu += u[i] * u[i];
+    }
+    return c->w_pos * ep + c->w_vel * ev + c->w_control * eu;
+}
+double pm3_term_d(const double* s, const uint8_t*, const uint8_t* cp, const mppi_time&) {
+    const Pm3Cost* c = reinterpret_cast<const Pm3Cost*>(cp);
+    double ep = 0;
+    for (int i = 0; i < 3; ++i) {
+        const double e = s[i] - c->goal[i];
+        ep += e * e;
+    }
+    return c->w_terminal * ep;
 }
 
 // ---- dynamic bicycle (mppi_metal_amd/plugins/bicycle.hip) ----
@@ -348,6 +383,21 @@ const Callable kCallables[] = {
          return plug_bic::bicycle_terminal_cost(x, m, c, tm);
      }),
      FP(&bic_term_d)},
+    {"point_mass3", "pm3_dynamics", 0,
+     FP(+[](float* x, const float* u, const uint8_t* m, const RngCtx& r) {
+         plug_pm3::pm3_dynamics(x, u, m, to_rng(r));
+     }),
+     FP(&pm3_dyn_d)},
+    {"point_mass3", "pm3_stage_cost", 1,
+     FP(+[](const float* x, const float* u, const uint8_t* m, const uint8_t* c, const mppi_time&) {
+         return plug_pm3::pm3_stage_cost(x, u, m, c);
+     }),
+     FP(&pm3_stage_d)},
+    {"point_mass3", "pm3_terminal_cost", 2,
+     FP(+[](const float* x, const uint8_t* m, const uint8_t* c, const mppi_time&) {
+         return plug_pm3::pm3_terminal_cost(x, m, c);
+     }),
+     FP(&pm3_term_d)},
     {"quadrotor", "quad_dynamics", 0,
      FP(+[](float* x, const float* u, const uint8_t* m, const RngCtx& r) {
          plug_quad::quad_dynamics(x, u, m, to_rng(r));

@@ -2,6 +2,8 @@
 oracle (test infrastructure)."""
 from __future__ import annotations
 
+import struct
+
 import numpy as np
 
 import oracle_lib as ol
@@ -79,6 +81,37 @@ def quadrotor(K, H, numeric_mode=0, seed=SEED, goal=(2.0, -1.0, 1.5)):
     return hip, ora
 
 
+PM3_SIGMA = (0.8, 0.6, 0.7)
+PM3_LO, PM3_HI = (-2.0, -2.0, -1.5), (2.0, 2.0, 1.5)
+
+
+def pm3_blobs(dt=0.05, process_std=0.2, goal=(1.0, -0.5, 0.8), w_pos=1.0, w_vel=0.1, w_control=0.05,
+              w_terminal=5.0):
+    """Model and cost blobs of the three-axis point mass (plugins/point_mass3.hip)."""
+    mb = struct.pack("<4f", dt, process_std, 0.0, 0.0)
+    cb = struct.pack("<8f", *goal, w_pos, w_vel, w_control, w_terminal, 0.0)
+    return mb, cb
+
+
+def point_mass3(K, H, seed=SEED, lam=2.0, process_std=0.2, numeric_mode=0):
+    """Three-control point mass whose dynamics draws process noise from its
+    random handle: noise blocks straddle timesteps (C = 3), and the GPU's and
+    the oracle's dynamics streams must agree."""
+    mb, cb = pm3_blobs(process_std=process_std)
+    hip = HipMPPI(6, 3, H, K, lam, PM3_SIGMA, PM3_LO, PM3_HI, numeric_mode=numeric_mode)
+    hip.link("point_mass3.hip", "pm3_dynamics", "pm3_stage_cost", "pm3_terminal_cost", len(mb), len(cb), seed)
+    hip.bind_params(mb, cb)
+    ora = point_mass3_oracle(K, H, seed, lam, process_std)
+    return hip, ora
+
+
+def point_mass3_oracle(K, H, seed=SEED, lam=2.0, process_std=0.2):
+    mb, cb = pm3_blobs(process_std=process_std)
+    return ol.Problem("point_mass3", 6, 3, H, K, lam, np.array(PM3_SIGMA, np.float32),
+                      np.array(PM3_LO, np.float32), np.array(PM3_HI, np.float32), seed, "pm3_dynamics",
+                      "pm3_stage_cost", "pm3_terminal_cost", mb, cb)
+
+
 def rel_err(a, b, floor=0.0):
     """max |a - b| / max(|b|, floor), elementwise."""
     a = np.asarray(a, np.float64)
@@ -91,3 +124,34 @@ def seq_rel_err(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+# ------------------------------------------------------ GPU-vs-oracle step --
+# FP32 bars: sample costs and the best cost bit-exact; weights 1e-5 relative
+# elementwise; u_next 1e-5 of the sequence's largest magnitude (tree versus
+# sequential sums); mean cost and ESS 1e-5 relative.  Errors against the FP64
+# oracle are printed and the costs' bounded.
+def report(name, **errs):
+    print(f"[{name}] " + " ".join(f"{k}={v:.3e}" for k, v in errs.items()), flush=True)
+
+
+def check_step(hip, ora, step, x0, u_nom, name, check_fp64=True):
+    u_g, d_g = hip.step(step, x0, u_nom)
+    c_g, w_g = hip.read_costs(), hip.read_weights()
+    u_o, d_o, c_o, w_o = ora.step(step, x0, u_nom, threads=8)
+    ulps = np.abs(c_g.view(np.int32).astype(np.int64) - c_o.view(np.int32).astype(np.int64))
+    assert np.array_equal(c_g, c_o), f"costs differ: {np.count_nonzero(ulps)} samples, max {ulps.max()} ulp"
+    assert d_g[0] == d_o[0]
+    assert rel_err(w_g, w_o, 1e-37) <= 1e-5
+    assert abs(w_g.sum() - 1.0) <= 1e-5
+    assert seq_rel_err(u_g, u_o) <= 1e-5
+    assert rel_err(d_g[1:], d_o[1:]) <= 1e-5
+    errs = dict(u_vs_fp32=seq_rel_err(u_g, u_o), w_vs_fp32=rel_err(w_g, w_o, 1e-37),
+                ess_vs_fp32=rel_err(d_g[2:], d_o[2:]))
+    if check_fp64:
+        u_d, d_d, c_d, w_d = ora.step(step, x0, np.asarray(u_nom, np.float64), threads=8, fp64=True)
+        errs.update(cost_vs_fp64=rel_err(c_g, c_d), u_vs_fp64=seq_rel_err(u_g, u_d),
+                    oracle32_u_vs_fp64=seq_rel_err(u_o, u_d))
+        assert errs["cost_vs_fp64"] < 1e-3
+    report(name, **errs)
+    return u_g, d_g

@@ -9,43 +9,18 @@ Tolerances (FP32):
                     w_k z_k in a tree, the oracle sequentially in k)
   mean cost, ESS    1e-5 relative
 Every result's error against the FP64 oracle is printed (pytest -s) and
-bounded.  The single-step cases run on both step paths (two kernels, and the
-one-kernel step that keeps the noise in LDS); the rest run on the default
-path, which is the one-kernel step for every shape here except K = 2^18.
+bounded.  The single-step and closed-loop cases run on both step paths (two
+kernels, and the one-kernel step that keeps the noise in LDS); the rest run
+on the default path, which is the one-kernel step for every shape here
+except K = 2^18.
 """
 import numpy as np
 import pytest
 
-from problems import double_integrator, rel_err, seq_rel_err, unicycle
+from problems import check_step as _check_step
+from problems import double_integrator, rel_err, report as _report, seq_rel_err, unicycle
 
 pytestmark = pytest.mark.gpu
-
-
-def _report(name, **errs):
-    print(f"[{name}] " + " ".join(f"{k}={v:.3e}" for k, v in errs.items()), flush=True)
-
-
-def _check_step(hip, ora, step, x0, u_nom, name, check_fp64=True):
-    u_g, d_g = hip.step(step, x0, u_nom)
-    c_g, w_g = hip.read_costs(), hip.read_weights()
-    u_o, d_o, c_o, w_o = ora.step(step, x0, u_nom, threads=8)
-    ulps = np.abs(c_g.view(np.int32).astype(np.int64) - c_o.view(np.int32).astype(np.int64))
-    assert np.array_equal(c_g, c_o), f"costs differ: {np.count_nonzero(ulps)} samples, max {ulps.max()} ulp"
-    assert d_g[0] == d_o[0]
-    assert rel_err(w_g, w_o, 1e-37) <= 1e-5
-    assert abs(w_g.sum() - 1.0) <= 1e-5
-    assert seq_rel_err(u_g, u_o) <= 1e-5
-    assert rel_err(d_g[1:], d_o[1:]) <= 1e-5
-    errs = dict(u_vs_fp32=seq_rel_err(u_g, u_o), w_vs_fp32=rel_err(w_g, w_o, 1e-37),
-                ess_vs_fp32=rel_err(d_g[2:], d_o[2:]))
-    if check_fp64:
-        u_d, d_d, c_d, w_d = ora.step(step, x0, np.asarray(u_nom, np.float64), threads=8, fp64=True)
-        errs.update(cost_vs_fp64=rel_err(c_g, c_d), u_vs_fp64=seq_rel_err(u_g, u_d),
-                    oracle32_u_vs_fp64=seq_rel_err(u_o, u_d))
-        assert errs["cost_vs_fp64"] < 1e-3
-    _report(name, **errs)
-    return u_g, d_g
-
 
 PATHS = pytest.mark.parametrize("path", [1, 2], ids=["two_kernel", "fused"])
 
@@ -114,8 +89,10 @@ def test_simulate_matches_oracle(require_gpu, path):
     assert rel_err(kt, ko[:, 0]) <= CLOSED_LOOP_TOL
 
 
-def test_simulate_batch_matches_oracle(require_gpu):
+@PATHS
+def test_simulate_batch_matches_oracle(require_gpu, path):
     hip, ora, sc = unicycle(512, 16, 8)
+    hip.set_step_path(path)
     x0 = np.array([[0, 0, 0, 0], [1, -1, 0.5, 0.2], [-2, 1, -0.3, 0.0]], np.float32)
     st, ct, kt, tt = hip.simulate_batch(3, 4, 10, x0)
     so, co, ko, to = ora.simulate(4, 10, x0, n_agents=3, threads=8)

@@ -9,7 +9,9 @@ here before the oracle is used to judge the GPU.
   reduce   uniform weights give the sample mean of the noise; one-hot
            weights give that sample's noise; clamping to the bounds.
   rollout  the double integrator is linear, so its costs have a closed form
-           given the noise: x_{t+1} = A x_t + B u_t in matrix form.
+           given the noise: x_{t+1} = A x_t + B u_t in matrix form; the
+           three-axis point mass adds process noise from the dynamics stream
+           and is checked from running sums of the same draws.
   noise    normal n = t*C + d of a sample comes from Philox block n // 4,
            lane n % 4; full covariance noise has covariance L L^T = Sigma.
   closed loop  the reference tests' relations: simulate == loop of step +
@@ -20,7 +22,7 @@ import pytest
 from scipy.special import softmax
 
 import oracle_lib as ol
-from problems import unicycle_oracle
+from problems import PM3_HI, PM3_LO, PM3_SIGMA, point_mass3_oracle, unicycle_oracle
 
 SIG = np.array([0.7, 0.4], np.float32)
 LO, HI = np.array([-1.5, -1.0], np.float32), np.array([1.5, 1.0], np.float32)
@@ -142,6 +144,46 @@ def test_rollout_double_integrator_closed_form():
     p2 = di(K=24, H=12, terminal="")
     assert np.max(np.abs(p2.rollout(5, x0, u_nom, fp64=True) -
                          closed_form_di_costs(p2, x0, u_nom, 5, with_terminal=False)) / want) < 1e-12
+
+
+def closed_form_pm3_costs(p, x0, u_nom, step, agent=0, process_std=0.2):
+    """Costs of the three-axis point mass from running sums: with a_t the
+    clamped controls and w_t the process normals (draw 0 of the dynamics
+    stream at timestep t, lanes 0..2),
+        v_t = v_0 + sum_{j<t} (dt a_j + std w_j),   p_{t+1} = p_0 + dt sum_{i<=t} v_i."""
+    dt = np.float64(np.float32(0.05))
+    std = np.float64(np.float32(process_std))
+    goal = np.array([1.0, -0.5, 0.8], np.float32).astype(np.float64)
+    w_pos, w_vel, w_u, w_term = 1.0, np.float64(np.float32(0.1)), np.float64(np.float32(0.05)), 5.0
+    sig, lo, hi = (np.array(v, np.float32).astype(np.float64) for v in (PM3_SIGMA, PM3_LO, PM3_HI))
+    x0 = x0.astype(np.float64)
+    out = []
+    for k in range(p.K):
+        a = np.clip(u_nom + noise64(p, k, step, agent) * sig, lo, hi)
+        w = np.array([ol.normals4(t << 8, k, step, 1, agent, p.s.seed, fp64=True)[:3] for t in range(p.H)])
+        dv = dt * a + std * w
+        v = x0[3:] + np.vstack([np.zeros(3), np.cumsum(dv, axis=0)])    # v_0 .. v_H
+        pos = x0[:3] + dt * np.cumsum(v[:-1], axis=0)                   # p_1 .. p_H
+        c = (w_pos * np.sum((pos - goal) ** 2) + w_vel * np.sum(v[1:] ** 2) + w_u * np.sum(a ** 2) +
+             w_term * np.sum((pos[-1] - goal) ** 2))
+        out.append(c)
+    return np.array(out)
+
+
+def test_rollout_point_mass_with_process_noise_closed_form():
+    H = 9                                 # H*C = 27: noise blocks straddle timesteps
+    p = point_mass3_oracle(K=20, H=H)
+    x0 = np.array([0.2, -0.1, 0.4, 0.3, 0.0, -0.2], np.float32)
+    u_nom = np.random.default_rng(8).uniform(-0.5, 0.5, (H, 3))
+    for agent in (0, 3):
+        want = closed_form_pm3_costs(p, x0, u_nom, 4, agent)
+        got64 = p.rollout(4, x0, u_nom, agent=agent, fp64=True)
+        assert np.max(np.abs(got64 - want) / want) < 1e-12
+        got32 = p.rollout(4, x0, u_nom.astype(np.float32), agent=agent)
+        assert np.max(np.abs(got32 - want) / want) < 1e-5
+    # without process noise the costs change: the dynamics stream is used
+    quiet = point_mass3_oracle(K=20, H=H, process_std=0.0)
+    assert not np.allclose(quiet.rollout(4, x0, u_nom, fp64=True), want)
 
 
 def test_rollout_depends_on_step_agent_and_seed():
