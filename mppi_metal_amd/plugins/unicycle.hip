@@ -76,23 +76,26 @@ MPPI_CALLABLE float unicycle_stage_cost_obstacles(const float* state, const floa
     float pen = 0.0f;
     uint32_t n = p->n_obstacles < UNICYCLE_MAX_OBSTACLES ? p->n_obstacles : UNICYCLE_MAX_OBSTACLES;
     // max(0, r - |p - c|) summed over the obstacles.  The square root is
-    // only needed inside an obstacle: a first pass finds whether this sample
-    // is inside any, and the penalty pass runs only in wavefronts where some
-    // sample is.  Four obstacles per iteration so their parameter loads are
-    // issued together; slots past n have radius 0 and never add a penalty.
-    const uint32_t n4 = (n + 3u) & ~3u;
-    bool inside_any = false;
-    for (uint32_t i = 0; i < n4; i += 4) {
+    // only needed inside an obstacle, so a cheap first pass tests the square
+    // |ox|, |oy| < r around each circle: inside the circle (d2 < r*r in the
+    // rounded arithmetic below) implies inside the square, since rounding is
+    // monotone.  The exact pass runs only in wavefronts where some sample is
+    // in some square.  The first pass reads obstacles in groups of eight so
+    // their parameter loads are issued together; slots in [n, n8) are read
+    // but only ever send a wavefront to the exact pass, which stops at n.
+    const uint32_t n8 = (n + 7u) & ~7u;
+    float margin = 1.0f;        // min over obstacles of max(|ox|, |oy|) - r
+    for (uint32_t i = 0; i < n8; i += 8) {
         MPPI_UNROLL
-        for (uint32_t j = 0; j < 4; ++j) {
+        for (uint32_t j = 0; j < 8; ++j) {
             float ox = state[0] - p->obstacles[i + j][0];
             float oy = state[1] - p->obstacles[i + j][1];
             float r = p->obstacles[i + j][2];
-            inside_any |= ox * ox + oy * oy < r * r;
+            margin = __builtin_fminf(margin, __builtin_fmaxf(__builtin_fabsf(ox), __builtin_fabsf(oy)) - r);
         }
     }
-    if (mppi_any(inside_any)) {
-        for (uint32_t i = 0; i < n4; ++i) {
+    if (mppi_any(margin < 0.0f)) {
+        for (uint32_t i = 0; i < n; ++i) {
             float ox = state[0] - p->obstacles[i][0];
             float oy = state[1] - p->obstacles[i][1];
             float r = p->obstacles[i][2];
