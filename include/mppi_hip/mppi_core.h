@@ -47,13 +47,7 @@
 // Guard a loop or a large block: the optimiser evaluates a small guarded
 // block unconditionally and selects its result.  Results are the same either
 // way; on the host it is the identity.
-MPPI_HD bool mppi_any(bool c) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_amdgcn_ballot_w64(c) != 0;
-#else
-    return c;
-#endif
-}
+MPPI_HD bool mppi_any(bool c) { return mppi_wave_any_(c); }
 
 // Random-number handle given to the dynamics callable.  Draw j of timestep t
 // of sample k uses counter (t << 8 | j, k, step, tag << 28 | agent); a

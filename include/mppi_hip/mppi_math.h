@@ -67,6 +67,17 @@ MPPI_HD uint32_t mppi_f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
 MPPI_HD float mppi_u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
 
 MPPI_HD float mppi_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// True when c holds in any lane of the wavefront (on the host: c itself).
+// Guards a rarely needed slow path so that it costs a uniform branch rather
+// than divergent control flow in the common case.
+MPPI_HD bool mppi_wave_any_(bool c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ballot_w64(c) != 0;
+#else
+    return c;
+#endif
+}
 MPPI_HD float mppi_sqrtf(float x) {
 #if MPPI_FAST
     return __builtin_amdgcn_sqrtf(x);
@@ -197,8 +208,17 @@ MPPI_HD void mppi_quadrant(int q, float s, float c, float* so, float* co) {
     *co = ((q + 1) & 2) ? -ca : ca;
 }
 
+// The quadrant (mod 4) of an integral float k.  Converting a float beyond
+// the int range is undefined in C++ and differs between targets (x86 gives
+// INT_MIN, the GPU saturates), so such k (and NaN) map to 0: every float
+// with |k| >= 2^26 is a multiple of 4 anyway.
+MPPI_HD int mppi_quadrant_index(float k) {
+    return mppi_fabsf(k) < 67108864.0f ? int(k) : 0;
+}
+
 // sin(x), cos(x).  Accurate for |x| <= 1e5 (the three-part pi/2 keeps the
-// reduced argument exact enough there); deterministic everywhere finite.
+// reduced argument exact enough there); the same bits on every target for
+// every input.
 MPPI_HD void mppi_sincosf(float x, float* s, float* c) {
 #if MPPI_FAST
     {
@@ -219,7 +239,7 @@ MPPI_HD void mppi_sincosf(float x, float* s, float* c) {
     float r = mppi_fma(-kf, pio2_1, x);
     r = mppi_fma(-kf, pio2_2, r);
     r = mppi_fma(-kf, pio2_3, r);
-    mppi_quadrant(int(kf), mppi_sin_poly(r), mppi_cos_poly(r), s, c);
+    mppi_quadrant(mppi_quadrant_index(kf), mppi_sin_poly(r), mppi_cos_poly(r), s, c);
 }
 
 MPPI_HD float mppi_sinf(float x) { float s, c; mppi_sincosf(x, &s, &c); return s; }
@@ -237,31 +257,36 @@ MPPI_HD void mppi_sincospif(float x, float* s, float* c) {
     float n = mppi_rintf(x * 2.0f);                // exact scaling
     float f = x - n * 0.5f;                        // exact: f in [-1/4, 1/4]
     const float pi_f = 3.14159274101257324219f;
-    mppi_quadrant(int(n), mppi_sin_poly(f * pi_f), mppi_cos_poly(f * pi_f), s, c);
+    mppi_quadrant(mppi_quadrant_index(n), mppi_sin_poly(f * pi_f), mppi_cos_poly(f * pi_f), s, c);
 }
 
 // ---------------------------------------------------------------------------
 // fmod: IEEE remainder toward zero, exact.
 // ---------------------------------------------------------------------------
 MPPI_HD float mppi_fmodf(float x, float y) {
-    float ax = mppi_fabsf(x), ay = mppi_fabsf(y);
-    if (!(ax == ax) || !(ay == ay) || ay == 0.0f || ax == mppi_u2f(0x7f800000u))
-        return mppi_u2f(0x7fc00000u);
-    if (ax < ay) return x;
-    // One subtraction suffices when ay <= ax < 2 ay (exact by Sterbenz).
-    if (ax < ay * 2.0f) return x < 0.0f ? -(ax - ay) : ax - ay;
-    // Binary long division: yj runs over ay*2^i from the largest value with
-    // ax < 2*yj down to ay.  With yj <= r < 2*yj each r - yj is exact
-    // (Sterbenz), and halving yj is exact, so the remainder is exact.
-    float yj = ay;
-    while (!(yj > ax * 0.5f)) yj = yj * 2.0f;
-    float r = ax;
-    for (;;) {
-        if (yj <= r) r = r - yj;
-        if (yj == ay) break;
-        yj = yj * 0.5f;
+    const float ax = mppi_fabsf(x), ay = mppi_fabsf(y);
+    const bool nan_result = !(ax == ax) || !(ay == ay) || ay == 0.0f || ax == mppi_u2f(0x7f800000u);
+    // ax < ay: the remainder is |x|.  ay <= ax < 2 ay: one subtraction, exact
+    // by Sterbenz.  Both are selects, so the usual case has no branch.
+    float r = ax < ay ? ax : ax - ay;
+    const bool long_div = !nan_result && !(ax < ay * 2.0f);
+    // Otherwise binary long division: yj runs over ay*2^i from the largest
+    // value with ax < 2*yj down to ay.  With yj <= r < 2*yj each r - yj is
+    // exact (Sterbenz), and halving yj is exact, so the remainder is exact.
+    // On the GPU the loop is entered only by wavefronts that need it.
+    if (mppi_wave_any_(long_div) && long_div) {
+        float yj = ay;
+        while (!(yj > ax * 0.5f)) yj = yj * 2.0f;
+        r = ax;
+        for (;;) {
+            if (yj <= r) r = r - yj;
+            if (yj == ay) break;
+            yj = yj * 0.5f;
+        }
     }
-    return x < 0.0f ? -r : r;
+    // The result has the sign of x (x itself when |x| < |y|, -0 included).
+    r = mppi_u2f(mppi_f2u(r) | (mppi_f2u(x) & 0x80000000u));
+    return nan_result ? mppi_u2f(0x7fc00000u) : r;
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,98 @@
+"""The shared single-precision elementary functions (include/mppi_hip/mppi_math.h)
+evaluated on the GPU inside a rollout, bit for bit against the same source
+compiled for the host (the oracle), over random arguments and special values
+(signed zeros, infinities, NaN, subnormals, overflow and underflow edges,
+long-division fmod quotients).
+
+A test plugin evaluates function `fn` on input `sample` of its model blob in
+the dynamics callable, and the terminal cost returns it, so with H = 1 and a
+zero stage cost sample k's cost is 0 + f(x_k) (the addition turns -0 into +0;
+the host side applies the same addition).  FP32 mode only: the MIXED and FP16
+modes use the hardware's approximate instructions instead.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from mppi_metal_amd import HipMPPI
+
+pytestmark = pytest.mark.gpu
+
+MATH_PLUGIN = r'''
+#include "mppi_core.h"
+// blob: [fn, 0, 0, 0, x_0, y_0, x_1, y_1, ...]; sample k evaluates fn on (x_k, y_k).
+MPPI_CALLABLE void eval_fn(float* s, const float* u, const uint8_t* m, mppi_rng r) {
+    const float* p = reinterpret_cast<const float*>(m);
+    const uint32_t fn = reinterpret_cast<const uint32_t*>(m)[0];
+    const float x = p[4 + 2 * r.sample], y = p[5 + 2 * r.sample];
+    float v = 0.0f, sn, cs;
+    switch (fn) {
+        case 0: v = mppi_expf(x); break;
+        case 1: v = mppi_logf(x); break;
+        case 2: v = mppi_sinf(x); break;
+        case 3: v = mppi_cosf(x); break;
+        case 4: mppi_sincospif(x, &sn, &cs); v = sn; break;
+        case 5: mppi_sincospif(x, &sn, &cs); v = cs; break;
+        case 6: v = mppi_tanhf(x); break;
+        case 7: v = mppi_sqrtf(x); break;
+        default: v = mppi_fmodf(x, y); break;
+    }
+    s[0] = v;
+}
+MPPI_CALLABLE float zero_stage(const float* s, const float* u, const uint8_t* m, const uint8_t* c) {
+    return 0.0f;
+}
+MPPI_CALLABLE float value(const float* s, const uint8_t* m, const uint8_t* c) { return s[0]; }
+'''
+
+F = np.float32
+SPECIAL = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1e-45, -1e-45, 1e-40, -1e-40, 1.17549435e-38,
+                    1.0, -1.0, 0.5, -0.5, 0.25, 2.0, 88.72, 88.73, -87.33, -103.9, 9.0, -9.0, 0.55,
+                    -0.55, 3.14159265, 1.5707963, 6.2831853, 256.0, 1e4, -1e4, 1e30, -3.4e38, 3.4e38],
+                   F)
+
+
+def _inputs(fn, rng):
+    if fn == 0:
+        x = rng.uniform(-110, 95, 3000)
+    elif fn in (1, 7):
+        x = np.concatenate([10.0 ** rng.uniform(-45, 38.5, 2500), -rng.uniform(0, 10, 100)])
+    elif fn in (2, 3):
+        x = np.concatenate([rng.uniform(-20, 20, 2000), rng.uniform(-1e5, 1e5, 1000)])
+    elif fn in (4, 5):
+        x = np.concatenate([rng.uniform(-4, 4, 2000), rng.uniform(-1e6, 1e6, 500),
+                            np.round(rng.uniform(-64, 64, 500)) / 4])
+    elif fn == 6:
+        x = np.concatenate([rng.uniform(-1, 1, 1500), rng.uniform(-25, 25, 1500)])
+    else:                               # fmod: random pairs and every pair of specials
+        gx, gy = (a.ravel() for a in np.meshgrid(SPECIAL, SPECIAL))
+        yr = rng.uniform(0.05, 60, 2000) * rng.choice([-1, 1], 2000)
+        return (np.concatenate([rng.uniform(-1e4, 1e4, 2000).astype(F), gx]),
+                np.concatenate([yr.astype(F), gy]))
+    x = np.concatenate([x.astype(F), SPECIAL])
+    return x, np.zeros_like(x)
+
+
+def _host(fn, x, y):
+    return ol.fmodf(x, y) if fn == 8 else ol.mathf(fn, x)
+
+
+@pytest.mark.parametrize("fn", range(9), ids=["exp", "log", "sin", "cos", "sinpi", "cospi", "tanh", "sqrt",
+                                              "fmod"])
+def test_math_bit_for_bit(require_gpu, fn):
+    x, y = _inputs(fn, np.random.default_rng(100 + fn))
+    K = x.size
+    blob = np.zeros(4 + 2 * K, F)
+    blob[:4].view(np.uint32)[0] = fn
+    blob[4::2], blob[5::2] = x, y
+    hip = HipMPPI(1, 1, 1, K, 1.0, [1.0], [-1.0], [1.0])
+    hip.link_source(MATH_PLUGIN, "eval_fn", "zero_stage", "value", blob.nbytes, 0, 1, name="math_probe.hip")
+    hip.bind_params(blob.tobytes(), None)
+    hip.step(0, np.zeros(1, F), np.zeros(1, F))
+    got = hip.read_costs()
+    want = F(0.0) + _host(fn, x, y)
+    nan = np.isnan(want)
+    assert np.array_equal(np.isnan(got), nan)
+    diff = got[~nan].view(np.uint32) != want[~nan].view(np.uint32)
+    assert not diff.any(), f"{diff.sum()} of {K} differ, e.g. x={x[~nan][diff][:4]} gpu={got[~nan][diff][:4]} " \
+                           f"host={want[~nan][diff][:4]}"

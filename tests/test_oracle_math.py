@@ -79,3 +79,17 @@ def test_fmod_exact():
     two_pi = np.float32(2.0 * np.float32(3.1415926535))
     th = np.linspace(-10, 10, 100001, dtype=np.float32)
     assert np.array_equal(ol.fmodf(th, np.full_like(th, two_pi)), np.fmod(th, two_pi))
+
+
+def test_fmod_special_values_bit_for_bit():
+    # signed zeros, infinities, NaN, negative and subnormal divisors, and
+    # quotients up to 2^200 (the long-division path), against numpy's fmod
+    # bit for bit (any NaN matches any NaN)
+    f = np.float32
+    vals = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1e-45, -1e-45, 1e-40, 3.0, -3.0, 6.0, -7.5,
+                     2.5, 1e30, -3.4e38, 3.4e38, 0.1, -0.1], f)
+    x, y = (a.ravel() for a in np.meshgrid(vals, vals))
+    got, want = ol.fmodf(x, y), np.fmod(x, y)
+    nan = np.isnan(want)
+    assert np.array_equal(np.isnan(got), nan)
+    assert np.array_equal(got[~nan].view(np.uint32), want[~nan].view(np.uint32))
