@@ -255,7 +255,8 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
 
     // The nominal sequence goes to LDS and the noise scale and bounds to
     // registers, so the time loop issues no memory waits of its own.
-    __shared__ float s_un[FUSED ? MPPI_FUSED_MAX_HC : MPPI_MAX_HC];
+    // 2 * MPPI_NG slots of padding: the time loop reads one group ahead.
+    __shared__ __attribute__((aligned(16))) float s_un[(FUSED ? MPPI_FUSED_MAX_HC : MPPI_MAX_HC) + 2 * MPPI_NG];
     // [lane][H*C + 1]: a lane's writes and a column's reads are both
     // conflict-free.
     constexpr int ZS = MPPI_FUSED_MAX_HC + 1;
@@ -276,7 +277,19 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
         float x[SDIM];
 #pragma unroll
         for (int i = 0; i < SDIM; ++i) x[i] = a.x[size_t(slot) * SDIM + i];
+        // The nominal controls of group t0 + G are read while group t0 runs,
+        // so the LDS latency hides behind a group's work.  The read past the
+        // horizon lands in the padding (t0 < H, so (t0 + G) C + NG <= H C + 2 NG).
+        float un_next[MPPI_NG];
+#pragma unroll
+        for (int i = 0; i < MPPI_NG; ++i) un_next[i] = s_un[i];
         for (uint32_t t0 = 0; t0 < H; t0 += MPPI_G) {
+            float un[MPPI_NG];
+#pragma unroll
+            for (int i = 0; i < MPPI_NG; ++i) {
+                un[i] = un_next[i];
+                un_next[i] = s_un[(t0 + MPPI_G) * CDIM + i];
+            }
             float z[MPPI_NG];
             const uint32_t b0 = (t0 * CDIM) >> 2;
 #pragma unroll
@@ -295,7 +308,7 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
                 const uint32_t t = t0 + g;
                 if (t < H) {
                     float u[CDIM];
-                    mppi_make_control(s_un + t * CDIM, z + g * CDIM, scale, lo, hi, u);
+                    mppi_make_control(un + g * CDIM, z + g * CDIM, scale, lo, hi, u);
                     mppi_rng rng{t << 8, k, step, dyn_tag, a.key0, a.key1};
                     MPPI_DYNAMICS(mppi_state_ref{x}, u, mppi_blob_ref{model}, rng);
                     total += mppi_stage(mppi_form_timed{}, mppi_cstate_ref{x}, (const float*)u,

@@ -15,11 +15,13 @@
 //   mppi_stage_cost / mppi_terminal_cost — the reference's goal costs; they
 //       read only the first 24 bytes, so the reference's CostParams works.
 //   unicycle_stage_cost_obstacles       — goal cost plus
-//       obstacle_weight * sum_i max(0, r_i - |p - c_i|)
+//       obstacle_weight * sum_i max(0, r_i - |p - c_i|) over the first
+//       n_obstacles slots (at most 16); it needs the whole UnicycleCost
+//       (224 bytes).
 //   unicycle_terminal_cost              — 10 * pos_weight * |p - g|^2
 #include "mppi_core.h"
 
-#define UNICYCLE_MAX_OBSTACLES 32
+#define UNICYCLE_MAX_OBSTACLES 16
 
 struct UnicycleModel {
     float v_max;
@@ -80,19 +82,17 @@ MPPI_CALLABLE float unicycle_stage_cost_obstacles(const float* state, const floa
     // |ox|, |oy| < r around each circle: inside the circle (d2 < r*r in the
     // rounded arithmetic below) implies inside the square, since rounding is
     // monotone.  The exact pass runs only in wavefronts where some sample is
-    // in some square.  The first pass reads obstacles in groups of eight so
-    // their parameter loads are issued together; slots in [n, n8) are read
-    // but only ever send a wavefront to the exact pass, which stops at n.
-    const uint32_t n8 = (n + 7u) & ~7u;
+    // in some square.  The first pass tests all 16 slots whatever n is, so
+    // the obstacle parameters do not depend on n and stay in registers across
+    // the whole horizon.  Slots at or past n may hold anything: they can only
+    // send a wavefront to the exact pass, which stops at n.
     float margin = 1.0f;        // min over obstacles of max(|ox|, |oy|) - r
-    for (uint32_t i = 0; i < n8; i += 8) {
-        MPPI_UNROLL
-        for (uint32_t j = 0; j < 8; ++j) {
-            float ox = state[0] - p->obstacles[i + j][0];
-            float oy = state[1] - p->obstacles[i + j][1];
-            float r = p->obstacles[i + j][2];
-            margin = __builtin_fminf(margin, __builtin_fmaxf(__builtin_fabsf(ox), __builtin_fabsf(oy)) - r);
-        }
+    MPPI_UNROLL
+    for (uint32_t j = 0; j < UNICYCLE_MAX_OBSTACLES; ++j) {
+        float ox = state[0] - p->obstacles[j][0];
+        float oy = state[1] - p->obstacles[j][1];
+        float r = p->obstacles[j][2];
+        margin = __builtin_fminf(margin, __builtin_fmaxf(__builtin_fabsf(ox), __builtin_fabsf(oy)) - r);
     }
     if (mppi_any(margin < 0.0f)) {
         for (uint32_t i = 0; i < n; ++i) {

@@ -21,7 +21,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 TAG_SCENE = 3
-UNICYCLE_MAX_OBSTACLES = 32
+UNICYCLE_MAX_OBSTACLES = 16
 UNICYCLE_MODEL_BYTES = 8
 UNICYCLE_COST_BYTES = 8 * 4 + UNICYCLE_MAX_OBSTACLES * 12
 

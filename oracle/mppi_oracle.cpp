@@ -161,7 +161,7 @@ struct UniCost {
     float x_goal, y_goal, goal_radius, pos_weight, heading_weight, control_weight;
     float obstacle_weight;
     uint32_t n_obstacles;
-    float obstacles[32][3];
+    float obstacles[16][3];
 };
 void uni_dyn_d(double* s, const double* u, const uint8_t* mp, const RngCtx&) {
     const UniModel* p = reinterpret_cast<const UniModel*>(mp);
@@ -187,7 +187,7 @@ double uni_stage_obs_d(const double* s, const double* u, const uint8_t* mp, cons
                        const mppi_time& tm) {
     const UniCost* p = reinterpret_cast<const UniCost*>(cp);
     double c = uni_stage_d(s, u, mp, cp, tm), pen = 0.0;
-    uint32_t n = std::min<uint32_t>(p->n_obstacles, 32);
+    uint32_t n = std::min<uint32_t>(p->n_obstacles, 16);
     for (uint32_t i = 0; i < n; ++i) {
         double ox = s[0] - p->obstacles[i][0], oy = s[1] - p->obstacles[i][1];
         pen += std::max(0.0, double(p->obstacles[i][2]) - std::sqrt(ox * ox + oy * oy));
