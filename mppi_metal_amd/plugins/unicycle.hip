@@ -94,13 +94,18 @@ MPPI_CALLABLE float unicycle_stage_cost_obstacles(const float* state, const floa
         float r = p->obstacles[j][2];
         margin = __builtin_fminf(margin, __builtin_fmaxf(__builtin_fabsf(ox), __builtin_fabsf(oy)) - r);
     }
-    if (mppi_any(margin < 0.0f)) {
-        for (uint32_t i = 0; i < n; ++i) {
+    // The exact pass reads the same 16 slots (already in registers) and
+    // takes a square root only for obstacles that some sample of the
+    // wavefront is inside, in slot order.
+    if (__builtin_expect(mppi_any(margin < 0.0f), 0)) {
+        MPPI_UNROLL
+        for (uint32_t i = 0; i < UNICYCLE_MAX_OBSTACLES; ++i) {
             float ox = state[0] - p->obstacles[i][0];
             float oy = state[1] - p->obstacles[i][1];
             float r = p->obstacles[i][2];
             float d2 = ox * ox + oy * oy;
-            if (d2 < r * r) pen += r - mppi_sqrtf(d2);
+            const bool inside = i < n && d2 < r * r;
+            if (__builtin_expect(mppi_any(inside), 0) && inside) pen += r - mppi_sqrtf(d2);
         }
     }
     return cost + p->obstacle_weight * pen;
