@@ -320,6 +320,9 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
         total += mppi_terminal(mppi_form_timed{}, mppi_cstate_ref{x}, mppi_blob_ref{model},
                                mppi_blob_ref{cost}, mppi_time{H, step});
         a.costs[d.cost_off + k] = total;
+    } else if (FUSED) {
+        // Lanes past K hold zeros, so the tile's noise sums need no select.
+        for (uint32_t i = 0; i < H * CDIM; ++i) s_z[threadIdx.x * ZS + i] = 0.0f;
     }
 
     if (FUSED) mppi_fused_stats<BLOCK>(a, agent, d, blk, active, total, s_z);
@@ -565,7 +568,7 @@ __device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const 
     constexpr int WAVES = BLOCK / 64;
     constexpr int ZS = MPPI_FUSED_MAX_HC + 1;
     __shared__ float s_red[4][WAVES];
-    __shared__ float s_e[BLOCK];
+    __shared__ __attribute__((aligned(16))) float s_e[BLOCK];
     __shared__ float s_h[BLOCK / 2];
     __shared__ int s_last;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -592,27 +595,27 @@ __device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const 
         bs[0] = m; bs[1] = S; bs[2] = Cs; bs[3] = Q;
     }
     // V_g[v] = sum over the tile's lanes of e_l z_l[v]: two threads per
-    // column, each over half the lanes in order, then added.  Zero weights
-    // select (lanes past K hold no noise).
+    // column, each over half the lanes in order, then added.  Lanes past K
+    // have e = 0 and zero noise, and a lane with a non-finite cost has e = 0
+    // and finite noise, so every term is finite and no select is needed.
     {
-        // threads [0, BLOCK/2) sum lanes [0, BLOCK/2) of column j, the other
+        // threads [0, BLOCK/2) sum lanes [0, BLOCK/2) of column v, the other
         // half the other lanes of the same column (distinct banks per wave)
         const uint32_t half = threadIdx.x / (BLOCK / 2), v = threadIdx.x % (BLOCK / 2);
         float acc = 0.0f;
         if (v < HC) {
-            // four independent chains keep the LDS reads in flight
-            float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, c3 = 0.0f;
+            // eight independent chains keep the LDS reads in flight; the
+            // weights are read four at a time (the same address in every lane)
+            float c[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
             const uint32_t l0 = half * (BLOCK / 2);
-            for (uint32_t l = l0; l < l0 + BLOCK / 2; l += 4) {
-                const float e0 = s_e[l], e1 = s_e[l + 1], e2 = s_e[l + 2], e3 = s_e[l + 3];
-                const float z0 = s_z[l * ZS + v], z1 = s_z[(l + 1) * ZS + v];
-                const float z2 = s_z[(l + 2) * ZS + v], z3 = s_z[(l + 3) * ZS + v];
-                c0 = e0 != 0.0f ? mppi_fma(e0, z0, c0) : c0;
-                c1 = e1 != 0.0f ? mppi_fma(e1, z1, c1) : c1;
-                c2 = e2 != 0.0f ? mppi_fma(e2, z2, c2) : c2;
-                c3 = e3 != 0.0f ? mppi_fma(e3, z3, c3) : c3;
+            const float4* e4 = reinterpret_cast<const float4*>(s_e);
+            for (uint32_t l = l0; l < l0 + BLOCK / 2; l += 8) {
+                const float4 ea = e4[l / 4], eb = e4[l / 4 + 1];
+                const float e[8] = {ea.x, ea.y, ea.z, ea.w, eb.x, eb.y, eb.z, eb.w};
+#pragma unroll
+                for (int j = 0; j < 8; ++j) c[j] = mppi_fma(e[j], s_z[(l + j) * ZS + v], c[j]);
             }
-            acc = (c0 + c1) + (c2 + c3);
+            acc = ((c[0] + c[1]) + (c[2] + c[3])) + ((c[4] + c[5]) + (c[6] + c[7]));
         }
         if (half == 1) s_h[v] = acc;
         __syncthreads();

@@ -85,27 +85,30 @@ MPPI_CALLABLE float unicycle_stage_cost_obstacles(const float* state, const floa
     // in some square.  The first pass tests all 16 slots whatever n is, so
     // the obstacle parameters do not depend on n and stay in registers across
     // the whole horizon.  Slots at or past n may hold anything: they can only
-    // send a wavefront to the exact pass, which stops at n.
-    float margin = 1.0f;        // min over obstacles of max(|ox|, |oy|) - r
+    // send a wavefront to the exact pass, which skips them.
+    float sq[UNICYCLE_MAX_OBSTACLES];   // max(|ox|, |oy|) - r: < 0 inside the square
+    float margin = 1.0f;                // their minimum
     MPPI_UNROLL
     for (uint32_t j = 0; j < UNICYCLE_MAX_OBSTACLES; ++j) {
         float ox = state[0] - p->obstacles[j][0];
         float oy = state[1] - p->obstacles[j][1];
-        float r = p->obstacles[j][2];
-        margin = __builtin_fminf(margin, __builtin_fmaxf(__builtin_fabsf(ox), __builtin_fabsf(oy)) - r);
+        sq[j] = __builtin_fmaxf(__builtin_fabsf(ox), __builtin_fabsf(oy)) - p->obstacles[j][2];
+        margin = __builtin_fminf(margin, sq[j]);
     }
-    // The exact pass reads the same 16 slots (already in registers) and
-    // takes a square root only for obstacles that some sample of the
-    // wavefront is inside, in slot order.
+    // The exact pass keeps slot order and takes the distance only for
+    // obstacles whose square holds some sample of the wavefront, and a square
+    // root only for samples inside the circle.  Marked unlikely so that it is
+    // compiled out of line rather than evaluated on every step.
     if (__builtin_expect(mppi_any(margin < 0.0f), 0)) {
         MPPI_UNROLL
         for (uint32_t i = 0; i < UNICYCLE_MAX_OBSTACLES; ++i) {
-            float ox = state[0] - p->obstacles[i][0];
-            float oy = state[1] - p->obstacles[i][1];
-            float r = p->obstacles[i][2];
-            float d2 = ox * ox + oy * oy;
-            const bool inside = i < n && d2 < r * r;
-            if (__builtin_expect(mppi_any(inside), 0) && inside) pen += r - mppi_sqrtf(d2);
+            if (__builtin_expect(mppi_any(sq[i] < 0.0f), 0)) {
+                float ox = state[0] - p->obstacles[i][0];
+                float oy = state[1] - p->obstacles[i][1];
+                float r = p->obstacles[i][2];
+                float d2 = ox * ox + oy * oy;
+                if (i < n && d2 < r * r) pen += r - mppi_sqrtf(d2);
+            }
         }
     }
     return cost + p->obstacle_weight * pen;
