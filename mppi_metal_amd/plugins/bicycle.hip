@@ -48,18 +48,23 @@ MPPI_CALLABLE void bicycle_dynamics(float* s, const float* u, const uint8_t* mod
     const uint32_t n = p->substeps > 0 ? p->substeps : 1u;
     const float h = p->dt / float(n);
     const float steer = u[0], acc = u[1];
+    // Reciprocals of the parameters: they do not change along a rollout, so
+    // the compiler computes them once per kernel rather than dividing twice
+    // per substep.
+    const float inv_mass = 1.0f / p->mass, inv_iz = 1.0f / p->yaw_inertia;
     for (uint32_t i = 0; i < n; ++i) {
         float sp, cp;
         mppi_sincosf(s[2], &sp, &cp);
         const float vx = s[3], vy = s[4], r = s[5];
         const float uu = vx > p->v_min ? vx : p->v_min;
-        const float fyf = p->cf * (steer - (vy + p->lf * r) / uu);
-        const float fyr = p->cr * (-(vy - p->lr * r) / uu);
+        const float inv_u = 1.0f / uu;
+        const float fyf = p->cf * (steer - (vy + p->lf * r) * inv_u);
+        const float fyr = p->cr * (-(vy - p->lr * r) * inv_u);
         const float dx = vx * cp - vy * sp;
         const float dy = vx * sp + vy * cp;
         const float dvx = acc - p->drag * vx + r * vy;
-        const float dvy = (fyf + fyr) / p->mass - r * vx;
-        const float dr = (p->lf * fyf - p->lr * fyr) / p->yaw_inertia;
+        const float dvy = (fyf + fyr) * inv_mass - r * vx;
+        const float dr = (p->lf * fyf - p->lr * fyr) * inv_iz;
         s[0] += h * dx;
         s[1] += h * dy;
         s[2] += h * r;
@@ -81,23 +86,35 @@ MPPI_CALLABLE float bicycle_pursuit_cost(const float* s, const float* u, const u
     cost += c->w_control * (u[0] * u[0] + u[1] * u[1]);
     float pen = 0.0f;
     const uint32_t nz = c->n_zones < BICYCLE_MAX_ZONES ? c->n_zones : BICYCLE_MAX_ZONES;
-    // The square root is only needed inside a zone: a first pass finds
-    // whether this sample is inside any; the penalty pass runs only in
-    // wavefronts where some sample is.
-    bool inside_any = false;
-    for (uint32_t i = 0; i < nz; ++i) {
-        const float ox = s[0] - c->zones[i][0];
-        const float oy = s[1] - c->zones[i][1];
-        const float rz = c->zones[i][2];
-        inside_any |= ox * ox + oy * oy < rz * rz;
+    // The square root is only needed inside a zone.  A first pass tests the
+    // square |ox|, |oy| < r around each zone (inside the circle in the
+    // rounded arithmetic below implies inside the square, rounding being
+    // monotone), eight zones per group so that their parameter loads issue
+    // together; slots in [nz, nz rounded up to 8) may hold anything and only
+    // send a wavefront to the exact pass.  The exact pass, taken only by
+    // wavefronts with some sample in some square, visits the zones in order
+    // and measures the distance only for zones whose square holds some sample
+    // of the wavefront.
+    const uint32_t nz8 = (nz + 7u) & ~7u;
+    float margin = 1.0f;        // min over zones of max(|ox|, |oy|) - r
+    for (uint32_t i = 0; i < nz8; i += 8) {
+        MPPI_UNROLL
+        for (uint32_t j = 0; j < 8; ++j) {
+            const float ox = s[0] - c->zones[i + j][0];
+            const float oy = s[1] - c->zones[i + j][1];
+            margin = __builtin_fminf(margin, __builtin_fmaxf(__builtin_fabsf(ox), __builtin_fabsf(oy)) -
+                                                 c->zones[i + j][2]);
+        }
     }
-    if (mppi_any(inside_any)) {
+    if (__builtin_expect(mppi_any(margin < 0.0f), 0)) {
         for (uint32_t i = 0; i < nz; ++i) {
             const float ox = s[0] - c->zones[i][0];
             const float oy = s[1] - c->zones[i][1];
             const float rz = c->zones[i][2];
-            const float d2 = ox * ox + oy * oy;
-            if (d2 < rz * rz) pen += rz - mppi_sqrtf(d2);
+            if (mppi_any(__builtin_fmaxf(__builtin_fabsf(ox), __builtin_fabsf(oy)) < rz)) {
+                const float d2 = ox * ox + oy * oy;
+                if (d2 < rz * rz) pen += rz - mppi_sqrtf(d2);
+            }
         }
     }
     return cost + c->w_zone * pen;
