@@ -291,6 +291,18 @@ def bench_surrogate(args):
     }
 
 
+def copy_calibration():
+    """Three clones of a 256 MiB tensor (the runtime's copy kernel): 256 MiB
+    read and 256 MiB written each, to scale the HBM byte counters."""
+    import torch
+    x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y = x.clone()
+        torch.cuda.synchronize()
+    print("copy calibration done", float(y[0]), file=sys.stderr)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,7 +323,13 @@ def main():
     ap.add_argument("--requests", type=int, default=1024)
     ap.add_argument("--surrogate-steps", type=int, default=500)
     ap.add_argument("--train-steps", type=int, default=4000)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three device-to-device copies of 256 MiB (the reference kernel "
+                         "for scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(agents or samples would shard with no or one small exchange per step).",
