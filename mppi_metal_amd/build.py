@@ -8,7 +8,8 @@ Steps (each compiles every time it is run):
      source fails the build here rather than at hipRTC time on the GPU box;
   3. link libmppi_hip.so (C ABI + the C++ MPPIDriver facade) with hipcc;
   4. build the C++ port of the reference's driver tests (tests/cpp);
-  5. build the CPU oracle (oracle/liboracle.so) — test infrastructure.
+  5. build the example programs (examples/);
+  6. build the CPU oracle (oracle/liboracle.so) — test infrastructure.
 
 Usage: python -m mppi_metal_amd.build   (or build() from __graft_entry__).
 """
@@ -111,6 +112,15 @@ def build_cpp_tests() -> Path:
     return out
 
 
+def build_examples() -> None:
+    """examples/unicycle_closed_loop, and, where the reference's sources are
+    present, the reference's unicycle examples compiled unchanged against this
+    library (examples/_ref/, outside the repository's history)."""
+    _run(["make", "-s", "-C", ROOT / "examples", "-B", "all"])
+    if Path("/root/reference/examples").is_dir():
+        _run(["make", "-s", "-C", ROOT / "examples", "-B", "reference"])
+
+
 def build_oracle() -> Path:
     _run(["make", "-s", "-C", ROOT / "oracle", "-B"])
     return ROOT / "oracle" / "liboracle.so"
@@ -121,6 +131,7 @@ def build_all():
     lib = build_library()
     check_kernels(lib)
     build_cpp_tests()
+    build_examples()
     build_oracle()
     return lib
 

@@ -20,6 +20,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -29,6 +30,7 @@
 #include <mutex>
 #include <sstream>
 #include <sys/stat.h>
+#include <unistd.h>
 
 namespace mppi_hip_impl {
 
@@ -108,6 +110,11 @@ uint64_t fnv1a(const std::string& s, uint64_t h) {
 std::mutex g_cache_mutex;
 std::map<std::string, std::vector<char>> g_code_objects;  // key -> code object
 std::map<std::pair<std::string, int>, std::shared_ptr<LoadedModule>> g_modules;  // (key, device)
+// One lock per code-object key, held while that object is read from disk or
+// compiled: contexts created together on several host threads compile a
+// plugin once, the others wait and take the result.
+std::map<std::string, std::shared_ptr<std::mutex>> g_key_locks;
+std::atomic<uint64_t> g_tmp_counter{0};
 
 std::string cache_dir() {
     const char* d = std::getenv("MPPI_HIP_CACHE_DIR");
@@ -276,6 +283,14 @@ int load_module(mppi_hip_ctx* ctx, bool batch, bool surrogate, std::shared_ptr<L
             return MPPI_HIP_OK;
         }
     }
+    std::shared_ptr<std::mutex> key_lock;
+    {
+        std::lock_guard<std::mutex> lock(g_cache_mutex);
+        auto& l = g_key_locks[key];
+        if (!l) l = std::make_shared<std::mutex>();
+        key_lock = l;
+    }
+    std::lock_guard<std::mutex> key_guard(*key_lock);
     std::vector<char> code;
     {
         std::lock_guard<std::mutex> lock(g_cache_mutex);
@@ -294,7 +309,8 @@ int load_module(mppi_hip_ctx* ctx, bool batch, bool surrogate, std::shared_ptr<L
             return fail(ctx, MPPI_HIP_E_COMPILE, "Plugin '" + ctx->plugin_name + "' failed to compile: " + log);
         if (!cpath.empty()) {
             mkdirs(dir);
-            std::string tmp = cpath + ".tmp" + std::to_string(uint64_t(std::rand()));
+            std::string tmp = cpath + ".tmp" + std::to_string(uint64_t(::getpid())) + "." +
+                              std::to_string(g_tmp_counter.fetch_add(1));
             std::ofstream f(tmp, std::ios::binary);
             if (f) {
                 f.write(code.data(), std::streamsize(code.size()));
