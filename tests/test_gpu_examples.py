@@ -11,6 +11,7 @@
   plugin path.  They run to completion, write their CSV and reach the goal.
 """
 import csv
+import importlib.util
 import subprocess
 from pathlib import Path
 
@@ -85,3 +86,10 @@ def test_reference_example_runs_unchanged(require_gpu, tmp_path, name, min_reach
     n = 512 if name.endswith("batch") else 1000
     assert len(sims) == n
     assert goal_hits >= min_reached * n, f"{goal_hits}/{n} ended near the goal"
+    # the plotter reads the reference examples' CSV too
+    spec = importlib.util.spec_from_file_location("plot_trajectory", ROOT / "examples" / "plot_trajectory.py")
+    plot = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(plot)
+    runs = plot.load(out)
+    assert plot.summary(runs)["runs"] == n
+    plot.plot(runs, tmp_path / "t.png")
