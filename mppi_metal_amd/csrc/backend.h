@@ -25,7 +25,8 @@ struct LoadedModule {
     hipModule_t module = nullptr;
     hipFunction_t rollout = nullptr;  // grid-mapped kernels
     hipFunction_t reduce = nullptr;
-    hipFunction_t rollout_fused = nullptr;  // one-kernel step
+    hipFunction_t rollout_fused = nullptr;  // one-pass step: tiles
+    hipFunction_t fused_combine = nullptr;  // one-pass step: combine and update
     hipFunction_t weights = nullptr;        // per-sample weights after a one-kernel step
     // batch kernels, indexed [block 64/128/256 -> 0/1/2][table 0 / persistent 1]
     hipFunction_t rollout_b[3][2] = {};

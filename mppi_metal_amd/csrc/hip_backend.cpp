@@ -330,6 +330,7 @@ int load_module(mppi_hip_ctx* ctx, bool batch, bool surrogate, std::shared_ptr<L
     bool ok = hipModuleGetFunction(&m->rollout, m->module, "mppi_rollout") == hipSuccess &&
               hipModuleGetFunction(&m->reduce, m->module, "mppi_reduce") == hipSuccess &&
               hipModuleGetFunction(&m->rollout_fused, m->module, "mppi_rollout_fused") == hipSuccess &&
+              hipModuleGetFunction(&m->fused_combine, m->module, "mppi_fused_combine") == hipSuccess &&
               hipModuleGetFunction(&m->weights, m->module, "mppi_weights") == hipSuccess;
     if (ok && batch) {
         const char* blocks[3] = {"64", "128", "256"};
@@ -456,6 +457,7 @@ int ensure_work_buffers(mppi_hip_ctx* ctx, const std::vector<mppi_agent_desc>& d
 // Layout of the per-call upload block (identical on host and device).
 struct UploadPlan {
     size_t desc = 0, agents = 0, x0 = 0, u_nom = 0, model = 0, cost = 0, total = 0;
+    size_t u_nom2 = 0, device_total = 0;   // closed loop: second warm-start buffer, device only
 };
 
 UploadPlan plan_upload(const mppi_hip_ctx* ctx, uint32_t N, size_t model_bytes, size_t cost_bytes) {
@@ -474,6 +476,9 @@ UploadPlan plan_upload(const mppi_hip_ctx* ctx, uint32_t N, size_t model_bytes, 
     p.cost = off;
     off = align_up(off + cost_bytes, 256);
     p.total = off;
+    p.u_nom2 = off;
+    off = align_up(off + sizeof(float) * size_t(N) * ctx->H * ctx->C, 256);
+    p.device_total = off;
     return p;
 }
 
@@ -504,7 +509,7 @@ int upload(mppi_hip_ctx* ctx, const std::vector<mppi_agent_desc>& desc, uint64_t
     const uint32_t N = uint32_t(desc.size());
     *plan = plan_upload(ctx, N, mo_bytes, co_bytes);
     int rc;
-    if ((rc = ensure_device_buffer(ctx, &ctx->up, plan->total))) return rc;
+    if ((rc = ensure_device_buffer(ctx, &ctx->up, plan->device_total))) return rc;
     if ((rc = ensure_pinned_buffer(ctx, &ctx->up_host, plan->total))) return rc;
     uint8_t* h = static_cast<uint8_t*>(ctx->up_host.ptr);
     std::memcpy(h + plan->desc, desc.data(), sizeof(mppi_agent_desc) * N);
@@ -555,13 +560,19 @@ int launch_iteration(mppi_hip_ctx* ctx, const std::vector<mppi_agent_desc>& desc
     if (use_fused(ctx, d, N, sur)) {
         MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(ctx->mod->rollout_fused, d.n_blk, N, 1, kBlock, 1, 1, 0,
                                                 ctx->stream, params, nullptr));
+        if (ctx->profiling) {
+            if (e1) (void)hipEventRecord(e1, ctx->stream);
+            if (e2) (void)hipEventRecord(e2, ctx->stream);
+        }
+        // the combine's workgroups: slices of 16 / C timesteps
+        const uint32_t tpw = ctx->C < 16 ? 16 / ctx->C : 1;
+        MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(ctx->mod->fused_combine, (d.H + tpw - 1) / tpw, N, 1, kBlock,
+                                                1, 1, 0, ctx->stream, params, nullptr));
         ctx->weights_pending = true;
         ctx->weights_args = args;
         ctx->weights_grid_x = (d.K + kBlock - 1) / kBlock;
         ctx->weights_grid_y = N;
         if (ctx->profiling) {
-            if (e1) (void)hipEventRecord(e1, ctx->stream);
-            if (e2) (void)hipEventRecord(e2, ctx->stream);
             if (e3) (void)hipEventRecord(e3, ctx->stream);
             if (e0 && e1) ctx->pending_rollout.push_back({e0, e1});
             if (e2 && e3) ctx->pending_reduce.push_back({e2, e3});
@@ -642,8 +653,15 @@ int run_closed_loop(mppi_hip_ctx* ctx, uint32_t N, uint32_t T, uint64_t start_st
     a.hist_controls = controls_out ? static_cast<float*>(ctx->hist_controls.ptr) : nullptr;
     a.hist_costs = costs_out ? static_cast<float*>(ctx->hist_costs.ptr) : nullptr;
     a.terminal = terminal_out ? static_cast<float*>(ctx->terminal.ptr) : nullptr;
-    for (uint32_t s = 0; s < T; ++s)
+    // The warm start alternates between two device buffers: step s reads one
+    // and its update writes the shifted sequence into the other.
+    uint8_t* up = static_cast<uint8_t*>(ctx->up.ptr);
+    float* warm[2] = {reinterpret_cast<float*>(up + plan.u_nom), reinterpret_cast<float*>(up + plan.u_nom2)};
+    for (uint32_t s = 0; s < T; ++s) {
+        a.u_nom = warm[s & 1u];
+        a.u_nom_next = warm[(s + 1) & 1u];
         if ((rc = launch_iteration(ctx, desc, a))) return rc;
+    }
     if (states_out)
         MPPI_HIP_TRY(ctx, hipMemcpyAsync(states_out, ctx->hist_states.ptr, sizeof(float) * T * N * D,
                                          hipMemcpyDeviceToHost, ctx->stream));

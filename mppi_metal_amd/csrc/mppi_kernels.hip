@@ -428,6 +428,48 @@ __device__ void mppi_sum_partials(const mppi_step_args& a, const mppi_agent_desc
     }
 }
 
+// Closed loop, one thread: log the history row (state, first control, best
+// cost), step the plant with the plugin dynamics and the first control, and
+// advance the agent's step and history counters.
+__device__ inline void mppi_plant_step(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
+                                       const float* u0) {
+    const uint32_t slot = d.slot;
+    mppi_agent_state* ag = &a.agents[slot];
+    const uint32_t step = ag->step, row = ag->hist;
+    float* xs = a.x + size_t(slot) * SDIM;
+    float x[SDIM];
+    float u[CDIM];
+#pragma unroll
+    for (int i = 0; i < SDIM; ++i) x[i] = xs[i];
+#pragma unroll
+    for (int i = 0; i < CDIM; ++i) u[i] = u0[i];
+    if (a.hist_states)
+        for (int i = 0; i < SDIM; ++i) a.hist_states[(size_t(row) * a.N + agent) * SDIM + i] = x[i];
+    if (a.hist_controls)
+        for (int i = 0; i < CDIM; ++i) a.hist_controls[(size_t(row) * a.N + agent) * CDIM + i] = u[i];
+    if (a.hist_costs) a.hist_costs[size_t(row) * a.N + agent] = float(ag->min_cost);
+    mppi_rng rng{0u, 0xffffffffu, step, (MPPI_TAG_PLANT << 28) | (slot & MPPI_AGENT_MASK), a.key0, a.key1};
+    MPPI_DYNAMICS(mppi_state_ref{x}, u, mppi_blob_ref{a.model + slot * a.model_stride}, rng);
+#pragma unroll
+    for (int i = 0; i < SDIM; ++i) xs[i] = x[i];
+    if (a.terminal)
+        for (int i = 0; i < SDIM; ++i) a.terminal[size_t(agent) * SDIM + i] = x[i];
+    ag->hist = row + 1;
+    ag->step = step + 1;
+}
+
+// u = clamp(u_nom + sigma z) (diagonal) or clamp(u_nom + L z) (full) for
+// control n = t*C + i of a sequence, z the averaged noise of timestep t.
+__device__ inline float mppi_update_control(const mppi_step_args& a, float un, const float* zt, uint32_t i) {
+#if MPPI_COV_FULL
+    float e = 0.0f;
+    for (uint32_t j = 0; j <= i; ++j) e = mppi_fma(a.noise_scale[i * CDIM + j], zt[j], e);
+#else
+    const float e = a.noise_scale[i] * zt[i];
+#endif
+    return mppi_clampf(un + e, a.u_min[i], a.u_max[i]);
+}
+
 // Apply the averaged noise zs (H*C values, already divided by the
 // normaliser): u_next, diagnostics, and in closed loop the history, the plant
 // step and the warm-start shift.
@@ -435,21 +477,12 @@ template <int BLOCK>
 __device__ void mppi_apply_update(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
                                   const float* zs) {
     const uint32_t H = d.H, HC = H * CDIM;
-    const uint32_t slot = d.slot;
-    mppi_agent_state* ag = &a.agents[slot];
-    const uint32_t step = ag->step;
+    mppi_agent_state* ag = &a.agents[d.slot];
     float* uo = a.u_out + d.out_off;
     float* un = a.u_nom + d.seq_off;
     for (uint32_t n = threadIdx.x; n < HC; n += BLOCK) {
         const uint32_t t = n / CDIM, i = n % CDIM;
-#if MPPI_COV_FULL
-        float e = 0.0f;
-        for (uint32_t j = 0; j <= i; ++j) e = mppi_fma(a.noise_scale[i * CDIM + j], zs[t * CDIM + j], e);
-#else
-        (void)t;
-        float e = a.noise_scale[i] * zs[n];
-#endif
-        uo[n] = mppi_clampf(un[n] + e, a.u_min[i], a.u_max[i]);
+        uo[n] = mppi_update_control(a, un[n], zs + t * CDIM, i);
     }
     __syncthreads();
     if (threadIdx.x == 0 && a.diag) {
@@ -459,35 +492,11 @@ __device__ void mppi_apply_update(const mppi_step_args& a, uint32_t agent, const
     }
     if (a.first_out && threadIdx.x < CDIM) a.first_out[agent * CDIM + threadIdx.x] = uo[threadIdx.x];
     if (a.closed_loop) {
-        if (threadIdx.x == 0) {
-            const uint32_t row = ag->hist;
-            float* xs = a.x + size_t(slot) * SDIM;
-            float x[SDIM];
-            float u[CDIM];
-#pragma unroll
-            for (int i = 0; i < SDIM; ++i) x[i] = xs[i];
-#pragma unroll
-            for (int i = 0; i < CDIM; ++i) u[i] = uo[i];
-            if (a.hist_states)
-                for (int i = 0; i < SDIM; ++i)
-                    a.hist_states[(size_t(row) * a.N + agent) * SDIM + i] = x[i];
-            if (a.hist_controls)
-                for (int i = 0; i < CDIM; ++i)
-                    a.hist_controls[(size_t(row) * a.N + agent) * CDIM + i] = u[i];
-            if (a.hist_costs) a.hist_costs[size_t(row) * a.N + agent] = float(ag->min_cost);
-            mppi_rng rng{0u, 0xffffffffu, step,
-                         (MPPI_TAG_PLANT << 28) | (slot & MPPI_AGENT_MASK), a.key0, a.key1};
-            MPPI_DYNAMICS(mppi_state_ref{x}, u, mppi_blob_ref{a.model + slot * a.model_stride}, rng);
-#pragma unroll
-            for (int i = 0; i < SDIM; ++i) xs[i] = x[i];
-            if (a.terminal)
-                for (int i = 0; i < SDIM; ++i) a.terminal[size_t(agent) * SDIM + i] = x[i];
-            ag->hist = row + 1;
-            ag->step = step + 1;
-        }
+        if (threadIdx.x == 0) mppi_plant_step(a, agent, d, uo);
+        float* next = a.u_nom_next ? a.u_nom_next + d.seq_off : un;
         for (uint32_t n = threadIdx.x; n < HC; n += BLOCK) {
             const uint32_t t = n / CDIM, i = n % CDIM;
-            un[n] = uo[(t + 1 < H ? t + 1 : H - 1) * CDIM + i];
+            next[n] = uo[(t + 1 < H ? t + 1 : H - 1) * CDIM + i];
         }
     }
     if (threadIdx.x == 0) {
@@ -507,61 +516,19 @@ __device__ void mppi_finalize(const mppi_step_args& a, uint32_t agent, const mpp
 }
 
 // ---------------------------------------------------------------------------
-// One-kernel step (FUSED rollout tiles).  A tile's weights are taken against
-// its own minimum m_g, e_k = exp(-(c_k - m_g)/lambda), and its weighted noise
-// sums V_g = sum_k e_k z_k are formed from the normals kept in LDS; the last
-// tile of the agent scales each tile's sums by exp(-(m_g - m)/lambda) (m the
-// agent's minimum, as in the normaliser), divides by the normaliser and
-// applies the update.  A tile with no finite cost keeps its plain sum of the
-// noise and gets scale 0, which matters only when no cost is finite (uniform
-// weights, as in the two-kernel step).
+// One-pass step (FUSED rollout tiles, then mppi_fused_combine).  A tile's
+// weights are taken against its own minimum m_g, e_k = exp(-(c_k - m_g)/lambda),
+// and its weighted noise sums V_g = sum_k e_k z_k are formed from the normals
+// kept in LDS; the tile writes (m_g, S_g, C_g, Q_g) and V_g.  A second, small
+// launch combines the tiles: with m = min m_g and f_g = exp(-(m_g - m)/lambda),
+// S = sum f_g S_g and the update's average is sum f_g V_g / S.  A tile with no
+// finite cost keeps the plain sum of its noise and gets f = 0, which matters
+// only when no cost is finite (then f = 1 for every tile and S = K: uniform
+// weights, as in the two-kernel step).  The combine is spread over several
+// workgroups, one per slice of timesteps, so reading every tile's sums costs
+// a few memory latencies on many compute units instead of one compute unit
+// reading them all after the slowest tile.
 // ---------------------------------------------------------------------------
-// The last tile of a one-kernel step (n_blk <= BLOCK tiles, one per thread):
-// the normaliser and diagnostics as mppi_combine_stats forms them, from one
-// read of the tile statistics, then the scaled sum of the tiles' noise sums.
-template <int BLOCK>
-__device__ void mppi_fused_finish(const mppi_step_args& a, const mppi_agent_desc& d, float* zs) {
-    constexpr int WAVES = BLOCK / 64;
-    __shared__ float s_f[BLOCK];
-    __shared__ float s_min[WAVES];
-    __shared__ float s_sum[3][WAVES];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t g = threadIdx.x;
-    float4 st = {__builtin_inff(), 0.0f, 0.0f, 0.0f};
-    if (g < d.n_blk) st = reinterpret_cast<const float4*>(a.blk_stats + d.blk_off * 4)[g];
-    float mg = mppi_wave_min(st.x);
-    if (lane == 0) s_min[wave] = mg;
-    __syncthreads();
-    mg = s_min[0];
-#pragma unroll
-    for (int w = 1; w < WAVES; ++w) mg = s_min[w] < mg ? s_min[w] : mg;
-    const float f = (mppi_isfinite(st.x) && mppi_isfinite(mg)) ? mppi_expf(-(st.x - mg) / a.lambda) : 0.0f;
-    const bool uniform = !mppi_isfinite(mg);
-    s_f[g] = uniform ? 1.0f : f;
-    float S = mppi_wave_sum(st.y * f);
-    float Cs = mppi_wave_sum(st.z);
-    float Q = mppi_wave_sum(st.w * (f * f));
-    if (lane == 0) { s_sum[0][wave] = S; s_sum[1][wave] = Cs; s_sum[2][wave] = Q; }
-    __syncthreads();
-    S = s_sum[0][0]; Cs = s_sum[1][0]; Q = s_sum[2][0];
-#pragma unroll
-    for (int w = 1; w < WAVES; ++w) { S += s_sum[0][w]; Cs += s_sum[1][w]; Q += s_sum[2][w]; }
-    if (uniform) {
-        S = float(d.K);
-        Q = float(d.K);
-    }
-    if (threadIdx.x == 0) {
-        mppi_agent_state* ag = &a.agents[d.slot];
-        ag->min_cost = mg;
-        ag->weight_sum = double(S);
-        ag->mean_cost = double(Cs / float(d.K));
-        ag->ess = double((S * S) / Q);
-        ag->rollout_arrivals = 0u;
-    }
-    __syncthreads();
-    mppi_sum_rows<BLOCK, 32>(a.partials + d.part_off, d.n_blk, d.NB * 4, s_f, S, zs);
-}
-
 template <int BLOCK>
 __device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
                                  uint32_t blk, bool active, float total, float* s_z) {
@@ -570,7 +537,6 @@ __device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const 
     __shared__ float s_red[4][WAVES];
     __shared__ __attribute__((aligned(16))) float s_e[BLOCK];
     __shared__ float s_h[BLOCK / 2];
-    __shared__ int s_last;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t HC = d.H * CDIM, W = d.NB * 4;
     float c = (active && mppi_isfinite(total)) ? total : __builtin_inff();
@@ -621,19 +587,117 @@ __device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const 
         __syncthreads();
         if (half == 0 && v < W) a.partials[d.part_off + size_t(blk) * W + v] = v < HC ? acc + s_h[v] : 0.0f;
     }
-    // The workgroup barrier completes every thread's stores; thread 0's
-    // device-scope fence then publishes them before the arrival count.
+    // The rows are read by mppi_fused_combine, a later launch.
+}
+
+// Combine of the one-pass step for one agent: workgroup `slice` handles the
+// timesteps [t0, t1) = [slice*TPW, min((slice+1)*TPW, H)) and, for the
+// warm-start shift, forms timestep t1 as well.  Every workgroup reads all
+// tile statistics (one row per thread) and forms the same normaliser in the
+// same order; its columns' sums over the tiles are requested before that, so
+// the two reads overlap.  Slice 0 also writes the diagnostics and agent
+// state and steps the plant.  In closed loop the shifted warm start goes to
+// u_nom_next: shifting in place would overwrite the timestep t1 that the
+// previous slice reads.
+constexpr uint32_t MPPI_COMBINE_TPW = 16 / CDIM > 0 ? 16 / CDIM : 1;
+
+template <int BLOCK>
+__device__ void mppi_fused_combine_slice(const mppi_step_args& a, uint32_t agent, uint32_t slice) {
+    constexpr int WAVES = BLOCK / 64;
+    constexpr uint32_t TPW = MPPI_COMBINE_TPW;
+    constexpr uint32_t MAXCOL = (TPW + 1) * CDIM;
+    // rows (tiles) per thread: at most BLOCK tiles over BLOCK / MAXCOL row groups
+    constexpr int RPT = int((BLOCK + BLOCK / MAXCOL - 1) / (BLOCK / MAXCOL));
+    __shared__ float s_f[BLOCK];
+    __shared__ float s_min[WAVES];
+    __shared__ float s_sum[3][WAVES];
+    __shared__ float s_part[BLOCK];
+    __shared__ float s_z[MAXCOL];
+    __shared__ float s_u[MAXCOL];
+    const mppi_agent_desc d = a.desc[agent];
+    const uint32_t H = d.H, t0 = slice * TPW;
+    if (t0 >= H) return;                               // the whole workgroup
+    const uint32_t t1 = t0 + TPW < H ? t0 + TPW : H;   // own timesteps [t0, t1)
+    const uint32_t te = t1 + 1 < H ? t1 + 1 : H;       // formed: [t0, te)
+    const uint32_t c0 = t0 * CDIM, ncol = (te - t0) * CDIM, nown = (t1 - t0) * CDIM;
+    const uint32_t W = d.NB * 4, n_rows = d.n_blk;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+
+    const uint32_t RG = BLOCK / ncol, col = threadIdx.x % ncol, rg = threadIdx.x / ncol;
+    const float* rows = a.partials + d.part_off + c0 + col;
+    float v[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const uint32_t r = rg + uint32_t(q) * RG;
+        v[q] = (rg < RG && r < n_rows) ? rows[size_t(r) * W] : 0.0f;
+    }
+    float4 st = {__builtin_inff(), 0.0f, 0.0f, 0.0f};
+    if (threadIdx.x < n_rows) st = reinterpret_cast<const float4*>(a.blk_stats + d.blk_off * 4)[threadIdx.x];
+    float mg = mppi_wave_min(st.x);
+    if (lane == 0) s_min[wave] = mg;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        uint32_t prev = atomicAdd(&a.agents[d.slot].rollout_arrivals, 1u);
-        s_last = (prev == d.n_blk - 1);
+    mg = s_min[0];
+#pragma unroll
+    for (int w = 1; w < WAVES; ++w) mg = s_min[w] < mg ? s_min[w] : mg;
+    const float f = (mppi_isfinite(st.x) && mppi_isfinite(mg)) ? mppi_expf(-(st.x - mg) / a.lambda) : 0.0f;
+    const bool uniform = !mppi_isfinite(mg);
+    s_f[threadIdx.x] = uniform ? 1.0f : f;
+    float S = mppi_wave_sum(st.y * f);
+    float Cs = mppi_wave_sum(st.z);
+    float Q = mppi_wave_sum(st.w * (f * f));
+    if (lane == 0) { s_sum[0][wave] = S; s_sum[1][wave] = Cs; s_sum[2][wave] = Q; }
+    __syncthreads();
+    S = s_sum[0][0]; Cs = s_sum[1][0]; Q = s_sum[2][0];
+#pragma unroll
+    for (int w = 1; w < WAVES; ++w) { S += s_sum[0][w]; Cs += s_sum[1][w]; Q += s_sum[2][w]; }
+    if (uniform) {
+        S = float(d.K);
+        Q = float(d.K);
+    }
+    // the scaled sums of this workgroup's columns: rows in order per thread,
+    // then row groups in order
+    float acc = 0.0f;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const uint32_t r = rg + uint32_t(q) * RG;
+        if (rg < RG && r < n_rows) acc = mppi_fma(s_f[r], v[q], acc);
+    }
+    s_part[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < ncol) {
+        float t = 0.0f;
+        for (uint32_t g = 0; g < RG; ++g) t += s_part[g * ncol + threadIdx.x];
+        s_z[threadIdx.x] = t / S;
     }
     __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    mppi_fused_finish<BLOCK>(a, d, s_z);
-    mppi_apply_update<BLOCK>(a, agent, d, s_z);
+    const uint32_t seq = d.seq_off;
+    if (threadIdx.x < ncol) {
+        const uint32_t n = threadIdx.x, i = n % CDIM;
+        const float u = mppi_update_control(a, a.u_nom[seq + c0 + n], s_z + (n - i), i);
+        s_u[n] = u;
+        if (n < nown) a.u_out[d.out_off + c0 + n] = u;
+    }
+    __syncthreads();
+    if (slice == 0) {
+        mppi_agent_state* ag = &a.agents[d.slot];
+        if (threadIdx.x == 0) {
+            ag->min_cost = mg;
+            ag->weight_sum = double(S);
+            ag->mean_cost = double(Cs / float(d.K));
+            ag->ess = double((S * S) / Q);
+            if (a.diag) {
+                a.diag[agent * 3 + 0] = mg;
+                a.diag[agent * 3 + 1] = float(double(Cs / float(d.K)));
+                a.diag[agent * 3 + 2] = float(double((S * S) / Q));
+            }
+            if (a.closed_loop) mppi_plant_step(a, agent, d, s_u);
+        }
+        if (a.first_out && threadIdx.x < CDIM) a.first_out[agent * CDIM + threadIdx.x] = s_u[threadIdx.x];
+    }
+    if (a.closed_loop && a.u_nom_next && threadIdx.x < nown) {
+        const uint32_t n = threadIdx.x, t = t0 + n / CDIM;
+        a.u_nom_next[seq + c0 + n] = t + 1 < H ? s_u[n + CDIM] : s_u[n];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -751,6 +815,11 @@ extern "C" __global__ void __launch_bounds__(256) mppi_rollout_fused(mppi_step_a
 // Per-sample weights of the last iteration, w_k = exp(-(c_k - m)/lambda) / S
 // (the reduce kernel's formula), for a one-kernel step, which does not form
 // them; grid (ceil(K / 256), N).
+// Combine of the one-pass step (grid: timestep slices x agents).
+extern "C" __global__ void __launch_bounds__(256) mppi_fused_combine(mppi_step_args a) {
+    mppi_fused_combine_slice<256>(a, blockIdx.y, blockIdx.x);
+}
+
 extern "C" __global__ void __launch_bounds__(256) mppi_weights(mppi_step_args a) {
     const uint32_t agent = blockIdx.y;
     const mppi_agent_desc d = a.desc[agent];

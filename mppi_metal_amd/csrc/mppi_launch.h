@@ -53,6 +53,7 @@ struct mppi_step_args {
     uint32_t N;
     float* x;                    // [N][D] state (x0; plant state in closed loop)
     float* u_nom;                // packed by desc.seq_off
+    float* u_nom_next;           // closed loop: the shifted warm start goes here (null: into u_nom)
     float* u_out;                // packed by desc.seq_off
     const uint8_t* model;        // model blob of agent 0
     const uint8_t* cost;         // cost blob of agent 0
