@@ -87,6 +87,23 @@ MPPI_HD float mppi_sqrtf(float x) {
 }
 MPPI_HD float mppi_fabsf(float x) { return mppi_u2f(mppi_f2u(x) & 0x7fffffffu); }
 
+// Correctly rounded square root of x in [2^-96, 2^126] or +-0 (finite,
+// no subnormal, no scaling needed): the same result as mppi_sqrtf there.  On
+// the GPU, the hardware square root (within 1 ulp) is corrected by the
+// residuals of its two neighbours, the compiler's own correctly rounded
+// sequence without the range handling that the argument does not need.
+MPPI_HD float mppi_sqrt_boundedf(float x) {
+#if defined(__HIP_DEVICE_COMPILE__) && !MPPI_FAST
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const uint32_t si = mppi_f2u(s);
+    const float down = mppi_u2f(si - 1u), up = mppi_u2f(si + 1u);
+    float r = mppi_fma(-down, s, x) <= 0.0f ? down : s;
+    return mppi_fma(-up, s, x) > 0.0f ? up : r;
+#else
+    return mppi_sqrtf(x);
+#endif
+}
+
 // Clamp with explicit comparisons (NaN passes through unchanged on both
 // targets, unlike min/max instructions whose NaN rules differ).
 MPPI_HD float mppi_clampf(float x, float lo, float hi) {
@@ -137,21 +154,10 @@ MPPI_HD float mppi_expf(float x) {
 // ---------------------------------------------------------------------------
 // log
 // ---------------------------------------------------------------------------
-MPPI_HD float mppi_logf(float x) {
-#if MPPI_FAST
-    return __builtin_amdgcn_logf(x) * 0.693147180559945309417f;
-#endif
-    if (!(x == x)) return x;
-    if (x < 0.0f) return mppi_u2f(0x7fc00000u);    // NaN
-    if (x == 0.0f) return mppi_u2f(0xff800000u);   // -inf
-    uint32_t ix = mppi_f2u(x);
-    if (ix == 0x7f800000u) return x;               // +inf
-    int e = 0;
-    if (ix < 0x00800000u) {                        // subnormal: scale by 2^25
-        x = x * 33554432.0f;
-        ix = mppi_f2u(x);
-        e = -25;
-    }
+// log of 2^e * x for a positive normal float x (bit pattern ix): the core of
+// mppi_logf, also called directly where the argument is known to be a
+// positive normal number (Box-Muller), with the same result.
+MPPI_HD float mppi_log_core_(uint32_t ix, int e) {
     e += int(ix >> 23) - 127;
     uint32_t mi = (ix & 0x007fffffu) | 0x3f800000u;  // m in [1, 2)
     if (mi > 0x3fb504f3u) {                        // m > sqrt(2): halve it
@@ -173,6 +179,28 @@ MPPI_HD float mppi_logf(float x) {
     const float ln2_lo = 1.428606765330187045e-06f;
     float ef = float(e);
     return mppi_fma(ef, ln2_hi, mppi_fma(ef, ln2_lo, lm));
+}
+
+MPPI_HD float mppi_logf(float x) {
+#if MPPI_FAST
+    return __builtin_amdgcn_logf(x) * 0.693147180559945309417f;
+#endif
+    if (!(x == x)) return x;
+    if (x < 0.0f) return mppi_u2f(0x7fc00000u);    // NaN
+    if (x == 0.0f) return mppi_u2f(0xff800000u);   // -inf
+    uint32_t ix = mppi_f2u(x);
+    if (ix == 0x7f800000u) return x;               // +inf
+    const bool subnormal = ix < 0x00800000u;       // scaled by 2^25
+    return mppi_log_core_(subnormal ? mppi_f2u(x * 33554432.0f) : ix, subnormal ? -25 : 0);
+}
+
+// mppi_logf for an argument known to be a positive normal float: the same
+// result without the special-value tests.
+MPPI_HD float mppi_log_normalf(float x) {
+#if MPPI_FAST
+    return __builtin_amdgcn_logf(x) * 0.693147180559945309417f;
+#endif
+    return mppi_log_core_(mppi_f2u(x), 0);
 }
 
 // ---------------------------------------------------------------------------

@@ -96,7 +96,9 @@ MPPI_HD float mppi_round_f16(float x) {
 MPPI_HD void mppi_box_muller(uint32_t a, uint32_t b, float* z0, float* z1) {
     float u1 = mppi_u01(a);
     float u2 = mppi_u01(b);
-    float r = mppi_sqrtf(-2.0f * mppi_logf(u1));
+    // u1 in [2^-24, 1 - 2^-24] is a positive normal number and -2 log(u1) lies
+    // in [1.1e-7, 33.3], so the range-free forms give the same bits
+    float r = mppi_sqrt_boundedf(-2.0f * mppi_log_normalf(u1));
     float s, c;
     mppi_sincospif(2.0f * u2, &s, &c);
     *z0 = r * c;

@@ -2,7 +2,9 @@
 evaluated on the GPU inside a rollout, bit for bit against the same source
 compiled for the host (the oracle), over random arguments and special values
 (signed zeros, infinities, NaN, subnormals, overflow and underflow edges,
-long-division fmod quotients).
+long-division fmod quotients).  The range-restricted forms used by Box-Muller
+(log of a positive normal number, square root of [2^-96, 2^126] or zero)
+are checked over their domains against mppi_logf and the IEEE square root.
 
 A test plugin evaluates function `fn` on input `sample` of its model blob in
 the dynamics callable, and the terminal cost returns it, so with H = 1 and a
@@ -35,7 +37,9 @@ MPPI_CALLABLE void eval_fn(float* s, const float* u, const uint8_t* m, mppi_rng 
         case 5: mppi_sincospif(x, &sn, &cs); v = cs; break;
         case 6: v = mppi_tanhf(x); break;
         case 7: v = mppi_sqrtf(x); break;
-        default: v = mppi_fmodf(x, y); break;
+        case 8: v = mppi_fmodf(x, y); break;
+        case 9: v = mppi_log_normalf(x); break;
+        default: v = mppi_sqrt_boundedf(x); break;
     }
     s[0] = v;
 }
@@ -64,6 +68,14 @@ def _inputs(fn, rng):
                             np.round(rng.uniform(-64, 64, 500)) / 4])
     elif fn == 6:
         x = np.concatenate([rng.uniform(-1, 1, 1500), rng.uniform(-25, 25, 1500)])
+    elif fn == 9:                       # positive normal floats
+        x = np.concatenate([10.0 ** rng.uniform(-37.9, 38.5, 4000), rng.uniform(2.0 ** -24, 1.0, 4000),
+                            [1.17549435e-38, 1.0, 1.0 - 2.0 ** -24, 2.0 ** -24, 3.4e38]])
+        return x.astype(F), np.zeros(x.size, F)
+    elif fn == 10:                      # [2^-96, 2^126] and signed zeros, dense where Box-Muller lives
+        x = np.concatenate([2.0 ** rng.uniform(-96, 126, 4000), rng.uniform(0.0, 34.0, 20000),
+                            [0.0, -0.0, 2.0 ** -96, 2.0 ** 126, 1.0, 4.0, 33.27]])
+        return x.astype(F), np.zeros(x.size, F)
     else:                               # fmod: random pairs and every pair of specials
         gx, gy = (a.ravel() for a in np.meshgrid(SPECIAL, SPECIAL))
         yr = rng.uniform(0.05, 60, 2000) * rng.choice([-1, 1], 2000)
@@ -74,11 +86,17 @@ def _inputs(fn, rng):
 
 
 def _host(fn, x, y):
-    return ol.fmodf(x, y) if fn == 8 else ol.mathf(fn, x)
+    if fn == 8:
+        return ol.fmodf(x, y)
+    if fn == 9:
+        return ol.mathf(1, x)          # mppi_logf: the same core for normal arguments
+    if fn == 10:
+        return np.sqrt(x)              # IEEE, correctly rounded
+    return ol.mathf(fn, x)
 
 
-@pytest.mark.parametrize("fn", range(9), ids=["exp", "log", "sin", "cos", "sinpi", "cospi", "tanh", "sqrt",
-                                              "fmod"])
+@pytest.mark.parametrize("fn", range(11), ids=["exp", "log", "sin", "cos", "sinpi", "cospi", "tanh", "sqrt",
+                                               "fmod", "log_normal", "sqrt_bounded"])
 def test_math_bit_for_bit(require_gpu, fn):
     x, y = _inputs(fn, np.random.default_rng(100 + fn))
     K = x.size
