@@ -69,6 +69,7 @@
 #define MPPI_FP64_MIN_K (1u << 18)
 #define MPPI_MAX_HC 4096
 #define MPPI_FUSED_MAX_HC 128             // horizon * control_dim limit of the one-kernel step
+#define MPPI_FUSED_ZS (MPPI_FUSED_MAX_HC + 4)  // LDS row of a lane's normals (16-byte multiple)
 
 constexpr int mppi_gcd(int a, int b) { return b == 0 ? a : mppi_gcd(b, a % b); }
 // Timesteps whose C*G noise values fill whole Philox blocks.
@@ -257,10 +258,10 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
     // registers, so the time loop issues no memory waits of its own.
     // 2 * MPPI_NG slots of padding: the time loop reads one group ahead.
     __shared__ __attribute__((aligned(16))) float s_un[(FUSED ? MPPI_FUSED_MAX_HC : MPPI_MAX_HC) + 2 * MPPI_NG];
-    // [lane][H*C + 1]: a lane's writes and a column's reads are both
-    // conflict-free.
-    constexpr int ZS = MPPI_FUSED_MAX_HC + 1;
-    __shared__ float s_z[FUSED ? BLOCK * ZS : 1];
+    // [lane][ZS]: rows 16-byte aligned, so a lane writes a group's normals
+    // and the tile's sums read them four columns at a time.
+    constexpr int ZS = MPPI_FUSED_ZS;
+    __shared__ __attribute__((aligned(16))) float s_z[FUSED ? BLOCK * ZS : 1];
     for (uint32_t i = threadIdx.x; i < H * CDIM; i += BLOCK) s_un[i] = a.u_nom[d.seq_off + i];
     float scale[MPPI_COV_FULL ? CDIM * CDIM : CDIM], lo[CDIM], hi[CDIM];
 #pragma unroll
@@ -299,9 +300,16 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
                 z[4 * q + 2] = v.z; z[4 * q + 3] = v.w;
             }
             if (FUSED) {
+                float* row = s_z + threadIdx.x * ZS + t0 * CDIM;
+                if (t0 * CDIM + MPPI_NG <= H * CDIM) {
 #pragma unroll
-                for (int i = 0; i < MPPI_NG; ++i)
-                    if (t0 * CDIM + i < H * CDIM) s_z[threadIdx.x * ZS + t0 * CDIM + i] = z[i];
+                    for (int q = 0; q < MPPI_BG; ++q)
+                        reinterpret_cast<float4*>(row)[q] = float4{z[4 * q], z[4 * q + 1], z[4 * q + 2], z[4 * q + 3]};
+                } else {
+#pragma unroll
+                    for (int i = 0; i < MPPI_NG; ++i)
+                        if (t0 * CDIM + i < H * CDIM) row[i] = z[i];
+                }
             }
 #pragma unroll
             for (int g = 0; g < MPPI_G; ++g) {
@@ -533,10 +541,9 @@ template <int BLOCK>
 __device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
                                  uint32_t blk, bool active, float total, float* s_z) {
     constexpr int WAVES = BLOCK / 64;
-    constexpr int ZS = MPPI_FUSED_MAX_HC + 1;
+    constexpr int ZS = MPPI_FUSED_ZS;
     __shared__ float s_red[4][WAVES];
     __shared__ __attribute__((aligned(16))) float s_e[BLOCK];
-    __shared__ float s_h[BLOCK / 2];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t HC = d.H * CDIM, W = d.NB * 4;
     float c = (active && mppi_isfinite(total)) ? total : __builtin_inff();
@@ -560,32 +567,48 @@ __device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const 
         float* bs = a.blk_stats + (d.blk_off + blk) * 4;
         bs[0] = m; bs[1] = S; bs[2] = Cs; bs[3] = Q;
     }
-    // V_g[v] = sum over the tile's lanes of e_l z_l[v]: two threads per
-    // column, each over half the lanes in order, then added.  Lanes past K
+    // V_g[v] = sum over the tile's lanes of e_l z_l[v].  Thread (quad, sg)
+    // sums four columns over the 32 lanes of sample group sg in order (two
+    // interleaved accumulators), reading four normals and four weights per
+    // LDS access; the eight groups are then added in order.  Lanes past K
     // have e = 0 and zero noise, and a lane with a non-finite cost has e = 0
     // and finite noise, so every term is finite and no select is needed.
     {
-        // threads [0, BLOCK/2) sum lanes [0, BLOCK/2) of column v, the other
-        // half the other lanes of the same column (distinct banks per wave)
-        const uint32_t half = threadIdx.x / (BLOCK / 2), v = threadIdx.x % (BLOCK / 2);
-        float acc = 0.0f;
-        if (v < HC) {
-            // eight independent chains keep the LDS reads in flight; the
-            // weights are read four at a time (the same address in every lane)
-            float c[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-            const uint32_t l0 = half * (BLOCK / 2);
+        constexpr uint32_t QUADS = MPPI_FUSED_MAX_HC / 4, GROUPS = BLOCK / QUADS, PER = BLOCK / GROUPS;
+        __shared__ float4 s_q[GROUPS][QUADS];
+        const uint32_t quad = threadIdx.x % QUADS, sg = threadIdx.x / QUADS, Q = W / 4;
+        float4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (quad < Q) {
+            float4 a0 = {0.0f, 0.0f, 0.0f, 0.0f}, a1 = {0.0f, 0.0f, 0.0f, 0.0f};
             const float4* e4 = reinterpret_cast<const float4*>(s_e);
-            for (uint32_t l = l0; l < l0 + BLOCK / 2; l += 8) {
-                const float4 ea = e4[l / 4], eb = e4[l / 4 + 1];
-                const float e[8] = {ea.x, ea.y, ea.z, ea.w, eb.x, eb.y, eb.z, eb.w};
+            const uint32_t l0 = sg * PER;
+            for (uint32_t l = l0; l < l0 + PER; l += 4) {
+                const float4 e = e4[l / 4];
+                const float ev[4] = {e.x, e.y, e.z, e.w};
 #pragma unroll
-                for (int j = 0; j < 8; ++j) c[j] = mppi_fma(e[j], s_z[(l + j) * ZS + v], c[j]);
+                for (int j = 0; j < 4; ++j) {
+                    const float4 z = reinterpret_cast<const float4*>(s_z + (l + j) * ZS)[quad];
+                    float4& t = (j & 1) ? a1 : a0;
+                    t.x = mppi_fma(ev[j], z.x, t.x);
+                    t.y = mppi_fma(ev[j], z.y, t.y);
+                    t.z = mppi_fma(ev[j], z.z, t.z);
+                    t.w = mppi_fma(ev[j], z.w, t.w);
+                }
             }
-            acc = ((c[0] + c[1]) + (c[2] + c[3])) + ((c[4] + c[5]) + (c[6] + c[7]));
+            acc = float4{a0.x + a1.x, a0.y + a1.y, a0.z + a1.z, a0.w + a1.w};
         }
-        if (half == 1) s_h[v] = acc;
+        s_q[sg][quad] = acc;
         __syncthreads();
-        if (half == 0 && v < W) a.partials[d.part_off + size_t(blk) * W + v] = v < HC ? acc + s_h[v] : 0.0f;
+        if (sg == 0 && quad < Q) {
+            float4 t = s_q[0][quad];
+            for (uint32_t g = 1; g < GROUPS; ++g) {
+                const float4 u = s_q[g][quad];
+                t.x += u.x; t.y += u.y; t.z += u.z; t.w += u.w;
+            }
+            // columns past H*C hold normals of the next group or zeros; the
+            // combine never reads them
+            reinterpret_cast<float4*>(a.partials + d.part_off + size_t(blk) * W)[quad] = t;
+        }
     }
     // The rows are read by mppi_fused_combine, a later launch.
 }
