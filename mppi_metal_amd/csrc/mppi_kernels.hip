@@ -94,20 +94,55 @@ template <class X, class M, class C>
 __device__ inline auto mppi_terminal(mppi_form_plain, X x, M m, C c, mppi_time)
     -> decltype(MPPI_TERMINAL_COST(x, m, c)) { return MPPI_TERMINAL_COST(x, m, c); }
 
+// Wavefront reductions (every lane of the wavefront active).  Within each
+// row of 16 lanes the values are combined through DPP lane patterns, which
+// cost one VALU instruction each instead of an LDS permute and its wait:
+// quad_perm [1,0,3,2] (lane ^ 1), quad_perm [2,3,0,1] (lane ^ 2), then the
+// half-row mirror (lane i of 8 takes lane 7 - i, which lies in the other
+// quad) and the row mirror (lane i of 16 takes lane 15 - i, in the other
+// half).  After these four steps every lane of a row holds the row's value;
+// the four rows are read out to scalar registers and combined as
+// (r0 op r1) op (r2 op r3), so every lane returns the same value.
+template <int CTRL>
+__device__ inline uint32_t mppi_dpp_(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_mov_dpp(int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ inline float mppi_dpp_(float v) {
+    return __builtin_bit_cast(float, mppi_dpp_<CTRL>(__builtin_bit_cast(uint32_t, v)));
+}
+template <int CTRL>
+__device__ inline double mppi_dpp_(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = mppi_dpp_<CTRL>(uint32_t(b)), hi = mppi_dpp_<CTRL>(uint32_t(b >> 32));
+    return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+__device__ inline float mppi_readlane_(float v, int lane) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+__device__ inline double mppi_readlane_(double v, int lane) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(b)), lane));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(b >> 32)), lane));
+    return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+
+template <typename T, typename Op>
+__device__ inline T mppi_wave_reduce_(T v, Op op) {
+    v = op(v, mppi_dpp_<0xB1>(v));
+    v = op(v, mppi_dpp_<0x4E>(v));
+    v = op(v, mppi_dpp_<0x141>(v));
+    v = op(v, mppi_dpp_<0x140>(v));
+    return op(op(mppi_readlane_(v, 0), mppi_readlane_(v, 16)), op(mppi_readlane_(v, 32), mppi_readlane_(v, 48)));
+}
+
 __device__ inline float mppi_wave_min(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        float w = __shfl_xor(v, o, 64);
-        v = w < v ? w : v;
-    }
-    return v;
+    return mppi_wave_reduce_(v, [](float x, float w) { return w < x ? w : x; });
 }
 
 template <typename T>
 __device__ inline T mppi_wave_sum(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    return mppi_wave_reduce_(v, [](T x, T w) { return x + w; });
 }
 
 __device__ inline bool mppi_isfinite(float x) {
@@ -438,32 +473,47 @@ __device__ void mppi_sum_partials(const mppi_step_args& a, const mppi_agent_desc
 
 // Closed loop, one thread: log the history row (state, first control, best
 // cost), step the plant with the plugin dynamics and the first control, and
-// advance the agent's step and history counters.
+// advance the agent's step and history counters.  The inputs are read by
+// mppi_plant_load, which a caller may issue early to hide their latency.
+struct mppi_plant_in {
+    float x[SDIM];
+    uint32_t step, row;
+};
+
+__device__ inline mppi_plant_in mppi_plant_load(const mppi_step_args& a, const mppi_agent_desc& d) {
+    mppi_plant_in in;
+    const float* xs = a.x + size_t(d.slot) * SDIM;
+#pragma unroll
+    for (int i = 0; i < SDIM; ++i) in.x[i] = xs[i];
+    in.step = a.agents[d.slot].step;
+    in.row = a.agents[d.slot].hist;
+    return in;
+}
+
 __device__ inline void mppi_plant_step(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
-                                       const float* u0) {
-    const uint32_t slot = d.slot;
-    mppi_agent_state* ag = &a.agents[slot];
-    const uint32_t step = ag->step, row = ag->hist;
-    float* xs = a.x + size_t(slot) * SDIM;
+                                       const float* u0, const mppi_plant_in& in, float best) {
+    const uint32_t slot = d.slot, row = in.row;
     float x[SDIM];
     float u[CDIM];
 #pragma unroll
-    for (int i = 0; i < SDIM; ++i) x[i] = xs[i];
+    for (int i = 0; i < SDIM; ++i) x[i] = in.x[i];
 #pragma unroll
     for (int i = 0; i < CDIM; ++i) u[i] = u0[i];
     if (a.hist_states)
         for (int i = 0; i < SDIM; ++i) a.hist_states[(size_t(row) * a.N + agent) * SDIM + i] = x[i];
     if (a.hist_controls)
         for (int i = 0; i < CDIM; ++i) a.hist_controls[(size_t(row) * a.N + agent) * CDIM + i] = u[i];
-    if (a.hist_costs) a.hist_costs[size_t(row) * a.N + agent] = float(ag->min_cost);
-    mppi_rng rng{0u, 0xffffffffu, step, (MPPI_TAG_PLANT << 28) | (slot & MPPI_AGENT_MASK), a.key0, a.key1};
+    if (a.hist_costs) a.hist_costs[size_t(row) * a.N + agent] = best;
+    mppi_rng rng{0u, 0xffffffffu, in.step, (MPPI_TAG_PLANT << 28) | (slot & MPPI_AGENT_MASK), a.key0, a.key1};
     MPPI_DYNAMICS(mppi_state_ref{x}, u, mppi_blob_ref{a.model + slot * a.model_stride}, rng);
+    float* xs = a.x + size_t(slot) * SDIM;
 #pragma unroll
     for (int i = 0; i < SDIM; ++i) xs[i] = x[i];
     if (a.terminal)
         for (int i = 0; i < SDIM; ++i) a.terminal[size_t(agent) * SDIM + i] = x[i];
+    mppi_agent_state* ag = &a.agents[slot];
     ag->hist = row + 1;
-    ag->step = step + 1;
+    ag->step = in.step + 1;
 }
 
 // u = clamp(u_nom + sigma z) (diagonal) or clamp(u_nom + L z) (full) for
@@ -500,7 +550,7 @@ __device__ void mppi_apply_update(const mppi_step_args& a, uint32_t agent, const
     }
     if (a.first_out && threadIdx.x < CDIM) a.first_out[agent * CDIM + threadIdx.x] = uo[threadIdx.x];
     if (a.closed_loop) {
-        if (threadIdx.x == 0) mppi_plant_step(a, agent, d, uo);
+        if (threadIdx.x == 0) mppi_plant_step(a, agent, d, uo, mppi_plant_load(a, d), float(ag->min_cost));
         float* next = a.u_nom_next ? a.u_nom_next + d.seq_off : un;
         for (uint32_t n = threadIdx.x; n < HC; n += BLOCK) {
             const uint32_t t = n / CDIM, i = n % CDIM;
@@ -646,16 +696,41 @@ __device__ void mppi_fused_combine_slice(const mppi_step_args& a, uint32_t agent
     const uint32_t W = d.NB * 4, n_rows = d.n_blk;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
-    const uint32_t RG = BLOCK / ncol, col = threadIdx.x % ncol, rg = threadIdx.x / ncol;
+    // Every global read is issued here, before any reduction: the tiles'
+    // sums of this slice's columns, the tile statistics, this thread's
+    // nominal control and its bounds and noise factors, and (slice 0) the
+    // plant's state, so that the kernel waits on memory once.
+    const uint32_t seq = d.seq_off;
+    const uint32_t ci = threadIdx.x % CDIM;
+    float un = 0.0f, lo = 0.0f, hi = 0.0f;
+    float sc[MPPI_COV_FULL ? CDIM : 1];
+    if (threadIdx.x < ncol) {
+        un = a.u_nom[seq + c0 + threadIdx.x];
+        lo = a.u_min[ci];
+        hi = a.u_max[ci];
+#if MPPI_COV_FULL
+#pragma unroll
+        for (uint32_t j = 0; j < CDIM; ++j) sc[j] = j <= ci ? a.noise_scale[ci * CDIM + j] : 0.0f;
+#else
+        sc[0] = a.noise_scale[ci];
+#endif
+    }
+    // Columns are spread over MAXCOL threads whatever ncol is (threads of
+    // columns past ncol load nothing), so the layout is fixed at compile time.
+    constexpr uint32_t RG = BLOCK / MAXCOL;
+    const uint32_t col = threadIdx.x % MAXCOL, rg = threadIdx.x / MAXCOL;
+    const bool col_ok = col < ncol && rg < RG;
     const float* rows = a.partials + d.part_off + c0 + col;
     float v[RPT];
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
         const uint32_t r = rg + uint32_t(q) * RG;
-        v[q] = (rg < RG && r < n_rows) ? rows[size_t(r) * W] : 0.0f;
+        v[q] = (col_ok && r < n_rows) ? rows[size_t(r) * W] : 0.0f;
     }
     float4 st = {__builtin_inff(), 0.0f, 0.0f, 0.0f};
     if (threadIdx.x < n_rows) st = reinterpret_cast<const float4*>(a.blk_stats + d.blk_off * 4)[threadIdx.x];
+    mppi_plant_in plant{};
+    if (slice == 0 && threadIdx.x == 0 && a.closed_loop) plant = mppi_plant_load(a, d);
     float mg = mppi_wave_min(st.x);
     if (lane == 0) s_min[wave] = mg;
     __syncthreads();
@@ -683,20 +758,29 @@ __device__ void mppi_fused_combine_slice(const mppi_step_args& a, uint32_t agent
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
         const uint32_t r = rg + uint32_t(q) * RG;
-        if (rg < RG && r < n_rows) acc = mppi_fma(s_f[r], v[q], acc);
+        if (col_ok && r < n_rows) acc = mppi_fma(s_f[r], v[q], acc);
     }
     s_part[threadIdx.x] = acc;
     __syncthreads();
     if (threadIdx.x < ncol) {
         float t = 0.0f;
-        for (uint32_t g = 0; g < RG; ++g) t += s_part[g * ncol + threadIdx.x];
+#pragma unroll
+        for (uint32_t g = 0; g < RG; ++g) t += s_part[g * MAXCOL + threadIdx.x];
         s_z[threadIdx.x] = t / S;
     }
     __syncthreads();
-    const uint32_t seq = d.seq_off;
     if (threadIdx.x < ncol) {
-        const uint32_t n = threadIdx.x, i = n % CDIM;
-        const float u = mppi_update_control(a, a.u_nom[seq + c0 + n], s_z + (n - i), i);
+        const uint32_t n = threadIdx.x;
+        const float* zt = s_z + (n - ci);
+#if MPPI_COV_FULL
+        float e = 0.0f;
+#pragma unroll
+        for (uint32_t j = 0; j < CDIM; ++j)
+            if (j <= ci) e = mppi_fma(sc[j], zt[j], e);
+#else
+        const float e = sc[0] * zt[ci];
+#endif
+        const float u = mppi_clampf(un + e, lo, hi);
         s_u[n] = u;
         if (n < nown) a.u_out[d.out_off + c0 + n] = u;
     }
@@ -713,7 +797,7 @@ __device__ void mppi_fused_combine_slice(const mppi_step_args& a, uint32_t agent
                 a.diag[agent * 3 + 1] = float(double(Cs / float(d.K)));
                 a.diag[agent * 3 + 2] = float(double((S * S) / Q));
             }
-            if (a.closed_loop) mppi_plant_step(a, agent, d, s_u);
+            if (a.closed_loop) mppi_plant_step(a, agent, d, s_u, plant, mg);
         }
         if (a.first_out && threadIdx.x < CDIM) a.first_out[agent * CDIM + threadIdx.x] = s_u[threadIdx.x];
     }
