@@ -301,16 +301,20 @@ MPPI_HD float mppi_fmodf(float x, float y) {
     // Otherwise binary long division: yj runs over ay*2^i from the largest
     // value with ax < 2*yj down to ay.  With yj <= r < 2*yj each r - yj is
     // exact (Sterbenz), and halving yj is exact, so the remainder is exact.
-    // On the GPU the loop is entered only by wavefronts that need it.
-    if (mppi_wave_any_(long_div) && long_div) {
-        float yj = ay;
-        while (!(yj > ax * 0.5f)) yj = yj * 2.0f;
-        r = ax;
+    // On the GPU the loop is entered only by wavefronts that need it, through
+    // a uniform branch; lanes of such a wavefront that need no long division
+    // run it on (1, 1), which ends after one pass, and keep their r.
+    if (mppi_wave_any_(long_div)) {
+        const float axl = long_div ? ax : 1.0f, ayl = long_div ? ay : 1.0f;
+        float yj = ayl;
+        while (!(yj > axl * 0.5f)) yj = yj * 2.0f;
+        float rl = axl;
         for (;;) {
-            if (yj <= r) r = r - yj;
-            if (yj == ay) break;
+            if (yj <= rl) rl = rl - yj;
+            if (yj == ayl) break;
             yj = yj * 0.5f;
         }
+        r = long_div ? rl : r;
     }
     // The result has the sign of x (x itself when |x| < |y|, -0 included).
     r = mppi_u2f(mppi_f2u(r) | (mppi_f2u(x) & 0x80000000u));

@@ -76,6 +76,10 @@ constexpr int mppi_gcd(int a, int b) { return b == 0 ? a : mppi_gcd(b, a % b); }
 constexpr int MPPI_G = 4 / mppi_gcd(CDIM, 4);
 constexpr int MPPI_NG = MPPI_G * CDIM;     // normals per group (multiple of 4)
 constexpr int MPPI_BG = MPPI_NG / 4;       // Philox blocks per group
+// The one-kernel step writes each group's normals whole when a group that
+// starts inside the horizon (t0 <= H - 1) and ends past it still fits the
+// padding of a lane's row: (G - 1) C <= ZS - MAX_HC.
+constexpr bool MPPI_FUSED_OVERHANG_OK = CDIM * (MPPI_G - 1) <= MPPI_FUSED_ZS - MPPI_FUSED_MAX_HC;
 
 // Cost calls with or without the optional trailing mppi_time argument: the
 // first overload is viable only when the plugin's function takes it.
@@ -336,7 +340,10 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
             }
             if (FUSED) {
                 float* row = s_z + threadIdx.x * ZS + t0 * CDIM;
-                if (t0 * CDIM + MPPI_NG <= H * CDIM) {
+                // A group that runs past the horizon still fits the row when
+                // its overhang (at most (G - 1) C values) fits the padding;
+                // the values there are never read back.
+                if (MPPI_FUSED_OVERHANG_OK || t0 * CDIM + MPPI_NG <= H * CDIM) {
 #pragma unroll
                     for (int q = 0; q < MPPI_BG; ++q)
                         reinterpret_cast<float4*>(row)[q] = float4{z[4 * q], z[4 * q + 1], z[4 * q + 2], z[4 * q + 3]};
