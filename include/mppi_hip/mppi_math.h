@@ -87,6 +87,19 @@ MPPI_HD float mppi_sqrtf(float x) {
 }
 MPPI_HD float mppi_fabsf(float x) { return mppi_u2f(mppi_f2u(x) & 0x7fffffffu); }
 
+// 1/x within one ulp for normal x and results: the hardware estimate on the
+// GPU, a division on the host.  The two may differ in the last bit, so it is
+// only for bounds whose proof allows that error on each side (conservative
+// tests that gate an exact computation), never for a value that reaches a
+// result.
+MPPI_HD float mppi_rcp_estimatef(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
 // Correctly rounded square root of x in [2^-96, 2^126] or +-0 (finite,
 // no subnormal, no scaling needed): the same result as mppi_sqrtf there.  On
 // the GPU, the hardware square root (within 1 ulp) is corrected by the

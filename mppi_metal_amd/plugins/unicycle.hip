@@ -70,6 +70,35 @@ MPPI_CALLABLE float mppi_stage_cost(const float* state, const float* control,
     return cost;
 }
 
+// The first pass's square test for obstacle (cx, cy, r), as scaled
+// coordinates ox = x a + bx, oy = y a + by with a ~ 1 / (r t), bx = -cx a,
+// by = -cy a: a sample inside the square has max(|ox|, |oy|) < 1.  Proof that
+// every sample the exact pass counts (d2 < r*r, rounded as written there) is
+// flagged: d2 >= fl(ox^2) and fl(r^2) <= r^2 (1 + 2^-24) give |x - cx| <
+// r (1 + 2^-22) (likewise y; r in [2^-60, 2^60] keeps r^2 normal and finite,
+// and an underflowed ox^2 means |x - cx| < 2^-62).  The estimates of a
+// (one ulp each for 1/r and 1/(1 + s), and two roundings) and the roundings
+// of bx and of the fused multiply-add move ox by less than
+// 1.5 * 2^-21 + (|cx| / r) 2^-24 beyond (x - cx) / (r t); the effective t,
+// (1 + s) / (1 - 2^-18) with s = (|cx| + |cy|) / r 2^-21, exceeds 1 by
+// more than four times that.  Radii outside [2^-60, 2^60] are always sent to
+// the exact pass (a = 0); a slot with r <= 0 or NaN is never counted, as its
+// hinge max(0, r - |p - c|) is 0 (bx = by = 2 keeps its square empty).  The
+// gate only decides where the exact pass runs, so the estimate may differ
+// between the GPU and the host in its last bit: the costs do not.
+struct UnicycleGate {
+    float a, bx, by;
+};
+
+MPPI_HD UnicycleGate unicycle_gate(float cx, float cy, float r) {
+    if (!(r > 0.0f)) return {0.0f, 2.0f, 2.0f};
+    if (!(r >= 0x1p-60f && r <= 0x1p60f)) return {0.0f, 0.0f, 0.0f};
+    const float q = mppi_rcp_estimatef(r);
+    const float s = (mppi_fabsf(cx) + mppi_fabsf(cy)) * q * 0x1p-21f;
+    const float a = (q * 0.999996185302734375f) * mppi_rcp_estimatef(1.0f + s);   // 1 - 2^-18
+    return {a, -cx * a, -cy * a};
+}
+
 MPPI_CALLABLE float unicycle_stage_cost_obstacles(const float* state, const float* control,
                                                   const uint8_t* model_params,
                                                   const uint8_t* cost_params_raw) {
@@ -79,30 +108,31 @@ MPPI_CALLABLE float unicycle_stage_cost_obstacles(const float* state, const floa
     uint32_t n = p->n_obstacles < UNICYCLE_MAX_OBSTACLES ? p->n_obstacles : UNICYCLE_MAX_OBSTACLES;
     // max(0, r - |p - c|) summed over the obstacles.  The square root is
     // only needed inside an obstacle, so a cheap first pass tests the square
-    // |ox|, |oy| < r around each circle: inside the circle (d2 < r*r in the
-    // rounded arithmetic below) implies inside the square, since rounding is
-    // monotone.  The exact pass runs only in wavefronts where some sample is
-    // in some square.  The first pass tests all 16 slots whatever n is, so
-    // the obstacle parameters do not depend on n and stay in registers across
-    // the whole horizon.  Slots at or past n may hold anything: they can only
-    // send a wavefront to the exact pass, which skips them.
-    float sq[UNICYCLE_MAX_OBSTACLES];   // max(|ox|, |oy|) - r: < 0 inside the square
-    float margin = 1.0f;                // their minimum
+    // |ox|, |oy| < r around each circle, in coordinates scaled by the radius
+    // (two fused multiply-adds per obstacle; unicycle_gate).  The exact pass
+    // runs only in wavefronts where some sample is in some square.  The first
+    // pass tests all 16 slots whatever n is, so the obstacle parameters do
+    // not depend on n and stay in registers across the whole horizon.  Slots
+    // at or past n may hold anything: they can only send a wavefront to the
+    // exact pass, which skips them.
+    float sq[UNICYCLE_MAX_OBSTACLES];   // max(|ox|, |oy|) / r, scaled: < 1 inside the square
+    float margin = 2.0f;                // their minimum
     MPPI_UNROLL
     for (uint32_t j = 0; j < UNICYCLE_MAX_OBSTACLES; ++j) {
-        float ox = state[0] - p->obstacles[j][0];
-        float oy = state[1] - p->obstacles[j][1];
-        sq[j] = __builtin_fmaxf(__builtin_fabsf(ox), __builtin_fabsf(oy)) - p->obstacles[j][2];
+        const UnicycleGate g = unicycle_gate(p->obstacles[j][0], p->obstacles[j][1], p->obstacles[j][2]);
+        const float ox = mppi_fma(state[0], g.a, g.bx);
+        const float oy = mppi_fma(state[1], g.a, g.by);
+        sq[j] = __builtin_fmaxf(__builtin_fabsf(ox), __builtin_fabsf(oy));
         margin = __builtin_fminf(margin, sq[j]);
     }
     // The exact pass keeps slot order and takes the distance only for
     // obstacles whose square holds some sample of the wavefront, and a square
     // root only for samples inside the circle.  Marked unlikely so that it is
     // compiled out of line rather than evaluated on every step.
-    if (__builtin_expect(mppi_any(margin < 0.0f), 0)) {
+    if (__builtin_expect(mppi_any(margin < 1.0f), 0)) {
         MPPI_UNROLL
         for (uint32_t i = 0; i < UNICYCLE_MAX_OBSTACLES; ++i) {
-            if (__builtin_expect(mppi_any(sq[i] < 0.0f), 0)) {
+            if (__builtin_expect(mppi_any(sq[i] < 1.0f), 0)) {
                 float ox = state[0] - p->obstacles[i][0];
                 float oy = state[1] - p->obstacles[i][1];
                 float r = p->obstacles[i][2];

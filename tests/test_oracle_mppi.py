@@ -276,3 +276,62 @@ def test_pursuit_realised_cost_matches_oracle_stage_cost():
     c64 = scenes.pursuit_realised_cost(sc, a, states, controls, steps)
     assert np.max(np.abs(c32 - c64) / np.maximum(1.0, np.abs(c64))) <= 1e-4
     assert np.count_nonzero(c64 > scenes.pursuit_realised_cost(sc, a, states, controls, steps, w_zone=0.0)) > 10
+
+
+def _unicycle_cost32(sc, x, y, u0, u1):
+    """unicycle_stage_cost_obstacles restated in numpy FP32, one rounding per
+    operation in the plugin's order: goal and control terms, then the hinge
+    max(0, r - |p - c|) summed over slots in order (d2 < r*r, r - sqrt(d2);
+    a slot with r <= 0 adds nothing)."""
+    f = np.float32
+    dx, dy = f(x) - f(sc.goal[0]), f(y) - f(sc.goal[1])
+    cost = f(sc.pos_weight) * (dx * dx + dy * dy)
+    cost = cost + f(sc.control_weight) * (f(u0) * f(u0) + f(u1) * f(u1))
+    pen = f(0.0)
+    for cx, cy, r in list(sc.obstacles)[:16]:
+        ox, oy, r = f(x) - f(cx), f(y) - f(cy), f(r)
+        d2 = ox * ox + oy * oy
+        if r > 0 and d2 < r * r:
+            pen = pen + (r - np.sqrt(d2))
+    return cost + f(sc.obstacle_weight) * pen, pen
+
+
+@pytest.mark.parametrize("obstacles", [
+    [(1.0, 2.0, 0.5), (3.0e4, -2.0e4, 0.01), (-7.5, 0.25, 3.0), (1e-3, 1e-3, 1e-3),
+     (2.0, -1.0, 0.0), (2.0, -1.0, -0.7), (4.0, 4.0, float("nan")), (0.5, 0.5, 0.25)],
+    [(0.0, 0.0, 2.0 ** -61), (1.0, 1.0, 2.0 ** -59), (-3.0, 2.0, 1e-12)],
+    [(5.0, 5.0, 2.0 ** 61)],
+])
+def test_unicycle_obstacle_square_gate_misses_no_sample(obstacles):
+    """The obstacle cost's first pass (a square test in coordinates scaled by
+    the radius) only decides where the exact hinge is evaluated: on states
+    within a few ulps of every circle, on the axes (where square and circle
+    touch) and diagonals, including far centres with small radii, radii at
+    and beyond the scaled range, zero, negative and NaN radii, the stage
+    cost equals the FP32 hinge evaluated for every slot."""
+    from mppi_metal_amd import scenes
+    sc = scenes.unicycle_scene(4, 42)
+    sc.obstacles = obstacles
+    mb, cb = sc.model_blob(), sc.cost_blob()
+    ora = ol.Problem("unicycle", 4, 2, 4, 16, sc.lam, np.array(sc.sigma, np.float32), sc.u_min, sc.u_max,
+                     42, "mppi_dynamics", "unicycle_stage_cost_obstacles", "unicycle_terminal_cost", mb, cb)
+    pts = []
+    for cx, cy, r in obstacles:
+        if not (r > 0.0):
+            r = 0.3
+        for ang in np.arange(8) * (np.pi / 4):
+            for k in range(-6, 7):
+                d = np.float64(r) * (1.0 + k * 2.0 ** -23)
+                pts.append((cx + d * np.cos(ang), cy + d * np.sin(ang)))
+                # the float next to the axis point, toward and away from the centre
+                if ang % (np.pi / 2) == 0:
+                    px = np.float32(cx + d * np.cos(ang))
+                    pts.append((float(np.nextafter(px, np.float32(cx))), cy + d * np.sin(ang)))
+    states = np.zeros((len(pts), 4), np.float32)
+    states[:, :2] = np.array(pts, np.float64).astype(np.float32)
+    controls = np.tile(np.array([0.3, -0.2], np.float32), (len(pts), 1))
+    got = ora.stage_costs(states, controls, np.zeros(len(pts), np.uint32))
+    want, pen = np.array([_unicycle_cost32(sc, s[0], s[1], 0.3, -0.2) for s in states], np.float32).T
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    # the states straddle the circles: some pay the hinge, some do not
+    assert 0 < np.count_nonzero(pen) < len(pts)
