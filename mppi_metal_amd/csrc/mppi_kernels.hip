@@ -302,6 +302,18 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
     constexpr int ZS = MPPI_FUSED_ZS;
     __shared__ __attribute__((aligned(16))) float s_z[FUSED ? BLOCK * ZS : 1];
     for (uint32_t i = threadIdx.x; i < H * CDIM; i += BLOCK) s_un[i] = a.u_nom[d.seq_off + i];
+    // FUSED: the Philox words that depend only on a noise block's index, the
+    // step and the tag (mppi_philox_shared) are the same for every sample of
+    // the tile, so they are formed once here, one block per thread, and read
+    // with each group's nominal controls; each lane then runs only the part
+    // of the generator that depends on its sample index.  2 * MPPI_BG blocks
+    // of padding for the read one group ahead.
+    constexpr int PH = FUSED ? MPPI_FUSED_MAX_HC / 4 + 2 * MPPI_BG : 1;
+    __shared__ __attribute__((aligned(16))) mppi_philox_shared_words s_ph[PH];
+    if (FUSED)
+        for (uint32_t b = threadIdx.x; b < uint32_t(PH); b += BLOCK)
+            s_ph[b] = mppi_philox_shared(b, step, tag_agent, a.key0, a.key1);
+    const uint32_t ph_e = mppi_philox_lane_key(step, a.key0);
     float scale[MPPI_COV_FULL ? CDIM * CDIM : CDIM], lo[CDIM], hi[CDIM];
 #pragma unroll
     for (int i = 0; i < (MPPI_COV_FULL ? CDIM * CDIM : CDIM); ++i) scale[i] = a.noise_scale[i];
@@ -317,24 +329,35 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
         float x[SDIM];
 #pragma unroll
         for (int i = 0; i < SDIM; ++i) x[i] = a.x[size_t(slot) * SDIM + i];
-        // The nominal controls of group t0 + G are read while group t0 runs,
-        // so the LDS latency hides behind a group's work.  The read past the
-        // horizon lands in the padding (t0 < H, so (t0 + G) C + NG <= H C + 2 NG).
+        // The nominal controls (and FUSED, the shared Philox words) of group
+        // t0 + G are read while group t0 runs, so the LDS latency hides
+        // behind a group's work.  The read past the horizon lands in the
+        // padding (t0 < H, so (t0 + G) C + NG <= H C + 2 NG).
         float un_next[MPPI_NG];
+        mppi_philox_shared_words ph_next[MPPI_BG];
 #pragma unroll
         for (int i = 0; i < MPPI_NG; ++i) un_next[i] = s_un[i];
+#pragma unroll
+        for (int q = 0; q < MPPI_BG; ++q) ph_next[q] = s_ph[FUSED ? q : 0];
         for (uint32_t t0 = 0; t0 < H; t0 += MPPI_G) {
             float un[MPPI_NG];
+            mppi_philox_shared_words ph[MPPI_BG];
 #pragma unroll
             for (int i = 0; i < MPPI_NG; ++i) {
                 un[i] = un_next[i];
                 un_next[i] = s_un[(t0 + MPPI_G) * CDIM + i];
             }
-            float z[MPPI_NG];
             const uint32_t b0 = (t0 * CDIM) >> 2;
 #pragma unroll
             for (int q = 0; q < MPPI_BG; ++q) {
-                mppi_f32x4 v = mppi_normal4(mppi_u32x4{b0 + q, k, step, tag_agent}, a.key0, a.key1);
+                ph[q] = ph_next[q];
+                if (FUSED) ph_next[q] = s_ph[b0 + MPPI_BG + q];
+            }
+            float z[MPPI_NG];
+#pragma unroll
+            for (int q = 0; q < MPPI_BG; ++q) {
+                mppi_f32x4 v = FUSED ? mppi_normal4_words(mppi_philox_lanes(k, ph_e, ph[q], a.key0, a.key1))
+                                     : mppi_normal4(mppi_u32x4{b0 + q, k, step, tag_agent}, a.key0, a.key1);
                 z[4 * q + 0] = v.x; z[4 * q + 1] = v.y;
                 z[4 * q + 2] = v.z; z[4 * q + 3] = v.w;
             }

@@ -799,6 +799,14 @@ void oracle_library_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uin
     out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
 }
 
+// The same block through the header's split form (the shared words of x, z,
+// w, then the lane part from y), as the GPU rollout computes it.
+void oracle_library_philox4x32_split(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    const mppi_philox_shared_words s = mppi_philox_shared(ctr[0], ctr[2], ctr[3], key[0], key[1]);
+    mppi_u32x4 r = mppi_philox_lanes(ctr[1], mppi_philox_lane_key(ctr[2], key[0]), s, key[0], key[1]);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
+}
+
 // Costs of samples [k0, k1) only (costs is indexed by absolute sample).
 int oracle_rollout_range(const oracle_problem* p, uint32_t step, uint32_t agent, const float* x0,
                          const float* u_nom, uint32_t k0, uint32_t k1, float* costs) {

@@ -43,6 +43,7 @@ def lib():
         sig = {
             "oracle_philox4x32": (None, [_U, _U, _U]),
             "oracle_library_philox4x32": (None, [_U, _U, _U]),
+            "oracle_library_philox4x32_split": (None, [_U, _U, _U]),
             "oracle_philox2x32": (None, [_U, ctypes.c_uint32, _U]),
             "oracle_normals4": (None, [ctypes.c_uint32] * 5 + [ctypes.c_uint64, _F]),
             "oracle_normals4_64": (None, [ctypes.c_uint32] * 5 + [ctypes.c_uint64, _D]),
@@ -208,10 +209,13 @@ class Problem:
 
 
 def philox4x32(ctr, key, library=False):
+    """library: False (this oracle's own generator), True (the library
+    header's compiled for the host) or "split" (the header's split form)."""
     c = np.asarray(ctr, np.uint32)
     k = np.asarray(key, np.uint32)
     out = np.empty(4, np.uint32)
-    fn = lib().oracle_library_philox4x32 if library else lib().oracle_philox4x32
+    fn = {False: lib().oracle_philox4x32, True: lib().oracle_library_philox4x32,
+          "split": lib().oracle_library_philox4x32_split}[library]
     fn(_p(c, _U), _p(k, _U), _p(out, _U))
     return [int(v) for v in out]
 

@@ -31,6 +31,7 @@ KAT_2x32 = [
 def test_philox4x32_kat(ctr, key, want):
     assert ol.philox4x32(ctr, key) == list(want)
     assert ol.philox4x32(ctr, key, library=True) == list(want)
+    assert ol.philox4x32(ctr, key, library="split") == list(want)
     assert scenes.philox4x32_10(ctr, key) == list(want)
 
 
@@ -45,6 +46,7 @@ def test_library_philox_matches_oracle_on_random_counters():
         c = [int(v) for v in rng.integers(0, 2**32, 4, dtype=np.uint64)]
         k = [int(v) for v in rng.integers(0, 2**32, 2, dtype=np.uint64)]
         assert ol.philox4x32(c, k, library=True) == ol.philox4x32(c, k)
+        assert ol.philox4x32(c, k, library="split") == ol.philox4x32(c, k)
 
 
 def test_uniform_conversion_edges():
