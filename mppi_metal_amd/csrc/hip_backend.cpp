@@ -17,6 +17,7 @@
 // copy (u_next and the diagnostics), then synchronises the context's stream.
 #include "backend.h"
 
+#include <hip/hip_ext.h>
 #include <hip/hiprtc.h>
 
 #include <algorithm>
@@ -540,6 +541,18 @@ bool use_fused(const mppi_hip_ctx* ctx, const mppi_agent_desc& d, uint32_t N, bo
     return uint64_t(d.n_blk) * N <= uint64_t(ctx->num_cus);
 }
 
+// Launch f (grid in workgroups of kBlock) on the context's stream.  With a
+// start and stop event (profiling), the runtime attaches them to the dispatch
+// itself, so the pair times the kernel alone, as a kernel trace does, rather
+// than the event packets and the launch gap around it.
+hipError_t launch_kernel(mppi_hip_ctx* ctx, hipFunction_t f, uint32_t gx, uint32_t gy, uint32_t gz,
+                         void** params, hipEvent_t start, hipEvent_t stop) {
+    if (start && stop)
+        return hipExtModuleLaunchKernel(f, gx * kBlock, gy, gz, kBlock, 1, 1, 0, ctx->stream, params, nullptr,
+                                        start, stop, 0);
+    return hipModuleLaunchKernel(f, gx, gy, gz, kBlock, 1, 1, 0, ctx->stream, params, nullptr);
+}
+
 // Enqueue one MPPI iteration for N agents: grid-mapped rollout + reduce, or
 // the one-kernel step.
 int launch_iteration(mppi_hip_ctx* ctx, const std::vector<mppi_agent_desc>& desc,
@@ -551,29 +564,21 @@ int launch_iteration(mppi_hip_ctx* ctx, const std::vector<mppi_agent_desc>& desc
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
     if (ctx->profiling) {
         e0 = take_event(ctx); e1 = take_event(ctx); e2 = take_event(ctx); e3 = take_event(ctx);
-        if (e0) (void)hipEventRecord(e0, ctx->stream);
     }
     alignas(16) unsigned char sur_args[1024];
     hipFunction_t sur_fn = nullptr;
     const bool sur = surrogate_rollout(ctx, args, &sur_fn, sur_args, sizeof(sur_args));
     ctx->weights_pending = false;
     if (use_fused(ctx, d, N, sur)) {
-        MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(ctx->mod->rollout_fused, d.n_blk, N, 1, kBlock, 1, 1, 0,
-                                                ctx->stream, params, nullptr));
-        if (ctx->profiling) {
-            if (e1) (void)hipEventRecord(e1, ctx->stream);
-            if (e2) (void)hipEventRecord(e2, ctx->stream);
-        }
+        MPPI_HIP_TRY(ctx, launch_kernel(ctx, ctx->mod->rollout_fused, d.n_blk, N, 1, params, e0, e1));
         // the combine's workgroups: slices of 16 / C timesteps
         const uint32_t tpw = ctx->C < 16 ? 16 / ctx->C : 1;
-        MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(ctx->mod->fused_combine, (d.H + tpw - 1) / tpw, N, 1, kBlock,
-                                                1, 1, 0, ctx->stream, params, nullptr));
+        MPPI_HIP_TRY(ctx, launch_kernel(ctx, ctx->mod->fused_combine, (d.H + tpw - 1) / tpw, N, 1, params, e2, e3));
         ctx->weights_pending = true;
         ctx->weights_args = args;
         ctx->weights_grid_x = (d.K + kBlock - 1) / kBlock;
         ctx->weights_grid_y = N;
         if (ctx->profiling) {
-            if (e3) (void)hipEventRecord(e3, ctx->stream);
             if (e0 && e1) ctx->pending_rollout.push_back({e0, e1});
             if (e2 && e3) ctx->pending_reduce.push_back({e2, e3});
             ctx->prof_launches += 1;
@@ -582,20 +587,12 @@ int launch_iteration(mppi_hip_ctx* ctx, const std::vector<mppi_agent_desc>& desc
     }
     if (sur) {
         void* sparams[] = {sur_args};
-        MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(sur_fn, d.n_blk, N, 1, kBlock, 1, 1, 0, ctx->stream,
-                                                sparams, nullptr));
+        MPPI_HIP_TRY(ctx, launch_kernel(ctx, sur_fn, d.n_blk, N, 1, sparams, e0, e1));
     } else {
-        MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(ctx->mod->rollout, d.n_blk, N, 1, kBlock, 1, 1, 0,
-                                                ctx->stream, params, nullptr));
+        MPPI_HIP_TRY(ctx, launch_kernel(ctx, ctx->mod->rollout, d.n_blk, N, 1, params, e0, e1));
     }
+    MPPI_HIP_TRY(ctx, launch_kernel(ctx, ctx->mod->reduce, d.n_chunks, d.n_groups, N, params, e2, e3));
     if (ctx->profiling) {
-        if (e1) (void)hipEventRecord(e1, ctx->stream);
-        if (e2) (void)hipEventRecord(e2, ctx->stream);
-    }
-    MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(ctx->mod->reduce, d.n_chunks, d.n_groups, N, kBlock, 1, 1,
-                                            0, ctx->stream, params, nullptr));
-    if (ctx->profiling) {
-        if (e3) (void)hipEventRecord(e3, ctx->stream);
         if (e0 && e1) ctx->pending_rollout.push_back({e0, e1});
         if (e2 && e3) ctx->pending_reduce.push_back({e2, e3});
         ctx->prof_launches += 1;
