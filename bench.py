@@ -13,7 +13,7 @@ Printed as one JSON line:
   step_p50/p99_ms  host wall time of a blocking MPPIDriver-style step()
                    (uploads x0 and the nominal, downloads u_next), 200 steps
   roofline         the rollout kernel (the dominant one) against the VALU
-                   lane-op peak, from HIP-event kernel times; reduce_roofline
+                   lane-op peak, from kernel-exact HIP event times; reduce_roofline
                    the reduce kernel against HBM bandwidth
   cpu_baseline     the CPU oracle (scalar C++, same algorithm and noise
                    stream) on the host's cores over a bounded sample of steps
@@ -363,7 +363,8 @@ def main():
     value = K * H * args.steps / elapsed
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f}s", file=sys.stderr, flush=True)
 
-    # Kernel times from HIP events on the library's stream.
+    # Kernel times from start/stop events attached to each profiled dispatch
+    # (the library launches with hipExtModuleLaunchKernel while profiling).
     from mppi_metal_amd import _native
     fused = H * C <= 128 and K < 2 ** 18 and -(-K // 256) <= 256
     hip.profile(True)
@@ -372,7 +373,8 @@ def main():
     roll_ms, red_ms, launches = hip.profile_read()
     hip.profile(False)
     roll_s = roll_ms / launches / 1e3
-    # The one-kernel step also keeps and averages the noise: 2 C more slots
+    second_s = red_ms / launches / 1e3     # the combine launch (fused) or the reduce
+    # The one-pass step also keeps and averages the noise: 2 C more slots
     # (an LDS store and an FMA per normal).
     w_ss = valu_ops_per_sample_step(n_obs) + (2 * C if fused else 0)
     achieved_ops = K * H * w_ss / roll_s
@@ -435,13 +437,17 @@ def main():
         "config": {"workload": "plain MPPI closed loop, kinematic unicycle", "K": K, "H": H,
                    "obstacles": n_obs, "closed_loop_steps": args.steps, "lambda": 1.0,
                    "sigma": [0.5, 0.3]},
-        "step_path": "fused (one kernel)" if fused else "two kernels",
+        "step_path": "one pass (rollout keeping the noise, then combine)" if fused else "two kernels",
         "roofline": {"bound": "valu", "kernel": "mppi_rollout_fused" if fused else "mppi_rollout",
                      "achieved": achieved_ops / 1e12, "peak": VALU_PEAK_LANE_OPS / 1e12,
                      "unit": "T lane-op/s", "frac": achieved_ops / VALU_PEAK_LANE_OPS,
                      "ops_per_sample_step": w_ss, "avg_launch_us": roll_s * 1e6,
                      "traffic": measured_traffic("mppi_rollout_fused" if fused else "mppi_rollout")
                      if default_workload else None},
+        "second_launch": {"kernel": "mppi_fused_combine" if fused else "mppi_reduce",
+                          "avg_launch_us": second_s * 1e6,
+                          "traffic": measured_traffic("mppi_fused_combine" if fused else "mppi_reduce")
+                          if default_workload else None},
         "two_kernel_path": {"value": K * H * args.steps / el2, "ms_per_step": el2 / args.steps * 1e3,
                             "rollout_us": roll2_s * 1e6, "reduce_us": red_s * 1e6,
                             "rollout_frac": K * H * valu_ops_per_sample_step(n_obs) / roll2_s / VALU_PEAK_LANE_OPS},

@@ -18,7 +18,7 @@ namespace mppi_hip_impl {
 constexpr uint32_t kBlock = 256;      // workgroup size of the grid-mapped kernels
 constexpr uint32_t kReduceBPG = 8;    // noise blocks per reduce tile
 constexpr uint32_t kMaxHC = 4096;     // horizon * control_dim limit
-constexpr uint32_t kFusedMaxHC = 128; // horizon * control_dim limit of the one-kernel step
+constexpr uint32_t kFusedMaxHC = 128; // horizon * control_dim limit of the one-pass step
 
 // One loaded code object and the kernels looked up in it.
 struct LoadedModule {
@@ -27,7 +27,7 @@ struct LoadedModule {
     hipFunction_t reduce = nullptr;
     hipFunction_t rollout_fused = nullptr;  // one-pass step: tiles
     hipFunction_t fused_combine = nullptr;  // one-pass step: combine and update
-    hipFunction_t weights = nullptr;        // per-sample weights after a one-kernel step
+    hipFunction_t weights = nullptr;        // per-sample weights after a one-pass step
     // batch kernels, indexed [block 64/128/256 -> 0/1/2][table 0 / persistent 1]
     hipFunction_t rollout_b[3][2] = {};
     hipFunction_t reduce_b[3][2] = {};
@@ -95,7 +95,7 @@ struct mppi_hip_ctx {
     const float* last_weights = nullptr;
     std::vector<mppi_agent_desc> last_desc;
 
-    // step path (mppi_hip_set_step_path); a one-kernel step leaves the
+    // step path (mppi_hip_set_step_path); a one-pass step leaves the
     // per-sample weights to be formed when they are read
     uint32_t step_path = MPPI_HIP_PATH_AUTO;
     bool weights_pending = false;

@@ -431,7 +431,7 @@ std::vector<mppi_agent_desc> uniform_descs(const mppi_hip_ctx* ctx, uint32_t N) 
         v[a].seq_off = uint64_t(a) * ctx->H * ctx->C;
         v[a].out_off = v[a].seq_off;
         v[a].blk_off = uint64_t(a) * d.n_blk;
-        // room for either the reduce's chunk partials or the one-kernel
+        // room for either the reduce's chunk partials or the one-pass
         // step's per-workgroup sums
         v[a].part_off = uint64_t(a) * std::max(d.n_chunks, d.n_blk) * d.NB * 4;
     }
@@ -529,7 +529,7 @@ int upload(mppi_hip_ctx* ctx, const std::vector<mppi_agent_desc>& desc, uint64_t
     return MPPI_HIP_OK;
 }
 
-// The one-kernel step applies when the normals of a 256-sample workgroup fit
+// The one-pass step applies when the normals of a 256-sample workgroup fit
 // in LDS (H*C <= 128), K is below the FP64 threshold, and the surrogate is
 // not in use.  AUTO takes it when the rollout grid has at most one workgroup
 // per compute unit: the kernel's LDS then costs no occupancy, since each
@@ -554,7 +554,7 @@ hipError_t launch_kernel(mppi_hip_ctx* ctx, hipFunction_t f, uint32_t gx, uint32
 }
 
 // Enqueue one MPPI iteration for N agents: grid-mapped rollout + reduce, or
-// the one-kernel step.
+// the one-pass step.
 int launch_iteration(mppi_hip_ctx* ctx, const std::vector<mppi_agent_desc>& desc,
                      const mppi_step_args& a) {
     const uint32_t N = uint32_t(desc.size());
@@ -995,7 +995,7 @@ int read_per_sample(mppi_hip_ctx* ctx, const float* base, uint32_t agent, float*
         return fail(ctx, MPPI_HIP_E_INVALID, std::string(what) + ": out must hold the agent's sample count.");
     const bool weights = std::strcmp(what, "read_weights") == 0;
     if (weights && ctx->weights_pending) {
-        // A one-kernel step: form the weights of its iteration now.
+        // A one-pass step: form the weights of its iteration now.
         mppi_step_args wa = ctx->weights_args;
         void* params[] = {&wa};
         MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(ctx->mod->weights, ctx->weights_grid_x, ctx->weights_grid_y, 1,

@@ -68,7 +68,7 @@
 #define MPPI_NV (4 * MPPI_REDUCE_BPG)     // accumulators per reduce thread
 #define MPPI_FP64_MIN_K (1u << 18)
 #define MPPI_MAX_HC 4096
-#define MPPI_FUSED_MAX_HC 128             // horizon * control_dim limit of the one-kernel step
+#define MPPI_FUSED_MAX_HC 128             // horizon * control_dim limit of the one-pass step
 #define MPPI_FUSED_ZS (MPPI_FUSED_MAX_HC + 4)  // LDS row of a lane's normals (16-byte multiple)
 
 constexpr int mppi_gcd(int a, int b) { return b == 0 ? a : mppi_gcd(b, a % b); }
@@ -76,7 +76,7 @@ constexpr int mppi_gcd(int a, int b) { return b == 0 ? a : mppi_gcd(b, a % b); }
 constexpr int MPPI_G = 4 / mppi_gcd(CDIM, 4);
 constexpr int MPPI_NG = MPPI_G * CDIM;     // normals per group (multiple of 4)
 constexpr int MPPI_BG = MPPI_NG / 4;       // Philox blocks per group
-// The one-kernel step writes each group's normals whole when a group that
+// The one-pass step writes each group's normals whole when a group that
 // starts inside the horizon (t0 <= H - 1) and ends past it still fits the
 // padding of a lane's row: (G - 1) C <= ZS - MAX_HC.
 constexpr bool MPPI_FUSED_OVERHANG_OK = CDIM * (MPPI_G - 1) <= MPPI_FUSED_ZS - MPPI_FUSED_MAX_HC;
@@ -275,7 +275,7 @@ __device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const 
 
 // ---------------------------------------------------------------------------
 // Rollout tile: samples [blk*BLOCK, blk*BLOCK + BLOCK) of one agent.
-// FUSED (one-kernel step, H*C <= MPPI_FUSED_MAX_HC): every lane also keeps its
+// FUSED (one-pass step, H*C <= MPPI_FUSED_MAX_HC): every lane also keeps its
 // normals in LDS, and the tile ends with its weighted noise sums
 // (mppi_fused_stats) instead of the plain softmax partials.
 // ---------------------------------------------------------------------------
@@ -942,7 +942,7 @@ extern "C" __global__ void __launch_bounds__(256) mppi_reduce(mppi_step_args a) 
     mppi_reduce_tile<256>(a, agent, blockIdx.x, blockIdx.y);
 }
 
-// One-kernel step (grid as mppi_rollout; H*C <= MPPI_FUSED_MAX_HC, K < 2^18).
+// Rollout of the one-pass step (grid as mppi_rollout; H*C <= MPPI_FUSED_MAX_HC, K < 2^18).
 extern "C" __global__ void __launch_bounds__(256) mppi_rollout_fused(mppi_step_args a) {
     const uint32_t agent = blockIdx.y;
     if (blockIdx.x >= a.desc[agent].n_blk) return;
@@ -950,7 +950,7 @@ extern "C" __global__ void __launch_bounds__(256) mppi_rollout_fused(mppi_step_a
 }
 
 // Per-sample weights of the last iteration, w_k = exp(-(c_k - m)/lambda) / S
-// (the reduce kernel's formula), for a one-kernel step, which does not form
+// (the reduce kernel's formula), for a one-pass step, which does not form
 // them; grid (ceil(K / 256), N).
 // Combine of the one-pass step (grid: timestep slices x agents).
 extern "C" __global__ void __launch_bounds__(256) mppi_fused_combine(mppi_step_args a) {
