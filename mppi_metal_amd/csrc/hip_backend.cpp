@@ -541,16 +541,16 @@ bool use_fused(const mppi_hip_ctx* ctx, const mppi_agent_desc& d, uint32_t N, bo
     return uint64_t(d.n_blk) * N <= uint64_t(ctx->num_cus);
 }
 
-// Launch f (grid in workgroups of kBlock) on the context's stream.  With a
-// start and stop event (profiling), the runtime attaches them to the dispatch
-// itself, so the pair times the kernel alone, as a kernel trace does, rather
-// than the event packets and the launch gap around it.
+// Launch f (grid in workgroups of `block` threads) on the context's stream.
+// With a start and stop event (profiling), the runtime attaches them to the
+// dispatch itself, so the pair times the kernel alone, as a kernel trace
+// does, rather than the event packets and the launch gap around it.
 hipError_t launch_kernel(mppi_hip_ctx* ctx, hipFunction_t f, uint32_t gx, uint32_t gy, uint32_t gz,
-                         void** params, hipEvent_t start, hipEvent_t stop) {
+                         void** params, hipEvent_t start, hipEvent_t stop, uint32_t block = kBlock) {
     if (start && stop)
-        return hipExtModuleLaunchKernel(f, gx * kBlock, gy, gz, kBlock, 1, 1, 0, ctx->stream, params, nullptr,
+        return hipExtModuleLaunchKernel(f, gx * block, gy, gz, block, 1, 1, 0, ctx->stream, params, nullptr,
                                         start, stop, 0);
-    return hipModuleLaunchKernel(f, gx, gy, gz, kBlock, 1, 1, 0, ctx->stream, params, nullptr);
+    return hipModuleLaunchKernel(f, gx, gy, gz, block, 1, 1, 0, ctx->stream, params, nullptr);
 }
 
 // Enqueue one MPPI iteration for N agents: grid-mapped rollout + reduce, or
@@ -570,7 +570,8 @@ int launch_iteration(mppi_hip_ctx* ctx, const std::vector<mppi_agent_desc>& desc
     const bool sur = surrogate_rollout(ctx, args, &sur_fn, sur_args, sizeof(sur_args));
     ctx->weights_pending = false;
     if (use_fused(ctx, d, N, sur)) {
-        MPPI_HIP_TRY(ctx, launch_kernel(ctx, ctx->mod->rollout_fused, d.n_blk, N, 1, params, e0, e1));
+        // two threads per sample: noise producers and rollout consumers
+        MPPI_HIP_TRY(ctx, launch_kernel(ctx, ctx->mod->rollout_fused, d.n_blk, N, 1, params, e0, e1, 2 * kBlock));
         // the combine's workgroups: slices of 16 / C timesteps
         const uint32_t tpw = ctx->C < 16 ? 16 / ctx->C : 1;
         MPPI_HIP_TRY(ctx, launch_kernel(ctx, ctx->mod->fused_combine, (d.H + tpw - 1) / tpw, N, 1, params, e2, e3));

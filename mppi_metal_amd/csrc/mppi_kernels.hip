@@ -269,7 +269,7 @@ __device__ void mppi_block_stats(const mppi_step_args& a, uint32_t agent, const 
     else mppi_combine_stats<BLOCK, float>(a, agent, d);
 }
 
-template <int BLOCK>
+template <int SAMPLES, int THREADS = SAMPLES>
 __device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
                                  uint32_t blk, bool active, float total, float* s_z);
 
@@ -400,6 +400,157 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
 
     if (FUSED) mppi_fused_stats<BLOCK>(a, agent, d, blk, active, total, s_z);
     else mppi_block_stats<BLOCK>(a, agent, d, blk, active, total);
+}
+
+// ---------------------------------------------------------------------------
+// Rollout tile of the one-pass step with two wavefronts per SIMD: SAMPLES
+// samples and 2 * SAMPLES threads.  Threads [SAMPLES, 2 SAMPLES) produce the
+// noise (Philox and Box-Muller) into the LDS rows that the tile keeps for
+// its noise sums anyway; threads [0, SAMPLES) run the rollout and read each
+// group's normals from their row.  Every SIMD holds one wavefront of each
+// kind, so the scalar instructions, branches, hazard waits and quarter-rate
+// multiplies of one issue beside the other's vector work.  The horizon runs
+// in phases of MPPI_PHASE_G groups: a consumer runs phase p once its
+// producer has published it, while the producer goes on with the next ones.
+// ---------------------------------------------------------------------------
+constexpr uint32_t MPPI_PHASE_G = 4;    // groups of timesteps per phase
+
+template <int SAMPLES>
+__device__ void mppi_fused_tile(const mppi_step_args& a, uint32_t agent, uint32_t blk) {
+    constexpr int THREADS = 2 * SAMPLES;
+    constexpr int ZS = MPPI_FUSED_ZS;
+    const mppi_agent_desc d = a.desc[agent];
+    const bool producer = threadIdx.x >= uint32_t(SAMPLES);
+    const uint32_t sid = producer ? threadIdx.x - SAMPLES : threadIdx.x;
+    const uint32_t k = blk * SAMPLES + sid;
+    const uint32_t H = d.H;
+    const bool active = k < d.K;
+    const uint32_t slot = d.slot;
+    const uint32_t step = a.agents[slot].step;
+    const uint32_t tag_agent = (MPPI_TAG_CONTROL_NOISE << 28) | (slot & MPPI_AGENT_MASK);
+    const uint32_t dyn_tag = (MPPI_TAG_DYNAMICS << 28) | (slot & MPPI_AGENT_MASK);
+    const uint8_t* model = a.model + slot * a.model_stride;
+    const uint8_t* cost = a.cost + slot * a.cost_stride;
+    const uint32_t n_groups = (H + MPPI_G - 1) / MPPI_G;
+    const uint32_t n_phases = (n_groups + MPPI_PHASE_G - 1) / MPPI_PHASE_G;
+
+    // The nominal sequence (read by the consumers a phase at a time, so the
+    // LDS latency is waited out once per phase), the shared Philox words of
+    // every noise block (mppi_philox_shared) and the normals, one row per
+    // sample ([sample][ZS], rows 16-byte aligned).
+    __shared__ __attribute__((aligned(16))) float s_un[MPPI_FUSED_MAX_HC + MPPI_NG];
+    __shared__ __attribute__((aligned(16))) mppi_philox_shared_words s_ph[MPPI_FUSED_MAX_HC / 4 + MPPI_BG];
+    __shared__ __attribute__((aligned(16))) float s_z[SAMPLES * ZS];
+    for (uint32_t i = threadIdx.x; i < H * CDIM; i += THREADS) s_un[i] = a.u_nom[d.seq_off + i];
+    for (uint32_t b = threadIdx.x; b < n_groups * MPPI_BG; b += THREADS)
+        s_ph[b] = mppi_philox_shared(b, step, tag_agent, a.key0, a.key1);
+    // Progress of producer wavefront w (the rows of samples [64 w, 64 w + 64),
+    // which consumer wavefront w reads): phases completed.
+    __shared__ uint32_t s_prog[SAMPLES / 64];
+    if (threadIdx.x < uint32_t(SAMPLES / 64)) s_prog[threadIdx.x] = 0;
+    float* row = s_z + sid * ZS;
+    float total = 0.0f;
+    __syncthreads();                         // s_un, s_ph and s_prog written
+
+    // Producer wavefront w finishes a phase with a workgroup release fence
+    // (its LDS writes are complete) and a relaxed store of the phase count
+    // to s_prog[w]; consumer wavefront w reads s_prog[w] with relaxed loads
+    // at the start of each phase until the phase is there, and adds a zero
+    // taken from that value (bit 31, never set) to its row address, so its
+    // reads of the phase's normals are issued only after the count arrived.
+    // No barrier or fence sits in the consumers' time loop: both keep the
+    // compiler from hoisting the plugin's loop-invariant parameter loads and
+    // set-up out of it.  The producers never wait, so the wait ends; it is
+    // bounded all the same.
+    constexpr uint32_t SPIN_MAX = 1u << 20;
+    const uint32_t wave = (threadIdx.x & (SAMPLES - 1)) >> 6;
+    if (producer) {
+        const uint32_t ph_e = mppi_philox_lane_key(step, a.key0);
+        for (uint32_t p = 0; p < n_phases; ++p) {
+#pragma unroll 1
+            for (uint32_t gi = 0; gi < MPPI_PHASE_G; ++gi) {
+                const uint32_t g = p * MPPI_PHASE_G + gi;
+                if (g < n_groups) {
+#pragma unroll
+                    for (int q = 0; q < MPPI_BG; ++q) {
+                        const uint32_t b = g * MPPI_BG + q;
+                        mppi_f32x4 v = mppi_normal4_words(mppi_philox_lanes(k, ph_e, s_ph[b], a.key0, a.key1));
+                        // samples past K keep zero noise, so the tile's
+                        // noise sums need no select
+                        if (!active) v = mppi_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                        if (MPPI_FUSED_OVERHANG_OK || (b + 1) * 4 <= H * CDIM) {
+                            reinterpret_cast<float4*>(row)[b] = float4{v.x, v.y, v.z, v.w};
+                        } else {
+                            const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+                                if (b * 4 + j < H * CDIM) row[b * 4 + j] = vv[j];
+                        }
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            if ((threadIdx.x & 63) == 0)
+                __hip_atomic_store(&s_prog[wave], p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    } else {
+        float scale[MPPI_COV_FULL ? CDIM * CDIM : CDIM], lo[CDIM], hi[CDIM];
+#pragma unroll
+        for (int i = 0; i < (MPPI_COV_FULL ? CDIM * CDIM : CDIM); ++i) scale[i] = a.noise_scale[i];
+#pragma unroll
+        for (int i = 0; i < CDIM; ++i) {
+            lo[i] = a.u_min[i];
+            hi[i] = a.u_max[i];
+        }
+        float x[SDIM];
+#pragma unroll
+        for (int i = 0; i < SDIM; ++i) x[i] = a.x[size_t(slot) * SDIM + i];
+        // Samples past K run the loop too (their noise is zero and their
+        // cost is not stored), so the loop holds no per-lane branch around
+        // the barriers.  Blocks of four values past the horizon read the
+        // last block inside it (their timesteps are skipped).
+        const uint32_t last_b = (H * CDIM - 1) / 4;
+        uint32_t ready = 0, zero = 0;
+        for (uint32_t t0 = 0; t0 < H; t0 += MPPI_G) {
+            const uint32_t g = t0 / MPPI_G;
+            const uint32_t need = g / MPPI_PHASE_G + 1;
+            if (ready < need) {
+                uint32_t got = __hip_atomic_load(&s_prog[wave], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                for (uint32_t spin = 0; got < need && spin < SPIN_MAX; ++spin)
+                    got = __hip_atomic_load(&s_prog[wave], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                ready = got;
+                zero = got & 0x80000000u;
+            }
+            float z[MPPI_NG], un[MPPI_NG];
+#pragma unroll
+            for (int q = 0; q < MPPI_BG; ++q) {
+                const uint32_t b0 = g * MPPI_BG + q, b = b0 < last_b ? b0 : last_b;
+                const float4 zq = reinterpret_cast<const float4*>(row + zero)[b];
+                const float4 uq = reinterpret_cast<const float4*>(s_un)[b];
+                z[4 * q + 0] = zq.x; z[4 * q + 1] = zq.y;
+                z[4 * q + 2] = zq.z; z[4 * q + 3] = zq.w;
+                un[4 * q + 0] = uq.x; un[4 * q + 1] = uq.y;
+                un[4 * q + 2] = uq.z; un[4 * q + 3] = uq.w;
+            }
+#pragma unroll
+            for (int gg = 0; gg < MPPI_G; ++gg) {
+                const uint32_t t = t0 + gg;
+                if (t < H) {
+                    float u[CDIM];
+                    mppi_make_control(un + gg * CDIM, z + gg * CDIM, scale, lo, hi, u);
+                    mppi_rng rng{t << 8, k, step, dyn_tag, a.key0, a.key1};
+                    MPPI_DYNAMICS(mppi_state_ref{x}, u, mppi_blob_ref{model}, rng);
+                    total += mppi_stage(mppi_form_timed{}, mppi_cstate_ref{x}, (const float*)u,
+                                        mppi_blob_ref{model}, mppi_blob_ref{cost},
+                                        mppi_time{t, step});
+                }
+            }
+        }
+        total += mppi_terminal(mppi_form_timed{}, mppi_cstate_ref{x}, mppi_blob_ref{model},
+                               mppi_blob_ref{cost}, mppi_time{H, step});
+        if (active) a.costs[d.cost_off + k] = total;
+    }
+    mppi_fused_stats<SAMPLES, THREADS>(a, agent, d, blk, active, total, s_z);
 }
 
 // ---------------------------------------------------------------------------
@@ -617,15 +768,19 @@ __device__ void mppi_finalize(const mppi_step_args& a, uint32_t agent, const mpp
 // a few memory latencies on many compute units instead of one compute unit
 // reading them all after the slowest tile.
 // ---------------------------------------------------------------------------
-template <int BLOCK>
+// SAMPLES samples, THREADS >= SAMPLES threads: thread i < SAMPLES holds
+// sample i's cost (`active`, `total`); the other threads add nothing to the
+// statistics and share the work of the noise sums.
+template <int SAMPLES, int THREADS>
 __device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
                                  uint32_t blk, bool active, float total, float* s_z) {
-    constexpr int WAVES = BLOCK / 64;
+    constexpr int WAVES = THREADS / 64;
     constexpr int ZS = MPPI_FUSED_ZS;
     __shared__ float s_red[4][WAVES];
-    __shared__ __attribute__((aligned(16))) float s_e[BLOCK];
+    __shared__ __attribute__((aligned(16))) float s_e[SAMPLES];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t HC = d.H * CDIM, W = d.NB * 4;
+    const uint32_t W = d.NB * 4;
+    active = active && threadIdx.x < uint32_t(SAMPLES);
     float c = (active && mppi_isfinite(total)) ? total : __builtin_inff();
     float m = mppi_wave_min(c);
     if (lane == 0) s_red[0][wave] = m;
@@ -634,7 +789,7 @@ __device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const 
 #pragma unroll
     for (int w = 1; w < WAVES; ++w) m = s_red[0][w] < m ? s_red[0][w] : m;
     const float e = (mppi_isfinite(c) && mppi_isfinite(m)) ? mppi_expf(-(c - m) / a.lambda) : 0.0f;
-    s_e[threadIdx.x] = mppi_isfinite(m) ? e : (active ? 1.0f : 0.0f);
+    if (threadIdx.x < uint32_t(SAMPLES)) s_e[threadIdx.x] = mppi_isfinite(m) ? e : (active ? 1.0f : 0.0f);
     const float se = mppi_wave_sum(e);
     const float sq = mppi_wave_sum(e * e);
     const float sc = mppi_wave_sum(active ? total : 0.0f);
@@ -654,7 +809,8 @@ __device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const 
     // have e = 0 and zero noise, and a lane with a non-finite cost has e = 0
     // and finite noise, so every term is finite and no select is needed.
     {
-        constexpr uint32_t QUADS = MPPI_FUSED_MAX_HC / 4, GROUPS = BLOCK / QUADS, PER = BLOCK / GROUPS;
+        constexpr uint32_t QUADS = MPPI_FUSED_MAX_HC / 4, GROUPS = THREADS / QUADS, PER = SAMPLES / GROUPS;
+        static_assert(PER % 4 == 0, "sample groups of whole quads");
         __shared__ float4 s_q[GROUPS][QUADS];
         const uint32_t quad = threadIdx.x % QUADS, sg = threadIdx.x / QUADS, Q = W / 4;
         float4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -942,11 +1098,12 @@ extern "C" __global__ void __launch_bounds__(256) mppi_reduce(mppi_step_args a) 
     mppi_reduce_tile<256>(a, agent, blockIdx.x, blockIdx.y);
 }
 
-// Rollout of the one-pass step (grid as mppi_rollout; H*C <= MPPI_FUSED_MAX_HC, K < 2^18).
-extern "C" __global__ void __launch_bounds__(256) mppi_rollout_fused(mppi_step_args a) {
+// Rollout of the one-pass step (grid as mppi_rollout, 512 threads per tile of
+// 256 samples; H*C <= MPPI_FUSED_MAX_HC, K < 2^18).
+extern "C" __global__ void __launch_bounds__(512) mppi_rollout_fused(mppi_step_args a) {
     const uint32_t agent = blockIdx.y;
     if (blockIdx.x >= a.desc[agent].n_blk) return;
-    mppi_rollout_tile<256, true>(a, agent, blockIdx.x);
+    mppi_fused_tile<256>(a, agent, blockIdx.x);
 }
 
 // Per-sample weights of the last iteration, w_k = exp(-(c_k - m)/lambda) / S
