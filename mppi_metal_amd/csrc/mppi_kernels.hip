@@ -269,19 +269,16 @@ __device__ void mppi_block_stats(const mppi_step_args& a, uint32_t agent, const 
     else mppi_combine_stats<BLOCK, float>(a, agent, d);
 }
 
-template <int SAMPLES, int THREADS = SAMPLES>
+template <int SAMPLES, int THREADS>
 __device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
                                  uint32_t blk, bool active, float total, float* s_z);
 
 // ---------------------------------------------------------------------------
-// Rollout tile: samples [blk*BLOCK, blk*BLOCK + BLOCK) of one agent.
-// FUSED (one-pass step, H*C <= MPPI_FUSED_MAX_HC): every lane also keeps its
-// normals in LDS, and the tile ends with its weighted noise sums
-// (mppi_fused_stats) instead of the plain softmax partials.
+// Rollout tile: samples [blk*BLOCK, blk*BLOCK + BLOCK) of one agent; the
+// tile ends with its softmax partials (mppi_block_stats).
 // ---------------------------------------------------------------------------
-template <int BLOCK, bool FUSED = false>
+template <int BLOCK>
 __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint32_t blk) {
-    constexpr int WAVES = BLOCK / 64;
     const mppi_agent_desc d = a.desc[agent];
     const uint32_t k = blk * BLOCK + threadIdx.x;
     const uint32_t H = d.H;
@@ -296,24 +293,8 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
     // The nominal sequence goes to LDS and the noise scale and bounds to
     // registers, so the time loop issues no memory waits of its own.
     // 2 * MPPI_NG slots of padding: the time loop reads one group ahead.
-    __shared__ __attribute__((aligned(16))) float s_un[(FUSED ? MPPI_FUSED_MAX_HC : MPPI_MAX_HC) + 2 * MPPI_NG];
-    // [lane][ZS]: rows 16-byte aligned, so a lane writes a group's normals
-    // and the tile's sums read them four columns at a time.
-    constexpr int ZS = MPPI_FUSED_ZS;
-    __shared__ __attribute__((aligned(16))) float s_z[FUSED ? BLOCK * ZS : 1];
+    __shared__ __attribute__((aligned(16))) float s_un[MPPI_MAX_HC + 2 * MPPI_NG];
     for (uint32_t i = threadIdx.x; i < H * CDIM; i += BLOCK) s_un[i] = a.u_nom[d.seq_off + i];
-    // FUSED: the Philox words that depend only on a noise block's index, the
-    // step and the tag (mppi_philox_shared) are the same for every sample of
-    // the tile, so they are formed once here, one block per thread, and read
-    // with each group's nominal controls; each lane then runs only the part
-    // of the generator that depends on its sample index.  2 * MPPI_BG blocks
-    // of padding for the read one group ahead.
-    constexpr int PH = FUSED ? MPPI_FUSED_MAX_HC / 4 + 2 * MPPI_BG : 1;
-    __shared__ __attribute__((aligned(16))) mppi_philox_shared_words s_ph[PH];
-    if (FUSED)
-        for (uint32_t b = threadIdx.x; b < uint32_t(PH); b += BLOCK)
-            s_ph[b] = mppi_philox_shared(b, step, tag_agent, a.key0, a.key1);
-    const uint32_t ph_e = mppi_philox_lane_key(step, a.key0);
     float scale[MPPI_COV_FULL ? CDIM * CDIM : CDIM], lo[CDIM], hi[CDIM];
 #pragma unroll
     for (int i = 0; i < (MPPI_COV_FULL ? CDIM * CDIM : CDIM); ++i) scale[i] = a.noise_scale[i];
@@ -329,52 +310,26 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
         float x[SDIM];
 #pragma unroll
         for (int i = 0; i < SDIM; ++i) x[i] = a.x[size_t(slot) * SDIM + i];
-        // The nominal controls (and FUSED, the shared Philox words) of group
-        // t0 + G are read while group t0 runs, so the LDS latency hides
-        // behind a group's work.  The read past the horizon lands in the
-        // padding (t0 < H, so (t0 + G) C + NG <= H C + 2 NG).
+        // The nominal controls of group t0 + G are read while group t0 runs,
+        // so the LDS latency hides behind a group's work.  The read past the
+        // horizon lands in the padding (t0 < H, so (t0 + G) C + NG <= H C + 2 NG).
         float un_next[MPPI_NG];
-        mppi_philox_shared_words ph_next[MPPI_BG];
 #pragma unroll
         for (int i = 0; i < MPPI_NG; ++i) un_next[i] = s_un[i];
-#pragma unroll
-        for (int q = 0; q < MPPI_BG; ++q) ph_next[q] = s_ph[FUSED ? q : 0];
         for (uint32_t t0 = 0; t0 < H; t0 += MPPI_G) {
             float un[MPPI_NG];
-            mppi_philox_shared_words ph[MPPI_BG];
 #pragma unroll
             for (int i = 0; i < MPPI_NG; ++i) {
                 un[i] = un_next[i];
                 un_next[i] = s_un[(t0 + MPPI_G) * CDIM + i];
             }
             const uint32_t b0 = (t0 * CDIM) >> 2;
-#pragma unroll
-            for (int q = 0; q < MPPI_BG; ++q) {
-                ph[q] = ph_next[q];
-                if (FUSED) ph_next[q] = s_ph[b0 + MPPI_BG + q];
-            }
             float z[MPPI_NG];
 #pragma unroll
             for (int q = 0; q < MPPI_BG; ++q) {
-                mppi_f32x4 v = FUSED ? mppi_normal4_words(mppi_philox_lanes(k, ph_e, ph[q], a.key0, a.key1))
-                                     : mppi_normal4(mppi_u32x4{b0 + q, k, step, tag_agent}, a.key0, a.key1);
+                mppi_f32x4 v = mppi_normal4(mppi_u32x4{b0 + q, k, step, tag_agent}, a.key0, a.key1);
                 z[4 * q + 0] = v.x; z[4 * q + 1] = v.y;
                 z[4 * q + 2] = v.z; z[4 * q + 3] = v.w;
-            }
-            if (FUSED) {
-                float* row = s_z + threadIdx.x * ZS + t0 * CDIM;
-                // A group that runs past the horizon still fits the row when
-                // its overhang (at most (G - 1) C values) fits the padding;
-                // the values there are never read back.
-                if (MPPI_FUSED_OVERHANG_OK || t0 * CDIM + MPPI_NG <= H * CDIM) {
-#pragma unroll
-                    for (int q = 0; q < MPPI_BG; ++q)
-                        reinterpret_cast<float4*>(row)[q] = float4{z[4 * q], z[4 * q + 1], z[4 * q + 2], z[4 * q + 3]};
-                } else {
-#pragma unroll
-                    for (int i = 0; i < MPPI_NG; ++i)
-                        if (t0 * CDIM + i < H * CDIM) row[i] = z[i];
-                }
             }
 #pragma unroll
             for (int g = 0; g < MPPI_G; ++g) {
@@ -393,13 +348,8 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
         total += mppi_terminal(mppi_form_timed{}, mppi_cstate_ref{x}, mppi_blob_ref{model},
                                mppi_blob_ref{cost}, mppi_time{H, step});
         a.costs[d.cost_off + k] = total;
-    } else if (FUSED) {
-        // Lanes past K hold zeros, so the tile's noise sums need no select.
-        for (uint32_t i = 0; i < H * CDIM; ++i) s_z[threadIdx.x * ZS + i] = 0.0f;
     }
-
-    if (FUSED) mppi_fused_stats<BLOCK>(a, agent, d, blk, active, total, s_z);
-    else mppi_block_stats<BLOCK>(a, agent, d, blk, active, total);
+    mppi_block_stats<BLOCK>(a, agent, d, blk, active, total);
 }
 
 // ---------------------------------------------------------------------------
