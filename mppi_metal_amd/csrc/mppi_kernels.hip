@@ -414,6 +414,10 @@ __device__ void mppi_fused_tile(const mppi_step_args& a, uint32_t agent, uint32_
     // bounded all the same.
     constexpr uint32_t SPIN_MAX = 1u << 20;
     const uint32_t wave = (threadIdx.x & (SAMPLES - 1)) >> 6;
+    // The consumers are the critical path (their timesteps depend on each
+    // other; the producers run ahead and have slack), so they take the
+    // higher wavefront priority: the SIMD issues a consumer instruction
+    // whenever one is ready and fills the gaps with producer work.
     if (producer) {
         const uint32_t ph_e = mppi_philox_lane_key(step, a.key0);
         for (uint32_t p = 0; p < n_phases; ++p) {
@@ -444,6 +448,7 @@ __device__ void mppi_fused_tile(const mppi_step_args& a, uint32_t agent, uint32_
                 __hip_atomic_store(&s_prog[wave], p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     } else {
+        __builtin_amdgcn_s_setprio(3);
         float scale[MPPI_COV_FULL ? CDIM * CDIM : CDIM], lo[CDIM], hi[CDIM];
 #pragma unroll
         for (int i = 0; i < (MPPI_COV_FULL ? CDIM * CDIM : CDIM); ++i) scale[i] = a.noise_scale[i];
@@ -500,6 +505,7 @@ __device__ void mppi_fused_tile(const mppi_step_args& a, uint32_t agent, uint32_
                                mppi_blob_ref{cost}, mppi_time{H, step});
         if (active) a.costs[d.cost_off + k] = total;
     }
+    __builtin_amdgcn_s_setprio(0);
     mppi_fused_stats<SAMPLES, THREADS>(a, agent, d, blk, active, total, s_z);
 }
 
