@@ -14,9 +14,12 @@ kernels, and the one-kernel step that keeps the noise in LDS); the rest run
 on the default path, which is the one-kernel step for every shape here
 except K = 2^18.
 """
+import struct
+
 import numpy as np
 import pytest
 
+from mppi_metal_amd import HipMPPI, scenes
 from problems import check_step as _check_step
 from problems import double_integrator, rel_err, report as _report, seq_rel_err, unicycle
 
@@ -101,6 +104,70 @@ def test_simulate_batch_matches_oracle(require_gpu, path):
     assert np.max(np.abs(st - so)) <= CLOSED_LOOP_TOL
     assert np.max(np.abs(ct - co)) <= CLOSED_LOOP_TOL
     assert np.max(np.abs(tt - to)) <= CLOSED_LOOP_TOL
+
+
+def _obstacle_scene(kind):
+    """Obstacles across the paths of samples that start at speed 1.8 heading
+    +x: `dense` puts 16 of them where most samples pass; `edges` mixes live
+    slots with radii that are tiny (always tested), zero, negative or NaN
+    and centres at infinity or NaN (never count), and junk slots past n = 11
+    that would cover every sample; `huge` holds a radius above 2^60 (always
+    tested; every sample counts)."""
+    rng = np.random.default_rng(7)
+    if kind == "dense":
+        obs = [(float(rng.uniform(0.3, 7.0)), float(rng.uniform(-2.5, 2.5)), float(rng.uniform(0.2, 1.0)))
+               for _ in range(16)]
+        n = 16
+    elif kind in ("edges", "huge"):
+        nan, inf = float("nan"), float("inf")
+        obs = [(1.5, 0.2, 0.6), (3.0, -0.5, 1e-25), (2.0, 0.0, 0.0), (2.5, 0.3, -1.0),
+               (4.0, 0.0, nan), (inf, 0.0, 1.0), (3.5, nan, 1.0), (2.2, -0.6, 0.5),
+               (5.0, 0.1, 0.7), (0.9, 0.4, 0.3), (3.3, 0.9, 0.8),
+               (2.0, 0.0, 50.0), (3.0, 0.0, 1e30), (1.0, 1.0, nan), (0.0, 0.0, 9.0), (4.0, 4.0, 20.0)]
+        n = 11
+    if kind == "huge":
+        obs = [(1.5, 0.2, 0.6), (5.0, 0.1, 3e19), (2.2, -0.6, 0.5)]
+        n = 3
+    sc = scenes.UnicycleScene(goal=(9.0, 0.5), obstacles=obs)
+    blob = bytearray(sc.cost_blob())
+    blob[28:32] = struct.pack("<I", n)
+    return sc, bytes(blob)
+
+
+@PATHS
+@pytest.mark.parametrize("kind", ["dense", "edges", "huge"])
+def test_unicycle_obstacle_window_parity(require_gpu, path, kind):
+    """The GPU's stage cost takes the exact obstacle distance only in
+    wavefronts where some sample lies in an obstacle's square gate (scaled
+    coordinates, unicycle_gate), with the sample noise of both step paths;
+    the oracle runs the same plugin source on the host.  Costs bit-exact over
+    all samples, and the penalty must be live in most of them, so that the
+    gate decides real cases."""
+    import oracle_lib as ol
+    sc, cb = _obstacle_scene(kind)
+    mb = sc.model_blob()
+    K, H = 4096, 64
+    sigma = np.array(sc.sigma, np.float32)
+    hip = HipMPPI(4, 2, H, K, sc.lam, sigma, sc.u_min, sc.u_max)
+    hip.link("unicycle.hip", "mppi_dynamics", "unicycle_stage_cost_obstacles", "unicycle_terminal_cost",
+             len(mb), len(cb), 5)
+    hip.bind_params(mb, cb)
+    hip.set_step_path(path)
+    ora = ol.Problem("unicycle", 4, 2, H, K, sc.lam, sigma, sc.u_min, sc.u_max, 5, "mppi_dynamics",
+                     "unicycle_stage_cost_obstacles", "unicycle_terminal_cost", mb, cb)
+    free = bytearray(cb)
+    free[24:28] = struct.pack("<f", 0.0)          # obstacle_weight 0: goal and control terms only
+    ora_free = ol.Problem("unicycle", 4, 2, H, K, sc.lam, sigma, sc.u_min, sc.u_max, 5, "mppi_dynamics",
+                          "unicycle_stage_cost_obstacles", "unicycle_terminal_cost", mb, bytes(free))
+    x0 = np.array([0.0, 0.0, 0.1, 1.8], np.float32)
+    u_nom = np.tile(np.array([0.3, 0.05], np.float32), (H, 1))
+    for step in (0, 3):
+        hip.step(step, x0, u_nom)
+        c_g = hip.read_costs()
+        c_o = ora.rollout(step, x0, u_nom, threads=8)
+        assert np.array_equal(c_g, c_o, equal_nan=True), f"{kind} step {step}"
+        hit = np.mean(c_o != ora_free.rollout(step, x0, u_nom, threads=8))
+        assert hit > 0.5, hit
 
 
 @PATHS
