@@ -38,6 +38,19 @@ def test_log_ulp():
     assert ulp_err(ol.mathf(1, u), np.log(u.astype(np.float64))).max() <= 2.0
 
 
+def test_log_every_float_near_one():
+    """The logarithm's core is f + f^2 P(f) on the mantissa's f in
+    [sqrt(1/2) - 1, sqrt(2) - 1]: every float in [1/2, 2) runs every f the
+    polynomial sees (at k = -1, 0 and 1); other binades add k log 2, checked
+    on a stride over [2^-30, 2^8).  Bound: one ulp of the double result."""
+    lo, hi = np.float32(0.5).view(np.uint32), np.float32(2.0).view(np.uint32)
+    x = np.arange(lo, hi, dtype=np.uint32).view(np.float32)
+    assert ulp_err(ol.mathf(1, x), np.log(x.astype(np.float64))).max() <= 1.0
+    lo, hi = np.float32(2.0**-30).view(np.uint32), np.float32(2.0**8).view(np.uint32)
+    x = np.arange(lo, hi, 97, dtype=np.uint32).view(np.float32)
+    assert ulp_err(ol.mathf(1, x), np.log(x.astype(np.float64))).max() <= 1.0
+
+
 def test_sin_cos_abs():
     x = np.linspace(-200.0, 200.0, 400001, dtype=np.float32)
     xd = x.astype(np.float64)
