@@ -136,7 +136,7 @@ def _obstacle_scene(kind):
 
 @PATHS
 @pytest.mark.parametrize("kind", ["dense", "edges", "huge"])
-def test_unicycle_obstacle_window_parity(require_gpu, path, kind):
+def test_unicycle_obstacle_gate_parity(require_gpu, path, kind):
     """The GPU's stage cost takes the exact obstacle distance only in
     wavefronts where some sample lies in an obstacle's square gate (scaled
     coordinates, unicycle_gate), with the sample noise of both step paths;
