@@ -162,11 +162,13 @@ int mppi_hip_read_weights(mppi_hip_ctx* ctx, uint32_t agent, float* out, size_t 
  *   TWO_KERNEL  rollout, then a reduce that regenerates the noise from its
  *               counters and forms the weighted average in parallel over
  *               samples;
- *   FUSED       one kernel: the rollout keeps each workgroup's normals in LDS
- *               and forms its weighted sums, the last workgroup combines them
- *               (needs horizon * control_dim <= 128 and K < 2^18, else the
- *               two kernels run); per-sample weights are then formed when
- *               mppi_hip_read_weights asks for them;
+ *   FUSED       one pass over the noise: the rollout keeps each tile's
+ *               normals in LDS (producer wavefronts write them, consumer
+ *               wavefronts roll out from them) and forms the tile's weighted
+ *               sums, then a small combine launch forms the update (needs
+ *               horizon * control_dim <= 128 and at most 256 tiles of 256
+ *               samples, K <= 65536, else the two kernels run); per-sample
+ *               weights are then formed when mppi_hip_read_weights asks;
  *   AUTO        FUSED where it applies and the rollout has at most one
  *               workgroup per compute unit, else TWO_KERNEL (the default). */
 enum mppi_hip_step_path { MPPI_HIP_PATH_AUTO = 0, MPPI_HIP_PATH_TWO_KERNEL = 1, MPPI_HIP_PATH_FUSED = 2 };

@@ -6,7 +6,7 @@ bit-exact, weights 1e-5 relative, u_next 1e-5 of its largest magnitude):
                    with H*C not a multiple of 4 (ragged last block) and K
                    not a multiple of the 256-sample workgroup; the dynamics
                    draws process noise from its random handle
-  one-kernel edge  H*C = 126 and 128 (the one-kernel step's limit) and
+  one-pass edge    H*C = 126 and 128 (the one-pass step's limit) and
                    129, 130 (past it, where the two-kernel step runs),
                    requested and automatic
   largest horizon  H*C = 4096, the library's limit
@@ -39,7 +39,7 @@ def test_three_controls_ragged(require_gpu, path, K, H):
         check_step(hip, ora, step, X0_PM3, _u_nom(H, 3), f"point mass K={K} H={H} step {step}")
 
 
-# (control_dim, horizon): H*C on both sides of the one-kernel limit of 128
+# (control_dim, horizon): H*C on both sides of the one-pass limit of 128
 @pytest.mark.parametrize("path", [0, 2], ids=["auto", "fused_requested"])
 @pytest.mark.parametrize("C,H", [(2, 64), (2, 65), (3, 42), (3, 43)])
 def test_one_kernel_limit(require_gpu, path, C, H):

@@ -10,8 +10,8 @@ Tolerances (FP32):
   mean cost, ESS    1e-5 relative
 Every result's error against the FP64 oracle is printed (pytest -s) and
 bounded.  The single-step and closed-loop cases run on both step paths (two
-kernels, and the one-kernel step that keeps the noise in LDS); the rest run
-on the default path, which is the one-kernel step for every shape here
+kernels, and the one-pass step that keeps the noise in LDS); the rest run
+on the default path, which is the one-pass step for every shape here
 except K = 2^18.
 """
 import struct
