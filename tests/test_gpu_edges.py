@@ -42,7 +42,7 @@ def test_three_controls_ragged(require_gpu, path, K, H):
 # (control_dim, horizon): H*C on both sides of the one-pass limit of 128
 @pytest.mark.parametrize("path", [0, 2], ids=["auto", "fused_requested"])
 @pytest.mark.parametrize("C,H", [(2, 64), (2, 65), (3, 42), (3, 43)])
-def test_one_kernel_limit(require_gpu, path, C, H):
+def test_one_pass_limit(require_gpu, path, C, H):
     if C == 2:
         hip, ora, _ = unicycle(1500, H, 8)
         x0 = np.array([0.5, -0.5, 0.3, 0.4], np.float32)
