@@ -109,7 +109,7 @@ def unicycle_scene(n_obstacles: int, seed: int = 42) -> UnicycleScene:
     return UnicycleScene(goal=goal, obstacles=obstacles)
 
 
-# BASELINE workloads: (name, n_obstacles, H, K, closed-loop steps)
+# Benchmark workloads of the unicycle: (name, n_obstacles, H, K, closed-loop steps)
 UNICYCLE_C1 = dict(name="unicycle_k1024_h32_obs8", n_obstacles=8, H=32, K=1024, steps=200)
 UNICYCLE_C2 = dict(name="unicycle_k65536_h64_obs16", n_obstacles=16, H=64, K=65536, steps=500)
 
