@@ -102,7 +102,34 @@ def measured_traffic(kernel: str):
     return None
 
 
+def host_cpus():
+    """The host's CPUs as this process sees them: nproc (the affinity mask),
+    os.cpu_count(), the cgroup CPU quota, the model name; `usable` is what a
+    worker pool may use (affinity, capped by the quota)."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, per = Path("/sys/fs/cgroup/cpu.max").read_text().split()
+        if q != "max":
+            quota = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    usable = aff if quota is None else max(1, min(aff, int(quota)))
+    return {"nproc": aff, "os_cpu_count": os.cpu_count(), "cgroup_cpu_quota": quota, "model": model,
+            "usable": usable}
+
+
 def cpu_baseline(n_obs, H, K, steps, threads):
+    """sample-steps/s of the CPU oracle over the first `steps` closed-loop
+    steps of the workload, on `threads` threads."""
     sys.path.insert(0, str(ROOT / "tests"))
     from problems import unicycle_oracle
     ora, sc = unicycle_oracle(K, H, n_obs)
@@ -111,6 +138,61 @@ def cpu_baseline(n_obs, H, K, steps, threads):
     ora.simulate(steps, 0, x0, threads=threads)
     dt = time.perf_counter() - t0
     return K * H * steps / dt, dt
+
+
+def small_workload(hip_cls, threads, steps=200, warm=10, no_cpu=False):
+    """The CPU-runnable workload (unicycle K = 1024, H = 32, 8 obstacles, seed
+    42, 200 closed-loop steps from x0 = 0), on the CPU oracle and on the GPU:
+    sample-steps/s of the closed loop, p50/p99 of a blocking step (warm start
+    shifted on the host between steps; 10 warm-up steps excluded) and the
+    mean realised closed-loop cost."""
+    from mppi_metal_amd import scenes
+    K, H, n_obs = 1024, 32, 8
+    sc = scenes.unicycle_scene(n_obs, 42)
+    mb, cb = sc.model_blob(), sc.cost_blob()
+    x0 = np.zeros(4, np.float32)
+    u0 = np.zeros((H, 2), np.float32)
+    out = {"workload": "plain MPPI closed loop, kinematic unicycle, K=1024, H=32, 8 obstacles, seed 42",
+           "steps": steps}
+
+    def latencies(step_fn):
+        lat, u_nom = [], u0.copy()
+        for i in range(warm + steps):
+            t1 = time.perf_counter()
+            u = step_fn(i, u_nom)
+            t2 = time.perf_counter()
+            if i >= warm:
+                lat.append((t2 - t1) * 1e3)
+            u_nom = np.concatenate([u[1:], u[-1:]], axis=0)
+        return np.array(lat)
+
+    hip = hip_cls(4, 2, H, K, sc.lam, sc.sigma, sc.u_min, sc.u_max)
+    hip.link("unicycle.hip", "mppi_dynamics", "unicycle_stage_cost_obstacles", "unicycle_terminal_cost",
+             len(mb), len(cb), 42)
+    hip.bind_params(mb, cb)
+    sys.path.insert(0, str(ROOT / "tests"))
+    from problems import unicycle_oracle
+    ora, _ = unicycle_oracle(K, H, n_obs)
+    hip.simulate(warm, 0, x0, u0)
+    t0 = time.perf_counter()
+    st, ct, kt, tt = hip.simulate_batch(1, steps, 0, x0, u0)     # one agent: simulate() plus its end state
+    el = time.perf_counter() - t0
+    cost = ora.closed_loop_cost(st[:, 0], ct[:, 0], tt[0], 0)
+    lat = latencies(lambda i, un: hip.step(i, x0, un)[0])
+    out["gpu"] = {"value": K * H * steps / el, "unit": "sample-steps/s", "ms_per_step": el / steps * 1e3,
+                  "step_p50_ms": float(np.percentile(lat, 50)), "step_p99_ms": float(np.percentile(lat, 99)),
+                  "mean_realised_cost": float(np.mean(cost))}
+    hip.close()
+    if not no_cpu:
+        t0 = time.perf_counter()
+        so, co, ko, to = ora.simulate(steps, 0, x0, threads=threads)
+        el = time.perf_counter() - t0
+        cost_o = ora.closed_loop_cost(so[:, 0], co[:, 0], to[0], 0)
+        lat = latencies(lambda i, un: ora.step(i, x0, un, threads=threads)[0])
+        out["cpu"] = {"value": K * H * steps / el, "unit": "sample-steps/s", "ms_per_step": el / steps * 1e3,
+                      "step_p50_ms": float(np.percentile(lat, 50)), "step_p99_ms": float(np.percentile(lat, 99)),
+                      "mean_realised_cost": float(np.mean(cost_o)), "threads": threads, "kind": "port"}
+    return out
 
 
 def bench_batch(args):
@@ -310,6 +392,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--latency-steps", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=200)
+    ap.add_argument("--cpu-steps-1t", type=int, default=3)
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the K = 1024, H = 32 closed loop (GPU and CPU oracle, 200 steps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--K", type=int, default=65536)
     ap.add_argument("--H", type=int, default=64)
@@ -408,12 +493,19 @@ def main():
     lat = np.array(lat)
 
     cpu = None
+    hosts = host_cpus()
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = hosts["usable"]
         cps, cdt = cpu_baseline(n_obs, H, K, args.cpu_steps, threads)
+        one, one_dt = cpu_baseline(n_obs, H, K, args.cpu_steps_1t, 1)
         cpu = {"value": cps, "unit": "sample-steps/s", "cores": threads, "kind": "port",
-               "sample": f"CPU oracle (scalar C++, FP32, same noise stream), first {args.cpu_steps} "
-                         f"closed-loop steps of the same workload, {threads} threads, {cdt:.1f}s"}
+               "sample": f"CPU oracle (scalar C++ at -O3 -march=native, FP32, same noise stream and "
+                         f"plugin source), first {args.cpu_steps} closed-loop steps of the same workload "
+                         f"on {threads} threads ({cdt:.1f} s); 1 thread: first {args.cpu_steps_1t} steps "
+                         f"({one_dt:.1f} s)",
+               "one_thread": {"value": one, "steps": args.cpu_steps_1t, "seconds": one_dt},
+               "host": hosts, "gpu_over_cpu": value / cps, "gpu_over_one_thread": value / one}
+    small = None if args.no_small else small_workload(HipMPPI, hosts["usable"], no_cpu=args.no_cpu)
 
     batch = None if args.no_batch else bench_batch(args)
     surrogate = None if args.no_surrogate else bench_surrogate(args)
@@ -457,6 +549,7 @@ def main():
                             "bytes_per_launch": rb, "avg_launch_us": red_s * 1e6,
                             "traffic": measured_traffic("mppi_reduce") if default_workload else None},
         "cpu_baseline": cpu,
+        "small_workload": small,
         "final_closed_loop_cost": float(kt[-1]),
         "batch": batch,
         "surrogate": surrogate,

@@ -178,6 +178,16 @@ int mppi_hip_set_step_path(mppi_hip_ctx* ctx, uint32_t path);
  * the built-in choice.  No reference counterpart; used by the autotuner. */
 int mppi_hip_set_reduce_samples_per_thread(mppi_hip_ctx* ctx, uint32_t spt);
 
+/* The one-pass step's rollout wavefronts wait (bounded) for the noise that
+ * their producer wavefronts write to LDS.  A wavefront that reaches the bound
+ * flags its agent, and the call (step, simulate, simulate_batch) then fails
+ * with MPPI_HIP_E_DEVICE instead of returning results computed from missing
+ * noise.  polls: loads of the producer's progress word before giving up (0:
+ * the default, 2^20).  force_timeout != 0 makes every wait target unreachable
+ * so that each wait runs to the bound and gives up (every kernel still ends
+ * normally); it exists to test that error path.  No reference counterpart. */
+int mppi_hip_set_wait_limit(mppi_hip_ctx* ctx, uint32_t polls, int force_timeout);
+
 /* Device timing: when enabled, every rollout and reduce launch is bracketed by
  * HIP events on the context's stream and their durations are summed.
  * mppi_hip_profile_read returns the sums in milliseconds and the launch count
@@ -192,13 +202,20 @@ int mppi_hip_profile_read(mppi_hip_ctx* ctx, double* rollout_ms, double* reduce_
  * (src/metal_backend.mm:935-1124) requires one K and one H for all agents.
  * A batch is bound to a context (plugin, dimensions, noise model, lambda,
  * bounds, seed) and holds N agents, agent a with K_a samples and horizon
- * H_a (H_a * control_dim <= 4096).  Each MPPI step of all agents is two
- * kernel launches over a tile list (rollout: agent x 256-sample blocks;
- * reduce: agent x sample chunks x noise-block groups), sorted longest first
- * (a persistent grid pulling tiles from an atomic queue is built but switched
- * off, see mppi_hip_batch_tuning.persistent), and is captured once
- * in a HIP graph that is replayed for every step.  The launch shape is chosen
- * by mppi_hip_batch_autotune or set explicitly.
+ * H_a (H_a * control_dim <= 4096).  An MPPI step of all agents runs over
+ * tile lists sorted longest first, in one of two forms:
+ *   two launches  a rollout launch (tiles: agent x block of samples) and a
+ *                 reduce launch that regenerates the noise (tiles: agent x
+ *                 sample chunk x noise-block group);
+ *   one launch    every tile (agent x block of samples) rolls out its samples,
+ *                 weights them against its own best cost and sums its
+ *                 weighted noise; the last tile of each agent to finish forms
+ *                 the agent's update (needs K_a < 2^18 for every agent).
+ * Each launch runs one workgroup per tile (tile table) or a persistent grid
+ * of compute units x wgs_per_cu workgroups taking tiles from a ticket
+ * counter.  The step is captured once in a HIP graph that is replayed for
+ * every step.  The launch shape is chosen by mppi_hip_batch_autotune or set
+ * explicitly.
  * Agent a draws its noise from counter word 3 = a, so a batch agent and a
  * single-agent context see the same noise when their indices agree.
  * ------------------------------------------------------------------------- */
@@ -206,11 +223,13 @@ typedef struct mppi_hip_batch mppi_hip_batch;
 
 typedef struct mppi_hip_batch_tuning {
     uint32_t block;        /* workgroup size: 64, 128 or 256 */
-    uint32_t persistent;   /* 0: one workgroup per tile; 1: persistent grid with a work queue
-                              (switched off: rejected with MPPI_HIP_E_INVALID) */
+    uint32_t persistent;   /* 0: one workgroup per tile; 1: persistent grid taking tiles
+                              from a ticket counter */
     uint32_t wgs_per_cu;   /* persistent grid size = compute units * wgs_per_cu (1..16) */
-    uint32_t reduce_spt;   /* samples per reduce thread (1..64); 0: built-in heuristic */
+    uint32_t reduce_spt;   /* samples per reduce thread (1..64); 0: built-in heuristic
+                              (two-launch form only) */
     uint32_t order;        /* 0: longest tiles first; 1: agent order */
+    uint32_t launches;     /* kernel launches per step: 2 (rollout, reduce) or 1 */
 } mppi_hip_batch_tuning;
 
 /* sample_counts, horizons: num_agents entries each. */

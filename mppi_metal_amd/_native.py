@@ -48,6 +48,7 @@ class MppiHipBatchTuning(ctypes.Structure):
         ("wgs_per_cu", ctypes.c_uint32),
         ("reduce_spt", ctypes.c_uint32),
         ("order", ctypes.c_uint32),
+        ("launches", ctypes.c_uint32),
     ]
 
 
@@ -120,6 +121,7 @@ SIGNATURES = [
     ("mppi_hip_read_weights", ctypes.c_int, [_CTX, ctypes.c_uint32, _F, ctypes.c_size_t]),
     ("mppi_hip_set_reduce_samples_per_thread", ctypes.c_int, [_CTX, ctypes.c_uint32]),
     ("mppi_hip_set_step_path", ctypes.c_int, [_CTX, ctypes.c_uint32]),
+    ("mppi_hip_set_wait_limit", ctypes.c_int, [_CTX, ctypes.c_uint32, ctypes.c_int]),
     ("mppi_hip_profile_enable", ctypes.c_int, [_CTX, ctypes.c_int]),
     ("mppi_hip_profile_read", ctypes.c_int,
      [_CTX, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),

@@ -177,6 +177,11 @@ class HipMPPI:
         """_native.PATH_AUTO, PATH_TWO_KERNEL or PATH_FUSED."""
         self._check(self.lib.mppi_hip_set_step_path(self.ctx, int(path)))
 
+    def set_wait_limit(self, polls=0, force_timeout=False):
+        """Bound of the one-pass rollout's wait for its noise (0: default);
+        force_timeout makes every wait give up (tests the error path)."""
+        self._check(self.lib.mppi_hip_set_wait_limit(self.ctx, int(polls), 1 if force_timeout else 0))
+
     def set_reduce_samples_per_thread(self, spt):
         self._check(self.lib.mppi_hip_set_reduce_samples_per_thread(self.ctx, int(spt)))
 
@@ -286,15 +291,15 @@ class HipBatch:
         cp, cn, ck = _blob(cost)
         self.ctx._check(self.lib.mppi_hip_batch_set_params(self.b, mp, model_stride, cp, cost_stride))
 
-    def set_tuning(self, block=256, persistent=0, wgs_per_cu=4, reduce_spt=0, order=0):
-        t = _native.MppiHipBatchTuning(block, persistent, wgs_per_cu, reduce_spt, order)
+    def set_tuning(self, block=256, persistent=0, wgs_per_cu=4, reduce_spt=0, order=0, launches=2):
+        t = _native.MppiHipBatchTuning(block, persistent, wgs_per_cu, reduce_spt, order, launches)
         self.ctx._check(self.lib.mppi_hip_batch_set_tuning(self.b, ctypes.byref(t)))
 
     def get_tuning(self):
         t = _native.MppiHipBatchTuning()
         self.ctx._check(self.lib.mppi_hip_batch_get_tuning(self.b, ctypes.byref(t)))
         return dict(block=t.block, persistent=t.persistent, wgs_per_cu=t.wgs_per_cu,
-                    reduce_spt=t.reduce_spt, order=t.order)
+                    reduce_spt=t.reduce_spt, order=t.order, launches=t.launches)
 
     def autotune(self, trial_steps=3):
         ms = ctypes.c_double()

@@ -31,6 +31,7 @@ struct LoadedModule {
     // batch kernels, indexed [block 64/128/256 -> 0/1/2][table 0 / persistent 1]
     hipFunction_t rollout_b[3][2] = {};
     hipFunction_t reduce_b[3][2] = {};
+    hipFunction_t step_b[3][2] = {};        // one-launch step
     // surrogate kernels (MPPI_SURROGATE_KERNELS)
     hipFunction_t surrogate_data = nullptr;
     hipFunction_t rollout_surrogate = nullptr;
@@ -79,6 +80,8 @@ struct mppi_hip_ctx {
     hipStream_t stream = nullptr;
     std::string error;
     uint32_t spt_override = 0;
+    // bounded wait of the one-pass rollout (mppi_hip_set_wait_limit)
+    uint32_t wait_limit = 0, wait_bias = 0;
 
     // device buffers of the single-agent / uniform-batch path
     mppi_hip_impl::DeviceBuffer consts;  // noise_scale | u_min | u_max | zero blob
@@ -131,6 +134,8 @@ hipEvent_t take_event(mppi_hip_ctx* ctx);
 void collect_profile(mppi_hip_ctx* ctx);
 // Common kernel arguments (constants, bound blobs, work buffers) of a step.
 void fill_common_args(mppi_hip_ctx* ctx, mppi_step_args* a);
+// MPPI_HIP_E_DEVICE when any of the n status words reports a failed device wait.
+int check_status(mppi_hip_ctx* ctx, const uint32_t* status, uint32_t n);
 
 }  // namespace mppi_hip_impl
 

@@ -3,13 +3,20 @@
 //
 // A batch keeps every agent's state on the device: descriptors (K_a, H_a,
 // buffer offsets), step counters, states, warm starts, parameter blobs.  One
-// MPPI step of all agents is a rollout launch and a reduce launch over tile
-// tables; both are captured once in a HIP graph and the graph is replayed for
-// every step (the step counters advance on the device, so the captured
-// arguments never change).  The launch shape — workgroup size, one workgroup
-// per tile or a persistent grid pulling tiles from an atomic queue, reduce
+// MPPI step of all agents is either a rollout launch and a reduce launch over
+// tile tables, or one launch whose tiles roll out, sum their weighted noise
+// and, in the last tile of each agent, form the update; the step is captured
+// once in a HIP graph and the graph is replayed for every step (the step
+// counters advance on the device, so the captured arguments never change).
+// The launch shape — one or two launches, workgroup size, one workgroup per
+// tile or a persistent grid taking tiles from a ticket counter, reduce
 // samples per thread, tile order — is a mppi_hip_batch_tuning, chosen by the
 // autotuner below by timing candidates with HIP events.
+//
+// Reference: MetalBackend::dispatch_batch_simulation
+// (src/metal_backend.mm:935-1124) encodes the batched rollout, weights,
+// reduce and propagate kernels (shaders/mppi_rollout.metal:103-170,
+// shaders/mppi_reduce.metal:242-488) for agents that share K and H.
 #include "backend.h"
 
 #include <algorithm>
@@ -22,7 +29,7 @@ struct mppi_hip_batch {
     mppi_hip_ctx* ctx = nullptr;
     uint32_t N = 0;
     std::vector<uint32_t> K, H;
-    mppi_hip_batch_tuning tuning{256, 0, 4, 0, 0};
+    mppi_hip_batch_tuning tuning{256, 0, 4, 0, 0, 2};
     std::shared_ptr<LoadedModule> mod;
 
     // layout of the current tuning
@@ -31,7 +38,7 @@ struct mppi_hip_batch {
     size_t seq_total = 0;                                // sum H_a * C
 
     // device memory
-    DeviceBuffer d_desc, d_agents, d_x, d_unom, d_uout, d_diag, d_rtiles, d_dtiles, d_queue;
+    DeviceBuffer d_desc, d_agents, d_x, d_unom, d_uout, d_diag, d_rtiles, d_dtiles, d_ticket;
     DeviceBuffer d_costs, d_weights, d_blk, d_part, d_model, d_cost;
     DeviceBuffer d_hist_states, d_hist_controls, d_hist_costs;
     size_t hist_rows = 0;
@@ -63,6 +70,14 @@ void drop_graphs(mppi_hip_batch* b) {
 }
 
 int block_index(uint32_t block) { return block == 64 ? 0 : block == 128 ? 1 : 2; }
+
+// The one-launch form sums across tiles in FP32, which the library keeps
+// below 2^18 samples per agent.
+bool one_launch_ok(const mppi_hip_batch* b) {
+    for (uint32_t k : b->K)
+        if (k >= (1u << 18)) return false;
+    return true;
+}
 
 // Descriptors and tile tables for the current tuning; (re)allocates the
 // tuning-dependent work buffers.
@@ -98,7 +113,9 @@ int build_layout(mppi_hip_batch* b) {
         cost_off += d.K;
         seq_off += uint64_t(d.H) * C;
         blk_off += d.n_blk;
-        part_off += uint64_t(d.n_chunks) * d.NB * 4;
+        // rows of partials: the reduce's sample chunks, or one per tile in
+        // the one-launch form
+        part_off += uint64_t(b->tuning.launches == 1 ? d.n_blk : d.n_chunks) * d.NB * 4;
     }
     b->seq_total = seq_off;
     // Tile tables.  Rollout tile cost ~ H_a; reduce tile cost ~ spt * noise blocks.
@@ -137,6 +154,9 @@ int build_layout(mppi_hip_batch* b) {
     MPPI_HIP_TRY(ctx, hipMemcpyAsync(b->d_dtiles.ptr, b->reduce_tiles.data(),
                                      sizeof(uint32_t) * b->reduce_tiles.size(), hipMemcpyHostToDevice,
                                      ctx->stream));
+    // The persistent kernels' ticket counters start every launch at a
+    // multiple of n_tiles + grid; both may have changed.
+    MPPI_HIP_TRY(ctx, hipMemsetAsync(b->d_ticket.ptr, 0, sizeof(uint64_t) * 3, ctx->stream));
     MPPI_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     drop_graphs(b);
     return MPPI_HIP_OK;
@@ -148,7 +168,6 @@ mppi_step_args batch_args(mppi_hip_batch* b, int closed_loop) {
     fill_common_args(ctx, &a);
     a.desc = static_cast<const mppi_agent_desc*>(b->d_desc.ptr);
     a.agents = static_cast<mppi_agent_state*>(b->d_agents.ptr);
-    a.queue = static_cast<uint32_t*>(b->d_queue.ptr);
     a.N = b->N;
     a.x = static_cast<float*>(b->d_x.ptr);
     a.u_nom = static_cast<float*>(b->d_unom.ptr);
@@ -166,7 +185,8 @@ mppi_step_args batch_args(mppi_hip_batch* b, int closed_loop) {
     return a;
 }
 
-// Enqueue one step (rollout launch + reduce launch) on the context's stream.
+// Enqueue one step on the context's stream: the rollout and reduce launches,
+// or the one-launch step.
 int enqueue_step(mppi_hip_batch* b, const mppi_step_args& base) {
     mppi_hip_ctx* ctx = b->ctx;
     const int bi = block_index(b->tuning.block);
@@ -174,6 +194,7 @@ int enqueue_step(mppi_hip_batch* b, const mppi_step_args& base) {
     const uint32_t B = b->tuning.block;
     const uint32_t nr = uint32_t(b->rollout_tiles.size() / 3), nd = uint32_t(b->reduce_tiles.size() / 3);
     const uint32_t pgrid = uint32_t(ctx->num_cus) * std::max(1u, b->tuning.wgs_per_cu);
+    uint64_t* tickets = static_cast<uint64_t*>(b->d_ticket.ptr);
     mppi_step_args ra = base, da = base;
     ra.tiles = static_cast<const uint32_t*>(b->d_rtiles.ptr);
     ra.n_tiles = nr;
@@ -183,6 +204,14 @@ int enqueue_step(mppi_hip_batch* b, const mppi_step_args& base) {
     void* dp[] = {&da};
     const uint32_t rgrid = pm ? std::min(nr, pgrid) : nr;
     const uint32_t dgrid = pm ? std::min(nd, pgrid) : nd;
+    if (b->tuning.launches == 1) {
+        ra.ticket = tickets + 2;
+        MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(b->mod->step_b[bi][pm], rgrid, 1, 1, B, 1, 1, 0,
+                                                ctx->stream, rp, nullptr));
+        return MPPI_HIP_OK;
+    }
+    ra.ticket = tickets + 0;
+    da.ticket = tickets + 1;
     MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(b->mod->rollout_b[bi][pm], rgrid, 1, 1, B, 1, 1, 0,
                                             ctx->stream, rp, nullptr));
     MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(b->mod->reduce_b[bi][pm], dgrid, 1, 1, B, 1, 1, 0,
@@ -266,8 +295,7 @@ int mppi_hip_batch_create(mppi_hip_ctx* ctx, uint32_t num_agents, const uint32_t
     if ((rc = ensure_device_buffer(ctx, &b->d_unom, sizeof(float) * seq))) return rc;
     if ((rc = ensure_device_buffer(ctx, &b->d_uout, sizeof(float) * seq))) return rc;
     if ((rc = ensure_device_buffer(ctx, &b->d_diag, sizeof(float) * 3 * num_agents))) return rc;
-    if ((rc = ensure_device_buffer(ctx, &b->d_queue, sizeof(uint32_t) * 4))) return rc;
-    MPPI_HIP_TRY(ctx, hipMemsetAsync(b->d_queue.ptr, 0, sizeof(uint32_t) * 4, ctx->stream));
+    if ((rc = ensure_device_buffer(ctx, &b->d_ticket, sizeof(uint64_t) * 3))) return rc;
     MPPI_HIP_TRY(ctx, hipMemsetAsync(b->d_agents.ptr, 0, sizeof(mppi_agent_state) * num_agents, ctx->stream));
     MPPI_HIP_TRY(ctx, hipMemsetAsync(b->d_x.ptr, 0, sizeof(float) * num_agents * ctx->D, ctx->stream));
     MPPI_HIP_TRY(ctx, hipMemsetAsync(b->d_unom.ptr, 0, sizeof(float) * seq, ctx->stream));
@@ -299,10 +327,11 @@ void mppi_hip_batch_destroy(mppi_hip_batch* b) {
         ctx->last_agents = 0;
         ctx->last_costs = nullptr;
         ctx->last_weights = nullptr;
+        ctx->weights_pending = false;
         ctx->last_desc.clear();
     }
     for (DeviceBuffer* d : {&b->d_desc, &b->d_agents, &b->d_x, &b->d_unom, &b->d_uout, &b->d_diag,
-                            &b->d_rtiles, &b->d_dtiles, &b->d_queue, &b->d_costs, &b->d_weights,
+                            &b->d_rtiles, &b->d_dtiles, &b->d_ticket, &b->d_costs, &b->d_weights,
                             &b->d_blk, &b->d_part, &b->d_model, &b->d_cost, &b->d_hist_states,
                             &b->d_hist_controls, &b->d_hist_costs})
         d->release();
@@ -338,11 +367,9 @@ int mppi_hip_batch_set_tuning(mppi_hip_batch* b, const mppi_hip_batch_tuning* t)
     if (t->block != 64 && t->block != 128 && t->block != 256)
         return bfail(b, MPPI_HIP_E_INVALID, "tuning.block must be 64, 128 or 256.");
     if (t->persistent > 1 || t->order > 1) return bfail(b, MPPI_HIP_E_INVALID, "tuning.persistent and tuning.order must be 0 or 1.");
-    if (t->persistent)
-        return bfail(b, MPPI_HIP_E_INVALID,
-                     "tuning.persistent = 1 is switched off: a launch of the persistent work-queue "
-                     "kernels did not complete on gfx950 and the cause is not yet found; use the "
-                     "tile table (persistent = 0).");
+    if (t->launches != 1 && t->launches != 2) return bfail(b, MPPI_HIP_E_INVALID, "tuning.launches must be 1 or 2.");
+    if (t->launches == 1 && !one_launch_ok(b))
+        return bfail(b, MPPI_HIP_E_INVALID, "tuning.launches = 1 needs K < 2^18 for every agent.");
     if (t->persistent && (t->wgs_per_cu < 1 || t->wgs_per_cu > 16))
         return bfail(b, MPPI_HIP_E_INVALID, "tuning.wgs_per_cu must be in [1, 16].");
     if (t->reduce_spt > 64) return bfail(b, MPPI_HIP_E_INVALID, "tuning.reduce_spt must be <= 64.");
@@ -427,6 +454,14 @@ int mppi_hip_batch_run(mppi_hip_batch* b, uint32_t num_steps, int closed_loop, f
     ctx->last_costs = static_cast<const float*>(b->d_costs.ptr);
     ctx->last_weights = static_cast<const float*>(b->d_weights.ptr);
     ctx->weights_pending = false;
+    if (b->tuning.launches == 1) {
+        // The one-launch form does not form per-sample weights; they are
+        // formed from the costs and the agents' normalisers when read.
+        ctx->weights_pending = true;
+        ctx->weights_args = batch_args(b, closed_loop);
+        ctx->weights_grid_x = (*std::max_element(b->K.begin(), b->K.end()) + kBlock - 1) / kBlock;
+        ctx->weights_grid_y = b->N;
+    }
     ctx->last_desc = b->desc;
     return MPPI_HIP_OK;
 }
@@ -455,10 +490,20 @@ int mppi_hip_batch_autotune(mppi_hip_batch* b, uint32_t trial_steps, double* bes
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (trial_steps == 0) trial_steps = 3;
+    // Two launches: tile tables over block x reduce spt x order, and the
+    // persistent grid at two sizes.  One launch (when it applies): tile
+    // table or persistent grid, block x order.
     std::vector<mppi_hip_batch_tuning> cands;
-    for (uint32_t block : {64u, 128u, 256u})
+    for (uint32_t block : {64u, 128u, 256u}) {
         for (uint32_t spt : {1u, 2u, 4u, 8u})
-            for (uint32_t order : {0u, 1u}) cands.push_back({block, 0, 0, spt, order});
+            for (uint32_t order : {0u, 1u}) cands.push_back({block, 0, 0, spt, order, 2});
+        for (uint32_t w : {2u, 4u}) cands.push_back({block, 1, w, 0, 0, 2});
+        if (one_launch_ok(b))
+            for (uint32_t order : {0u, 1u}) {
+                cands.push_back({block, 0, 0, 0, order, 1});
+                for (uint32_t w : {2u, 4u}) cands.push_back({block, 1, w, 0, order, 1});
+            }
+    }
     hipEvent_t e0 = take_event(ctx), e1 = take_event(ctx);
     if (!e0 || !e1) return fail(ctx, MPPI_HIP_E_DEVICE, "hipEventCreate failed.");
     double best = 1e300;

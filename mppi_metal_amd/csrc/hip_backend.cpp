@@ -340,8 +340,10 @@ int load_module(mppi_hip_ctx* ctx, bool batch, bool surrogate, std::shared_ptr<L
             for (int md = 0; md < 2 && ok; ++md) {
                 std::string r = std::string("mppi_rollout_") + modes[md] + "_b" + blocks[b];
                 std::string d = std::string("mppi_reduce_") + modes[md] + "_b" + blocks[b];
+                std::string st = std::string("mppi_step_") + modes[md] + "_b" + blocks[b];
                 ok = hipModuleGetFunction(&m->rollout_b[b][md], m->module, r.c_str()) == hipSuccess &&
-                     hipModuleGetFunction(&m->reduce_b[b][md], m->module, d.c_str()) == hipSuccess;
+                     hipModuleGetFunction(&m->reduce_b[b][md], m->module, d.c_str()) == hipSuccess &&
+                     hipModuleGetFunction(&m->step_b[b][md], m->module, st.c_str()) == hipSuccess;
             }
     }
     if (ok && surrogate)
@@ -398,6 +400,17 @@ void fill_common_args(mppi_hip_ctx* ctx, mppi_step_args* a) {
     a->key0 = ctx->key0;
     a->key1 = ctx->key1;
     a->lambda = ctx->lambda;
+    a->wait_limit = ctx->wait_limit;
+    a->wait_bias = ctx->wait_bias;
+}
+
+int check_status(mppi_hip_ctx* ctx, const uint32_t* status, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i)
+        if (status[i] & MPPI_STATUS_WAIT_TIMEOUT)
+            return fail(ctx, MPPI_HIP_E_DEVICE,
+                        "agent " + std::to_string(i) + ": a rollout wavefront of the one-pass step gave up "
+                        "waiting for its noise (the bounded wait ran out); the results are not valid.");
+    return MPPI_HIP_OK;
 }
 
 namespace {
@@ -449,7 +462,8 @@ int ensure_work_buffers(mppi_hip_ctx* ctx, const std::vector<mppi_agent_desc>& d
     if ((rc = ensure_device_buffer(ctx, &ctx->partials,
                                    sizeof(float) * (last.part_off + size_t(std::max(last.n_chunks, last.n_blk)) *
                                                                         last.NB * 4)))) return rc;
-    const size_t down = sizeof(float) * (last.seq_off + size_t(last.H) * ctx->C + 3 * size_t(N));
+    // u_out | diag (3 per agent) | status (one word per agent)
+    const size_t down = sizeof(float) * (last.seq_off + size_t(last.H) * ctx->C + 4 * size_t(N));
     if ((rc = ensure_device_buffer(ctx, &ctx->down, down))) return rc;
     if ((rc = ensure_pinned_buffer(ctx, &ctx->down_host, down))) return rc;
     return MPPI_HIP_OK;
@@ -626,6 +640,7 @@ mppi_step_args plain_args(mppi_hip_ctx* ctx, const UploadPlan& plan, uint32_t N,
     a.blk_stats = static_cast<float*>(ctx->blk_stats.ptr);
     a.partials = static_cast<float*>(ctx->partials.ptr);
     a.diag = a.u_out + size_t(N) * ctx->H * ctx->C;
+    a.status_out = reinterpret_cast<uint32_t*>(a.diag + 3 * size_t(N));
     return a;
 }
 
@@ -672,7 +687,11 @@ int run_closed_loop(mppi_hip_ctx* ctx, uint32_t N, uint32_t T, uint64_t start_st
     if (terminal_out)
         MPPI_HIP_TRY(ctx, hipMemcpyAsync(terminal_out, ctx->terminal.ptr, sizeof(float) * N * D,
                                          hipMemcpyDeviceToHost, ctx->stream));
+    uint32_t* status = static_cast<uint32_t*>(ctx->down_host.ptr);
+    MPPI_HIP_TRY(ctx, hipMemcpyAsync(status, a.status_out, sizeof(uint32_t) * N, hipMemcpyDeviceToHost,
+                                     ctx->stream));
     if ((rc = finish(ctx))) return rc;
+    if ((rc = check_status(ctx, status, N))) return rc;
     ctx->last_agents = N;
     ctx->last_costs = static_cast<const float*>(ctx->costs.ptr);
     ctx->last_weights = static_cast<const float*>(ctx->weights.ptr);
@@ -945,10 +964,11 @@ int mppi_hip_step(mppi_hip_ctx* ctx, uint64_t step_index, const float* x0, const
     mppi_step_args a = plain_args(ctx, plan, 1, mo, mo_bytes, co, co_bytes);
     if ((rc = launch_iteration(ctx, desc, a))) return rc;
     const size_t hc = size_t(ctx->H) * ctx->C;
-    MPPI_HIP_TRY(ctx, hipMemcpyAsync(ctx->down_host.ptr, ctx->down.ptr, sizeof(float) * (hc + 3),
+    MPPI_HIP_TRY(ctx, hipMemcpyAsync(ctx->down_host.ptr, ctx->down.ptr, sizeof(float) * (hc + 4),
                                      hipMemcpyDeviceToHost, ctx->stream));
     if ((rc = finish(ctx))) return rc;
     const float* h = static_cast<const float*>(ctx->down_host.ptr);
+    if ((rc = check_status(ctx, reinterpret_cast<const uint32_t*>(h + hc + 3), 1))) return rc;
     std::memcpy(u_out, h, sizeof(float) * hc);
     if (diag3) std::memcpy(diag3, h + hc, sizeof(float) * 3);
     ctx->last_agents = 1;
@@ -1030,6 +1050,13 @@ int mppi_hip_set_reduce_samples_per_thread(mppi_hip_ctx* ctx, uint32_t spt) {
     if (!ctx) return fail(nullptr, MPPI_HIP_E_INVALID, "context is null.");
     if (spt > 1024) return fail(ctx, MPPI_HIP_E_INVALID, "samples per thread must be <= 1024.");
     ctx->spt_override = spt;
+    return MPPI_HIP_OK;
+}
+
+int mppi_hip_set_wait_limit(mppi_hip_ctx* ctx, uint32_t polls, int force_timeout) {
+    if (!ctx) return fail(nullptr, MPPI_HIP_E_INVALID, "context is null.");
+    ctx->wait_limit = polls;
+    ctx->wait_bias = force_timeout ? 0x40000000u : 0u;
     return MPPI_HIP_OK;
 }
 

@@ -34,10 +34,12 @@
 //            the warm start.  Reference: mppi_reduce (mppi_reduce.metal:130-168,
 //            :363-406) and mppi_propagate_and_shift (:175-231, :413-488).
 // Work is cut into tiles: rollout (agent, sample block), reduce (agent,
-// sample chunk, noise-block group).  Three dispatch modes share the tile
+// sample chunk, noise-block group), and the one-launch step's tile (agent,
+// sample block: rollout, then the tile's weighted noise sums, the last tile
+// of the agent forming the update).  Three dispatch modes share the tile
 // bodies: grid-mapped (one workgroup per tile in grid order), tile table
 // (workgroup i runs tile i of a host-built, longest-first table) and
-// persistent (a fixed grid pulls tiles from an atomic queue).
+// persistent (a fixed grid takes tiles from a ticket counter).
 // Cross-block sums (normaliser and weighted noise) run in FP64 when K >= 2^18.
 #if !defined(__HIPCC_RTC__)
 #include <hip/hip_runtime.h>   // hipRTC provides these builtins itself
@@ -263,7 +265,7 @@ __device__ void mppi_block_stats(const mppi_step_args& a, uint32_t agent, const 
         s_last = (prev == d.n_blk - 1);
     }
     __syncthreads();
-    if (!s_last) return;
+    if (!__builtin_amdgcn_readfirstlane(s_last)) return;
     __threadfence();
     if (d.K >= MPPI_FP64_MIN_K) mppi_combine_stats<BLOCK, double>(a, agent, d);
     else mppi_combine_stats<BLOCK, float>(a, agent, d);
@@ -274,12 +276,16 @@ __device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const 
                                  uint32_t blk, bool active, float total, float* s_z);
 
 // ---------------------------------------------------------------------------
-// Rollout tile: samples [blk*BLOCK, blk*BLOCK + BLOCK) of one agent; the
-// tile ends with its softmax partials (mppi_block_stats).
+// Rollout of samples [blk*BLOCK, blk*BLOCK + BLOCK) of one agent: returns
+// the sample's cost (0 for lanes past K) and stores it.  s_un: LDS of at
+// least MPPI_UN_LDS floats, free again once the function returns (after a
+// barrier).
 // ---------------------------------------------------------------------------
+constexpr int MPPI_UN_LDS = MPPI_MAX_HC + 2 * MPPI_NG;
+
 template <int BLOCK>
-__device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint32_t blk) {
-    const mppi_agent_desc d = a.desc[agent];
+__device__ float mppi_rollout_costs(const mppi_step_args& a, const mppi_agent_desc& d, uint32_t blk,
+                                    float* s_un) {
     const uint32_t k = blk * BLOCK + threadIdx.x;
     const uint32_t H = d.H;
     const bool active = k < d.K;
@@ -293,7 +299,6 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
     // The nominal sequence goes to LDS and the noise scale and bounds to
     // registers, so the time loop issues no memory waits of its own.
     // 2 * MPPI_NG slots of padding: the time loop reads one group ahead.
-    __shared__ __attribute__((aligned(16))) float s_un[MPPI_MAX_HC + 2 * MPPI_NG];
     for (uint32_t i = threadIdx.x; i < H * CDIM; i += BLOCK) s_un[i] = a.u_nom[d.seq_off + i];
     float scale[MPPI_COV_FULL ? CDIM * CDIM : CDIM], lo[CDIM], hi[CDIM];
 #pragma unroll
@@ -349,7 +354,17 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
                                mppi_blob_ref{cost}, mppi_time{H, step});
         a.costs[d.cost_off + k] = total;
     }
-    mppi_block_stats<BLOCK>(a, agent, d, blk, active, total);
+    return total;
+}
+
+// Rollout tile of the two-kernel step: the costs, then the tile's softmax
+// partials (mppi_block_stats; the last tile of an agent forms the normaliser).
+template <int BLOCK>
+__device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint32_t blk) {
+    __shared__ __attribute__((aligned(16))) float s_un[MPPI_UN_LDS];
+    const mppi_agent_desc d = a.desc[agent];
+    const float total = mppi_rollout_costs<BLOCK>(a, d, blk, s_un);
+    mppi_block_stats<BLOCK>(a, agent, d, blk, blk * BLOCK + threadIdx.x < d.K, total);
 }
 
 // ---------------------------------------------------------------------------
@@ -411,8 +426,11 @@ __device__ void mppi_fused_tile(const mppi_step_args& a, uint32_t agent, uint32_
     // No barrier or fence sits in the consumers' time loop: both keep the
     // compiler from hoisting the plugin's loop-invariant parameter loads and
     // set-up out of it.  The producers never wait, so the wait ends; it is
-    // bounded all the same.
-    constexpr uint32_t SPIN_MAX = 1u << 20;
+    // bounded all the same, and a consumer that reaches the bound sets
+    // MPPI_STATUS_WAIT_TIMEOUT in its agent's status, which the update copies
+    // to the host, so the call fails rather than return results computed
+    // from normals that were not there.
+    const uint32_t spin_max = a.wait_limit ? a.wait_limit : (1u << 20);
     const uint32_t wave = (threadIdx.x & (SAMPLES - 1)) >> 6;
     // The consumers are the critical path (their timesteps depend on each
     // other; the producers run ahead and have slack), so they take the
@@ -468,11 +486,20 @@ __device__ void mppi_fused_tile(const mppi_step_args& a, uint32_t agent, uint32_
         uint32_t ready = 0, zero = 0;
         for (uint32_t t0 = 0; t0 < H; t0 += MPPI_G) {
             const uint32_t g = t0 / MPPI_G;
-            const uint32_t need = g / MPPI_PHASE_G + 1;
+            const uint32_t need = g / MPPI_PHASE_G + 1 + a.wait_bias;
             if (ready < need) {
                 uint32_t got = __hip_atomic_load(&s_prog[wave], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                for (uint32_t spin = 0; got < need && spin < SPIN_MAX; ++spin)
+                uint32_t spin = 0;
+                for (; got < need && spin < spin_max; ++spin)
                     got = __hip_atomic_load(&s_prog[wave], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (got < need) {
+                    // Gave up: every later phase counts as ready (the rows
+                    // are read as they are) and the agent is flagged.
+                    if ((threadIdx.x & 63) == 0)
+                        __hip_atomic_fetch_or(&a.agents[slot].status, MPPI_STATUS_WAIT_TIMEOUT, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+                    got = 0x7fffffffu;
+                }
                 ready = got;
                 zero = got & 0x80000000u;
             }
@@ -515,13 +542,14 @@ __device__ void mppi_fused_tile(const mppi_step_args& a, uint32_t agent, uint32_
 // diagnostics; in closed loop also log history, step the plant and shift.
 // ---------------------------------------------------------------------------
 // zs[col] = (sum_r scale_r * rows[r * W + col]) / S for col < W (W a
-// multiple of 4), summed in a fixed order; scale = null means 1.  Rows were
+// multiple of 4), summed in a fixed order; scale_r = scale[r * sstride], or
+// 1 when scale is null.  Rows were
 // written by other workgroups, possibly on other XCDs: each thread owns a
 // float4 of columns and a stride of rows and keeps 16 row loads in flight,
 // so the sum costs a few memory latencies rather than one per row.
 template <int BLOCK, int U = 16>
-__device__ void mppi_sum_rows(const float* rows, uint32_t n_rows, uint32_t W, const float* scale, float S,
-                              float* zs) {
+__device__ void mppi_sum_rows(const float* rows, uint32_t n_rows, uint32_t W, const float* scale,
+                              uint32_t sstride, float S, float* zs) {
     __shared__ float4 s_part[BLOCK];
     const uint32_t Q = W / 4;                            // column quads
     const uint32_t T = Q < BLOCK ? BLOCK / Q : 1;        // threads per quad
@@ -538,7 +566,7 @@ __device__ void mppi_sum_rows(const float* rows, uint32_t n_rows, uint32_t W, co
                 for (int u = 0; u < U; ++u) v[u] = base[size_t(r + u * T) * Q];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const float f = scale ? scale[r + u * T] : 1.0f;
+                    const float f = scale ? scale[(r + u * T) * sstride] : 1.0f;
                     acc.x = mppi_fma(f, v[u].x, acc.x);
                     acc.y = mppi_fma(f, v[u].y, acc.y);
                     acc.z = mppi_fma(f, v[u].z, acc.z);
@@ -547,7 +575,7 @@ __device__ void mppi_sum_rows(const float* rows, uint32_t n_rows, uint32_t W, co
             }
             for (; r < n_rows; r += T) {
                 const float4 v = base[size_t(r) * Q];
-                const float f = scale ? scale[r] : 1.0f;
+                const float f = scale ? scale[r * sstride] : 1.0f;
                 acc.x = mppi_fma(f, v.x, acc.x);
                 acc.y = mppi_fma(f, v.y, acc.y);
                 acc.z = mppi_fma(f, v.z, acc.z);
@@ -685,6 +713,7 @@ __device__ void mppi_apply_update(const mppi_step_args& a, uint32_t agent, const
         a.diag[agent * 3 + 1] = float(ag->mean_cost);
         a.diag[agent * 3 + 2] = float(ag->ess);
     }
+    if (threadIdx.x == 0 && a.status_out) a.status_out[agent] = ag->status;
     if (a.first_out && threadIdx.x < CDIM) a.first_out[agent * CDIM + threadIdx.x] = uo[threadIdx.x];
     if (a.closed_loop) {
         if (threadIdx.x == 0) mppi_plant_step(a, agent, d, uo, mppi_plant_load(a, d), float(ag->min_cost));
@@ -705,7 +734,7 @@ __device__ void mppi_finalize(const mppi_step_args& a, uint32_t agent, const mpp
                               float* zs) {
     const mppi_agent_state* ag = &a.agents[d.slot];
     if (d.K >= MPPI_FP64_MIN_K) mppi_sum_partials<BLOCK, double>(a, d, ag->weight_sum, zs);
-    else mppi_sum_rows<BLOCK>(a.partials + d.part_off, d.n_chunks, d.NB * 4, nullptr, float(ag->weight_sum), zs);
+    else mppi_sum_rows<BLOCK>(a.partials + d.part_off, d.n_chunks, d.NB * 4, nullptr, 1, float(ag->weight_sum), zs);
     __syncthreads();
     mppi_apply_update<BLOCK>(a, agent, d, zs);
 }
@@ -727,16 +756,18 @@ __device__ void mppi_finalize(const mppi_step_args& a, uint32_t agent, const mpp
 // SAMPLES samples, THREADS >= SAMPLES threads: thread i < SAMPLES holds
 // sample i's cost (`active`, `total`); the other threads add nothing to the
 // statistics and share the work of the noise sums.
-template <int SAMPLES, int THREADS>
-__device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
-                                 uint32_t blk, bool active, float total, float* s_z) {
+// Softmax partials of one tile against the tile's own minimum m_g: writes
+// (m_g, S_g, C_g, Q_g) to the tile's statistics row and returns the thread's
+// weight factor e = exp(-(c - m_g)/lambda), 0 for a lane without a finite
+// cost, or, in a tile with no finite cost, 1 for every active lane (the tile
+// then keeps the plain sum of its noise).  When s_e is not null the factor is
+// also stored at s_e[threadIdx.x] before the function's last barrier.
+template <int THREADS>
+__device__ float mppi_tile_stats(const mppi_step_args& a, const mppi_agent_desc& d, uint32_t blk, bool active,
+                                 float total, float* s_e) {
     constexpr int WAVES = THREADS / 64;
-    constexpr int ZS = MPPI_FUSED_ZS;
     __shared__ float s_red[4][WAVES];
-    __shared__ __attribute__((aligned(16))) float s_e[SAMPLES];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t W = d.NB * 4;
-    active = active && threadIdx.x < uint32_t(SAMPLES);
     float c = (active && mppi_isfinite(total)) ? total : __builtin_inff();
     float m = mppi_wave_min(c);
     if (lane == 0) s_red[0][wave] = m;
@@ -745,7 +776,8 @@ __device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const 
 #pragma unroll
     for (int w = 1; w < WAVES; ++w) m = s_red[0][w] < m ? s_red[0][w] : m;
     const float e = (mppi_isfinite(c) && mppi_isfinite(m)) ? mppi_expf(-(c - m) / a.lambda) : 0.0f;
-    if (threadIdx.x < uint32_t(SAMPLES)) s_e[threadIdx.x] = mppi_isfinite(m) ? e : (active ? 1.0f : 0.0f);
+    const float ew = mppi_isfinite(m) ? e : (active ? 1.0f : 0.0f);
+    if (s_e) s_e[threadIdx.x] = ew;
     const float se = mppi_wave_sum(e);
     const float sq = mppi_wave_sum(e * e);
     const float sc = mppi_wave_sum(active ? total : 0.0f);
@@ -758,6 +790,17 @@ __device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const 
         float* bs = a.blk_stats + (d.blk_off + blk) * 4;
         bs[0] = m; bs[1] = S; bs[2] = Cs; bs[3] = Q;
     }
+    return ew;
+}
+
+template <int SAMPLES, int THREADS>
+__device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
+                                 uint32_t blk, bool active, float total, float* s_z) {
+    constexpr int ZS = MPPI_FUSED_ZS;
+    __shared__ __attribute__((aligned(16))) float s_e[SAMPLES];
+    const uint32_t W = d.NB * 4;
+    active = active && threadIdx.x < uint32_t(SAMPLES);
+    (void)mppi_tile_stats<THREADS>(a, d, blk, active, total, threadIdx.x < uint32_t(SAMPLES) ? s_e : nullptr);
     // V_g[v] = sum over the tile's lanes of e_l z_l[v].  Thread (quad, sg)
     // sums four columns over the 32 lanes of sample group sg in order (two
     // interleaved accumulators), reading four normals and four weights per
@@ -939,6 +982,7 @@ __device__ void mppi_fused_combine_slice(const mppi_step_args& a, uint32_t agent
                 a.diag[agent * 3 + 1] = float(double(Cs / float(d.K)));
                 a.diag[agent * 3 + 2] = float(double((S * S) / Q));
             }
+            if (a.status_out) a.status_out[agent] = ag->status;
             if (a.closed_loop) mppi_plant_step(a, agent, d, s_u, plant, mg);
         }
         if (a.first_out && threadIdx.x < CDIM) a.first_out[agent * CDIM + threadIdx.x] = s_u[threadIdx.x];
@@ -1033,9 +1077,122 @@ __device__ void mppi_reduce_tile(const mppi_step_args& a, uint32_t agent, uint32
         s_last = (prev == d.n_chunks * d.n_groups - 1);
     }
     __syncthreads();
-    if (!s_last) return;
+    if (!__builtin_amdgcn_readfirstlane(s_last)) return;
     __threadfence();
     mppi_finalize<BLOCK>(a, agent, d, s_acc);
+}
+
+// ---------------------------------------------------------------------------
+// One-launch step tile (the uneven batch's one-launch form): samples
+// [blk*BLOCK, blk*BLOCK + BLOCK) of one agent.  The tile rolls out its
+// samples, takes their weights against its own minimum m_g, regenerates
+// their noise and forms its weighted noise sums V_g (one row of 4 NB
+// floats), as the one-pass step's tiles do with the noise kept in LDS; the
+// last tile of the agent to finish (an arrival counter; nothing waits) then
+// combines the tiles: m = min m_g, f_g = exp(-(m_g - m)/lambda),
+// S = sum f_g S_g, and the update's average is sum f_g V_g / S.  Every tile
+// of the agent has read the agent's state, nominal and step counter before
+// it arrives, so the last one updates them in place.  K < 2^18 (FP32
+// cross-tile sums).
+// ---------------------------------------------------------------------------
+// V_g[n] = sum over the tile's lanes of e_l z_l[n], n < 4 NB, the noise
+// regenerated from its counters: per group of MPPI_REDUCE_BPG noise blocks
+// every lane forms e z for its 32 values and the workgroup sums them through
+// an LDS transpose (the reduce tile's scheme, one sample per lane).
+// s_acc: BLOCK * (MPPI_NV + 1) floats of LDS.
+template <int BLOCK>
+__device__ void mppi_tile_noise_sums(const mppi_step_args& a, const mppi_agent_desc& d, uint32_t blk, float e,
+                                     float* s_acc) {
+    const uint32_t k = blk * BLOCK + threadIdx.x;
+    const uint32_t NB = d.NB;
+    const uint32_t step = a.agents[d.slot].step;
+    const uint32_t tag_agent = (MPPI_TAG_CONTROL_NOISE << 28) | (d.slot & MPPI_AGENT_MASK);
+    float* row = a.partials + d.part_off + size_t(blk) * NB * 4;
+    // A wavefront whose samples all have weight 0 adds nothing.
+    const bool any = mppi_any(e != 0.0f);
+    for (uint32_t b_begin = 0; b_begin < NB; b_begin += MPPI_REDUCE_BPG) {
+        float acc[MPPI_REDUCE_BPG][4];
+#pragma unroll
+        for (int q = 0; q < MPPI_REDUCE_BPG; ++q)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[q][j] = 0.0f;
+        if (any) {
+#pragma unroll
+            for (int q = 0; q < MPPI_REDUCE_BPG; ++q) {
+                if (b_begin + q < NB) {
+                    const mppi_f32x4 z = mppi_normal4(mppi_u32x4{b_begin + q, k, step, tag_agent}, a.key0, a.key1);
+                    acc[q][0] = e * z.x;
+                    acc[q][1] = e * z.y;
+                    acc[q][2] = e * z.z;
+                    acc[q][3] = e * z.w;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < MPPI_REDUCE_BPG; ++q)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) s_acc[threadIdx.x * (MPPI_NV + 1) + q * 4 + j] = acc[q][j];
+        __syncthreads();
+        constexpr int NSEG = BLOCK / MPPI_NV;     // row segments
+        constexpr int ROWS = BLOCK / NSEG;        // rows per segment
+        const int v = threadIdx.x % MPPI_NV, seg = threadIdx.x / MPPI_NV;
+        float sum = 0.0f;
+        for (int r = 0; r < ROWS; ++r) sum += s_acc[(seg * ROWS + r) * (MPPI_NV + 1) + v];
+        __syncthreads();
+        s_acc[seg * MPPI_NV + v] = sum;
+        __syncthreads();
+        if (threadIdx.x < MPPI_NV) {
+            float tot = 0.0f;
+#pragma unroll
+            for (int g = 0; g < NSEG; ++g) tot += s_acc[g * MPPI_NV + threadIdx.x];
+            const uint32_t b = b_begin + threadIdx.x / 4;
+            if (b < NB) row[b * 4 + (threadIdx.x & 3)] = tot;
+        }
+        __syncthreads();                          // s_acc is reused by the next group
+    }
+}
+
+// The last tile of an agent: normaliser, scaled row sums, update.
+template <int BLOCK>
+__device__ void mppi_onelaunch_combine(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
+                                       float* zs) {
+    mppi_combine_stats<BLOCK, float>(a, agent, d);
+    __syncthreads();
+    const mppi_agent_state* ag = &a.agents[d.slot];
+    const float m = float(ag->min_cost);
+    const bool uniform = !mppi_isfinite(m);
+    // Each tile row's factor f_g (the one mppi_combine_stats used for S)
+    // replaces the row's m_g, which is no longer needed.
+    float* st = a.blk_stats + d.blk_off * 4;
+    for (uint32_t i = threadIdx.x; i < d.n_blk; i += BLOCK) {
+        const float bm = st[i * 4];
+        st[i * 4] = uniform ? 1.0f : (mppi_isfinite(bm) ? mppi_expf(-(bm - m) / a.lambda) : 0.0f);
+    }
+    __syncthreads();
+    mppi_sum_rows<BLOCK>(a.partials + d.part_off, d.n_blk, d.NB * 4, st, 4, float(ag->weight_sum), zs);
+    __syncthreads();
+    mppi_apply_update<BLOCK>(a, agent, d, zs);
+}
+
+template <int BLOCK>
+__device__ void mppi_onelaunch_tile(const mppi_step_args& a, uint32_t agent, uint32_t blk) {
+    constexpr int LDS = MPPI_UN_LDS > BLOCK * (MPPI_NV + 1) ? MPPI_UN_LDS : BLOCK * (MPPI_NV + 1);
+    __shared__ __attribute__((aligned(16))) float s_lds[LDS];
+    __shared__ int s_last;
+    const mppi_agent_desc d = a.desc[agent];
+    const bool active = blk * BLOCK + threadIdx.x < d.K;
+    const float total = mppi_rollout_costs<BLOCK>(a, d, blk, s_lds);
+    const float e = mppi_tile_stats<BLOCK>(a, d, blk, active, total, nullptr);   // its barriers free s_lds
+    mppi_tile_noise_sums<BLOCK>(a, d, blk, e, s_lds);
+    if (threadIdx.x == 0) {
+        __threadfence();                          // the row and the statistics before the arrival
+        const uint32_t prev = atomicAdd(&a.agents[d.slot].rollout_arrivals, 1u);
+        s_last = (prev == d.n_blk - 1);
+    }
+    __syncthreads();
+    if (!__builtin_amdgcn_readfirstlane(s_last)) return;
+    __threadfence();
+    mppi_onelaunch_combine<BLOCK>(a, agent, d, s_lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1087,48 +1244,76 @@ extern "C" __global__ void __launch_bounds__(256) mppi_weights(mppi_step_args a)
 
 #if MPPI_BATCH_KERNELS
 // Tile-table and persistent kernels of the uneven multi-agent step.  Tiles
-// are (agent, x, y) triples: rollout (agent, block, 0), reduce (agent, chunk,
-// group).  The persistent form loops until the queue is empty; the last
-// workgroup to leave resets the queue for the next launch.
-template <int BLOCK, bool ROLLOUT>
+// are (agent, x, y) triples: rollout and one-launch step (agent, block, 0),
+// reduce (agent, chunk, group).
+//   table       workgroup i runs tile i;
+//   persistent  a grid of G workgroups takes tiles from a ticket counter
+//               (one 64-bit counter per kernel, never reset): ticket t names
+//               tile t mod P, P = n_tiles + G, and a workgroup leaves at its
+//               first ticket past the last tile.  Tickets are handed out in
+//               order, so a launch's first n_tiles tickets are the tiles and
+//               the next G are one exit each; each workgroup takes exactly
+//               one exit, so a launch takes exactly P tickets and the next
+//               launch starts at a multiple of P again (the host zeroes the
+//               counters whenever n_tiles or G changes).  Nothing waits on
+//               another workgroup: every tile runs to its end on its own,
+//               and every workgroup leaves after at most n_tiles + G + P
+//               tickets whatever the counter holds.
+enum { MPPI_KIND_ROLLOUT = 0, MPPI_KIND_REDUCE = 1, MPPI_KIND_STEP = 2 };
+
+template <int BLOCK, int KIND>
 __device__ inline void mppi_run_tile(const mppi_step_args& a, uint32_t i) {
     const uint32_t* t = a.tiles + size_t(i) * 3;
-    if (ROLLOUT) mppi_rollout_tile<BLOCK>(a, t[0], t[1]);
-    else mppi_reduce_tile<BLOCK>(a, t[0], t[1], t[2]);
+    if (KIND == MPPI_KIND_ROLLOUT) mppi_rollout_tile<BLOCK>(a, t[0], t[1]);
+    else if (KIND == MPPI_KIND_REDUCE) mppi_reduce_tile<BLOCK>(a, t[0], t[1], t[2]);
+    else mppi_onelaunch_tile<BLOCK>(a, t[0], t[1]);
 }
 
-template <int BLOCK, bool ROLLOUT>
+// Every branch of the loop is uniform to the compiler, and a barrier closes
+// each iteration.  Both matter: the tile index comes from LDS, which the
+// compiler must treat as varying across lanes, and the tiles end in regions
+// run by thread 0 alone; with a varying exit and no closing barrier, the
+// compiler may give thread 0's path back to the loop head (through the ticket
+// fetch) its own outer loop and let the other lanes re-enter the loop's
+// barriers without it, in the same wavefront -- they then re-run the tile
+// with a stale index and, once thread 0 is masked off, never leave.  The
+// tile index is read into a scalar register (readfirstlane), as is every
+// "last tile" flag the tiles branch on.
+template <int BLOCK, int KIND>
 __device__ inline void mppi_persistent(const mppi_step_args& a) {
     __shared__ uint32_t s_tile;
+    const uint64_t P = uint64_t(a.n_tiles) + gridDim.x;
     for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(&a.queue[0], 1u);
-        __syncthreads();
-        const uint32_t i = s_tile;
-        __syncthreads();
-        if (i >= a.n_tiles) break;
-        mppi_run_tile<BLOCK, ROLLOUT>(a, i);
-    }
-    if (threadIdx.x == 0) {
-        if (atomicAdd(&a.queue[1], 1u) == gridDim.x - 1) {
-            a.queue[0] = 0u;
-            a.queue[1] = 0u;
-            __threadfence();
+        if (threadIdx.x == 0) {
+            const uint64_t t = __hip_atomic_fetch_add(a.ticket, uint64_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_tile = uint32_t(t % P);
         }
+        __syncthreads();
+        const uint32_t i = __builtin_amdgcn_readfirstlane(s_tile);
+        if (i >= a.n_tiles) break;
+        mppi_run_tile<BLOCK, KIND>(a, i);
+        __syncthreads();       // the tile, thread 0's last writes included, ends here for every thread
     }
 }
 
-#define MPPI_BATCH_ENTRIES(B)                                                                   \
-    extern "C" __global__ void __launch_bounds__(B) mppi_rollout_table_b##B(mppi_step_args a) {   \
-        mppi_run_tile<B, true>(a, blockIdx.x);                                                  \
-    }                                                                                           \
-    extern "C" __global__ void __launch_bounds__(B) mppi_reduce_table_b##B(mppi_step_args a) {    \
-        mppi_run_tile<B, false>(a, blockIdx.x);                                                 \
-    }                                                                                           \
-    extern "C" __global__ void __launch_bounds__(B) mppi_rollout_persist_b##B(mppi_step_args a) { \
-        mppi_persistent<B, true>(a);                                                            \
-    }                                                                                           \
-    extern "C" __global__ void __launch_bounds__(B) mppi_reduce_persist_b##B(mppi_step_args a) {  \
-        mppi_persistent<B, false>(a);                                                           \
+#define MPPI_BATCH_ENTRIES(B)                                                                    \
+    extern "C" __global__ void __launch_bounds__(B) mppi_rollout_table_b##B(mppi_step_args a) {    \
+        mppi_run_tile<B, MPPI_KIND_ROLLOUT>(a, blockIdx.x);                                      \
+    }                                                                                            \
+    extern "C" __global__ void __launch_bounds__(B) mppi_reduce_table_b##B(mppi_step_args a) {     \
+        mppi_run_tile<B, MPPI_KIND_REDUCE>(a, blockIdx.x);                                       \
+    }                                                                                            \
+    extern "C" __global__ void __launch_bounds__(B) mppi_step_table_b##B(mppi_step_args a) {       \
+        mppi_run_tile<B, MPPI_KIND_STEP>(a, blockIdx.x);                                         \
+    }                                                                                            \
+    extern "C" __global__ void __launch_bounds__(B) mppi_rollout_persist_b##B(mppi_step_args a) {  \
+        mppi_persistent<B, MPPI_KIND_ROLLOUT>(a);                                                \
+    }                                                                                            \
+    extern "C" __global__ void __launch_bounds__(B) mppi_reduce_persist_b##B(mppi_step_args a) {   \
+        mppi_persistent<B, MPPI_KIND_REDUCE>(a);                                                 \
+    }                                                                                            \
+    extern "C" __global__ void __launch_bounds__(B) mppi_step_persist_b##B(mppi_step_args a) {     \
+        mppi_persistent<B, MPPI_KIND_STEP>(a);                                                   \
     }
 
 MPPI_BATCH_ENTRIES(64)

@@ -19,7 +19,12 @@ struct mppi_agent_state {
     double weight_sum;           // sum_k exp(-(c_k - min)/lambda)
     double mean_cost;
     double ess;
+    uint32_t status;             // MPPI_STATUS_* bits, sticky for a call
+    uint32_t reserved;
 };
+
+// mppi_agent_state::status bits.
+#define MPPI_STATUS_WAIT_TIMEOUT 1u   // a one-pass rollout consumer gave up waiting for its noise
 
 // Per-agent shape and buffer offsets (immutable for a batch shape).  All
 // offsets are in elements of the buffer they index.  An agent has two
@@ -48,7 +53,7 @@ struct mppi_step_args {
     const mppi_agent_desc* desc; // [N]
     mppi_agent_state* agents;    // [N]
     const uint32_t* tiles;       // table / persistent modes: [n_tiles][3]
-    uint32_t* queue;             // persistent mode: [2] head, exits (self-resetting)
+    uint64_t* ticket;            // persistent mode: the launch's ticket counter
     uint32_t n_tiles;
     uint32_t N;
     float* x;                    // [N][D] state (x0; plant state in closed loop)
@@ -75,6 +80,10 @@ struct mppi_step_args {
     uint32_t closed_loop;        // 1: plant step + warm-start shift after the reduce
     uint32_t key0, key1;
     float lambda;
+    uint32_t* status_out;        // [N] copy of each agent's status after the update, or null
+    uint32_t wait_limit;         // one-pass rollout: polls before a consumer gives up (0: default)
+    uint32_t wait_bias;          // added to the phase a consumer waits for; nonzero only to test
+                                 // the give-up path (the count then never arrives)
 };
 
 // Learned-dynamics surrogate (mppi_mlp.h): training data and rollout.

@@ -100,3 +100,25 @@ def test_simulate_with_process_noise(require_gpu, path):
     quiet_hip.set_step_path(path)
     sq, _, _ = quiet_hip.simulate(5, 2, X0_PM3, _u_nom(H, 3))
     assert seq_rel_err(sq[1:], st[1:]) > 1e-3
+
+
+def test_one_pass_wait_give_up_is_reported(require_gpu):
+    """The one-pass rollout's bounded wait: when a consumer wavefront runs out
+    of polls it flags its agent and the call fails with MPPI_HIP_E_DEVICE
+    instead of returning results computed from missing noise.  The host makes
+    every wait target unreachable (and the bound short), so every wait gives
+    up and every kernel still ends normally; the next calls, with the wait
+    restored, are correct again."""
+    from mppi_metal_amd import MppiHipError
+    hip, ora, sc = unicycle(2048, 32, 8)
+    hip.set_step_path(2)
+    x0 = np.zeros(4, np.float32)
+    hip.set_wait_limit(64, force_timeout=True)
+    with pytest.raises(MppiHipError, match="gave up waiting"):
+        hip.step(0, x0, _u_nom(32, 2))
+    with pytest.raises(MppiHipError, match="gave up waiting"):
+        hip.simulate(3, 0, x0, _u_nom(32, 2))
+    hip.set_wait_limit(0, force_timeout=False)
+    check_step(hip, ora, 0, x0, _u_nom(32, 2), "after a reported give-up")
+    st, ct, kt = hip.simulate(3, 0, x0, _u_nom(32, 2))
+    assert np.all(np.isfinite(kt))

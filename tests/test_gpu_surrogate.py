@@ -14,8 +14,19 @@ Tolerances (bf16 operands, FP32 accumulation on the matrix cores):
                     compounded; measured 4e-5 and 1.2e-4, median 0)
   training          the loss falls by 50x and the one-step relative RMSE of
                     every state component on unseen transitions is < 0.1
-  closed loop       mean realised cost within 5% of plain MPPI (bench.py
-                    reports the gap at the full workload)
+  closed loop       at the full workload (K = 2^18, H = 64, 500 steps, the
+                    benchmark's training): the surrogate's mean realised cost
+                    at most 1% above plain MPPI's, (sur - plain) / plain <=
+                    0.01 (a one-sided bar: doing better is no error); at a
+                    reduced size (K = 16384, 200 steps, smaller training)
+                    within 5% either way
+  full size         surrogate rollout costs at K = 2^18, H = 64 over three
+                    sample ranges: within 1e-3 relative for 99% of samples and
+                    3e-3 for all (four times the horizon of the bars above, so
+                    an activation tie has four times the steps to compound;
+                    measured q99 3.9e-4, max 6.3e-4); the update formed from the
+                    GPU's costs (FP64 cross-block sums at K >= 2^18) within
+                    1e-5 of its largest magnitude of the FP64 oracle's
 """
 import ctypes
 
@@ -23,7 +34,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as ol
-from problems import BICYCLE_ENVELOPE, bicycle
+from problems import BICYCLE_ENVELOPE, bicycle, seq_rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -203,3 +214,66 @@ def test_rejects_bad_calls(require_gpu):
         hip.surrogate_train(**BICYCLE_ENVELOPE, batch=100, steps=1)
     with pytest.raises(MppiHipError):
         hip.surrogate_import(np.zeros(10, np.float32), _identity_norm())
+
+
+@pytest.fixture(scope="module")
+def full_size():
+    """The learned-surrogate workload at its full size: pursuit agent 0,
+    K = 2^18, H = 64, the network trained as the benchmark trains it (4M
+    transitions, 4000 Adam steps of 4096, seed 7)."""
+    hip, ora, sc = bicycle(1 << 18, 64)
+    hip.surrogate_train(**BICYCLE_ENVELOPE, num_transitions=4 << 20, batch=4096, steps=4000,
+                        learning_rate=2e-3, seed=7)
+    return hip, ora, sc
+
+
+def test_surrogate_full_size_spot_checks(require_gpu, full_size):
+    hip, ora, sc = full_size
+    P, norm = hip.surrogate_export()
+    x0 = sc.x0[0]
+    K = 1 << 18
+    u_nom = (np.random.default_rng(8).standard_normal((64, 2)) * [0.1, 1.0]).astype(np.float32)
+    hip.surrogate_use(True)
+    try:
+        u_g, d_g = hip.step(5, x0, u_nom)
+        c_g = hip.read_costs()
+    finally:
+        hip.surrogate_use(False)
+    for k0 in (0, K // 2 + 77, K - 512):
+        c_o = ora.rollout_surrogate_range(5, x0, u_nom, k0, k0 + 512, P, norm)
+        rel = np.abs(c_g[k0:k0 + 512].astype(np.float64) - c_o) / np.maximum(np.abs(c_o), 1e-30)
+        _report(f"full-size surrogate rollout [{k0}, {k0 + 512})", q50=float(np.quantile(rel, 0.5)),
+                q99=float(np.quantile(rel, 0.99)), max=float(rel.max()))
+        assert np.quantile(rel, 0.99) <= 1e-3 and rel.max() <= 3e-3
+    assert np.all(np.isfinite(c_g))
+    # The update from these costs: the GPU sums across workgroups in FP64 at
+    # K >= 2^18; the oracle forms weights and the weighted noise in FP64.
+    w64, d64 = ora.weights(c_g.astype(np.float64), fp64=True)
+    u64 = ora.reduce(5, w64, u_nom.astype(np.float64), fp64=True)
+    err = seq_rel_err(u_g, u64)
+    _report("full-size surrogate update", u_vs_fp64=err, ess_vs_fp64=abs(d_g[2] - d64[2]) / d64[2])
+    assert d_g[0] == np.float32(d64[0])
+    assert err <= 1e-5
+    assert abs(d_g[2] - d64[2]) <= 1e-5 * d64[2]
+
+
+def test_closed_loop_gap_full_size(require_gpu, full_size):
+    """The north-star bar at the benchmark's workload: 500 closed-loop steps
+    from the same start and seed, plant = plugin dynamics; the surrogate's
+    mean realised cost is at most 1% above plain MPPI's."""
+    hip, ora, sc = full_size
+    x0 = sc.x0[0]
+    un = np.zeros((64, 2), np.float32)
+    T = 500
+    st, ct, _, tt = hip.simulate_batch(1, T, 0, x0, un)
+    plain = ora.closed_loop_cost(st[:, 0], ct[:, 0], tt[0])
+    hip.surrogate_use(True)
+    try:
+        st2, ct2, _, tt2 = hip.simulate_batch(1, T, 0, x0, un)
+    finally:
+        hip.surrogate_use(False)
+    sur = ora.closed_loop_cost(st2[:, 0], ct2[:, 0], tt2[0])
+    gap = (float(sur.mean()) - float(plain.mean())) / float(plain.mean())
+    _report("closed loop, full size", plain=float(plain.mean()), surrogate=float(sur.mean()), gap=gap)
+    assert np.all(np.isfinite(sur))
+    assert gap <= 0.01
