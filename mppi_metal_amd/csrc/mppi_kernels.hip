@@ -484,6 +484,7 @@ __device__ void mppi_fused_tile(const mppi_step_args& a, uint32_t agent, uint32_
         // last block inside it (their timesteps are skipped).
         const uint32_t last_b = (H * CDIM - 1) / 4;
         uint32_t ready = 0, zero = 0;
+        bool gave_up = false;
         for (uint32_t t0 = 0; t0 < H; t0 += MPPI_G) {
             const uint32_t g = t0 / MPPI_G;
             const uint32_t need = g / MPPI_PHASE_G + 1 + a.wait_bias;
@@ -494,10 +495,10 @@ __device__ void mppi_fused_tile(const mppi_step_args& a, uint32_t agent, uint32_
                     got = __hip_atomic_load(&s_prog[wave], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 if (got < need) {
                     // Gave up: every later phase counts as ready (the rows
-                    // are read as they are) and the agent is flagged.
-                    if ((threadIdx.x & 63) == 0)
-                        __hip_atomic_fetch_or(&a.agents[slot].status, MPPI_STATUS_WAIT_TIMEOUT, __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
+                    // are read as they are); the agent is flagged after the
+                    // loop (a global atomic in the loop could alias the
+                    // plugin's parameter blob and keep its loads in the loop).
+                    gave_up = true;
                     got = 0x7fffffffu;
                 }
                 ready = got;
@@ -531,6 +532,9 @@ __device__ void mppi_fused_tile(const mppi_step_args& a, uint32_t agent, uint32_
         total += mppi_terminal(mppi_form_timed{}, mppi_cstate_ref{x}, mppi_blob_ref{model},
                                mppi_blob_ref{cost}, mppi_time{H, step});
         if (active) a.costs[d.cost_off + k] = total;
+        if (gave_up && (threadIdx.x & 63) == 0)
+            __hip_atomic_fetch_or(&a.agents[slot].status, MPPI_STATUS_WAIT_TIMEOUT, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
     }
     __builtin_amdgcn_s_setprio(0);
     mppi_fused_stats<SAMPLES, THREADS>(a, agent, d, blk, active, total, s_z);
