@@ -122,6 +122,27 @@ int mppi_hip_step(mppi_hip_ctx* ctx, uint64_t step_index, const float* x0,
                   const void* cost_override, size_t cost_override_bytes, float* u_out,
                   float* diag3);
 
+/* mppi_hip_step without allocation: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid it).  No counterpart in the reference.  Inputs and outputs are
+ * device pointers owned by the caller: x0_dev (state_dim floats),
+ * u_nominal_dev (horizon*control_dim floats, read only), u_out_dev
+ * (horizon*control_dim floats), diag3_dev (3 floats, may be NULL); the
+ * bound parameter blobs are used (no overrides).  The step's work buffers
+ * (costs, workgroup statistics, partial sums, descriptor, agent state) live
+ * in a file-scope device array of the library, one per device, at most
+ * 8 MiB (K = 65536 at horizon*control_dim = 128 needs about 0.4 MiB);
+ * sizes beyond it fail with MPPI_HIP_E_INVALID.  The call issues no
+ * hipMalloc, hipFree or other allocating routine.  stream: a hipStream_t
+ * to enqueue on (the call then returns without waiting and without the
+ * wait check of mppi_hip_set_wait_limit), or NULL for the context's stream,
+ * waiting for the result like mppi_hip_step.  Every call on one device
+ * shares that device's array and overwrites the previous call's contents:
+ * calls that overlap on one device (concurrent streams or threads) race.
+ * Afterwards mppi_hip_read_costs / read_weights report no iteration. */
+int mppi_hip_step_noalloc(mppi_hip_ctx* ctx, uint64_t step_index, const float* x0_dev,
+                          const float* u_nominal_dev, float* u_out_dev, float* diag3_dev, void* stream);
+
 /* num_steps closed-loop steps on the device: each step runs the MPPI
  * iteration, logs (state, first optimal control, best cost), steps the plant
  * with the plugin dynamics and shifts the warm start (hold last).  The noise

@@ -112,6 +112,7 @@ SIGNATURES = [
     ("mppi_hip_bind_params", ctypes.c_int, [_CTX, _VP, ctypes.c_size_t, _VP, ctypes.c_size_t]),
     ("mppi_hip_step", ctypes.c_int,
      [_CTX, ctypes.c_uint64, _F, _F, _VP, ctypes.c_size_t, _VP, ctypes.c_size_t, _F, _F]),
+    ("mppi_hip_step_noalloc", ctypes.c_int, [_CTX, ctypes.c_uint64, _VP, _VP, _VP, _VP, _VP]),
     ("mppi_hip_simulate", ctypes.c_int,
      [_CTX, ctypes.c_uint32, ctypes.c_uint64, _F, _F, _VP, ctypes.c_size_t, _VP, ctypes.c_size_t,
       _F, _F, _F]),

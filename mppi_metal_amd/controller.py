@@ -132,6 +132,13 @@ class HipMPPI:
                                            cp, cn, _fptr(u_out), _fptr(diag)))
         return u_out.reshape(self.H, self.C), diag
 
+    def step_noalloc(self, step_index, x0_dev, u_nominal_dev, u_out_dev, diag_dev=0, stream=0):
+        """mppi_hip_step_noalloc: device addresses (integers) of the inputs
+        and outputs; stream 0 waits on the context's stream, any other value
+        is a hipStream_t to enqueue on."""
+        self._check(self.lib.mppi_hip_step_noalloc(self.ctx, int(step_index), x0_dev or None, u_nominal_dev or None,
+                                                   u_out_dev or None, diag_dev or None, stream or None))
+
     def simulate(self, num_steps, start_step, x0, u_nominal, model_override=None,
                  cost_override=None, outputs=("states", "controls", "costs")):
         x0 = _f32(x0, (self.D,))

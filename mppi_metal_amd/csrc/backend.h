@@ -19,6 +19,8 @@ constexpr uint32_t kBlock = 256;      // workgroup size of the grid-mapped kerne
 constexpr uint32_t kReduceBPG = 8;    // noise blocks per reduce tile
 constexpr uint32_t kMaxHC = 4096;     // horizon * control_dim limit
 constexpr uint32_t kFusedMaxHC = 128; // horizon * control_dim limit of the one-pass step
+// Size of the per-device scratch array of mppi_hip_step_noalloc (noalloc.hip).
+constexpr size_t kNoallocScratchBytes = size_t(8) << 20;
 
 // One loaded code object and the kernels looked up in it.
 struct LoadedModule {
@@ -136,6 +138,14 @@ void collect_profile(mppi_hip_ctx* ctx);
 void fill_common_args(mppi_hip_ctx* ctx, mppi_step_args* a);
 // MPPI_HIP_E_DEVICE when any of the n status words reports a failed device wait.
 int check_status(mppi_hip_ctx* ctx, const uint32_t* status, uint32_t n);
+// noalloc.hip: take the device address of the scratch array of `device` (the
+// device must be current; once per device), read it back (null with the
+// reason in *why when it could not be taken), and enqueue the kernel that
+// initialises a call's descriptor, agent state and status word.
+void noalloc_setup(int device);
+void* noalloc_scratch(int device, std::string* why);
+hipError_t noalloc_init(hipStream_t stream, const mppi_agent_desc& d, uint32_t step, mppi_agent_desc* desc_out,
+                        mppi_agent_state* agent_out, uint32_t* status_out);
 
 }  // namespace mppi_hip_impl
 

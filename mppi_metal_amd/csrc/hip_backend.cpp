@@ -426,8 +426,8 @@ uint32_t default_spt(const mppi_hip_ctx* ctx, uint32_t K, uint32_t n_groups, uin
     return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(64, spt)));
 }
 
-// Descriptors of N agents sharing the context's K and H (grid-mapped path).
-std::vector<mppi_agent_desc> uniform_descs(const mppi_hip_ctx* ctx, uint32_t N) {
+// Shape of agent 0 of N agents sharing the context's K and H (offsets 0).
+mppi_agent_desc uniform_desc(const mppi_hip_ctx* ctx, uint32_t N) {
     mppi_agent_desc d{};
     d.K = ctx->K;
     d.H = ctx->H;
@@ -436,6 +436,12 @@ std::vector<mppi_agent_desc> uniform_descs(const mppi_hip_ctx* ctx, uint32_t N) 
     d.n_groups = (d.NB + kReduceBPG - 1) / kReduceBPG;
     d.spt = default_spt(ctx, ctx->K, d.n_groups, N);
     d.n_chunks = (ctx->K + kBlock * d.spt - 1) / (kBlock * d.spt);
+    return d;
+}
+
+// Descriptors of N agents sharing the context's K and H (grid-mapped path).
+std::vector<mppi_agent_desc> uniform_descs(const mppi_hip_ctx* ctx, uint32_t N) {
+    const mppi_agent_desc d = uniform_desc(ctx, N);
     std::vector<mppi_agent_desc> v(N, d);
     for (uint32_t a = 0; a < N; ++a) {
         v[a].slot = a;
@@ -559,41 +565,45 @@ bool use_fused(const mppi_hip_ctx* ctx, const mppi_agent_desc& d, uint32_t N, bo
 // With a start and stop event (profiling), the runtime attaches them to the
 // dispatch itself, so the pair times the kernel alone, as a kernel trace
 // does, rather than the event packets and the launch gap around it.
-hipError_t launch_kernel(mppi_hip_ctx* ctx, hipFunction_t f, uint32_t gx, uint32_t gy, uint32_t gz,
+hipError_t launch_kernel(hipStream_t stream, hipFunction_t f, uint32_t gx, uint32_t gy, uint32_t gz,
                          void** params, hipEvent_t start, hipEvent_t stop, uint32_t block = kBlock) {
     if (start && stop)
-        return hipExtModuleLaunchKernel(f, gx * block, gy, gz, block, 1, 1, 0, ctx->stream, params, nullptr,
+        return hipExtModuleLaunchKernel(f, gx * block, gy, gz, block, 1, 1, 0, stream, params, nullptr,
                                         start, stop, 0);
-    return hipModuleLaunchKernel(f, gx, gy, gz, block, 1, 1, 0, ctx->stream, params, nullptr);
+    return hipModuleLaunchKernel(f, gx, gy, gz, block, 1, 1, 0, stream, params, nullptr);
 }
 
-// Enqueue one MPPI iteration for N agents: grid-mapped rollout + reduce, or
-// the one-pass step.
-int launch_iteration(mppi_hip_ctx* ctx, const std::vector<mppi_agent_desc>& desc,
-                     const mppi_step_args& a) {
-    const uint32_t N = uint32_t(desc.size());
-    const mppi_agent_desc& d = desc[0];
+// Enqueue one MPPI iteration for N agents of shape d (grid-mapped rollout +
+// reduce, or the one-pass step) on `stream` (the context's when null).
+// Without `bookkeep` nothing else is touched: no profiling events are taken
+// and the context's pending per-sample weights are left alone.
+int launch_iteration(mppi_hip_ctx* ctx, const mppi_agent_desc& d, uint32_t N, const mppi_step_args& a,
+                     hipStream_t stream = nullptr, bool bookkeep = true) {
+    hipStream_t st = stream ? stream : ctx->stream;
     mppi_step_args args = a;
     void* params[] = {&args};
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
-    if (ctx->profiling) {
+    const bool prof = bookkeep && ctx->profiling;
+    if (prof) {
         e0 = take_event(ctx); e1 = take_event(ctx); e2 = take_event(ctx); e3 = take_event(ctx);
     }
     alignas(16) unsigned char sur_args[1024];
     hipFunction_t sur_fn = nullptr;
     const bool sur = surrogate_rollout(ctx, args, &sur_fn, sur_args, sizeof(sur_args));
-    ctx->weights_pending = false;
+    if (bookkeep) ctx->weights_pending = false;
     if (use_fused(ctx, d, N, sur)) {
         // two threads per sample: noise producers and rollout consumers
-        MPPI_HIP_TRY(ctx, launch_kernel(ctx, ctx->mod->rollout_fused, d.n_blk, N, 1, params, e0, e1, 2 * kBlock));
+        MPPI_HIP_TRY(ctx, launch_kernel(st, ctx->mod->rollout_fused, d.n_blk, N, 1, params, e0, e1, 2 * kBlock));
         // the combine's workgroups: slices of 16 / C timesteps
         const uint32_t tpw = ctx->C < 16 ? 16 / ctx->C : 1;
-        MPPI_HIP_TRY(ctx, launch_kernel(ctx, ctx->mod->fused_combine, (d.H + tpw - 1) / tpw, N, 1, params, e2, e3));
-        ctx->weights_pending = true;
-        ctx->weights_args = args;
-        ctx->weights_grid_x = (d.K + kBlock - 1) / kBlock;
-        ctx->weights_grid_y = N;
-        if (ctx->profiling) {
+        MPPI_HIP_TRY(ctx, launch_kernel(st, ctx->mod->fused_combine, (d.H + tpw - 1) / tpw, N, 1, params, e2, e3));
+        if (bookkeep) {
+            ctx->weights_pending = true;
+            ctx->weights_args = args;
+            ctx->weights_grid_x = (d.K + kBlock - 1) / kBlock;
+            ctx->weights_grid_y = N;
+        }
+        if (prof) {
             if (e0 && e1) ctx->pending_rollout.push_back({e0, e1});
             if (e2 && e3) ctx->pending_reduce.push_back({e2, e3});
             ctx->prof_launches += 1;
@@ -602,12 +612,12 @@ int launch_iteration(mppi_hip_ctx* ctx, const std::vector<mppi_agent_desc>& desc
     }
     if (sur) {
         void* sparams[] = {sur_args};
-        MPPI_HIP_TRY(ctx, launch_kernel(ctx, sur_fn, d.n_blk, N, 1, sparams, e0, e1));
+        MPPI_HIP_TRY(ctx, launch_kernel(st, sur_fn, d.n_blk, N, 1, sparams, e0, e1));
     } else {
-        MPPI_HIP_TRY(ctx, launch_kernel(ctx, ctx->mod->rollout, d.n_blk, N, 1, params, e0, e1));
+        MPPI_HIP_TRY(ctx, launch_kernel(st, ctx->mod->rollout, d.n_blk, N, 1, params, e0, e1));
     }
-    MPPI_HIP_TRY(ctx, launch_kernel(ctx, ctx->mod->reduce, d.n_chunks, d.n_groups, N, params, e2, e3));
-    if (ctx->profiling) {
+    MPPI_HIP_TRY(ctx, launch_kernel(st, ctx->mod->reduce, d.n_chunks, d.n_groups, N, params, e2, e3));
+    if (prof) {
         if (e0 && e1) ctx->pending_rollout.push_back({e0, e1});
         if (e2 && e3) ctx->pending_reduce.push_back({e2, e3});
         ctx->prof_launches += 1;
@@ -673,7 +683,7 @@ int run_closed_loop(mppi_hip_ctx* ctx, uint32_t N, uint32_t T, uint64_t start_st
     for (uint32_t s = 0; s < T; ++s) {
         a.u_nom = warm[s & 1u];
         a.u_nom_next = warm[(s + 1) & 1u];
-        if ((rc = launch_iteration(ctx, desc, a))) return rc;
+        if ((rc = launch_iteration(ctx, desc[0], uint32_t(desc.size()), a))) return rc;
     }
     if (states_out)
         MPPI_HIP_TRY(ctx, hipMemcpyAsync(states_out, ctx->hist_states.ptr, sizeof(float) * T * N * D,
@@ -812,6 +822,9 @@ int mppi_hip_create(const mppi_hip_config* cfg, mppi_hip_ctx** out) {
     ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess)
         return fail(nullptr, MPPI_HIP_E_DEVICE, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
+    // Device address of the no-allocation entry point's scratch array on
+    // this device (once per device, with the device current).
+    noalloc_setup(dev);
 
     // Constants: noise scale | u_min | u_max | 256-byte zero blob.
     const size_t ns = ctx->noise_scale.size();
@@ -962,7 +975,7 @@ int mppi_hip_step(mppi_hip_ctx* ctx, uint64_t step_index, const float* x0, const
     UploadPlan plan;
     if ((rc = upload(ctx, desc, step_index, x0, u_nominal, mo, mo_bytes, co, co_bytes, &plan))) return rc;
     mppi_step_args a = plain_args(ctx, plan, 1, mo, mo_bytes, co, co_bytes);
-    if ((rc = launch_iteration(ctx, desc, a))) return rc;
+    if ((rc = launch_iteration(ctx, desc[0], uint32_t(desc.size()), a))) return rc;
     const size_t hc = size_t(ctx->H) * ctx->C;
     MPPI_HIP_TRY(ctx, hipMemcpyAsync(ctx->down_host.ptr, ctx->down.ptr, sizeof(float) * (hc + 4),
                                      hipMemcpyDeviceToHost, ctx->stream));
@@ -976,6 +989,59 @@ int mppi_hip_step(mppi_hip_ctx* ctx, uint64_t step_index, const float* x0, const
     ctx->last_weights = static_cast<const float*>(ctx->weights.ptr);
     ctx->last_desc = desc;
     return MPPI_HIP_OK;
+}
+
+int mppi_hip_step_noalloc(mppi_hip_ctx* ctx, uint64_t step_index, const float* x0_dev, const float* u_nominal_dev,
+                          float* u_out_dev, float* diag3_dev, void* stream) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!x0_dev || !u_nominal_dev || !u_out_dev)
+        return fail(ctx, MPPI_HIP_E_INVALID, "x0, u_nominal and u_out must not be null.");
+    std::string why;
+    uint8_t* base = static_cast<uint8_t*>(noalloc_scratch(ctx->device, &why));
+    if (!base) return fail(ctx, MPPI_HIP_E_DEVICE, "mppi_hip_step_noalloc: no scratch on this device: " + why);
+    // Scratch layout: descriptor | agent state | status word | costs |
+    // workgroup statistics | partial sums (offsets 256-byte aligned).
+    const mppi_agent_desc shape = uniform_desc(ctx, 1);
+    const size_t off_agent = 256, off_status = 512, off_costs = 768;
+    const size_t off_stats = off_costs + align_up(sizeof(float) * shape.K, 256);
+    const size_t off_part = off_stats + align_up(sizeof(float) * 4 * shape.n_blk, 256);
+    const size_t part_rows = std::max(shape.n_chunks, shape.n_blk);
+    const size_t total = off_part + align_up(sizeof(float) * part_rows * shape.NB * 4, 256);
+    if (total > kNoallocScratchBytes)
+        return fail(ctx, MPPI_HIP_E_INVALID,
+                    "mppi_hip_step_noalloc: sample_count and horizon need " + std::to_string(total) +
+                        " bytes of scratch; the limit is " + std::to_string(kNoallocScratchBytes) +
+                        " bytes (8 MiB) per device.");
+    mppi_step_args a{};
+    fill_common_args(ctx, &a);
+    a.desc = reinterpret_cast<const mppi_agent_desc*>(base);
+    a.agents = reinterpret_cast<mppi_agent_state*>(base + off_agent);
+    a.status_out = reinterpret_cast<uint32_t*>(base + off_status);
+    a.N = 1;
+    a.x = const_cast<float*>(x0_dev);              // read only: open loop
+    a.u_nom = const_cast<float*>(u_nominal_dev);   // read only: open loop
+    a.u_out = u_out_dev;
+    a.diag = diag3_dev;
+    a.model = resolve_blob(ctx, nullptr, 0, ctx->model_bound, ctx->model_is_bound, 0);
+    a.cost = resolve_blob(ctx, nullptr, 0, ctx->cost_bound, ctx->cost_is_bound, 0);
+    a.costs = reinterpret_cast<float*>(base + off_costs);
+    a.weights = nullptr;
+    a.blk_stats = reinterpret_cast<float*>(base + off_stats);
+    a.partials = reinterpret_cast<float*>(base + off_part);
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    MPPI_HIP_TRY(ctx, noalloc_init(st, shape, uint32_t(step_index), const_cast<mppi_agent_desc*>(a.desc), a.agents,
+                                   a.status_out));
+    if ((rc = launch_iteration(ctx, shape, 1, a, st, false))) return rc;
+    // read_costs / read_weights refer to the last call that used the
+    // context's own buffers; after this call they report no iteration.
+    ctx->last_agents = 0;
+    ctx->weights_pending = false;
+    if (stream) return MPPI_HIP_OK;
+    uint32_t status = 0;
+    MPPI_HIP_TRY(ctx, hipMemcpyAsync(&status, a.status_out, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    MPPI_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return check_status(ctx, &status, 1);
 }
 
 int mppi_hip_simulate(mppi_hip_ctx* ctx, uint32_t num_steps, uint64_t start_step, const float* x0,
