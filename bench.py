@@ -316,10 +316,12 @@ def bench_serving(args):
     return out
 
 
-def bench_surrogate(args):
-    """Learned bicycle dynamics in the rollout vs plugin dynamics."""
+def bench_surrogate(args, substeps=2):
+    """Learned bicycle dynamics in the rollout vs plugin dynamics (the
+    bicycle integrated with `substeps` Euler steps per MPPI step)."""
     from mppi_metal_amd import HipMPPI, scenes
     sc = scenes.bicycle_pursuit_scene(1, 42)
+    sc.model.substeps = substeps
     mb, cb = sc.model.blob(), sc.cost_blobs[0]
     K, H, T = 262144, 64, args.surrogate_steps
     hip = HipMPPI(6, 2, H, K, sc.lam, sc.sigma, sc.u_min, sc.u_max)
@@ -360,8 +362,8 @@ def bench_surrogate(args):
     sur = res["surrogate"]
     return {
         "workload": "learned dynamic-bicycle surrogate (MLP 8-128-128-6 tanh, bf16 MFMA, fp32 accumulate), "
-                    "pursuit, closed loop",
-        "K": K, "H": H, "closed_loop_steps": T,
+                    f"pursuit, closed loop, plugin dynamics with {substeps} Euler substeps per step",
+        "K": K, "H": H, "closed_loop_steps": T, "substeps": substeps,
         "train": {"transitions": 4 << 20, "batch": 4096, "steps": args.train_steps, "seconds": train_s,
                   "loss_first": float(hist[0]), "loss_last": float(hist[-20:].mean()),
                   "one_step_relative_rmse": [float(v) for v in rel]},
@@ -509,6 +511,9 @@ def main():
 
     batch = None if args.no_batch else bench_batch(args)
     surrogate = None if args.no_surrogate else bench_surrogate(args)
+    # The same workload on dynamics that cost more than the network (64
+    # Euler substeps per step): where the surrogate pays.
+    surrogate_costly = None if args.no_surrogate else bench_surrogate(args, substeps=64)
     serving = None if args.no_serving else bench_serving(args)
 
     out = {
@@ -553,6 +558,7 @@ def main():
         "final_closed_loop_cost": float(kt[-1]),
         "batch": batch,
         "surrogate": surrogate,
+        "surrogate_costly_dynamics": surrogate_costly,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

@@ -92,8 +92,10 @@ struct mppi_hip_ctx {
     mppi_hip_impl::DeviceBuffer up;      // desc | agents | x | u_nom | model ovr | cost ovr
     mppi_hip_impl::DeviceBuffer down;    // u_out | diag
     mppi_hip_impl::DeviceBuffer costs, blk_stats, weights, partials;
-    mppi_hip_impl::DeviceBuffer hist_states, hist_controls, hist_costs, terminal;
-    mppi_hip_impl::PinnedBuffer up_host, down_host;
+    // closed-loop results: states | controls | best costs | end states |
+    // status words, in one device block copied back in one transfer
+    mppi_hip_impl::DeviceBuffer hist;
+    mppi_hip_impl::PinnedBuffer up_host, down_host, hist_host;
     uint32_t last_agents = 0;
     // where read_costs / read_weights find the last iteration's data
     const float* last_costs = nullptr;

@@ -612,7 +612,8 @@ int launch_iteration(mppi_hip_ctx* ctx, const mppi_agent_desc& d, uint32_t N, co
     }
     if (sur) {
         void* sparams[] = {sur_args};
-        MPPI_HIP_TRY(ctx, launch_kernel(st, sur_fn, d.n_blk, N, 1, sparams, e0, e1));
+        // 512-sample workgroups (mppi_rollout_surrogate)
+        MPPI_HIP_TRY(ctx, launch_kernel(st, sur_fn, (d.K + 511) / 512, N, 1, sparams, e0, e1, 512));
     } else {
         MPPI_HIP_TRY(ctx, launch_kernel(st, ctx->mod->rollout, d.n_blk, N, 1, params, e0, e1));
     }
@@ -664,18 +665,26 @@ int run_closed_loop(mppi_hip_ctx* ctx, uint32_t N, uint32_t T, uint64_t start_st
     std::vector<mppi_agent_desc> desc = uniform_descs(ctx, N);
     if ((rc = ensure_work_buffers(ctx, desc))) return rc;
     const size_t D = ctx->D, C = ctx->C;
-    if (states_out && (rc = ensure_device_buffer(ctx, &ctx->hist_states, sizeof(float) * T * N * D))) return rc;
-    if (controls_out && (rc = ensure_device_buffer(ctx, &ctx->hist_controls, sizeof(float) * T * N * C))) return rc;
-    if (costs_out && (rc = ensure_device_buffer(ctx, &ctx->hist_costs, sizeof(float) * T * N))) return rc;
-    if (terminal_out && (rc = ensure_device_buffer(ctx, &ctx->terminal, sizeof(float) * N * D))) return rc;
+    // One block for everything the loop returns, so it comes back in one
+    // transfer (through pinned memory) however many outputs are asked for.
+    const size_t o_ctl = align_up(sizeof(float) * T * N * D, 256);
+    const size_t o_cost = o_ctl + align_up(sizeof(float) * T * N * C, 256);
+    const size_t o_term = o_cost + align_up(sizeof(float) * T * N, 256);
+    const size_t o_stat = o_term + align_up(sizeof(float) * N * D, 256);
+    const size_t hist_bytes = o_stat + sizeof(uint32_t) * N;
+    if ((rc = ensure_device_buffer(ctx, &ctx->hist, hist_bytes))) return rc;
+    if ((rc = ensure_pinned_buffer(ctx, &ctx->hist_host, hist_bytes))) return rc;
+    uint8_t* hd = static_cast<uint8_t*>(ctx->hist.ptr);
+    const uint8_t* hh = static_cast<const uint8_t*>(ctx->hist_host.ptr);
     UploadPlan plan;
     if ((rc = upload(ctx, desc, start_step, x0, u_nom, mo, mo_bytes, co, co_bytes, &plan))) return rc;
     mppi_step_args a = plain_args(ctx, plan, N, mo, mo_bytes, co, co_bytes);
     a.closed_loop = 1;
-    a.hist_states = states_out ? static_cast<float*>(ctx->hist_states.ptr) : nullptr;
-    a.hist_controls = controls_out ? static_cast<float*>(ctx->hist_controls.ptr) : nullptr;
-    a.hist_costs = costs_out ? static_cast<float*>(ctx->hist_costs.ptr) : nullptr;
-    a.terminal = terminal_out ? static_cast<float*>(ctx->terminal.ptr) : nullptr;
+    a.hist_states = states_out ? reinterpret_cast<float*>(hd) : nullptr;
+    a.hist_controls = controls_out ? reinterpret_cast<float*>(hd + o_ctl) : nullptr;
+    a.hist_costs = costs_out ? reinterpret_cast<float*>(hd + o_cost) : nullptr;
+    a.terminal = terminal_out ? reinterpret_cast<float*>(hd + o_term) : nullptr;
+    a.status_out = reinterpret_cast<uint32_t*>(hd + o_stat);
     // The warm start alternates between two device buffers: step s reads one
     // and its update writes the shifted sequence into the other.
     uint8_t* up = static_cast<uint8_t*>(ctx->up.ptr);
@@ -685,23 +694,19 @@ int run_closed_loop(mppi_hip_ctx* ctx, uint32_t N, uint32_t T, uint64_t start_st
         a.u_nom_next = warm[(s + 1) & 1u];
         if ((rc = launch_iteration(ctx, desc[0], uint32_t(desc.size()), a))) return rc;
     }
-    if (states_out)
-        MPPI_HIP_TRY(ctx, hipMemcpyAsync(states_out, ctx->hist_states.ptr, sizeof(float) * T * N * D,
-                                         hipMemcpyDeviceToHost, ctx->stream));
-    if (controls_out)
-        MPPI_HIP_TRY(ctx, hipMemcpyAsync(controls_out, ctx->hist_controls.ptr, sizeof(float) * T * N * C,
-                                         hipMemcpyDeviceToHost, ctx->stream));
-    if (costs_out)
-        MPPI_HIP_TRY(ctx, hipMemcpyAsync(costs_out, ctx->hist_costs.ptr, sizeof(float) * T * N,
-                                         hipMemcpyDeviceToHost, ctx->stream));
-    if (terminal_out)
-        MPPI_HIP_TRY(ctx, hipMemcpyAsync(terminal_out, ctx->terminal.ptr, sizeof(float) * N * D,
-                                         hipMemcpyDeviceToHost, ctx->stream));
-    uint32_t* status = static_cast<uint32_t*>(ctx->down_host.ptr);
-    MPPI_HIP_TRY(ctx, hipMemcpyAsync(status, a.status_out, sizeof(uint32_t) * N, hipMemcpyDeviceToHost,
+    // The requested parts and the status words, in one transfer: from the
+    // first requested part to the end of the block.
+    const size_t from = states_out ? 0 : controls_out ? o_ctl : costs_out ? o_cost : terminal_out ? o_term : o_stat;
+    MPPI_HIP_TRY(ctx, hipMemcpyAsync(ctx->hist_host.ptr, hd + from, hist_bytes - from, hipMemcpyDeviceToHost,
                                      ctx->stream));
     if ((rc = finish(ctx))) return rc;
-    if ((rc = check_status(ctx, status, N))) return rc;
+    // part at offset o (o >= from) of the device block
+    auto at = [&](size_t o) { return hh + (o - from); };
+    if ((rc = check_status(ctx, reinterpret_cast<const uint32_t*>(at(o_stat)), N))) return rc;
+    if (states_out) std::memcpy(states_out, at(0), sizeof(float) * T * N * D);
+    if (controls_out) std::memcpy(controls_out, at(o_ctl), sizeof(float) * T * N * C);
+    if (costs_out) std::memcpy(costs_out, at(o_cost), sizeof(float) * T * N);
+    if (terminal_out) std::memcpy(terminal_out, at(o_term), sizeof(float) * N * D);
     ctx->last_agents = N;
     ctx->last_costs = static_cast<const float*>(ctx->costs.ptr);
     ctx->last_weights = static_cast<const float*>(ctx->weights.ptr);
@@ -847,11 +852,11 @@ void mppi_hip_destroy(mppi_hip_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (DeviceBuffer* b : {&ctx->consts, &ctx->model_bound, &ctx->cost_bound, &ctx->up, &ctx->down,
-                            &ctx->costs, &ctx->blk_stats, &ctx->weights, &ctx->partials,
-                            &ctx->hist_states, &ctx->hist_controls, &ctx->hist_costs, &ctx->terminal})
+                            &ctx->costs, &ctx->blk_stats, &ctx->weights, &ctx->partials, &ctx->hist})
         b->release();
     ctx->up_host.release();
     ctx->down_host.release();
+    ctx->hist_host.release();
     collect_profile(ctx);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -1358,37 +1358,51 @@ extern "C" __global__ void __launch_bounds__(256) mppi_surrogate_data(mppi_surro
     for (int k = 0; k < SDIM; ++k) a.t[size_t(i) * SDIM + k] = xs[k] - v[k];
 }
 
-// Rollout with the MLP as dynamics (grid-mapped like mppi_rollout; the
-// reduce kernel is shared).  Per timestep each wave stages its 64 samples'
-// normalised (state, control) in its own LDS rows, evaluates the network on
-// four 16-sample blocks with the matrix cores (weight fragments in LDS), and
-// adds the denormalised delta to each lane's state; costs are the plugin's.
-// The staging rows are private to a wave, whose LDS operations complete in
-// order, so the timestep needs wave-level ordering only (no workgroup
-// barrier).  Two workgroups per CU (two waves per SIMD) bound the registers.
-extern "C" __global__ void __launch_bounds__(256, 2) mppi_rollout_surrogate(mppi_surrogate_rollout_args sa) {
+// Rollout with the MLP as dynamics (one sample per lane, 512-thread
+// workgroups, grid (ceil(K / 512), N); the reduce kernel is shared).  Per
+// timestep every wavefront evaluates the network for its 64 samples as four
+// 16-sample column blocks:
+//   layer 1   v_mfma_f32_16x16x16_bf16, one per (m-block, column block), the
+//             inputs staged per sample in the wavefront's LDS rows;
+//   layer 2   v_mfma_f32_16x16x32_bf16: each A fragment is read from LDS
+//             once and applied to the four column blocks; its B operands are
+//             the layer-1 results a lane already holds (mppi_mlp.h);
+//   layer 3   the same, accumulated as layer 2's m-blocks come out in pairs,
+//             so that only two m-blocks of the second hidden layer are live;
+// activations through the LDS table (mppi_act2).  The staging rows are
+// private to a wavefront, whose LDS operations complete in order, so a
+// timestep needs wavefront-level ordering only (no workgroup barrier).
+constexpr int MPPI_SUR_BLOCK = 512;
+
+extern "C" __global__ void __launch_bounds__(MPPI_SUR_BLOCK) mppi_rollout_surrogate(mppi_surrogate_rollout_args sa) {
     constexpr int IN = SDIM + CDIM;
-    constexpr int NFRAG = MPPI_MLP_FRAG_W2T;   // W1, W2, W3 fragments
-    __shared__ short s_frag[NFRAG * MPPI_MLP_FRAG_ELEMS];
+    constexpr int WAVES = MPPI_SUR_BLOCK / 64;
+    static_assert(IN <= 16 && SDIM <= 16, "surrogate network: inputs and outputs of at most 16");
+    __shared__ __attribute__((aligned(16))) short s_w1[8 * MPPI_MLP_FRAG_ELEMS];
+    __shared__ __attribute__((aligned(16))) short s_wx[MPPI_MLP_X32_FRAGS * MPPI_MLP_X32_ELEMS];
+    __shared__ unsigned short s_tab[MPPI_MLP_TANH_KEYS];
     __shared__ __attribute__((aligned(16))) float s_b[2 * MPPI_MLP_HID + 16];
-    __shared__ float s_io[4][64][IN + 1];
-    __shared__ float s_dx[4][64][SDIM + 1];
+    __shared__ __attribute__((aligned(16))) short s_in[WAVES][64][16];
+    __shared__ float s_dx[WAVES][64][SDIM + 1];
     const mppi_step_args& a = sa.s;
     const uint32_t agent = blockIdx.y;
     const mppi_agent_desc d = a.desc[agent];
-    if (blockIdx.x >= d.n_blk) return;
-    for (uint32_t e = threadIdx.x; e < NFRAG * MPPI_MLP_FRAG_ELEMS / 4; e += 256)
-        reinterpret_cast<uint2*>(s_frag)[e] = reinterpret_cast<const uint2*>(sa.frags)[e];
-    for (uint32_t e = threadIdx.x; e < MPPI_MLP_HID; e += 256) {
+    if (blockIdx.x * MPPI_SUR_BLOCK >= d.K) return;
+    for (uint32_t e = threadIdx.x; e < 8 * MPPI_MLP_FRAG_ELEMS / 8; e += MPPI_SUR_BLOCK)
+        reinterpret_cast<uint4*>(s_w1)[e] = reinterpret_cast<const uint4*>(sa.frags + MPPI_MLP_FRAG_W1 * MPPI_MLP_FRAG_ELEMS)[e];
+    for (uint32_t e = threadIdx.x; e < MPPI_MLP_X32_FRAGS * MPPI_MLP_X32_ELEMS / 8; e += MPPI_SUR_BLOCK)
+        reinterpret_cast<uint4*>(s_wx)[e] = reinterpret_cast<const uint4*>(sa.frags + MPPI_MLP_X32_BASE)[e];
+    for (uint32_t e = threadIdx.x; e < MPPI_MLP_TANH_KEYS; e += MPPI_SUR_BLOCK) s_tab[e] = sa.tanh_table[e];
+    for (uint32_t e = threadIdx.x; e < MPPI_MLP_HID; e += MPPI_SUR_BLOCK) {
         s_b[e] = sa.params[sa.b1 + e];
         s_b[MPPI_MLP_HID + e] = sa.params[sa.b2 + e];
     }
-    if (threadIdx.x < SDIM) s_b[2 * MPPI_MLP_HID + threadIdx.x] = sa.params[sa.b3 + threadIdx.x];
+    if (threadIdx.x < 16) s_b[2 * MPPI_MLP_HID + threadIdx.x] = threadIdx.x < SDIM ? sa.params[sa.b3 + threadIdx.x] : 0.0f;
     __syncthreads();
 
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
     const uint32_t blk = blockIdx.x;
-    const uint32_t k = blk * 256 + threadIdx.x;
+    const uint32_t k = blk * MPPI_SUR_BLOCK + threadIdx.x;
     const uint32_t H = d.H;
     const bool active = k < d.K;
     const uint32_t slot = d.slot;
@@ -1397,6 +1411,7 @@ extern "C" __global__ void __launch_bounds__(256, 2) mppi_rollout_surrogate(mppi
     const float* __restrict__ un = a.u_nom + d.seq_off;
     const uint8_t* model = a.model + slot * a.model_stride;
     const uint8_t* cost = a.cost + slot * a.cost_stride;
+    const mppi_v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
 
     float x[SDIM];
 #pragma unroll
@@ -1417,37 +1432,75 @@ extern "C" __global__ void __launch_bounds__(256, 2) mppi_rollout_surrogate(mppi
             if (t < H) {
                 float u[CDIM];
                 mppi_make_control(un + t * CDIM, z + gi * CDIM, a.noise_scale, a.u_min, a.u_max, u);
+                // this lane's sample: normalised inputs as bf16, zeros past IN
+                mppi_v4f xin[4];
 #pragma unroll
-                for (int i = 0; i < SDIM; ++i)
-                    s_io[wave][lane][i] = mppi_mlp_input(x[i], sa.in_center[i], sa.in_inv_half[i], sa.in_period[i]);
-#pragma unroll
-                for (int i = 0; i < CDIM; ++i)
-                    s_io[wave][lane][SDIM + i] = mppi_mlp_input(u[i], sa.in_center[SDIM + i], sa.in_inv_half[SDIM + i],
-                                                                sa.in_period[SDIM + i]);
-                __builtin_amdgcn_wave_barrier();
-                // The four 16-sample blocks of the wave one after the other
-                // (unrolling them would only multiply the live registers).
-#pragma unroll 1
-                for (int nb = 0; nb < 4; ++nb) {
-                    // An offset the compiler cannot see through keeps the
-                    // fragment reads inside this loop; hoisted out of it, the
-                    // 80 fragments would hold 160 registers per lane.
-                    uint32_t fo = 0;
-                    __asm__ volatile("" : "+s"(fo));
-                    mppi_v4f xv;
+                for (int q = 0; q < 4; ++q)
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        const int kk = 4 * g + j;
-                        xv[j] = kk < IN ? s_io[wave][16 * nb + c][kk < IN ? kk : 0] : 0.0f;
+                        const int kk = 4 * q + j;
+                        xin[q][j] = kk < SDIM ? mppi_mlp_input(x[kk < SDIM ? kk : 0], sa.in_center[kk], sa.in_inv_half[kk],
+                                                               sa.in_period[kk])
+                                  : kk < IN ? mppi_mlp_input(u[kk - SDIM < CDIM ? kk - SDIM : 0], sa.in_center[kk],
+                                                             sa.in_inv_half[kk], sa.in_period[kk])
+                                            : 0.0f;
                     }
-                    mppi_v4f y = mppi_mlp_block(s_frag + fo, s_b, s_b + MPPI_MLP_HID, mppi_pack_bf16(xv), lane,
-                                                nullptr, nullptr);
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    *reinterpret_cast<mppi_v4bf*>(&s_in[wave][lane][4 * q]) = mppi_pack_bf16(xin[q]);
+                __builtin_amdgcn_wave_barrier();
+                // layer 1: h1[nb][mb] = T(W1 x + b1) for column block nb
+                mppi_v4bf h1[4][8];
+#pragma unroll
+                for (int nb = 0; nb < 4; ++nb) {
+                    const mppi_v4bf xb = *reinterpret_cast<const mppi_v4bf*>(&s_in[wave][16 * nb + c][4 * g]);
+                    // the eight products first, then their activations, so
+                    // that table reads overlap the matrix cores
+                    mppi_v4f acc[8];
+#pragma unroll
+                    for (int mb = 0; mb < 8; ++mb)
+                        acc[mb] = mppi_mfma(mppi_frag(s_w1, mb, lane), xb,
+                                            *reinterpret_cast<const mppi_v4f*>(s_b + 16 * mb + 4 * g));
+#pragma unroll
+                    for (int mb = 0; mb < 8; ++mb) h1[nb][mb] = mppi_act4(acc[mb], s_tab);
+                }
+                // layers 2 and 3
+                mppi_v4f y[4] = {zero, zero, zero, zero};
+#pragma unroll
+                for (int kb3 = 0; kb3 < 4; ++kb3) {
+                    mppi_v4bf h2[4][2];
+#pragma unroll
+                    for (int half = 0; half < 2; ++half) {
+                        const int mb = 2 * kb3 + half;
+                        mppi_v8bf w[4];
+#pragma unroll
+                        for (int kb = 0; kb < 4; ++kb)
+                            w[kb] = *reinterpret_cast<const mppi_v8bf*>(
+                                s_wx + (MPPI_MLP_X32_W2 + mb * 4 + kb) * MPPI_MLP_X32_ELEMS + lane * 8);
+                        const mppi_v4f bias = *reinterpret_cast<const mppi_v4f*>(s_b + MPPI_MLP_HID + 16 * mb + 4 * g);
+                        // four independent accumulation chains (column blocks)
+                        mppi_v4f acc[4] = {bias, bias, bias, bias};
+#pragma unroll
+                        for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+                            for (int nb = 0; nb < 4; ++nb)
+                                acc[nb] = mppi_mfma32(w[kb], mppi_cat8(h1[nb][2 * kb], h1[nb][2 * kb + 1]), acc[nb]);
+#pragma unroll
+                        for (int nb = 0; nb < 4; ++nb) h2[nb][half] = mppi_act4(acc[nb], s_tab);
+                    }
+                    const mppi_v8bf w3 = *reinterpret_cast<const mppi_v8bf*>(
+                        s_wx + (MPPI_MLP_X32_W3 + kb3) * MPPI_MLP_X32_ELEMS + lane * 8);
+#pragma unroll
+                    for (int nb = 0; nb < 4; ++nb) y[nb] = mppi_mfma32(w3, mppi_cat8(h2[nb][0], h2[nb][1]), y[nb]);
+                }
+                // the state deltas: output row 4 g + j of sample 16 nb + c
+#pragma unroll
+                for (int nb = 0; nb < 4; ++nb)
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int r = 4 * g + j;
-                        if (r < SDIM) s_dx[wave][16 * nb + c][r] = y[j] + s_b[2 * MPPI_MLP_HID + r];
+                        if (r < SDIM) s_dx[wave][16 * nb + c][r] = y[nb][j] + s_b[2 * MPPI_MLP_HID + r];
                     }
-                }
                 __builtin_amdgcn_wave_barrier();
 #pragma unroll
                 for (int i = 0; i < SDIM; ++i) x[i] += s_dx[wave][lane][i] * sa.out_sd[i] + sa.out_mu[i];
@@ -1462,6 +1515,9 @@ extern "C" __global__ void __launch_bounds__(256, 2) mppi_rollout_surrogate(mppi
                                mppi_blob_ref{cost}, mppi_time{H, step});
         a.costs[d.cost_off + k] = total;
     }
-    mppi_block_stats<256>(a, agent, d, blk, active, total);
+    // softmax partials per workgroup of MPPI_SUR_BLOCK samples
+    mppi_agent_desc ds = d;
+    ds.n_blk = (d.K + MPPI_SUR_BLOCK - 1) / MPPI_SUR_BLOCK;
+    mppi_block_stats<MPPI_SUR_BLOCK>(a, agent, ds, blk, active, total);
 }
 #endif

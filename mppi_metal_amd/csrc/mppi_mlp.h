@@ -21,6 +21,25 @@
 //   W3   : 8 fragments, k-block kb:            A[i][k] = W3[i][16kb + k]       (i < out)
 //   W2T  : 64 fragments, mb * 8 + kb:          A[i][k] = W2[16kb + k][16mb + i]
 //   W3T  : 8 fragments, m-block mb:            A[i][k] = W3[k][16mb + i]       (k < out)
+// The rollout's layers 2 and 3 use gfx950's v_mfma_f32_16x16x32_bf16 (twice
+// the K per instruction), whose lane l holds A[l % 16][8 (l / 16) + j],
+// B[8 (l / 16) + j][l % 16], j = 0..7, and D as above.  Its fragments follow
+// the 152 above (512 bf16 each), with the K order of block kb
+//   u(kb, p) = 32 kb + (j < 4 ? 4 g + j : 16 + 4 g + (j - 4)),  g = p / 8, j = p % 8,
+// so that the two result tiles of m-blocks 2 kb and 2 kb + 1 that a lane
+// holds (rows 4 g + j of each), concatenated, are its B operand:
+//   W2X  : 32 fragments, mb * 4 + kb:          A[i][p] = W2[16mb + i][u(kb, p)]
+//   W3X  : 4 fragments, kb:                    A[i][p] = W3[i][u(kb, p)]       (i < out)
+//
+// Activation: the hidden layers apply T(z) = bf16(tanh(r16(z))), r16 the
+// rounding of the FP32 pre-activation to binary16 toward zero (the
+// hardware's packed conversion), bf16 the nearest-even rounding of the exact
+// tanh.  T is read from a table of the bf16 results of every binary16
+// magnitude in [0, 4] (17,409 entries, built on the host in FP64); larger
+// magnitudes (and infinities, NaN) give +-1, as bf16(tanh(4)) already is.
+// The activations are bf16 operands of the next layer anyway, so T is tanh
+// at the precision the network keeps, for about a quarter of the vector
+// work of an exp and a reciprocal.
 #ifndef MPPI_HIP_MLP_H
 #define MPPI_HIP_MLP_H
 
@@ -36,6 +55,13 @@
 #define MPPI_MLP_FRAG_W3T 144
 #define MPPI_MLP_NUM_FRAGS 152
 #define MPPI_MLP_FRAG_ELEMS 256
+#define MPPI_MLP_X32_BASE (MPPI_MLP_NUM_FRAGS * MPPI_MLP_FRAG_ELEMS)   // element offset of the x32 fragments
+#define MPPI_MLP_X32_ELEMS 512
+#define MPPI_MLP_X32_W2 0
+#define MPPI_MLP_X32_W3 32
+#define MPPI_MLP_X32_FRAGS 36
+#define MPPI_MLP_TOTAL_ELEMS (MPPI_MLP_X32_BASE + MPPI_MLP_X32_FRAGS * MPPI_MLP_X32_ELEMS)
+#define MPPI_MLP_TANH_KEYS 0x4401                                      // binary16 magnitudes 0 .. 4.0
 
 // Offsets of the FP32 parameter vector [W1 | b1 | W2 | b2 | W3 | b3] (row
 // major [out][in]) for in / out dimensions.
@@ -80,45 +106,63 @@ __device__ inline float mppi_bf2f(short h) {
     return __builtin_bit_cast(float, uint32_t((unsigned short)h) << 16);
 }
 
-// tanh(x) = 1 - 2 / (exp(2x) + 1) with the hardware exp2 and reciprocal
-// (about 2 ulp).  Large |x| saturates through IEEE infinities: exp2 gives
-// +inf (reciprocal 0, tanh 1) or 0 (tanh -1).
-__device__ inline float mppi_tanh_fast(float x) {
-    const float e = __builtin_amdgcn_exp2f(x * 2.88539008177792681472f);  // 2 / ln 2
-    return __builtin_fmaf(-2.0f, __builtin_amdgcn_rcpf(e + 1.0f), 1.0f);
+typedef short mppi_v8bf __attribute__((ext_vector_type(8)));
+typedef __bf16 mppi_v8bf16 __attribute__((ext_vector_type(8)));
+typedef unsigned short mppi_u16x2 __attribute__((ext_vector_type(2)));
+
+// T of two pre-activations, as a pair of bf16 (z0 in the low half): the
+// packed conversion to binary16 toward zero, magnitudes capped at 4.0
+// (0x4400), one table read each, the sign put back.  `tab` may be in LDS or
+// global memory.
+__device__ inline uint32_t mppi_act2(float z0, float z1, const unsigned short* tab) {
+    const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(z0, z1));
+    const mppi_u16x2 cap = {0x4400, 0x4400};
+    const mppi_u16x2 key = __builtin_elementwise_min(__builtin_bit_cast(mppi_u16x2, h & 0x7fff7fffu), cap);
+    const mppi_u16x2 r = {tab[key[0]], tab[key[1]]};
+    return __builtin_bit_cast(uint32_t, r) | (h & 0x80008000u);
 }
 
-__device__ inline mppi_v4f mppi_tanh4(mppi_v4f v) {
-    mppi_v4f r;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) r[j] = mppi_tanh_fast(v[j]);
-    return r;
+__device__ inline mppi_v4bf mppi_act4(mppi_v4f z, const unsigned short* tab) {
+    const uint32_t a = mppi_act2(z[0], z[1], tab), b = mppi_act2(z[2], z[3], tab);
+    return __builtin_bit_cast(mppi_v4bf, (uint64_t(b) << 32) | a);
 }
 
 __device__ inline mppi_v4f mppi_mfma(mppi_v4bf a, mppi_v4bf b, mppi_v4f c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// v_mfma_f32_16x16x32_bf16 (gfx950).
+__device__ inline mppi_v4f mppi_mfma32(mppi_v8bf a, mppi_v8bf b, mppi_v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mppi_v8bf16, a),
+                                                    __builtin_bit_cast(mppi_v8bf16, b), c, 0, 0, 0);
+}
+
+// B operand of x32 K-block kb from the two x16 result tiles 2 kb and 2 kb + 1.
+__device__ inline mppi_v8bf mppi_cat8(mppi_v4bf lo, mppi_v4bf hi) {
+    return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
 __device__ inline mppi_v4bf mppi_frag(const short* frags, uint32_t f, uint32_t lane) {
     return *reinterpret_cast<const mppi_v4bf*>(frags + size_t(f) * MPPI_MLP_FRAG_ELEMS + lane * 4);
 }
 
-// Forward pass of one 16-sample block.  xb: the B operand of the inputs
+// Forward pass of one 16-sample block (x16 fragments; training and
+// inference over sample lists).  xb: the B operand of the inputs
 // (rows 4*(lane/16)+j, zero beyond `in`).  Returns the output tile (rows
 // 4*(lane/16)+j < out valid, without the output bias); h1b / h2b receive the
 // bf16 hidden activations when non-null (for training).  b1 and b2 must be
 // 16-byte aligned: the biases initialise the accumulators (four rows per
 // lane), so the bias add costs no instruction.
 __device__ inline mppi_v4f mppi_mlp_block(const short* frags, const float* b1, const float* b2,
-                                            mppi_v4bf xb, uint32_t lane, mppi_v4bf* h1b_out,
-                                            mppi_v4bf* h2b_out) {
+                                            const unsigned short* tab, mppi_v4bf xb, uint32_t lane,
+                                            mppi_v4bf* h1b_out, mppi_v4bf* h2b_out) {
     const uint32_t g = lane >> 4;
     mppi_v4bf h1b[8], h2b[8];
     const mppi_v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int mb = 0; mb < 8; ++mb) {
         const mppi_v4f bias = *reinterpret_cast<const mppi_v4f*>(b1 + 16 * mb + 4 * g);
-        h1b[mb] = mppi_pack_bf16(mppi_tanh4(mppi_mfma(mppi_frag(frags, MPPI_MLP_FRAG_W1 + mb, lane), xb, bias)));
+        h1b[mb] = mppi_act4(mppi_mfma(mppi_frag(frags, MPPI_MLP_FRAG_W1 + mb, lane), xb, bias), tab);
     }
 #pragma unroll
     for (int mb = 0; mb < 8; ++mb) {
@@ -126,7 +170,7 @@ __device__ inline mppi_v4f mppi_mlp_block(const short* frags, const float* b1, c
 #pragma unroll
         for (int kb = 0; kb < 8; ++kb)
             acc = mppi_mfma(mppi_frag(frags, MPPI_MLP_FRAG_W2 + mb * 8 + kb, lane), h1b[kb], acc);
-        h2b[mb] = mppi_pack_bf16(mppi_tanh4(acc));
+        h2b[mb] = mppi_act4(acc, tab);
     }
     mppi_v4f out = zero;
 #pragma unroll

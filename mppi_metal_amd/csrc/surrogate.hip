@@ -36,6 +36,7 @@ constexpr int HID = MPPI_MLP_HID;
 
 struct train_args {
     const short* frags;
+    const unsigned short* tab;   // activation table
     const float* params;      // FP32 masters
     mppi_mlp_dims dims;
     const float* x;           // [n][in] normalised inputs
@@ -91,6 +92,7 @@ struct pack_args {
 
 struct forward_args {
     const short* frags;
+    const unsigned short* tab;   // activation table
     const float* params;
     mppi_mlp_dims dims;
     const float* x;           // [n][in] normalised
@@ -116,7 +118,7 @@ extern "C" __global__ void __launch_bounds__(256) mppi_mlp_train_fwd_bwd(train_a
         a.xt[size_t(k) * n + s] = xb[j];
     }
     mppi_v4bf h1b[8], h2b[8];
-    mppi_v4f y = mppi_mlp_block(a.frags, a.params + d.b1, a.params + d.b2, xb, lane, h1b, h2b);
+    mppi_v4f y = mppi_mlp_block(a.frags, a.params + d.b1, a.params + d.b2, a.tab, xb, lane, h1b, h2b);
 #pragma unroll
     for (int mb = 0; mb < 8; ++mb)
 #pragma unroll
@@ -252,14 +254,27 @@ extern "C" __global__ void __launch_bounds__(256) mppi_mlp_adam(adam_args a) {
     a.params[p] -= a.lr_t * m / (__builtin_sqrtf(v) + a.eps);
 }
 
-// One thread per fragment element (152 * 256).
+// One thread per fragment element (152 x16 fragments of 256, then 36 x32
+// fragments of 512).
 extern "C" __global__ void __launch_bounds__(256) mppi_mlp_pack(pack_args a) {
     const uint32_t e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= MPPI_MLP_NUM_FRAGS * MPPI_MLP_FRAG_ELEMS) return;
-    const uint32_t f = e / MPPI_MLP_FRAG_ELEMS, lane = (e / 4) % 64, j = e % 4;
-    const uint32_t i = lane % 16, k = 4 * (lane / 16) + j;   // A[i][k] of the fragment
+    if (e >= MPPI_MLP_TOTAL_ELEMS) return;
     const mppi_mlp_dims& d = a.dims;
     const float* P = a.params;
+    if (e >= MPPI_MLP_X32_BASE) {
+        const uint32_t q = e - MPPI_MLP_X32_BASE;
+        const uint32_t f = q / MPPI_MLP_X32_ELEMS, lane = (q / 8) % 64, j = q % 8;
+        const uint32_t i = lane % 16, g = lane / 16;          // A[i][8 g + j] of the fragment
+        const uint32_t kb = f % 4;
+        const uint32_t unit = 32 * kb + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
+        float v = 0.0f;
+        if (f < MPPI_MLP_X32_W3) v = P[d.w2 + (16 * (f / 4) + i) * HID + unit];      // W2X, mb = f / 4
+        else if (i < d.out) v = P[d.w3 + i * HID + unit];                              // W3X
+        a.frags[e] = mppi_f2bf(v);
+        return;
+    }
+    const uint32_t f = e / MPPI_MLP_FRAG_ELEMS, lane = (e / 4) % 64, j = e % 4;
+    const uint32_t i = lane % 16, k = 4 * (lane / 16) + j;   // A[i][k] of the fragment
     float v = 0.0f;
     if (f < MPPI_MLP_FRAG_W2) {                       // W1, mb = f
         if (k < d.in) v = P[d.w1 + (16 * f + i) * d.in + k];
@@ -292,7 +307,7 @@ extern "C" __global__ void __launch_bounds__(256) mppi_mlp_forward(forward_args 
         const uint32_t k = 4 * g + j;
         xb[j] = mppi_f2bf(k < d.in ? a.x[size_t(s) * d.in + k] : 0.0f);
     }
-    mppi_v4f y = mppi_mlp_block(a.frags, a.params + d.b1, a.params + d.b2, xb, lane, nullptr, nullptr);
+    mppi_v4f y = mppi_mlp_block(a.frags, a.params + d.b1, a.params + d.b2, a.tab, xb, lane, nullptr, nullptr);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const uint32_t r = 4 * g + j;
@@ -329,13 +344,13 @@ struct SurrogateState {
     float box_lo[16] = {}, box_hi[16] = {};   // sampling box of the training inputs
     bool ready = false, enabled = false;
     uint32_t adam_t = 0;
-    DeviceBuffer params, m, v, frags, partial, grad_sum, loss_part, loss_hist, out_norm;
+    DeviceBuffer params, m, v, frags, partial, grad_sum, loss_part, loss_hist, out_norm, tanh_table;
     DeviceBuffer xt, h1t, h2t, dz1t, dz2t, dyt;
     DeviceBuffer data_x, data_t, io_x, io_y;
     ~SurrogateState() {
         for (DeviceBuffer* b : {&params, &m, &v, &frags, &partial, &grad_sum, &loss_part, &loss_hist,
-                                &out_norm, &xt, &h1t, &h2t, &dz1t, &dz2t, &dyt, &data_x, &data_t,
-                                &io_x, &io_y})
+                                &out_norm, &tanh_table, &xt, &h1t, &h2t, &dz1t, &dz2t, &dyt, &data_x,
+                                &data_t, &io_x, &io_y})
             b->release();
     }
 };
@@ -360,6 +375,33 @@ mppi_mlp_dims dims_for(uint32_t in, uint32_t out) {
     return d;
 }
 
+// The activation table (mppi_mlp.h): entry `key` is the bf16 (nearest even)
+// of tanh of the binary16 value whose bits are `key`, for key in
+// [0, 0x4400] (magnitudes 0 .. 4.0), computed in FP64.
+uint16_t bf16_nearest_even(double t) {
+    float f = float(t);
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    const uint32_t lo = u & 0xffff0000u, hi = lo + 0x10000u;   // bf16 neighbours of t (t >= 0)
+    float flo, fhi;
+    std::memcpy(&flo, &lo, 4);
+    std::memcpy(&fhi, &hi, 4);
+    double dlo = t - double(flo), dhi = double(fhi) - t;
+    if (dlo < 0.0) return uint16_t(lo >> 16);            // t was rounded up past lo: t == flo - tiny
+    if (dlo < dhi || (dlo == dhi && ((lo >> 16) & 1u) == 0)) return uint16_t(lo >> 16);
+    return uint16_t(hi >> 16);
+}
+
+std::vector<uint16_t> tanh_table_host() {
+    std::vector<uint16_t> t(MPPI_MLP_TANH_KEYS);
+    for (uint32_t key = 0; key < MPPI_MLP_TANH_KEYS; ++key) {
+        const uint32_t ex = key >> 10, mant = key & 0x3ffu;
+        const double v = ex == 0 ? std::ldexp(double(mant), -24) : std::ldexp(double(1024 + mant), int(ex) - 25);
+        t[key] = bf16_nearest_even(std::tanh(v));
+    }
+    return t;
+}
+
 int ensure_state(mppi_hip_ctx* ctx) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
@@ -375,8 +417,13 @@ int ensure_state(mppi_hip_ctx* ctx) {
     if ((rc = ensure_device_buffer(ctx, &s.v, sizeof(float) * P))) return rc;
     if ((rc = ensure_device_buffer(ctx, &s.grad_sum, sizeof(float) * P))) return rc;
     if ((rc = ensure_device_buffer(ctx, &s.partial, sizeof(float) * P * kSplit))) return rc;
-    if ((rc = ensure_device_buffer(ctx, &s.frags, sizeof(short) * MPPI_MLP_NUM_FRAGS * MPPI_MLP_FRAG_ELEMS))) return rc;
+    if ((rc = ensure_device_buffer(ctx, &s.frags, sizeof(short) * MPPI_MLP_TOTAL_ELEMS))) return rc;
     if ((rc = ensure_device_buffer(ctx, &s.out_norm, sizeof(float) * 32))) return rc;
+    if (!s.tanh_table.ptr) {
+        const std::vector<uint16_t> t = tanh_table_host();
+        if ((rc = ensure_device_buffer(ctx, &s.tanh_table, sizeof(uint16_t) * t.size()))) return rc;
+        MPPI_HIP_TRY(ctx, hipMemcpy(s.tanh_table.ptr, t.data(), sizeof(uint16_t) * t.size(), hipMemcpyHostToDevice));
+    }
     return MPPI_HIP_OK;
 }
 
@@ -413,7 +460,7 @@ int upload_out_norm(mppi_hip_ctx* ctx) {
 int pack(mppi_hip_ctx* ctx) {
     SurrogateState& s = *ctx->sur;
     pack_args pa{static_cast<const float*>(s.params.ptr), s.dims, static_cast<short*>(s.frags.ptr)};
-    hipLaunchKernelGGL(mppi_mlp_pack, dim3(MPPI_MLP_NUM_FRAGS), dim3(256), 0, ctx->stream, pa);
+    hipLaunchKernelGGL(mppi_mlp_pack, dim3(MPPI_MLP_TOTAL_ELEMS / 256), dim3(256), 0, ctx->stream, pa);
     MPPI_HIP_TRY(ctx, hipGetLastError());
     return MPPI_HIP_OK;
 }
@@ -426,6 +473,7 @@ int train_step(mppi_hip_ctx* ctx, const float* x, const float* t, uint32_t n, bo
     const float* onorm = static_cast<const float*>(s.out_norm.ptr);
     train_args ta{};
     ta.frags = static_cast<const short*>(s.frags.ptr);
+    ta.tab = static_cast<const unsigned short*>(s.tanh_table.ptr);
     ta.params = static_cast<const float*>(s.params.ptr);
     ta.dims = s.dims;
     ta.x = x;
@@ -535,6 +583,7 @@ bool surrogate_rollout(mppi_hip_ctx* ctx, const mppi_step_args& a, hipFunction_t
     mppi_surrogate_rollout_args ra{};
     ra.s = a;
     ra.frags = static_cast<const short*>(s.frags.ptr);
+    ra.tanh_table = static_cast<const unsigned short*>(s.tanh_table.ptr);
     ra.params = static_cast<const float*>(s.params.ptr);
     ra.b1 = s.dims.b1;
     ra.b2 = s.dims.b2;
@@ -641,7 +690,8 @@ namespace {
 // Normalised network outputs (without denormalisation) for n inputs in io_x.
 int forward_io(mppi_hip_ctx* ctx, uint32_t n) {
     SurrogateState& s = *ctx->sur;
-    forward_args fa{static_cast<const short*>(s.frags.ptr), static_cast<const float*>(s.params.ptr), s.dims,
+    forward_args fa{static_cast<const short*>(s.frags.ptr), static_cast<const unsigned short*>(s.tanh_table.ptr),
+                    static_cast<const float*>(s.params.ptr), s.dims,
                     static_cast<const float*>(s.io_x.ptr), static_cast<float*>(s.io_y.ptr), n};
     hipLaunchKernelGGL(mppi_mlp_forward, dim3((n / 16 + 3) / 4), dim3(256), 0, ctx->stream, fa);
     MPPI_HIP_TRY(ctx, hipGetLastError());

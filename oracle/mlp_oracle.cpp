@@ -11,7 +11,12 @@
 // FP64 computation see the same rounded operands as the GPU (weights and
 // activations rounded to bfloat16, nearest-even), so the remaining difference
 // is FP32-versus-FP64 accumulation.
-// Pinned in tests/test_oracle_mlp.py by central finite differences.
+// With bf16 the hidden activations are T(z) = bf16(tanh(r16(float(z)))), the
+// GPU's activation (mppi_metal_amd/csrc/mppi_mlp.h): r16 rounds the FP32
+// pre-activation to binary16 toward zero, magnitudes above 4 (and
+// infinities, NaN) are capped at 4, bf16 is nearest-even of the FP64 tanh.
+// Pinned in tests/test_oracle_mlp.py by central finite differences (FP64
+// path) and, for T, against numpy over every binary16 magnitude.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -33,6 +38,42 @@ double round_bf16(double v) {
     return double(f);
 }
 
+// bf16 nearest-even of t >= 0, by scaling to 8 significant bits.
+double bf16_rne(double t) {
+    if (t == 0.0) return 0.0;
+    int e;
+    const double m = std::frexp(t, &e);            // t = m 2^e, m in [0.5, 1)
+    return std::ldexp(std::nearbyint(std::ldexp(m, 8)), e - 8);
+}
+
+// Binary16 bits of a >= 0 rounded toward zero (65504 for larger finite
+// values, 0x7c00 for infinity and NaN).
+uint32_t f16_rtz_bits(float a) {
+    uint32_t u;
+    std::memcpy(&u, &a, 4);
+    if (u >= 0x7f800000u) return 0x7c00u;
+    const int e = int(u >> 23) - 127;
+    if (e >= 16) return 0x7bffu;
+    if (e >= -14) return uint32_t((e + 15) << 10) | ((u >> 13) & 0x3ffu);
+    if (e < -24) return 0u;
+    return uint32_t(std::floor(std::ldexp(double(a), 24)));   // subnormal: m 2^-24
+}
+
+// The GPU's activation T of an FP32 pre-activation.
+double act(float z) {
+    uint32_t u;
+    std::memcpy(&u, &z, 4);
+    float a;
+    const uint32_t au = u & 0x7fffffffu;
+    std::memcpy(&a, &au, 4);
+    uint32_t key = f16_rtz_bits(a);
+    if (key > 0x4400u) key = 0x4400u;
+    const uint32_t ex = key >> 10, mant = key & 0x3ffu;
+    const double v = ex == 0 ? std::ldexp(double(mant), -24) : std::ldexp(double(1024 + mant), int(ex) - 25);
+    const double t = bf16_rne(std::tanh(v));
+    return (u >> 31) ? -t : t;
+}
+
 struct Params {
     const float *w1, *b1, *w2, *b2, *w3, *b3;  // row-major [out][in]
 };
@@ -52,12 +93,12 @@ void oracle_mlp_forward(const float* w1, const float* b1, const float* w2, const
         for (int o = 0; o < HID; ++o) {
             double z = b1[o];
             for (int i = 0; i < IN; ++i) z += r(w1[o * IN + i]) * r(x[s * IN + i]);
-            h1[o] = r(std::tanh(z));
+            h1[o] = bf16 ? act(float(z)) : std::tanh(z);
         }
         for (int o = 0; o < HID; ++o) {
             double z = b2[o];
             for (int i = 0; i < HID; ++i) z += r(w2[o * HID + i]) * h1[i];
-            h2[o] = r(std::tanh(z));
+            h2[o] = bf16 ? act(float(z)) : std::tanh(z);
         }
         for (int o = 0; o < OUT; ++o) {
             double z = b3[o];
@@ -91,12 +132,12 @@ double oracle_mlp_grad(const float* w1, const float* b1, const float* w2, const 
         for (int o = 0; o < HID; ++o) {
             double z = b1[o];
             for (int i = 0; i < IN; ++i) z += r(w1[o * IN + i]) * xr[i];
-            h1[o] = r(std::tanh(z));
+            h1[o] = bf16 ? act(float(z)) : std::tanh(z);
         }
         for (int o = 0; o < HID; ++o) {
             double z = b2[o];
             for (int i = 0; i < HID; ++i) z += r(w2[o * HID + i]) * h1[i];
-            h2[o] = r(std::tanh(z));
+            h2[o] = bf16 ? act(float(z)) : std::tanh(z);
         }
         for (int o = 0; o < OUT; ++o) {
             double z = b3[o];
@@ -136,9 +177,9 @@ double oracle_mlp_grad(const float* w1, const float* b1, const float* w2, const 
 
 // One sample through a network of any input / output width (<= 16) with
 // the parameter layout [W1 | b1 | W2 | b2 | W3 | b3] (row-major [out][in]),
-// as the GPU rollout evaluates it: bfloat16 weights, inputs and hidden
-// activations, sums rounded to FP32 before the FP32 bias is added (the
-// matrix cores accumulate in FP32), tanh in FP64.
+// as the GPU rollout evaluates it: bfloat16 weights and inputs, sums
+// rounded to FP32 before the FP32 bias is added (the matrix cores
+// accumulate in FP32), hidden activations T (above).
 void oracle_mlp_forward1(const float* P, uint32_t in, uint32_t out, const float* x, float* y) {
     const float* w1 = P;
     const float* b1 = w1 + HID * in;
@@ -150,18 +191,23 @@ void oracle_mlp_forward1(const float* P, uint32_t in, uint32_t out, const float*
     for (int o = 0; o < HID; ++o) {
         double z = 0.0;
         for (uint32_t i = 0; i < in; ++i) z += round_bf16(w1[o * in + i]) * round_bf16(x[i]);
-        h1[o] = round_bf16(std::tanh(double(float(z) + b1[o])));
+        h1[o] = act(float(z) + b1[o]);
     }
     for (int o = 0; o < HID; ++o) {
         double z = 0.0;
         for (int i = 0; i < HID; ++i) z += round_bf16(w2[o * HID + i]) * h1[i];
-        h2[o] = round_bf16(std::tanh(double(float(z) + b2[o])));
+        h2[o] = act(float(z) + b2[o]);
     }
     for (uint32_t o = 0; o < out; ++o) {
         double z = 0.0;
         for (int i = 0; i < HID; ++i) z += round_bf16(w3[o * HID + i]) * h2[i];
         y[o] = float(z) + b3[o];
     }
+}
+
+// T of n FP32 pre-activations (test access to the activation).
+void oracle_mlp_act(const float* z, float* out, size_t n) {
+    for (size_t i = 0; i < n; ++i) out[i] = float(act(z[i]));
 }
 
 }  // extern "C"

@@ -277,3 +277,39 @@ def test_closed_loop_gap_full_size(require_gpu, full_size):
     _report("closed loop, full size", plain=float(plain.mean()), surrogate=float(sur.mean()), gap=gap)
     assert np.all(np.isfinite(sur))
     assert gap <= 0.01
+
+
+def test_surrogate_gains_on_costly_dynamics(require_gpu):
+    """What the trained network gains over the method without it.  On the
+    bundled bicycle (two Euler substeps per step) the network costs more
+    than the dynamics it replaces; on a plugin whose dynamics costs more than
+    the network -- the same bicycle integrated with 64 substeps per step, a
+    high-fidelity integration -- the surrogate rollout runs more sample-steps
+    per second than the plugin rollout at the full K = 2^18, H = 64, and the
+    closed loop (plant: the plugin dynamics) keeps its mean realised cost
+    within 1% above plain MPPI's."""
+    K, H, T = 1 << 18, 64, 200
+    hip, ora, sc = bicycle(K, H, substeps=64)
+    hip.surrogate_train(**BICYCLE_ENVELOPE, num_transitions=4 << 20, batch=4096, steps=4000,
+                        learning_rate=2e-3, seed=7)
+    x0 = sc.x0[0]
+    un = np.zeros((H, 2), np.float32)
+    res = {}
+    try:
+        for use in (False, True):
+            hip.surrogate_use(use)
+            hip.simulate_batch(1, 3, 0, x0, un, outputs=())
+            hip.profile(True)
+            st, ct, _, tt = hip.simulate_batch(1, T, 0, x0, un)
+            roll_ms, _, n = hip.profile_read()
+            hip.profile(False)
+            cost = ora.closed_loop_cost(st[:, 0], ct[:, 0], tt[0])
+            res[use] = (roll_ms / n, float(cost.mean()))
+    finally:
+        hip.surrogate_use(False)
+    speed = res[False][0] / res[True][0]
+    gap = (res[True][1] - res[False][1]) / res[False][1]
+    _report("gain on costly dynamics", plugin_rollout_ms=res[False][0], surrogate_rollout_ms=res[True][0],
+            speedup=speed, plain_cost=res[False][1], surrogate_cost=res[True][1], gap=gap)
+    assert speed > 1.2
+    assert gap <= 0.01

@@ -93,15 +93,90 @@ def _flat(p):
     return np.concatenate([a.ravel() for a in p]).astype(np.float32)
 
 
+def _f16_rtz_keys(z):
+    """Binary16 bits of |z| (float32) rounded toward zero, capped at 4.0
+    (0x4400), infinities and NaN capped too; via numpy's binary16 and a
+    correction of its nearest rounding."""
+    a = np.abs(np.asarray(z, np.float32)).astype(np.float64)
+    big = ~(a < 4.0)                           # also NaN
+    a = np.where(big, 0.0, a)
+    h = a.astype(np.float16)                   # nearest
+    over = h.astype(np.float64) > a            # rounded up: step down one binary16 place
+    keys = h.view(np.uint16).astype(np.int64) - over.astype(np.int64)
+    return np.where(big, 0x4400, np.minimum(keys, 0x4400))
+
+
+def _bf16_nearest_even_exact(t):
+    """bf16 nearest-even of float64 values, exact (ties decided with
+    Fractions)."""
+    from fractions import Fraction
+    out = np.empty(t.shape, np.float64)
+    for i, v in enumerate(t.ravel()):
+        if v == 0.0:
+            out.flat[i] = 0.0
+            continue
+        f = np.float32(v)
+        u = int(np.array(f).view(np.uint32)) & 0xFFFF0000
+        lo = float(np.array(u, np.uint32).view(np.float32))
+        hi = float(np.array(u + 0x10000, np.uint32).view(np.float32))
+        if lo > v:                             # float rounding went up past a bf16 value
+            hi, lo = lo, float(np.array(u - 0x10000, np.uint32).view(np.float32))
+        d_lo, d_hi = Fraction(v) - Fraction(lo), Fraction(hi) - Fraction(v)
+        even_lo = ((int(np.array(lo, np.float32).view(np.uint32)) >> 16) & 1) == 0
+        out.flat[i] = lo if (d_lo < d_hi or (d_lo == d_hi and even_lo)) else hi
+    return out
+
+
+def _np_act(z):
+    """The GPU's hidden activation T(z) = bf16(tanh(r16(z)))."""
+    z = np.asarray(z, np.float32)
+    keys = _f16_rtz_keys(z)
+    v = keys.astype(np.uint16).view(np.float16).astype(np.float64)
+    t = _bf16_nearest_even_exact(np.tanh(v))
+    return np.where(np.signbit(z), -t, t)
+
+
 def _np_forward1(p, x):
-    """bf16 operands, FP64 sums rounded to FP32, FP32 bias, FP64 tanh."""
+    """bf16 operands, FP64 sums rounded to FP32, FP32 bias, activation T."""
     w1, b1, w2, b2, w3, b3 = p
     f64 = lambda a: _bf16(a).astype(np.float64)
     z1 = (f64(x) @ f64(w1).T).astype(np.float32) + b1
-    h1 = f64(np.tanh(z1.astype(np.float64)).astype(np.float32))
+    h1 = _np_act(z1)
     z2 = (h1 @ f64(w2).T).astype(np.float32) + b2
-    h2 = f64(np.tanh(z2.astype(np.float64)).astype(np.float32))
+    h2 = _np_act(z2)
     return (h2 @ f64(w3).T).astype(np.float32) + b3
+
+
+def _oracle_act(z):
+    z = np.ascontiguousarray(z, np.float32)
+    out = np.empty_like(z)
+    L = ol.lib()
+    L.oracle_mlp_act.restype = None
+    L.oracle_mlp_act(_fp(z), _fp(out), ctypes.c_size_t(z.size))
+    return out
+
+
+def test_activation_over_every_binary16_magnitude():
+    """T at every binary16 value in [0, 4] (each table entry) and just above
+    and below each, signed, against numpy's binary16 and exact bf16
+    rounding; plus the properties that pin it without numpy: odd, monotone,
+    within half a bf16 step of tanh, saturated to 1 from 4 on."""
+    keys = np.arange(0x4401, dtype=np.uint16)
+    v = keys.view(np.float16).astype(np.float32)
+    up = np.nextafter(v, np.float32(np.inf))
+    down = np.nextafter(v, np.float32(0))
+    for z in (v, up, down, -v, -up):
+        assert np.array_equal(_oracle_act(z), _np_act(z).astype(np.float32))
+    t = _oracle_act(v).astype(np.float64)
+    assert np.all(np.diff(t) >= 0) and t[0] == 0.0 and t[-1] == 1.0
+    ulp = np.array([np.spacing(np.float32(x)) * 65536 for x in t[1:]])   # bf16 step at each value
+    assert np.all(np.abs(t[1:] - np.tanh(v[1:].astype(np.float64))) <= 0.5 * ulp + 1e-300)
+    big = np.array([4.0, 4.5, 1e4, 7e4, 3e38, np.inf, np.nan], np.float32)
+    assert np.array_equal(_oracle_act(big), np.ones(7, np.float32))
+    assert np.array_equal(_oracle_act(-big[:-1]), -np.ones(6, np.float32))
+    # a value between binary16 neighbours takes the lower one (toward zero)
+    z = np.float32(1.0 + 2.0 ** -11)           # halfway between binary16 1.0 and 1.0 + 2^-10
+    assert _oracle_act(np.array([z]))[0] == _oracle_act(np.array([np.float32(1.0)]))[0]
 
 
 def _fp(a):
