@@ -58,16 +58,90 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9     # 256 CUs x 64 lanes/clk x 2.4 GHz = 39.3e12
+# Published MI355X peaks.  FP32 vector: 157.3 TFLOP/s, an FMA counted as two
+# flops, i.e. 78.6e12 lane-instructions/s (256 CUs x 4 SIMDs x 32 lanes x
+# 2.4 GHz); no VALU instruction issues faster (a v_fma_f32 stream measures
+# 22 lanes per cycle per SIMD).  Dense bf16 matrix cores: 2.5 PFLOP/s.
+VALU_PEAK_LANE_OPS = 157.3e12 / 2
+MFMA_PEAK_BF16 = 2.5e15
 HBM_PEAK_BPS = 8.0e12
+COUNTERS = ROOT / "profiles" / "mppi_counters_unicycle_pursuit_surrogate.md"
 
 
-def valu_ops_per_sample_step(n_obs: int) -> int:
-    """Algorithmic VALU lane-op slots per sample-step of the unicycle rollout
-    (transcendentals weighted x4): Philox4x32-10 60, uniform conversion 4,
-    Box-Muller 4 + 4*4, scale and clamp 6, unicycle 11 + 2*4, goal and
-    control cost 7, per obstacle 7 + 4."""
-    return 92 + 7 * n_obs + 4 * (6 + n_obs)
+def noise_ops(C: int) -> dict:
+    """Algorithmic VALU lane-ops per sample-step of the control noise (C
+    normals), transcendentals weighted x4: Philox4x32-10 per four normals
+    (10 rounds of 2 low and 2 high products and 4 xors; key bumps are
+    uniform), 3 per uniform word (shift, or, convert and scale), per
+    Box-Muller pair log, sqrt, sin, cos (x4 each) and 5 multiplies, and 3 per
+    control (scale by sigma, add the nominal, clamp)."""
+    return {"philox": 80 * C / 4, "uniform": 3 * C, "box_muller": 21 * C / 2, "control": 3 * C}
+
+
+def unicycle_w_ss(C: int = 2, fused: bool = True) -> dict:
+    """Algorithmic VALU lane-ops per sample-step of the unicycle rollout with
+    its 16-slot obstacle gate: noise; dynamics 26 (sincos x4 each, two
+    position updates of 2 products and an add, two FMAs, the heading wrap 8,
+    the speed clamp 2); goal and control cost 9; the gate 4 per slot (two
+    FMAs, max of the magnitudes, running min) over all 16 slots (the exact
+    pass runs only where a sample is in a square: none at this workload); the
+    cost accumulation 1; and in the one-pass step, 2 per control to keep and
+    average the noise."""
+    w = noise_ops(C)
+    w.update(dynamics=26, goal_cost=9, obstacle_gate=4 * 16, accumulate=1, keep_noise=2 * C if fused else 0)
+    w["total"] = sum(w.values())
+    return w
+
+
+def bicycle_w_ss(substeps: int = 2, zones: int = 32, C: int = 2, one_launch: bool = False) -> dict:
+    """Algorithmic VALU lane-ops per sample-step of the dynamic-bicycle
+    pursuit rollout: noise; per Euler substep 40 (sincos x4 each, the speed
+    floor 1, 1/u x4, the two tyre forces 7, the body-frame rates 12, six
+    state FMAs, the clamp 2); the tracking and control cost 9 (the target's
+    position is uniform); the zone test 5 per zone (two subtractions, max,
+    subtract the radius, running min); accumulation 1.  The one-launch step
+    also regenerates the noise for its weighted sum: the noise again and 2 per
+    control."""
+    w = noise_ops(C)
+    w.update(dynamics=40 * substeps, cost=9, zone_test=5 * zones, accumulate=1)
+    if one_launch:
+        w["noise_sums"] = sum(noise_ops(C).values()) + 2 * C
+    w["total"] = sum(w.values())
+    return w
+
+
+# Matrix-core flops per sample-step of the 8-128-128-6 surrogate: the
+# algorithm's 2 (8*128 + 128*128 + 128*6) and, with the input and output
+# padded to 16-wide tiles, the issued 2 (16*128 + 128*128 + 128*16).
+SURROGATE_FLOPS = 2 * (8 * 128 + 128 * 128 + 128 * 6)
+SURROGATE_FLOPS_ISSUED = 2 * (16 * 128 + 128 * 128 + 128 * 16)
+
+
+def profile_line(key: str) -> dict:
+    """The `key kernel value kernel value ...` line of the committed counter
+    summary, as {kernel: float}; empty without it."""
+    if not COUNTERS.exists():
+        return {}
+    for line in COUNTERS.read_text().splitlines():
+        if line.startswith(key + " "):
+            f = line.split()[1:]
+            return {k: float(v) for k, v in zip(f[0::2], f[1::2])}
+    return {}
+
+
+def issued(kernel: str, sample_steps: float, seconds: float):
+    """Counter-derived VALU issue of `kernel` (named kernel@workload): its issue slots per
+    sample-step from the committed rocprofv3 summary (SQ_INSTS_VALU, with
+    SQ_INSTS_VALU_TRANS_F32 and SQ_INSTS_VALU_INT64 weighted x4, times 64
+    lanes; an MFMA instruction, if the counter includes it, counts once),
+    times the sample-steps of a launch over the launch's duration measured
+    here, against the FP32 vector peak; or None without a summary."""
+    slots = profile_line("issue_slots_per_sample_step").get(kernel)
+    if slots is None:
+        return None
+    rate = slots * sample_steps / seconds
+    return {"slots_per_sample_step": slots, "achieved": rate / 1e12, "peak": VALU_PEAK_LANE_OPS / 1e12,
+            "unit": "T lane-op/s", "frac": rate / VALU_PEAK_LANE_OPS, "source": "profiles/" + COUNTERS.name}
 
 
 def reduce_bytes(K: int, H: int, C: int, n_chunks: int) -> int:
@@ -76,6 +150,15 @@ def reduce_bytes(K: int, H: int, C: int, n_chunks: int) -> int:
     read (2 * 16 * n_chunks * ceil(HC/4))."""
     nb = (H * C + 3) // 4
     return 8 * K + 8 * H * C + 2 * 16 * n_chunks * nb
+
+
+def combine_bytes(K: int, H: int, C: int) -> int:
+    """Algorithmic bytes of one combine launch of the one-pass step: the
+    tiles' weighted noise sums (4 HC per 256-sample tile) and statistics (16
+    per tile) read, the nominal read, the update and the next nominal
+    written (4 HC each)."""
+    tiles = -(-K // 256)
+    return tiles * (4 * H * C + 16) + 3 * 4 * H * C
 
 
 def reduce_chunks(K: int, H: int, C: int, N: int = 1, spt: int = 0, cus: int = 256) -> int:
@@ -88,18 +171,11 @@ def reduce_chunks(K: int, H: int, C: int, N: int = 1, spt: int = 0, cus: int = 2
 
 
 def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` at this workload, from the committed
+    """HBM bytes per launch of `kernel` (named kernel@workload), from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel
     calibration recorded there), or None."""
-    path = ROOT / "profiles" / "mppi_traffic_unicycle_k65536_h64.md"
-    if not path.exists():
-        return None
-    for line in path.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch"):
-            f = line.split()[1:]
-            d = dict(zip(f[0::2], f[1::2]))
-            return int(d[kernel]) if kernel in d else None
-    return None
+    v = profile_line("traffic_bytes_per_launch").get(kernel)
+    return None if v is None else int(v)
 
 
 def host_cpus():
@@ -219,6 +295,26 @@ def bench_batch(args):
     batch_rate = work * T / el
     print(f"[bench] batch: {T} steps in {el:.3f}s, tuning {tuning}", file=sys.stderr, flush=True)
 
+    # Kernel times from events attached to each dispatch (while profiling the
+    # batch launches its steps directly rather than replaying its graph).
+    hip.profile(True)
+    b.run(min(T, 20), closed_loop=True)
+    r_ms, d_ms, n_l = hip.profile_read()
+    hip.profile(False)
+    one = tuning["launches"] == 1
+    kern = "mppi_{}_{}_b{}".format("step" if one else "rollout", "persist" if tuning["persistent"] else "table",
+                                   tuning["block"])
+    w = bicycle_w_ss(2, 32, 2, one)
+    k_s = r_ms / n_l / 1e3
+    rate = work * w["total"] / k_s
+    roofline = {"bound": "valu", "kernel": kern, "achieved": rate / 1e12, "peak": VALU_PEAK_LANE_OPS / 1e12,
+                "unit": "T lane-op/s", "frac": rate / VALU_PEAK_LANE_OPS, "ops_per_sample_step": w["total"],
+                "ops_breakdown": w, "sample_steps_per_launch": work, "avg_launch_us": k_s * 1e6,
+                "second_launch_us": None if one else d_ms / n_l * 1e3,
+                "issued": issued(kern + "@pursuit", work, k_s),
+                "traffic": measured_traffic(kern + "@pursuit"),
+                "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/" + COUNTERS.name}
+
     # Plain GPU path, one agent at a time (blocking step per agent).
     ctxs = []
     for a in range(N):
@@ -259,7 +355,7 @@ def bench_batch(args):
                     "32 threat zones, closed loop",
         "agents": N, "sample_steps_per_step": work, "steps": T,
         "value": batch_rate, "unit": "sample-steps/s", "ms_per_step": el / T * 1e3,
-        "tuning": tuning, "autotune_best_ms": tune_ms,
+        "tuning": tuning, "autotune_best_ms": tune_ms, "roofline": roofline,
         "plain_per_agent": {"value": serial_rate, "ms_per_step": el_s / Ts * 1e3, "steps": Ts,
                             "how": "one context per agent, blocking step() per agent"},
         "padded_uniform_batch": {"value_useful": padded_rate, "ms_per_step": el_p / Tp * 1e3,
@@ -356,10 +452,20 @@ def bench_surrogate(args, substeps=2):
               flush=True)
     hip.surrogate_use(False)
     hip.close()
-    # Matrix-core flops of the network per sample-step (input and output
-    # padded to the 16-wide tiles): 2 * (16*128 + 128*128 + 128*16).
-    mfma_flops = 2 * (16 * 128 + 128 * 128 + 128 * 16)
     sur = res["surrogate"]
+    sur_s = sur["rollout_us"] * 1e-6
+    flops = K * H * SURROGATE_FLOPS / sur_s
+    key = "mppi_rollout_surrogate@surrogate"
+    roofline = {"bound": "mfma", "kernel": "mppi_rollout_surrogate", "achieved": flops / 1e12,
+                "peak": MFMA_PEAK_BF16 / 1e12, "unit": "TFLOP/s", "frac": flops / MFMA_PEAK_BF16,
+                "flops_per_sample_step": SURROGATE_FLOPS, "sample_steps_per_launch": K * H,
+                "avg_launch_us": sur["rollout_us"],
+                "issued_flops_per_sample_step": SURROGATE_FLOPS_ISSUED,
+                "issued_frac": K * H * SURROGATE_FLOPS_ISSUED / sur_s / MFMA_PEAK_BF16,
+                "valu_issued": issued(key, K * H, sur_s),
+                "lds_bank_conflict_frac": profile_line("lds_bank_conflict_cycles_over_lds_cycles").get(key),
+                "traffic": measured_traffic(key),
+                "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/" + COUNTERS.name}
     return {
         "workload": "learned dynamic-bicycle surrogate (MLP 8-128-128-6 tanh, bf16 MFMA, fp32 accumulate), "
                     f"pursuit, closed loop, plugin dynamics with {substeps} Euler substeps per step",
@@ -371,7 +477,7 @@ def bench_surrogate(args, substeps=2):
         "speed_ratio_surrogate_vs_plain": sur["value"] / res["plain"]["value"],
         "closed_loop_cost_gap": (sur["mean_realised_cost"] - res["plain"]["mean_realised_cost"])
                                 / res["plain"]["mean_realised_cost"],
-        "surrogate_rollout_mfma_tflops": K * H * mfma_flops / (sur["rollout_us"] * 1e-6) / 1e12,
+        "roofline": roofline if substeps == 2 else None,
     }
 
 
@@ -406,6 +512,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--serial-steps", type=int, default=5)
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-costly", action="store_true",
+                    help="skip the surrogate against 64-substep dynamics")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--requests", type=int, default=1024)
     ap.add_argument("--surrogate-steps", type=int, default=500)
@@ -460,11 +568,11 @@ def main():
     roll_ms, red_ms, launches = hip.profile_read()
     hip.profile(False)
     roll_s = roll_ms / launches / 1e3
+    roll_kernel = "mppi_rollout_fused" if fused else "mppi_rollout"
+    second_kernel = "mppi_fused_combine" if fused else "mppi_reduce"
     second_s = red_ms / launches / 1e3     # the combine launch (fused) or the reduce
-    # The one-pass step also keeps and averages the noise: 2 C more slots
-    # (an LDS store and an FMA per normal).
-    w_ss = valu_ops_per_sample_step(n_obs) + (2 * C if fused else 0)
-    achieved_ops = K * H * w_ss / roll_s
+    w_ss = unicycle_w_ss(C, fused)
+    achieved_ops = K * H * w_ss["total"] / roll_s
 
     # The two-kernel step (rollout, then the noise-regenerating reduce).
     hip.set_step_path(_native.PATH_TWO_KERNEL)
@@ -513,7 +621,7 @@ def main():
     surrogate = None if args.no_surrogate else bench_surrogate(args)
     # The same workload on dynamics that cost more than the network (64
     # Euler substeps per step): where the surrogate pays.
-    surrogate_costly = None if args.no_surrogate else bench_surrogate(args, substeps=64)
+    surrogate_costly = None if args.no_surrogate or args.no_costly else bench_surrogate(args, substeps=64)
     serving = None if args.no_serving else bench_serving(args)
 
     out = {
@@ -535,24 +643,27 @@ def main():
                    "obstacles": n_obs, "closed_loop_steps": args.steps, "lambda": 1.0,
                    "sigma": [0.5, 0.3]},
         "step_path": "one pass (rollout keeping the noise, then combine)" if fused else "two kernels",
-        "roofline": {"bound": "valu", "kernel": "mppi_rollout_fused" if fused else "mppi_rollout",
+        "roofline": {"bound": "valu", "kernel": roll_kernel,
                      "achieved": achieved_ops / 1e12, "peak": VALU_PEAK_LANE_OPS / 1e12,
                      "unit": "T lane-op/s", "frac": achieved_ops / VALU_PEAK_LANE_OPS,
-                     "ops_per_sample_step": w_ss, "avg_launch_us": roll_s * 1e6,
-                     "traffic": measured_traffic("mppi_rollout_fused" if fused else "mppi_rollout")
-                     if default_workload else None},
-        "second_launch": {"kernel": "mppi_fused_combine" if fused else "mppi_reduce",
-                          "avg_launch_us": second_s * 1e6,
-                          "traffic": measured_traffic("mppi_fused_combine" if fused else "mppi_reduce")
-                          if default_workload else None},
+                     "ops_per_sample_step": w_ss["total"], "ops_breakdown": w_ss,
+                     "sample_steps_per_launch": K * H, "avg_launch_us": roll_s * 1e6,
+                     "issued": issued(roll_kernel + "@unicycle", K * H, roll_s) if default_workload else None,
+                     "traffic": measured_traffic(roll_kernel + "@unicycle") if default_workload else None,
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/" + COUNTERS.name},
+        "second_launch": {"kernel": second_kernel, "avg_launch_us": second_s * 1e6,
+                          "traffic": measured_traffic(second_kernel + "@unicycle") if default_workload else None,
+                          "algorithmic_bytes": combine_bytes(K, H, C) if fused else rb},
         "two_kernel_path": {"value": K * H * args.steps / el2, "ms_per_step": el2 / args.steps * 1e3,
                             "rollout_us": roll2_s * 1e6, "reduce_us": red_s * 1e6,
-                            "rollout_frac": K * H * valu_ops_per_sample_step(n_obs) / roll2_s / VALU_PEAK_LANE_OPS},
+                            "rollout_frac": K * H * unicycle_w_ss(C, False)["total"] / roll2_s / VALU_PEAK_LANE_OPS,
+                            "rollout_issued": issued("mppi_rollout@unicycle", K * H, roll2_s)
+                            if default_workload else None},
         "reduce_roofline": {"bound": "hbm", "kernel": "mppi_reduce (two-kernel path)",
                             "achieved": rb / red_s / 1e9, "peak": HBM_PEAK_BPS / 1e9,
                             "unit": "GB/s", "frac": rb / red_s / HBM_PEAK_BPS,
                             "bytes_per_launch": rb, "avg_launch_us": red_s * 1e6,
-                            "traffic": measured_traffic("mppi_reduce") if default_workload else None},
+                            "traffic": measured_traffic("mppi_reduce@unicycle") if default_workload else None},
         "cpu_baseline": cpu,
         "small_workload": small,
         "final_closed_loop_cost": float(kt[-1]),

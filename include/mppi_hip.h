@@ -210,7 +210,10 @@ int mppi_hip_set_reduce_samples_per_thread(mppi_hip_ctx* ctx, uint32_t spt);
 int mppi_hip_set_wait_limit(mppi_hip_ctx* ctx, uint32_t polls, int force_timeout);
 
 /* Device timing: when enabled, every rollout and reduce launch is bracketed by
- * HIP events on the context's stream and their durations are summed.
+ * HIP events on the context's stream and their durations are summed.  This
+ * includes the launches of a batch on the context (mppi_hip_batch_run): its
+ * one-launch step counts as a rollout, and while profiling a batch launches
+ * each step directly instead of replaying its captured graph.
  * mppi_hip_profile_read returns the sums in milliseconds and the launch count
  * and clears them.  No reference counterpart. */
 int mppi_hip_profile_enable(mppi_hip_ctx* ctx, int enable);

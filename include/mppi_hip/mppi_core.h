@@ -19,6 +19,16 @@
 //     MPPI_CALLABLE float f(const float* state, const uint8_t* model_params,
 //                           const uint8_t* cost_params);
 //   Either cost may add a trailing `mppi_time` argument (new; see below).
+//   cost preparation (optional; new, no reference counterpart): a plugin
+//     that defines MPPI_COST_PREPARE as the name of a device function
+//       void f(const uint8_t* cost_params, uint32_t thread, uint32_t threads);
+//     has it called on the GPU by every thread of a workgroup (thread in
+//     [0, threads)) with the workgroup's cost parameters before a workgroup
+//     barrier that precedes the workgroup's first cost evaluation with them.
+//     It can fill workgroup memory (__shared__ arrays declared at file scope
+//     under `#if defined(__HIP_DEVICE_COMPILE__)`) that the costs then read;
+//     the costs' results must not depend on it (the host oracle has no
+//     workgroups and never calls it).
 //
 // The state argument is passed through mppi_state_ref / mppi_cstate_ref and
 // the parameter blobs through mppi_blob_ref, which convert implicitly to

@@ -19,6 +19,8 @@
 // shaders/mppi_reduce.metal:242-488) for agents that share K and H.
 #include "backend.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -185,9 +187,27 @@ mppi_step_args batch_args(mppi_hip_batch* b, int closed_loop) {
     return a;
 }
 
+// Launch f on the context's stream; with `prof`, a start and stop event
+// attached to the dispatch itself time the kernel alone and are queued on
+// `pending` for collect_profile.
+int launch_timed(mppi_hip_ctx* ctx, hipFunction_t f, uint32_t grid, uint32_t block, void** params, bool prof,
+                 std::vector<std::pair<hipEvent_t, hipEvent_t>>& pending) {
+    hipEvent_t e0 = prof ? take_event(ctx) : nullptr, e1 = prof ? take_event(ctx) : nullptr;
+    if (e0 && e1) {
+        MPPI_HIP_TRY(ctx, hipExtModuleLaunchKernel(f, grid * block, 1, 1, block, 1, 1, 0, ctx->stream, params,
+                                                   nullptr, e0, e1, 0));
+        pending.push_back({e0, e1});
+        return MPPI_HIP_OK;
+    }
+    MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, ctx->stream, params, nullptr));
+    return MPPI_HIP_OK;
+}
+
 // Enqueue one step on the context's stream: the rollout and reduce launches,
-// or the one-launch step.
-int enqueue_step(mppi_hip_batch* b, const mppi_step_args& base) {
+// or the one-launch step.  With `prof` (never under capture) each launch is
+// timed by its own events: the rollout, or the one-launch step, as the
+// profile's rollout time, the reduce as its reduce time.
+int enqueue_step(mppi_hip_batch* b, const mppi_step_args& base, bool prof = false) {
     mppi_hip_ctx* ctx = b->ctx;
     const int bi = block_index(b->tuning.block);
     const int pm = b->tuning.persistent ? 1 : 0;
@@ -204,18 +224,17 @@ int enqueue_step(mppi_hip_batch* b, const mppi_step_args& base) {
     void* dp[] = {&da};
     const uint32_t rgrid = pm ? std::min(nr, pgrid) : nr;
     const uint32_t dgrid = pm ? std::min(nd, pgrid) : nd;
+    int rc;
     if (b->tuning.launches == 1) {
         ra.ticket = tickets + 2;
-        MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(b->mod->step_b[bi][pm], rgrid, 1, 1, B, 1, 1, 0,
-                                                ctx->stream, rp, nullptr));
-        return MPPI_HIP_OK;
+        if ((rc = launch_timed(ctx, b->mod->step_b[bi][pm], rgrid, B, rp, prof, ctx->pending_rollout))) return rc;
+    } else {
+        ra.ticket = tickets + 0;
+        da.ticket = tickets + 1;
+        if ((rc = launch_timed(ctx, b->mod->rollout_b[bi][pm], rgrid, B, rp, prof, ctx->pending_rollout))) return rc;
+        if ((rc = launch_timed(ctx, b->mod->reduce_b[bi][pm], dgrid, B, dp, prof, ctx->pending_reduce))) return rc;
     }
-    ra.ticket = tickets + 0;
-    da.ticket = tickets + 1;
-    MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(b->mod->rollout_b[bi][pm], rgrid, 1, 1, B, 1, 1, 0,
-                                            ctx->stream, rp, nullptr));
-    MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(b->mod->reduce_b[bi][pm], dgrid, 1, 1, B, 1, 1, 0,
-                                            ctx->stream, dp, nullptr));
+    if (prof) ctx->prof_launches += 1;
     return MPPI_HIP_OK;
 }
 
@@ -255,6 +274,14 @@ int run_steps(mppi_hip_batch* b, uint32_t T, int closed_loop, float* hs, float* 
     a.hist_states = hs;
     a.hist_controls = hc;
     a.hist_costs = hk;
+    if (b->ctx->profiling) {
+        // Timed launches, one step at a time, outside any graph.
+        for (uint32_t s = 0; s < T; ++s) {
+            int rc = enqueue_step(b, a, true);
+            if (rc) return rc;
+        }
+        return MPPI_HIP_OK;
+    }
     hipGraphExec_t exec = nullptr;
     int rc = step_graph(b, a, &exec);
     if (rc) return rc;

@@ -426,6 +426,15 @@ uint32_t default_spt(const mppi_hip_ctx* ctx, uint32_t K, uint32_t n_groups, uin
     return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(64, spt)));
 }
 
+// Floats of partial sums one agent of shape d needs: the reduce's chunk
+// partials or the one-pass step's slice-major tile sums, rounded up to whole
+// 128-byte lines so that every agent's rows start on a line.
+uint64_t partial_floats(const mppi_agent_desc& d, uint32_t C) {
+    const uint64_t reduce = uint64_t(d.n_chunks) * d.NB * 4;
+    const uint64_t fused = uint64_t(d.n_blk) * mppi_combine_slices(d.H, C) * mppi_combine_width(C);
+    return (std::max(reduce, fused) + 31) / 32 * 32;
+}
+
 // Shape of agent 0 of N agents sharing the context's K and H (offsets 0).
 mppi_agent_desc uniform_desc(const mppi_hip_ctx* ctx, uint32_t N) {
     mppi_agent_desc d{};
@@ -450,9 +459,7 @@ std::vector<mppi_agent_desc> uniform_descs(const mppi_hip_ctx* ctx, uint32_t N) 
         v[a].seq_off = uint64_t(a) * ctx->H * ctx->C;
         v[a].out_off = v[a].seq_off;
         v[a].blk_off = uint64_t(a) * d.n_blk;
-        // room for either the reduce's chunk partials or the one-pass
-        // step's per-workgroup sums
-        v[a].part_off = uint64_t(a) * std::max(d.n_chunks, d.n_blk) * d.NB * 4;
+        v[a].part_off = uint64_t(a) * partial_floats(d, ctx->C);
     }
     return v;
 }
@@ -466,8 +473,7 @@ int ensure_work_buffers(mppi_hip_ctx* ctx, const std::vector<mppi_agent_desc>& d
     if ((rc = ensure_device_buffer(ctx, &ctx->weights, sizeof(float) * nk))) return rc;
     if ((rc = ensure_device_buffer(ctx, &ctx->blk_stats, sizeof(float) * 4 * (last.blk_off + last.n_blk)))) return rc;
     if ((rc = ensure_device_buffer(ctx, &ctx->partials,
-                                   sizeof(float) * (last.part_off + size_t(std::max(last.n_chunks, last.n_blk)) *
-                                                                        last.NB * 4)))) return rc;
+                                   sizeof(float) * (last.part_off + partial_floats(last, ctx->C))))) return rc;
     // u_out | diag (3 per agent) | status (one word per agent)
     const size_t down = sizeof(float) * (last.seq_off + size_t(last.H) * ctx->C + 4 * size_t(N));
     if ((rc = ensure_device_buffer(ctx, &ctx->down, down))) return rc;
@@ -594,9 +600,9 @@ int launch_iteration(mppi_hip_ctx* ctx, const mppi_agent_desc& d, uint32_t N, co
     if (use_fused(ctx, d, N, sur)) {
         // two threads per sample: noise producers and rollout consumers
         MPPI_HIP_TRY(ctx, launch_kernel(st, ctx->mod->rollout_fused, d.n_blk, N, 1, params, e0, e1, 2 * kBlock));
-        // the combine's workgroups: slices of 16 / C timesteps
-        const uint32_t tpw = ctx->C < 16 ? 16 / ctx->C : 1;
-        MPPI_HIP_TRY(ctx, launch_kernel(st, ctx->mod->fused_combine, (d.H + tpw - 1) / tpw, N, 1, params, e2, e3));
+        // the combine's workgroups: one per slice of timesteps
+        MPPI_HIP_TRY(ctx, launch_kernel(st, ctx->mod->fused_combine, mppi_combine_slices(d.H, ctx->C), N, 1, params,
+                                        e2, e3));
         if (bookkeep) {
             ctx->weights_pending = true;
             ctx->weights_args = args;
@@ -1011,8 +1017,7 @@ int mppi_hip_step_noalloc(mppi_hip_ctx* ctx, uint64_t step_index, const float* x
     const size_t off_agent = 256, off_status = 512, off_costs = 768;
     const size_t off_stats = off_costs + align_up(sizeof(float) * shape.K, 256);
     const size_t off_part = off_stats + align_up(sizeof(float) * 4 * shape.n_blk, 256);
-    const size_t part_rows = std::max(shape.n_chunks, shape.n_blk);
-    const size_t total = off_part + align_up(sizeof(float) * part_rows * shape.NB * 4, 256);
+    const size_t total = off_part + align_up(sizeof(float) * partial_floats(shape, ctx->C), 256);
     if (total > kNoallocScratchBytes)
         return fail(ctx, MPPI_HIP_E_INVALID,
                     "mppi_hip_step_noalloc: sample_count and horizon need " + std::to_string(total) +

@@ -100,6 +100,20 @@ template <class X, class M, class C>
 __device__ inline auto mppi_terminal(mppi_form_plain, X x, M m, C c, mppi_time)
     -> decltype(MPPI_TERMINAL_COST(x, m, c)) { return MPPI_TERMINAL_COST(x, m, c); }
 
+// The plugin's optional preparation of its costs for a workgroup
+// (MPPI_COST_PREPARE, mppi_core.h): every thread of the workgroup calls it
+// with the workgroup's cost parameters before the barrier that precedes the
+// time loop; the previous user of the workgroup's memory finished before an
+// earlier barrier.
+__device__ inline void mppi_prepare_costs(const uint8_t* cost, uint32_t threads) {
+#ifdef MPPI_COST_PREPARE
+    MPPI_COST_PREPARE(mppi_blob_ref{cost}, threadIdx.x, threads);
+#else
+    (void)cost;
+    (void)threads;
+#endif
+}
+
 // Wavefront reductions (every lane of the wavefront active).  Within each
 // row of 16 lanes the values are combined through DPP lane patterns, which
 // cost one VALU instruction each instead of an LDS permute and its wait:
@@ -308,6 +322,7 @@ __device__ float mppi_rollout_costs(const mppi_step_args& a, const mppi_agent_de
         lo[i] = a.u_min[i];
         hi[i] = a.u_max[i];
     }
+    mppi_prepare_costs(cost, BLOCK);
     __syncthreads();
 
     float total = 0.0f;
@@ -415,7 +430,8 @@ __device__ void mppi_fused_tile(const mppi_step_args& a, uint32_t agent, uint32_
     if (threadIdx.x < uint32_t(SAMPLES / 64)) s_prog[threadIdx.x] = 0;
     float* row = s_z + sid * ZS;
     float total = 0.0f;
-    __syncthreads();                         // s_un, s_ph and s_prog written
+    mppi_prepare_costs(cost, THREADS);
+    __syncthreads();                         // s_un, s_ph, s_prog and the costs' preparation written
 
     // Producer wavefront w finishes a phase with a workgroup release fence
     // (its LDS writes are complete) and a relaxed store of the phase count
@@ -844,32 +860,43 @@ __device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const 
                 const float4 u = s_q[g][quad];
                 t.x += u.x; t.y += u.y; t.z += u.z; t.w += u.w;
             }
-            // columns past H*C hold normals of the next group or zeros; the
-            // combine never reads them
-            reinterpret_cast<float4*>(a.partials + d.part_off + size_t(blk) * W)[quad] = t;
+            // Slice-major rows (mppi_combine_width): the quad's columns lie
+            // in one slice, as a slice spans a multiple of 32 columns or the
+            // whole sequence.  Columns past H*C hold normals of the next
+            // group or zeros; the combine never reads them.
+            constexpr uint32_t SPAN = mppi_combine_tpw(CDIM) * CDIM, WIDTH = mppi_combine_width(CDIM);
+            const uint32_t c = 4 * quad, sl = c / SPAN;
+            reinterpret_cast<float4*>(a.partials + d.part_off + (size_t(sl) * d.n_blk + blk) * WIDTH +
+                                      (c - sl * SPAN))[0] = t;
         }
     }
     // The rows are read by mppi_fused_combine, a later launch.
 }
 
 // Combine of the one-pass step for one agent: workgroup `slice` handles the
-// timesteps [t0, t1) = [slice*TPW, min((slice+1)*TPW, H)) and, for the
-// warm-start shift, forms timestep t1 as well.  Every workgroup reads all
-// tile statistics (one row per thread) and forms the same normaliser in the
-// same order; its columns' sums over the tiles are requested before that, so
-// the two reads overlap.  Slice 0 also writes the diagnostics and agent
-// state and steps the plant.  In closed loop the shifted warm start goes to
-// u_nom_next: shifting in place would overwrite the timestep t1 that the
-// previous slice reads.
-constexpr uint32_t MPPI_COMBINE_TPW = 16 / CDIM > 0 ? 16 / CDIM : 1;
+// timesteps [t0, t1) = [slice*TPW, min((slice+1)*TPW, H)), whose columns of
+// the tiles' sums are one block of whole 128-byte lines (slice-major rows,
+// mppi_combine_width).  Every workgroup reads all tile statistics (one row
+// per thread, 16 bytes per tile) and forms the same normaliser in the same
+// order; its columns' sums over the tiles are requested before that, so the
+// two reads overlap.  Slice 0 also writes the diagnostics and agent state and
+// steps the plant.  In closed loop the warm start shifted by one timestep
+// goes to u_nom_next: each slice writes its own timesteps t >= 1 to t - 1,
+// and the slice holding H - 1 also repeats it at H - 1, so no slice reads
+// another's timesteps.
+constexpr uint32_t MPPI_COMBINE_TPW = mppi_combine_tpw(CDIM);
 
 template <int BLOCK>
 __device__ void mppi_fused_combine_slice(const mppi_step_args& a, uint32_t agent, uint32_t slice) {
     constexpr int WAVES = BLOCK / 64;
     constexpr uint32_t TPW = MPPI_COMBINE_TPW;
-    constexpr uint32_t MAXCOL = (TPW + 1) * CDIM;
-    // rows (tiles) per thread: at most BLOCK tiles over BLOCK / MAXCOL row groups
-    constexpr int RPT = int((BLOCK + BLOCK / MAXCOL - 1) / (BLOCK / MAXCOL));
+    constexpr uint32_t MAXCOL = mppi_combine_width(CDIM);    // row width, >= TPW * CDIM
+    constexpr uint32_t RG = BLOCK / MAXCOL;                   // row groups
+    static_assert(RG >= 1 && MAXCOL <= MPPI_FUSED_MAX_HC, "a slice fits the workgroup");
+    // rows (tiles) per thread: at most BLOCK tiles over RG row groups,
+    // loaded in batches of at most 32
+    constexpr int RPT = int((BLOCK + RG - 1) / RG);
+    constexpr int BATCH = RPT < 32 ? RPT : 32;
     __shared__ float s_f[BLOCK];
     __shared__ float s_min[WAVES];
     __shared__ float s_sum[3][WAVES];
@@ -880,15 +907,14 @@ __device__ void mppi_fused_combine_slice(const mppi_step_args& a, uint32_t agent
     const uint32_t H = d.H, t0 = slice * TPW;
     if (t0 >= H) return;                               // the whole workgroup
     const uint32_t t1 = t0 + TPW < H ? t0 + TPW : H;   // own timesteps [t0, t1)
-    const uint32_t te = t1 + 1 < H ? t1 + 1 : H;       // formed: [t0, te)
-    const uint32_t c0 = t0 * CDIM, ncol = (te - t0) * CDIM, nown = (t1 - t0) * CDIM;
-    const uint32_t W = d.NB * 4, n_rows = d.n_blk;
+    const uint32_t c0 = t0 * CDIM, ncol = (t1 - t0) * CDIM;
+    const uint32_t n_rows = d.n_blk;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
     // Every global read is issued here, before any reduction: the tiles'
-    // sums of this slice's columns, the tile statistics, this thread's
-    // nominal control and its bounds and noise factors, and (slice 0) the
-    // plant's state, so that the kernel waits on memory once.
+    // sums of this slice's columns (the first batch), the tile statistics,
+    // this thread's nominal control and its bounds and noise factors, and
+    // (slice 0) the plant's state, so that the kernel waits on memory once.
     const uint32_t seq = d.seq_off;
     const uint32_t ci = threadIdx.x % CDIM;
     float un = 0.0f, lo = 0.0f, hi = 0.0f;
@@ -906,15 +932,14 @@ __device__ void mppi_fused_combine_slice(const mppi_step_args& a, uint32_t agent
     }
     // Columns are spread over MAXCOL threads whatever ncol is (threads of
     // columns past ncol load nothing), so the layout is fixed at compile time.
-    constexpr uint32_t RG = BLOCK / MAXCOL;
     const uint32_t col = threadIdx.x % MAXCOL, rg = threadIdx.x / MAXCOL;
     const bool col_ok = col < ncol && rg < RG;
-    const float* rows = a.partials + d.part_off + c0 + col;
-    float v[RPT];
+    const float* rows = a.partials + d.part_off + size_t(slice) * n_rows * MAXCOL + col;
+    float v[BATCH];
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) {
+    for (int q = 0; q < BATCH; ++q) {
         const uint32_t r = rg + uint32_t(q) * RG;
-        v[q] = (col_ok && r < n_rows) ? rows[size_t(r) * W] : 0.0f;
+        v[q] = (col_ok && r < n_rows) ? rows[size_t(r) * MAXCOL] : 0.0f;
     }
     float4 st = {__builtin_inff(), 0.0f, 0.0f, 0.0f};
     if (threadIdx.x < n_rows) st = reinterpret_cast<const float4*>(a.blk_stats + d.blk_off * 4)[threadIdx.x];
@@ -945,9 +970,21 @@ __device__ void mppi_fused_combine_slice(const mppi_step_args& a, uint32_t agent
     // then row groups in order
     float acc = 0.0f;
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) {
+    for (int q = 0; q < BATCH; ++q) {
         const uint32_t r = rg + uint32_t(q) * RG;
         if (col_ok && r < n_rows) acc = mppi_fma(s_f[r], v[q], acc);
+    }
+    for (int q0 = BATCH; q0 < RPT && rg + uint32_t(q0) * RG < n_rows; q0 += BATCH) {
+#pragma unroll
+        for (int q = 0; q < BATCH; ++q) {
+            const uint32_t r = rg + uint32_t(q0 + q) * RG;
+            v[q] = (col_ok && q0 + q < RPT && r < n_rows) ? rows[size_t(r) * MAXCOL] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < BATCH; ++q) {
+            const uint32_t r = rg + uint32_t(q0 + q) * RG;
+            if (col_ok && q0 + q < RPT && r < n_rows) acc = mppi_fma(s_f[r], v[q], acc);
+        }
     }
     s_part[threadIdx.x] = acc;
     __syncthreads();
@@ -971,7 +1008,13 @@ __device__ void mppi_fused_combine_slice(const mppi_step_args& a, uint32_t agent
 #endif
         const float u = mppi_clampf(un + e, lo, hi);
         s_u[n] = u;
-        if (n < nown) a.u_out[d.out_off + c0 + n] = u;
+        a.u_out[d.out_off + c0 + n] = u;
+        if (a.closed_loop && a.u_nom_next) {
+            // the shifted warm start: timestep t to t - 1, and H - 1 kept
+            const uint32_t t = t0 + n / CDIM;
+            if (t >= 1) a.u_nom_next[seq + c0 + n - CDIM] = u;
+            if (t + 1 == H) a.u_nom_next[seq + c0 + n] = u;
+        }
     }
     __syncthreads();
     if (slice == 0) {
@@ -990,10 +1033,6 @@ __device__ void mppi_fused_combine_slice(const mppi_step_args& a, uint32_t agent
             if (a.closed_loop) mppi_plant_step(a, agent, d, s_u, plant, mg);
         }
         if (a.first_out && threadIdx.x < CDIM) a.first_out[agent * CDIM + threadIdx.x] = s_u[threadIdx.x];
-    }
-    if (a.closed_loop && a.u_nom_next && threadIdx.x < nown) {
-        const uint32_t n = threadIdx.x, t = t0 + n / CDIM;
-        a.u_nom_next[seq + c0 + n] = t + 1 < H ? s_u[n + CDIM] : s_u[n];
     }
 }
 
@@ -1398,6 +1437,7 @@ extern "C" __global__ void __launch_bounds__(MPPI_SUR_BLOCK) mppi_rollout_surrog
         s_b[MPPI_MLP_HID + e] = sa.params[sa.b2 + e];
     }
     if (threadIdx.x < 16) s_b[2 * MPPI_MLP_HID + threadIdx.x] = threadIdx.x < SDIM ? sa.params[sa.b3 + threadIdx.x] : 0.0f;
+    mppi_prepare_costs(a.cost + d.slot * a.cost_stride, MPPI_SUR_BLOCK);
     __syncthreads();
 
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;

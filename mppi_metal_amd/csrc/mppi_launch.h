@@ -49,6 +49,23 @@ struct mppi_agent_desc {
     uint64_t out_off;            // u_out (floats, H*C per agent)
 };
 
+// Slices of the one-pass step's combine.  The combine splits the horizon
+// into slices of mppi_combine_tpw(C) timesteps, one workgroup each; the
+// rollout tiles store their weighted noise sums slice-major (slice s, tile g:
+// row s * n_blk + g), each row mppi_combine_width(C) floats, a multiple of 32
+// floats (128 bytes), so that a slice reads whole cache lines.  The slice
+// spans lcm(C, 32) columns of the sequence where that is at most 128 (the
+// one-pass step's H * C limit), else floor(128 / C) timesteps, which then
+// cover the whole horizon.
+constexpr uint32_t mppi_pow2_factor(uint32_t c) { return (c & (0u - c)) > 32u ? 32u : (c & (0u - c)); }
+constexpr uint32_t mppi_combine_tpw(uint32_t C) {
+    return (32u / mppi_pow2_factor(C)) * C <= 128u ? 32u / mppi_pow2_factor(C) : 128u / C;
+}
+constexpr uint32_t mppi_combine_width(uint32_t C) { return (mppi_combine_tpw(C) * C + 31u) / 32u * 32u; }
+constexpr uint32_t mppi_combine_slices(uint32_t H, uint32_t C) {
+    return (H + mppi_combine_tpw(C) - 1u) / mppi_combine_tpw(C);
+}
+
 struct mppi_step_args {
     const mppi_agent_desc* desc; // [N]
     mppi_agent_state* agents;    // [N]

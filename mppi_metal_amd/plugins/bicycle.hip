@@ -76,6 +76,121 @@ MPPI_CALLABLE void bicycle_dynamics(float* s, const float* u, const uint8_t* mod
     }
 }
 
+// Zone pre-filter (GPU).  A sample counts zone i (i < n_zones) when
+// d2 < r*r with d2 = fl(fl(ox^2) + fl(oy^2)), ox = fl(x - cx), oy = fl(y - cy);
+// that implies |x - cx| < r (1 + 2^-22) and likewise for y (fl(ox^2) <=
+// fl(d2) < fl(r^2), each rounding within 2^-24), so the sample lies in the
+// square of half-side r' = r (1 + 2^-20) around the centre.  Each workgroup
+// maps the plane to a BICYCLE_GRID x BICYCLE_GRID grid of square cells whose
+// frame (bicycle_frame) leaves two empty cells around the union of those
+// squares, and keeps in LDS, for every column and every row of cells, the
+// set of zones whose square, widened by 1/8 cell, meets it
+// (bicycle_prepare).  A sample's set, its column's and row's sets
+// intersected, holds every zone it can count.  The frame's arithmetic
+// (relative error a few 2^-24 of coordinates within G + 2 cells, and of the
+// squares' bounds) moves a sample or a bound by far less than the 1/8 cell
+// margin; a frame whose coordinates cannot resolve 1/64 of a cell is not
+// used, and its zones are tested by every wavefront instead, as are radii
+// outside [2^-60, 2^60].  Zones with r <= 0 or NaN, or a centre that is not
+// finite, never count and are in no set.  Only the exact pass decides what
+// counts, and it runs for a zone in every wavefront where some sample's set
+// holds it, so the cost is the exact sum over the zones, which is what the
+// host computes.
+#define BICYCLE_GRID 64
+
+struct BicycleFrame {
+    float x0, y0, inv;   // cell coordinates u = (x - x0) inv, v = (y - y0) inv
+    uint32_t grid;       // zones on the grid
+    uint32_t always;     // zones every wavefront tests
+};
+
+MPPI_HD bool bicycle_finite(float v) { return (mppi_f2u(v) & 0x7f800000u) != 0x7f800000u; }
+
+MPPI_HD float bicycle_square_half(float r) { return r * 1.00000095367431640625f; }   // r (1 + 2^-20)
+
+MPPI_HD BicycleFrame bicycle_frame(const BicycleCost* c) {
+    const uint32_t nz = c->n_zones < BICYCLE_MAX_ZONES ? c->n_zones : BICYCLE_MAX_ZONES;
+    float xlo = 3.0e38f, xhi = -3.0e38f, ylo = 3.0e38f, yhi = -3.0e38f;
+    uint32_t grid = 0, always = 0;
+    for (uint32_t j = 0; j < nz; ++j) {
+        const float cx = c->zones[j][0], cy = c->zones[j][1], r = c->zones[j][2];
+        if (!(r > 0.0f) || !bicycle_finite(cx) || !bicycle_finite(cy)) continue;
+        if (!(r >= 0x1p-60f && r <= 0x1p60f)) {
+            always |= 1u << j;
+            continue;
+        }
+        const float h = bicycle_square_half(r);
+        xlo = __builtin_fminf(xlo, cx - h);
+        xhi = __builtin_fmaxf(xhi, cx + h);
+        ylo = __builtin_fminf(ylo, cy - h);
+        yhi = __builtin_fmaxf(yhi, cy + h);
+        grid |= 1u << j;
+    }
+    BicycleFrame f{0.0f, 0.0f, 0.0f, grid, always};
+    if (grid) {
+        const float e = __builtin_fmaxf(xhi - xlo, yhi - ylo);
+        const float w = e * (1.0f / float(BICYCLE_GRID - 4));   // two cells of margin each side
+        const float inv = 1.0f / w;
+        const float m = __builtin_fmaxf(__builtin_fmaxf(mppi_fabsf(xlo), mppi_fabsf(xhi)),
+                                        __builtin_fmaxf(mppi_fabsf(ylo), mppi_fabsf(yhi)));
+        if (bicycle_finite(e) && w > 0.0f && bicycle_finite(inv) && m * 0x1p-17f <= w) {
+            f.x0 = xlo - 2.0f * w;
+            f.y0 = ylo - 2.0f * w;
+            f.inv = inv;
+        } else {
+            f.always |= grid;
+            f.grid = 0;
+        }
+    }
+    return f;
+}
+
+#if defined(__HIP_DEVICE_COMPILE__)
+// Zone sets of the columns (x) and rows (y) of cells; entry BICYCLE_GRID,
+// empty, takes every coordinate at or past the last cell.  The frame (x0,
+// y0, inv, always) is in LDS too, read by the costs with one broadcast load.
+__shared__ uint32_t bicycle_cols[BICYCLE_GRID + 1];
+__shared__ uint32_t bicycle_rows[BICYCLE_GRID + 1];
+__shared__ __attribute__((aligned(16))) float bicycle_frame_lds[4];
+
+// The workgroup's preparation of its costs (MPPI_COST_PREPARE, mppi_core.h).
+__device__ void bicycle_prepare(const uint8_t* cost_raw, uint32_t thread, uint32_t threads) {
+    const BicycleCost* c = reinterpret_cast<const BicycleCost*>(cost_raw);
+    const BicycleFrame f = bicycle_frame(c);
+    if (thread == 0) {
+        bicycle_frame_lds[0] = f.x0;
+        bicycle_frame_lds[1] = f.y0;
+        bicycle_frame_lds[2] = f.inv;
+        bicycle_frame_lds[3] = mppi_u2f(f.always);
+    }
+    for (uint32_t i = thread; i < 2 * (BICYCLE_GRID + 1); i += threads) {
+        const bool col = i <= BICYCLE_GRID;
+        const uint32_t band = col ? i : i - (BICYCLE_GRID + 1);
+        const float o = col ? f.x0 : f.y0;
+        uint32_t m = 0;
+        if (band < BICYCLE_GRID) {
+            for (uint32_t j = 0; j < BICYCLE_MAX_ZONES; ++j) {
+                if (!((f.grid >> j) & 1u)) continue;
+                const float uc = (c->zones[j][col ? 0 : 1] - o) * f.inv;
+                const float ur = bicycle_square_half(c->zones[j][2]) * f.inv;
+                if (uc + ur >= float(band) - 0.125f && uc - ur <= float(band) + 1.125f) m |= 1u << j;
+            }
+        }
+        (col ? bicycle_cols : bicycle_rows)[band] = m;
+    }
+}
+#define MPPI_COST_PREPARE bicycle_prepare
+
+// The zones sample (x, y) can count: its column's and row's sets, and the
+// zones every wavefront tests.
+__device__ uint32_t bicycle_candidates(float x, float y) {
+    const float4 f = *reinterpret_cast<const float4*>(bicycle_frame_lds);
+    const float u = __builtin_fminf(__builtin_fmaxf((x - f.x) * f.z, 0.0f), float(BICYCLE_GRID));
+    const float v = __builtin_fminf(__builtin_fmaxf((y - f.y) * f.z, 0.0f), float(BICYCLE_GRID));
+    return (bicycle_cols[uint32_t(u)] & bicycle_rows[uint32_t(v)]) | mppi_f2u(f.w);
+}
+#endif
+
 MPPI_CALLABLE float bicycle_pursuit_cost(const float* s, const float* u, const uint8_t* model,
                                          const uint8_t* cost_raw, mppi_time tm) {
     const BicycleCost* c = reinterpret_cast<const BicycleCost*>(cost_raw);
@@ -86,37 +201,31 @@ MPPI_CALLABLE float bicycle_pursuit_cost(const float* s, const float* u, const u
     cost += c->w_control * (u[0] * u[0] + u[1] * u[1]);
     float pen = 0.0f;
     const uint32_t nz = c->n_zones < BICYCLE_MAX_ZONES ? c->n_zones : BICYCLE_MAX_ZONES;
-    // The square root is only needed inside a zone.  A first pass tests the
-    // square |ox|, |oy| < r around each zone (inside the circle in the
-    // rounded arithmetic below implies inside the square, rounding being
-    // monotone), eight zones per group so that their parameter loads issue
-    // together; slots in [nz, nz rounded up to 8) may hold anything and only
-    // send a wavefront to the exact pass.  The exact pass, taken only by
-    // wavefronts with some sample in some square, visits the zones in order
-    // and measures the distance only for zones whose square holds some sample
-    // of the wavefront.
-    const uint32_t nz8 = (nz + 7u) & ~7u;
-    float margin = 1.0f;        // min over zones of max(|ox|, |oy|) - r
-    for (uint32_t i = 0; i < nz8; i += 8) {
-        MPPI_UNROLL
-        for (uint32_t j = 0; j < 8; ++j) {
-            const float ox = s[0] - c->zones[i + j][0];
-            const float oy = s[1] - c->zones[i + j][1];
-            margin = __builtin_fminf(margin, __builtin_fmaxf(__builtin_fabsf(ox), __builtin_fabsf(oy)) -
-                                                 c->zones[i + j][2]);
-        }
-    }
-    if (__builtin_expect(mppi_any(margin < 0.0f), 0)) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    // The exact pass keeps zone order and runs only in wavefronts where some
+    // sample can count some zone, and there only for those zones.
+    const uint32_t cand = bicycle_candidates(s[0], s[1]);
+    if (__builtin_expect(mppi_any(cand != 0u), 0)) {
         for (uint32_t i = 0; i < nz; ++i) {
-            const float ox = s[0] - c->zones[i][0];
-            const float oy = s[1] - c->zones[i][1];
-            const float rz = c->zones[i][2];
-            if (mppi_any(__builtin_fmaxf(__builtin_fabsf(ox), __builtin_fabsf(oy)) < rz)) {
+            if (mppi_any(((cand >> i) & 1u) != 0u)) {
+                const float ox = s[0] - c->zones[i][0];
+                const float oy = s[1] - c->zones[i][1];
+                const float rz = c->zones[i][2];
                 const float d2 = ox * ox + oy * oy;
                 if (d2 < rz * rz) pen += rz - mppi_sqrtf(d2);
             }
         }
     }
+#else
+    // The definition, zone by zone in the same order.
+    for (uint32_t i = 0; i < nz; ++i) {
+        const float ox = s[0] - c->zones[i][0];
+        const float oy = s[1] - c->zones[i][1];
+        const float rz = c->zones[i][2];
+        const float d2 = ox * ox + oy * oy;
+        if (rz > 0.0f && d2 < rz * rz) pen += rz - mppi_sqrtf(d2);
+    }
+#endif
     return cost + c->w_zone * pen;
 }
 

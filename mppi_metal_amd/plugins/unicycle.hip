@@ -84,8 +84,8 @@ MPPI_CALLABLE float mppi_stage_cost(const float* state, const float* control,
 // more than four times that.  Radii outside [2^-60, 2^60] are always sent to
 // the exact pass (a = 0); a slot with r <= 0 or NaN is never counted, as its
 // hinge max(0, r - |p - c|) is 0 (bx = by = 2 keeps its square empty).  The
-// gate only decides where the exact pass runs, so the estimate may differ
-// between the GPU and the host in its last bit: the costs do not.
+// gate only decides where the exact pass runs: the costs are the exact sum
+// over the obstacles, which is what the host computes (without the gate).
 struct UnicycleGate {
     float a, bx, by;
 };
@@ -106,6 +106,7 @@ MPPI_CALLABLE float unicycle_stage_cost_obstacles(const float* state, const floa
     float cost = mppi_stage_cost(state, control, model_params, cost_params_raw);
     float pen = 0.0f;
     uint32_t n = p->n_obstacles < UNICYCLE_MAX_OBSTACLES ? p->n_obstacles : UNICYCLE_MAX_OBSTACLES;
+#if defined(__HIP_DEVICE_COMPILE__)
     // max(0, r - |p - c|) summed over the obstacles.  The square root is
     // only needed inside an obstacle, so a cheap first pass tests the square
     // |ox|, |oy| < r around each circle, in coordinates scaled by the radius
@@ -141,6 +142,16 @@ MPPI_CALLABLE float unicycle_stage_cost_obstacles(const float* state, const floa
             }
         }
     }
+#else
+    // The definition, slot by slot in the same order.
+    for (uint32_t i = 0; i < n; ++i) {
+        const float ox = state[0] - p->obstacles[i][0];
+        const float oy = state[1] - p->obstacles[i][1];
+        const float r = p->obstacles[i][2];
+        const float d2 = ox * ox + oy * oy;
+        if (r > 0.0f && d2 < r * r) pen += r - mppi_sqrtf(d2);
+    }
+#endif
     return cost + p->obstacle_weight * pen;
 }
 

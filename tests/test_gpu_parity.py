@@ -21,7 +21,7 @@ import pytest
 
 from mppi_metal_amd import HipMPPI, scenes
 from problems import check_step as _check_step
-from problems import double_integrator, rel_err, report as _report, seq_rel_err, unicycle
+from problems import double_integrator, point_mass3, rel_err, report as _report, seq_rel_err, unicycle
 
 pytestmark = pytest.mark.gpu
 
@@ -85,6 +85,30 @@ def test_simulate_matches_oracle(require_gpu, path):
     x0 = np.zeros(4, np.float32)
     st, ct, kt = hip.simulate(6, 0, x0, np.zeros((32, 2), np.float32))
     so, co, ko, _ = ora.simulate(6, 0, x0, threads=8)
+    assert kt[0] == ko[0, 0]
+    assert np.max(np.abs(ct[0] - co[0, 0])) <= 1e-5
+    assert np.max(np.abs(st - so[:, 0])) <= CLOSED_LOOP_TOL
+    assert np.max(np.abs(ct - co[:, 0])) <= CLOSED_LOOP_TOL
+    assert rel_err(kt, ko[:, 0]) <= CLOSED_LOOP_TOL
+
+
+@PATHS
+@pytest.mark.parametrize("problem,H", [("unicycle", 50), ("point_mass3", 40)])
+def test_simulate_ragged_combine_slices(require_gpu, path, problem, H):
+    """Closed loop where the one-pass combine's last slice of timesteps is
+    partial (C = 2: slices of 16 timesteps, H = 50; C = 3: slices of 32, H =
+    40): the update of every slice and the shifted warm start, which each
+    slice writes for its own timesteps, against the oracle."""
+    if problem == "unicycle":
+        hip, ora, _ = unicycle(1024, H, 8)
+        x0, C = np.zeros(4, np.float32), 2
+    else:
+        hip, ora = point_mass3(1024, H)
+        x0, C = np.array([0.2, -0.1, 0.4, 0.3, 0.0, -0.2], np.float32), 3
+    hip.set_step_path(path)
+    u0 = (np.random.default_rng(3).standard_normal((H, C)) * 0.2).astype(np.float32)
+    st, ct, kt = hip.simulate(5, 0, x0, u0)
+    so, co, ko, _ = ora.simulate(5, 0, x0, u_nom_packed=u0, threads=8)
     assert kt[0] == ko[0, 0]
     assert np.max(np.abs(ct[0] - co[0, 0])) <= 1e-5
     assert np.max(np.abs(st - so[:, 0])) <= CLOSED_LOOP_TOL
