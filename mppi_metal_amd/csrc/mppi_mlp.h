@@ -34,12 +34,12 @@
 // Activation: the hidden layers apply T(z) = bf16(tanh(r16(z))), r16 the
 // rounding of the FP32 pre-activation to binary16 toward zero (the
 // hardware's packed conversion), bf16 the nearest-even rounding of the exact
-// tanh.  T is read from a table of the bf16 results of every binary16
-// magnitude in [0, 4] (17,409 entries, built on the host in FP64); larger
-// magnitudes (and infinities, NaN) give +-1, as bf16(tanh(4)) already is.
-// The activations are bf16 operands of the next layer anyway, so T is tanh
-// at the precision the network keeps, for about a quarter of the vector
-// work of an exp and a reciprocal.
+// tanh.  T is read from a table indexed by the binary16 magnitude bits
+// (32,768 entries, built on the host in FP64): magnitudes above 4 (and
+// infinities, NaN) hold the entry of 4, +-1, as bf16(tanh(4)) already is,
+// so that no cap is needed before the read.  The activations are bf16
+// operands of the next layer anyway, so T is tanh at the precision the
+// network keeps, for six vector instructions per pair of values.
 #ifndef MPPI_HIP_MLP_H
 #define MPPI_HIP_MLP_H
 
@@ -61,7 +61,7 @@
 #define MPPI_MLP_X32_W3 32
 #define MPPI_MLP_X32_FRAGS 36
 #define MPPI_MLP_TOTAL_ELEMS (MPPI_MLP_X32_BASE + MPPI_MLP_X32_FRAGS * MPPI_MLP_X32_ELEMS)
-#define MPPI_MLP_TANH_KEYS 0x4401                                      // binary16 magnitudes 0 .. 4.0
+#define MPPI_MLP_TANH_KEYS 0x8000                                      // every binary16 magnitude
 
 // Offsets of the FP32 parameter vector [W1 | b1 | W2 | b2 | W3 | b3] (row
 // major [out][in]) for in / out dimensions.
@@ -111,15 +111,23 @@ typedef __bf16 mppi_v8bf16 __attribute__((ext_vector_type(8)));
 typedef unsigned short mppi_u16x2 __attribute__((ext_vector_type(2)));
 
 // T of two pre-activations, as a pair of bf16 (z0 in the low half): the
-// packed conversion to binary16 toward zero, magnitudes capped at 4.0
-// (0x4400), one table read each, the sign put back.  `tab` may be in LDS or
-// global memory.
+// packed conversion to binary16 toward zero, one table read per magnitude
+// (the table repeats the entry of 4.0, 0x4400, for every larger magnitude,
+// infinities and NaN included, so no cap is applied here), the sign put
+// back.  `tab` may be in LDS or global memory.
 __device__ inline uint32_t mppi_act2(float z0, float z1, const unsigned short* tab) {
     const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(z0, z1));
-    const mppi_u16x2 cap = {0x4400, 0x4400};
-    const mppi_u16x2 key = __builtin_elementwise_min(__builtin_bit_cast(mppi_u16x2, h & 0x7fff7fffu), cap);
-    const mppi_u16x2 r = {tab[key[0]], tab[key[1]]};
-    return __builtin_bit_cast(uint32_t, r) | (h & 0x80008000u);
+    const uint32_t key = h & 0x7fff7fffu;
+    // Byte offsets of the two entries: the low key's by one shift of its
+    // 16-bit half (an SDWA operand select, which the compiler does not form
+    // here by itself), the high key's by one shift (bit 15 is clear).
+    uint32_t off_lo;
+    asm("v_lshlrev_b32_sdwa %0, 1, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_0"
+        : "=v"(off_lo) : "v"(key));
+    const char* base = reinterpret_cast<const char*>(tab);
+    const uint32_t lo = *reinterpret_cast<const unsigned short*>(base + off_lo);
+    const uint32_t hi = *reinterpret_cast<const unsigned short*>(base + (key >> 15));
+    return __builtin_amdgcn_perm(hi, lo, 0x05040100u) | (h & 0x80008000u);
 }
 
 __device__ inline mppi_v4bf mppi_act4(mppi_v4f z, const unsigned short* tab) {

@@ -376,8 +376,8 @@ mppi_mlp_dims dims_for(uint32_t in, uint32_t out) {
 }
 
 // The activation table (mppi_mlp.h): entry `key` is the bf16 (nearest even)
-// of tanh of the binary16 value whose bits are `key`, for key in
-// [0, 0x4400] (magnitudes 0 .. 4.0), computed in FP64.
+// of tanh of the binary16 value whose bits are min(key, 0x4400) (magnitudes
+// capped at 4.0), computed in FP64, for every key below 0x8000.
 uint16_t bf16_nearest_even(double t) {
     float f = float(t);
     uint32_t u;
@@ -394,11 +394,12 @@ uint16_t bf16_nearest_even(double t) {
 
 std::vector<uint16_t> tanh_table_host() {
     std::vector<uint16_t> t(MPPI_MLP_TANH_KEYS);
-    for (uint32_t key = 0; key < MPPI_MLP_TANH_KEYS; ++key) {
+    for (uint32_t key = 0; key <= 0x4400u; ++key) {
         const uint32_t ex = key >> 10, mant = key & 0x3ffu;
         const double v = ex == 0 ? std::ldexp(double(mant), -24) : std::ldexp(double(1024 + mant), int(ex) - 25);
         t[key] = bf16_nearest_even(std::tanh(v));
     }
+    for (uint32_t key = 0x4401u; key < MPPI_MLP_TANH_KEYS; ++key) t[key] = t[0x4400u];
     return t;
 }
 
