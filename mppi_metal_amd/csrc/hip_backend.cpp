@@ -332,6 +332,8 @@ int load_module(mppi_hip_ctx* ctx, bool batch, bool surrogate, std::shared_ptr<L
               hipModuleGetFunction(&m->reduce, m->module, "mppi_reduce") == hipSuccess &&
               hipModuleGetFunction(&m->rollout_fused, m->module, "mppi_rollout_fused") == hipSuccess &&
               hipModuleGetFunction(&m->fused_combine, m->module, "mppi_fused_combine") == hipSuccess &&
+              hipModuleGetFunction(&m->rollout_fused_one, m->module, "mppi_rollout_fused_one") == hipSuccess &&
+              hipModuleGetFunction(&m->fused_combine_one, m->module, "mppi_fused_combine_one") == hipSuccess &&
               hipModuleGetFunction(&m->weights, m->module, "mppi_weights") == hipSuccess;
     if (ok && batch) {
         const char* blocks[3] = {"64", "128", "256"};
@@ -598,11 +600,15 @@ int launch_iteration(mppi_hip_ctx* ctx, const mppi_agent_desc& d, uint32_t N, co
     const bool sur = surrogate_rollout(ctx, args, &sur_fn, sur_args, sizeof(sur_args));
     if (bookkeep) ctx->weights_pending = false;
     if (use_fused(ctx, d, N, sur)) {
-        // two threads per sample: noise producers and rollout consumers
-        MPPI_HIP_TRY(ctx, launch_kernel(st, ctx->mod->rollout_fused, d.n_blk, N, 1, params, e0, e1, 2 * kBlock));
+        // two threads per sample: noise producers and rollout consumers; one
+        // agent described in the arguments takes the kernels that read its
+        // descriptor and step counter from there
+        const bool one = N == 1 && args.one;
+        MPPI_HIP_TRY(ctx, launch_kernel(st, one ? ctx->mod->rollout_fused_one : ctx->mod->rollout_fused, d.n_blk,
+                                        N, 1, params, e0, e1, 2 * kBlock));
         // the combine's workgroups: one per slice of timesteps
-        MPPI_HIP_TRY(ctx, launch_kernel(st, ctx->mod->fused_combine, mppi_combine_slices(d.H, ctx->C), N, 1, params,
-                                        e2, e3));
+        MPPI_HIP_TRY(ctx, launch_kernel(st, one ? ctx->mod->fused_combine_one : ctx->mod->fused_combine,
+                                        mppi_combine_slices(d.H, ctx->C), N, 1, params, e2, e3));
         if (bookkeep) {
             ctx->weights_pending = true;
             ctx->weights_args = args;
@@ -695,9 +701,12 @@ int run_closed_loop(mppi_hip_ctx* ctx, uint32_t N, uint32_t T, uint64_t start_st
     // and its update writes the shifted sequence into the other.
     uint8_t* up = static_cast<uint8_t*>(ctx->up.ptr);
     float* warm[2] = {reinterpret_cast<float*>(up + plan.u_nom), reinterpret_cast<float*>(up + plan.u_nom2)};
+    a.one = N == 1 ? 1u : 0u;
+    a.desc0 = desc[0];
     for (uint32_t s = 0; s < T; ++s) {
         a.u_nom = warm[s & 1u];
         a.u_nom_next = warm[(s + 1) & 1u];
+        a.step0 = uint32_t(start_step + s);   // the device counter's value at step s
         if ((rc = launch_iteration(ctx, desc[0], uint32_t(desc.size()), a))) return rc;
     }
     // The requested parts and the status words, in one transfer: from the
@@ -986,6 +995,9 @@ int mppi_hip_step(mppi_hip_ctx* ctx, uint64_t step_index, const float* x0, const
     UploadPlan plan;
     if ((rc = upload(ctx, desc, step_index, x0, u_nominal, mo, mo_bytes, co, co_bytes, &plan))) return rc;
     mppi_step_args a = plain_args(ctx, plan, 1, mo, mo_bytes, co, co_bytes);
+    a.one = 1;
+    a.desc0 = desc[0];
+    a.step0 = uint32_t(step_index);
     if ((rc = launch_iteration(ctx, desc[0], uint32_t(desc.size()), a))) return rc;
     const size_t hc = size_t(ctx->H) * ctx->C;
     MPPI_HIP_TRY(ctx, hipMemcpyAsync(ctx->down_host.ptr, ctx->down.ptr, sizeof(float) * (hc + 4),
@@ -1039,6 +1051,9 @@ int mppi_hip_step_noalloc(mppi_hip_ctx* ctx, uint64_t step_index, const float* x
     a.weights = nullptr;
     a.blk_stats = reinterpret_cast<float*>(base + off_stats);
     a.partials = reinterpret_cast<float*>(base + off_part);
+    a.one = 1;
+    a.desc0 = shape;
+    a.step0 = uint32_t(step_index);
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
     MPPI_HIP_TRY(ctx, noalloc_init(st, shape, uint32_t(step_index), const_cast<mppi_agent_desc*>(a.desc), a.agents,
                                    a.status_out));

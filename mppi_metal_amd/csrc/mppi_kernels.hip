@@ -389,33 +389,33 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
 // its noise sums anyway; threads [0, SAMPLES) run the rollout and read each
 // group's normals from their row.  Every SIMD holds one wavefront of each
 // kind, so the scalar instructions, branches, hazard waits and quarter-rate
-// multiplies of one issue beside the other's vector work.  The horizon runs
-// in phases of MPPI_PHASE_G groups: a consumer runs phase p once its
-// producer has published it, while the producer goes on with the next ones.
+// multiplies of one issue beside the other's vector work.  A producer
+// publishes how many groups it has finished (after each of the first
+// MPPI_PHASE_G groups, then every MPPI_PHASE_G groups): a consumer runs group
+// g once the count is past g, while the producer goes on with the next ones.
 // ---------------------------------------------------------------------------
-constexpr uint32_t MPPI_PHASE_G = 4;    // groups of timesteps per phase
+constexpr uint32_t MPPI_PHASE_G = 4;    // groups of timesteps per published count
 
-template <int SAMPLES>
+template <int SAMPLES, bool ONE>
 __device__ void mppi_fused_tile(const mppi_step_args& a, uint32_t agent, uint32_t blk) {
     constexpr int THREADS = 2 * SAMPLES;
     constexpr int ZS = MPPI_FUSED_ZS;
-    const mppi_agent_desc d = a.desc[agent];
+    const mppi_agent_desc d = ONE ? a.desc0 : a.desc[agent];
     const bool producer = threadIdx.x >= uint32_t(SAMPLES);
     const uint32_t sid = producer ? threadIdx.x - SAMPLES : threadIdx.x;
     const uint32_t k = blk * SAMPLES + sid;
     const uint32_t H = d.H;
     const bool active = k < d.K;
     const uint32_t slot = d.slot;
-    const uint32_t step = a.agents[slot].step;
+    const uint32_t step = ONE ? a.step0 : a.agents[slot].step;
     const uint32_t tag_agent = (MPPI_TAG_CONTROL_NOISE << 28) | (slot & MPPI_AGENT_MASK);
     const uint32_t dyn_tag = (MPPI_TAG_DYNAMICS << 28) | (slot & MPPI_AGENT_MASK);
     const uint8_t* model = a.model + slot * a.model_stride;
     const uint8_t* cost = a.cost + slot * a.cost_stride;
     const uint32_t n_groups = (H + MPPI_G - 1) / MPPI_G;
-    const uint32_t n_phases = (n_groups + MPPI_PHASE_G - 1) / MPPI_PHASE_G;
 
-    // The nominal sequence (read by the consumers a phase at a time, so the
-    // LDS latency is waited out once per phase), the shared Philox words of
+    // The nominal sequence (read by the consumers with each group's
+    // normals), the shared Philox words of
     // every noise block (mppi_philox_shared) and the normals, one row per
     // sample ([sample][ZS], rows 16-byte aligned).
     __shared__ __attribute__((aligned(16))) float s_un[MPPI_FUSED_MAX_HC + MPPI_NG];
@@ -425,7 +425,7 @@ __device__ void mppi_fused_tile(const mppi_step_args& a, uint32_t agent, uint32_
     for (uint32_t b = threadIdx.x; b < n_groups * MPPI_BG; b += THREADS)
         s_ph[b] = mppi_philox_shared(b, step, tag_agent, a.key0, a.key1);
     // Progress of producer wavefront w (the rows of samples [64 w, 64 w + 64),
-    // which consumer wavefront w reads): phases completed.
+    // which consumer wavefront w reads): groups finished.
     __shared__ uint32_t s_prog[SAMPLES / 64];
     if (threadIdx.x < uint32_t(SAMPLES / 64)) s_prog[threadIdx.x] = 0;
     float* row = s_z + sid * ZS;
@@ -433,12 +433,13 @@ __device__ void mppi_fused_tile(const mppi_step_args& a, uint32_t agent, uint32_
     mppi_prepare_costs(cost, THREADS);
     __syncthreads();                         // s_un, s_ph, s_prog and the costs' preparation written
 
-    // Producer wavefront w finishes a phase with a workgroup release fence
-    // (its LDS writes are complete) and a relaxed store of the phase count
-    // to s_prog[w]; consumer wavefront w reads s_prog[w] with relaxed loads
-    // at the start of each phase until the phase is there, and adds a zero
-    // taken from that value (bit 31, never set) to its row address, so its
-    // reads of the phase's normals are issued only after the count arrived.
+    // Producer wavefront w publishes with a workgroup release fence (its LDS
+    // writes are complete) and a relaxed store of the count of finished
+    // groups to s_prog[w]; consumer wavefront w reads s_prog[w] with relaxed
+    // loads before a group past the count it last saw, until the group is
+    // there, and adds a zero taken from that value (bit 31, never set) to its
+    // row address, so its reads of the group's normals are issued only after
+    // the count arrived.
     // No barrier or fence sits in the consumers' time loop: both keep the
     // compiler from hoisting the plugin's loop-invariant parameter loads and
     // set-up out of it.  The producers never wait, so the wait ends; it is
@@ -454,32 +455,33 @@ __device__ void mppi_fused_tile(const mppi_step_args& a, uint32_t agent, uint32_
     // whenever one is ready and fills the gaps with producer work.
     if (producer) {
         const uint32_t ph_e = mppi_philox_lane_key(step, a.key0);
-        for (uint32_t p = 0; p < n_phases; ++p) {
 #pragma unroll 1
-            for (uint32_t gi = 0; gi < MPPI_PHASE_G; ++gi) {
-                const uint32_t g = p * MPPI_PHASE_G + gi;
-                if (g < n_groups) {
+        for (uint32_t g = 0; g < n_groups; ++g) {
 #pragma unroll
-                    for (int q = 0; q < MPPI_BG; ++q) {
-                        const uint32_t b = g * MPPI_BG + q;
-                        mppi_f32x4 v = mppi_normal4_words(mppi_philox_lanes(k, ph_e, s_ph[b], a.key0, a.key1));
-                        // samples past K keep zero noise, so the tile's
-                        // noise sums need no select
-                        if (!active) v = mppi_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-                        if (MPPI_FUSED_OVERHANG_OK || (b + 1) * 4 <= H * CDIM) {
-                            reinterpret_cast<float4*>(row)[b] = float4{v.x, v.y, v.z, v.w};
-                        } else {
-                            const float vv[4] = {v.x, v.y, v.z, v.w};
+            for (int q = 0; q < MPPI_BG; ++q) {
+                const uint32_t b = g * MPPI_BG + q;
+                mppi_f32x4 v = mppi_normal4_words(mppi_philox_lanes(k, ph_e, s_ph[b], a.key0, a.key1));
+                // samples past K keep zero noise, so the tile's noise sums
+                // need no select
+                if (!active) v = mppi_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                if (MPPI_FUSED_OVERHANG_OK || (b + 1) * 4 <= H * CDIM) {
+                    reinterpret_cast<float4*>(row)[b] = float4{v.x, v.y, v.z, v.w};
+                } else {
+                    const float vv[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-                            for (int j = 0; j < 4; ++j)
-                                if (b * 4 + j < H * CDIM) row[b * 4 + j] = vv[j];
-                        }
-                    }
+                    for (int j = 0; j < 4; ++j)
+                        if (b * 4 + j < H * CDIM) row[b * 4 + j] = vv[j];
                 }
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            if ((threadIdx.x & 63) == 0)
-                __hip_atomic_store(&s_prog[wave], p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            // The count of finished groups is published after each of the
+            // first MPPI_PHASE_G groups, so that the consumers start after
+            // one group, then every MPPI_PHASE_G groups and after the last.
+            const uint32_t done = g + 1;
+            if (done <= MPPI_PHASE_G || done % MPPI_PHASE_G == 0 || done == n_groups) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                if ((threadIdx.x & 63) == 0)
+                    __hip_atomic_store(&s_prog[wave], done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
         }
     } else {
         __builtin_amdgcn_s_setprio(3);
@@ -503,14 +505,14 @@ __device__ void mppi_fused_tile(const mppi_step_args& a, uint32_t agent, uint32_
         bool gave_up = false;
         for (uint32_t t0 = 0; t0 < H; t0 += MPPI_G) {
             const uint32_t g = t0 / MPPI_G;
-            const uint32_t need = g / MPPI_PHASE_G + 1 + a.wait_bias;
+            const uint32_t need = g + 1 + a.wait_bias;
             if (ready < need) {
                 uint32_t got = __hip_atomic_load(&s_prog[wave], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 uint32_t spin = 0;
                 for (; got < need && spin < spin_max; ++spin)
                     got = __hip_atomic_load(&s_prog[wave], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 if (got < need) {
-                    // Gave up: every later phase counts as ready (the rows
+                    // Gave up: every later group counts as ready (the rows
                     // are read as they are); the agent is flagged after the
                     // loop (a global atomic in the loop could alias the
                     // plugin's parameter blob and keep its loads in the loop).
@@ -886,7 +888,7 @@ __device__ void mppi_fused_stats(const mppi_step_args& a, uint32_t agent, const 
 // another's timesteps.
 constexpr uint32_t MPPI_COMBINE_TPW = mppi_combine_tpw(CDIM);
 
-template <int BLOCK>
+template <int BLOCK, bool ONE>
 __device__ void mppi_fused_combine_slice(const mppi_step_args& a, uint32_t agent, uint32_t slice) {
     constexpr int WAVES = BLOCK / 64;
     constexpr uint32_t TPW = MPPI_COMBINE_TPW;
@@ -903,7 +905,7 @@ __device__ void mppi_fused_combine_slice(const mppi_step_args& a, uint32_t agent
     __shared__ float s_part[BLOCK];
     __shared__ float s_z[MAXCOL];
     __shared__ float s_u[MAXCOL];
-    const mppi_agent_desc d = a.desc[agent];
+    const mppi_agent_desc d = ONE ? a.desc0 : a.desc[agent];
     const uint32_t H = d.H, t0 = slice * TPW;
     if (t0 >= H) return;                               // the whole workgroup
     const uint32_t t1 = t0 + TPW < H ? t0 + TPW : H;   // own timesteps [t0, t1)
@@ -1259,7 +1261,13 @@ extern "C" __global__ void __launch_bounds__(256) mppi_reduce(mppi_step_args a) 
 extern "C" __global__ void __launch_bounds__(512) mppi_rollout_fused(mppi_step_args a) {
     const uint32_t agent = blockIdx.y;
     if (blockIdx.x >= a.desc[agent].n_blk) return;
-    mppi_fused_tile<256>(a, agent, blockIdx.x);
+    mppi_fused_tile<256, false>(a, agent, blockIdx.x);
+}
+
+// The same for one agent whose descriptor and step counter are in the
+// arguments (desc0, step0; grid n_blk x 1).
+extern "C" __global__ void __launch_bounds__(512) mppi_rollout_fused_one(mppi_step_args a) {
+    mppi_fused_tile<256, true>(a, 0, blockIdx.x);
 }
 
 // Per-sample weights of the last iteration, w_k = exp(-(c_k - m)/lambda) / S
@@ -1267,7 +1275,11 @@ extern "C" __global__ void __launch_bounds__(512) mppi_rollout_fused(mppi_step_a
 // them; grid (ceil(K / 256), N).
 // Combine of the one-pass step (grid: timestep slices x agents).
 extern "C" __global__ void __launch_bounds__(256) mppi_fused_combine(mppi_step_args a) {
-    mppi_fused_combine_slice<256>(a, blockIdx.y, blockIdx.x);
+    mppi_fused_combine_slice<256, false>(a, blockIdx.y, blockIdx.x);
+}
+
+extern "C" __global__ void __launch_bounds__(256) mppi_fused_combine_one(mppi_step_args a) {
+    mppi_fused_combine_slice<256, true>(a, 0, blockIdx.x);
 }
 
 extern "C" __global__ void __launch_bounds__(256) mppi_weights(mppi_step_args a) {

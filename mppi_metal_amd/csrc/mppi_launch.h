@@ -99,8 +99,14 @@ struct mppi_step_args {
     float lambda;
     uint32_t* status_out;        // [N] copy of each agent's status after the update, or null
     uint32_t wait_limit;         // one-pass rollout: polls before a consumer gives up (0: default)
-    uint32_t wait_bias;          // added to the phase a consumer waits for; nonzero only to test
+    uint32_t wait_bias;          // added to the group count a consumer waits for; nonzero only to test
                                  // the give-up path (the count then never arrives)
+    // One-agent launches (N == 1, one != 0) of the one-pass step: agent 0's
+    // descriptor and step counter, as the host knows them, so that the
+    // kernels start without first loading desc[0] and agents[slot].step.
+    mppi_agent_desc desc0;
+    uint32_t one;
+    uint32_t step0;
 };
 
 // Learned-dynamics surrogate (mppi_mlp.h): training data and rollout.
