@@ -31,6 +31,7 @@ struct LoadedModule {
     hipFunction_t fused_combine = nullptr;  // one-pass step: combine and update
     hipFunction_t rollout_fused_one = nullptr;   // the same for one agent described in the arguments
     hipFunction_t fused_combine_one = nullptr;
+    hipFunction_t finalize = nullptr;       // two-kernel step: the update, one workgroup per agent
     hipFunction_t weights = nullptr;        // per-sample weights after a one-pass step
     // batch kernels, indexed [block 64/128/256 -> 0/1/2][table 0 / persistent 1]
     hipFunction_t rollout_b[3][2] = {};

@@ -233,6 +233,8 @@ int enqueue_step(mppi_hip_batch* b, const mppi_step_args& base, bool prof = fals
         da.ticket = tickets + 1;
         if ((rc = launch_timed(ctx, b->mod->rollout_b[bi][pm], rgrid, B, rp, prof, ctx->pending_rollout))) return rc;
         if ((rc = launch_timed(ctx, b->mod->reduce_b[bi][pm], dgrid, B, dp, prof, ctx->pending_reduce))) return rc;
+        // the update: one workgroup per agent (timed with the reduce)
+        if ((rc = launch_timed(ctx, b->mod->finalize, b->N, 256, dp, prof, ctx->pending_reduce))) return rc;
     }
     if (prof) ctx->prof_launches += 1;
     return MPPI_HIP_OK;

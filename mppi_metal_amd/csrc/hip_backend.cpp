@@ -330,6 +330,7 @@ int load_module(mppi_hip_ctx* ctx, bool batch, bool surrogate, std::shared_ptr<L
         return fail(ctx, MPPI_HIP_E_COMPILE, std::string("hipModuleLoadData failed: ") + hipGetErrorString(e));
     bool ok = hipModuleGetFunction(&m->rollout, m->module, "mppi_rollout") == hipSuccess &&
               hipModuleGetFunction(&m->reduce, m->module, "mppi_reduce") == hipSuccess &&
+              hipModuleGetFunction(&m->finalize, m->module, "mppi_finalize_agents") == hipSuccess &&
               hipModuleGetFunction(&m->rollout_fused, m->module, "mppi_rollout_fused") == hipSuccess &&
               hipModuleGetFunction(&m->fused_combine, m->module, "mppi_fused_combine") == hipSuccess &&
               hipModuleGetFunction(&m->rollout_fused_one, m->module, "mppi_rollout_fused_one") == hipSuccess &&
@@ -630,9 +631,13 @@ int launch_iteration(mppi_hip_ctx* ctx, const mppi_agent_desc& d, uint32_t N, co
         MPPI_HIP_TRY(ctx, launch_kernel(st, ctx->mod->rollout, d.n_blk, N, 1, params, e0, e1));
     }
     MPPI_HIP_TRY(ctx, launch_kernel(st, ctx->mod->reduce, d.n_chunks, d.n_groups, N, params, e2, e3));
+    // the update: one workgroup per agent (timed with the reduce)
+    hipEvent_t e4 = prof ? take_event(ctx) : nullptr, e5 = prof ? take_event(ctx) : nullptr;
+    MPPI_HIP_TRY(ctx, launch_kernel(st, ctx->mod->finalize, N, 1, 1, params, e4, e5));
     if (prof) {
         if (e0 && e1) ctx->pending_rollout.push_back({e0, e1});
         if (e2 && e3) ctx->pending_reduce.push_back({e2, e3});
+        if (e4 && e5) ctx->pending_reduce.push_back({e4, e5});
         ctx->prof_launches += 1;
     }
     return MPPI_HIP_OK;

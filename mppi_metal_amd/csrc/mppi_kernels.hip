@@ -16,23 +16,24 @@
 // them through a visible function table at every timestep,
 // shaders/mppi_rollout.metal:90-91).
 //
-// One MPPI step is two launches:
+// The two-kernel MPPI step is three launches:
 //   rollout  one sample per lane; control noise from Philox4x32-10 in
 //            registers (never stored); clamp; dynamics; stage cost on the
 //            post-dynamics state; terminal cost; FP32 cost accumulation.
 //            Each workgroup also reduces its costs to (min, sum exp, sum cost,
-//            sum exp^2) with wave shuffles and LDS, and the last workgroup of
-//            an agent combines those partials into the softmax normaliser
-//            (log-sum-exp form).  Reference: mppi_rollout
+//            sum exp^2) with wave shuffles and LDS.  Reference: mppi_rollout
 //            (mppi_rollout.metal:36-96, batch :103-170) and the min / sum
 //            passes of mppi_compute_weights (mppi_reduce.metal:14-121, :242-356).
-//   reduce   weighted-noise average, parallel over samples: every lane
-//            regenerates its noise blocks from the same counters and
-//            accumulates w~_k * z_k; workgroup partials are summed by the
-//            last workgroup of the agent, which writes u_next, the
+//   reduce   every workgroup first combines its agent's rollout partials into
+//            the softmax normaliser (log-sum-exp form), then the weighted-noise
+//            average, parallel over samples: every lane regenerates its noise
+//            blocks from the same counters and accumulates w~_k * z_k into
+//            workgroup partials.  Reference: mppi_reduce
+//            (mppi_reduce.metal:130-168, :363-406).
+//   finalize one workgroup per agent sums the partials, writes u_next, the
 //            diagnostics and, in closed-loop mode, steps the plant and shifts
-//            the warm start.  Reference: mppi_reduce (mppi_reduce.metal:130-168,
-//            :363-406) and mppi_propagate_and_shift (:175-231, :413-488).
+//            the warm start.  Reference: mppi_propagate_and_shift
+//            (mppi_reduce.metal:175-231, :413-488).
 // Work is cut into tiles: rollout (agent, sample block), reduce (agent,
 // sample chunk, noise-block group), and the one-launch step's tile (agent,
 // sample block: rollout, then the tile's weighted noise sums, the last tile
@@ -193,11 +194,22 @@ __device__ inline void mppi_make_control(const float* un, const float* z, const 
 // Softmax normaliser: combine the per-workgroup (min, sum e, sum c, sum e^2)
 // partials of one agent: m = min m_g, S = sum_g S_g exp(-(m_g - m)/lambda),
 // Q likewise with the squared factor, ESS = S^2 / Q (= 1 / sum w^2).
-// T is the accumulation type: float, or double when K >= 2^18.
+// T is the accumulation type: float, or double when K >= 2^18.  Every
+// workgroup that needs the normaliser forms it from the rows itself, in the
+// same order, so all get the same values: a workgroup that wrote rows would
+// otherwise have to publish them at agent scope before a last-arriving
+// workgroup combined them, and that release (a write-back of the XCD's L2,
+// whose caches are not coherent with the other XCDs') costs more, summed
+// over thousands of workgroups, than the rows' few loads per workgroup.
 // ---------------------------------------------------------------------------
+template <typename T>
+struct mppi_norm {
+    float m;          // min m_g (not finite: no finite cost; then S = Q = K)
+    T S, Cs, Q;
+};
+
 template <int BLOCK, typename T>
-__device__ void mppi_combine_stats(const mppi_step_args& a, uint32_t agent,
-                                   const mppi_agent_desc& d) {
+__device__ mppi_norm<T> mppi_normaliser(const mppi_step_args& a, const mppi_agent_desc& d) {
     constexpr int WAVES = BLOCK / 64;
     __shared__ float s_min[WAVES];
     __shared__ T s_sum[3][WAVES];
@@ -227,62 +239,70 @@ __device__ void mppi_combine_stats(const mppi_step_args& a, uint32_t agent,
     Q = mppi_wave_sum(Q);
     if (lane == 0) { s_sum[0][wave] = S; s_sum[1][wave] = Cs; s_sum[2][wave] = Q; }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        S = s_sum[0][0]; Cs = s_sum[1][0]; Q = s_sum[2][0];
+    S = s_sum[0][0]; Cs = s_sum[1][0]; Q = s_sum[2][0];
 #pragma unroll
-        for (int w = 1; w < WAVES; ++w) { S += s_sum[0][w]; Cs += s_sum[1][w]; Q += s_sum[2][w]; }
-        if (!mppi_isfinite(mg)) {
-            // Every cost is NaN or infinite: fall back to uniform weights.
-            S = T(d.K);
-            Q = T(d.K);
-        }
+    for (int w = 1; w < WAVES; ++w) { S += s_sum[0][w]; Cs += s_sum[1][w]; Q += s_sum[2][w]; }
+    if (!mppi_isfinite(mg)) {
+        // Every cost is NaN or infinite: fall back to uniform weights.
+        S = T(d.K);
+        Q = T(d.K);
+    }
+    __syncthreads();                        // s_min and s_sum are free again
+    return mppi_norm<T>{mg, S, Cs, Q};
+}
+
+// The normaliser into the agent's state (thread 0), as the update and the
+// per-sample weights read it; also re-arms the agent's tile arrival counter.
+template <typename T>
+__device__ void mppi_store_normaliser(const mppi_step_args& a, const mppi_agent_desc& d, const mppi_norm<T>& n) {
+    if (threadIdx.x == 0) {
         mppi_agent_state* ag = &a.agents[d.slot];
-        ag->min_cost = mg;
-        ag->weight_sum = double(S);
-        ag->mean_cost = double(Cs / T(d.K));
-        ag->ess = double((S * S) / Q);
+        ag->min_cost = n.m;
+        ag->weight_sum = double(n.S);
+        ag->mean_cost = double(n.Cs / T(d.K));
+        ag->ess = double((n.S * n.S) / n.Q);
         ag->rollout_arrivals = 0u;
-        __threadfence();
     }
 }
 
 // Workgroup softmax partials over one rollout tile's costs (min, sum e,
-// sum c, sum e^2); the last tile of an agent combines all partials.
-template <int BLOCK>
-__device__ void mppi_block_stats(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
-                                 uint32_t blk, bool active, float total) {
-    constexpr int WAVES = BLOCK / 64;
+// sum c, sum e^2), one statistics row per SUB samples (SUB = BLOCK: one row
+// per tile; a tile of BLOCK = 2 SUB samples writes rows 2 blk and 2 blk + 1,
+// the second only if it is inside the agent's n_blk rows).
+template <int BLOCK, int SUB = BLOCK>
+__device__ void mppi_block_stats(const mppi_step_args& a, const mppi_agent_desc& d, uint32_t blk, bool active,
+                                 float total) {
+    constexpr int WAVES = BLOCK / 64, WPR = SUB / 64, ROWS = BLOCK / SUB;
+    static_assert(BLOCK % SUB == 0 && SUB % 64 == 0, "rows of whole wavefronts");
     __shared__ float s_red[4][WAVES];
-    __shared__ int s_last;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r0 = (wave / WPR) * WPR;
     float c = (active && mppi_isfinite(total)) ? total : __builtin_inff();
     float m = mppi_wave_min(c);
     if (lane == 0) s_red[0][wave] = m;
     __syncthreads();
-    m = s_red[0][0];
+    m = s_red[0][r0];
 #pragma unroll
-    for (int w = 1; w < WAVES; ++w) m = s_red[0][w] < m ? s_red[0][w] : m;
+    for (int w = 1; w < WPR; ++w) m = s_red[0][r0 + w] < m ? s_red[0][r0 + w] : m;
     float e = (mppi_isfinite(c) && mppi_isfinite(m)) ? mppi_expf(-(c - m) / a.lambda) : 0.0f;
     float se = mppi_wave_sum(e);
     float sq = mppi_wave_sum(e * e);
     float sc = mppi_wave_sum(active ? total : 0.0f);
     if (lane == 0) { s_red[1][wave] = se; s_red[2][wave] = sc; s_red[3][wave] = sq; }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        float S = 0.0f, Cs = 0.0f, Q = 0.0f;
+    if (threadIdx.x < uint32_t(ROWS)) {
+        const int rr = int(threadIdx.x) * WPR;
+        float mr = s_red[0][rr], S = 0.0f, Cs = 0.0f, Q = 0.0f;
 #pragma unroll
-        for (int w = 0; w < WAVES; ++w) { S += s_red[1][w]; Cs += s_red[2][w]; Q += s_red[3][w]; }
-        float* bs = a.blk_stats + (d.blk_off + blk) * 4;
-        bs[0] = m; bs[1] = S; bs[2] = Cs; bs[3] = Q;
-        __threadfence();
-        uint32_t prev = atomicAdd(&a.agents[d.slot].rollout_arrivals, 1u);
-        s_last = (prev == d.n_blk - 1);
+        for (int w = 0; w < WPR; ++w) {
+            mr = s_red[0][rr + w] < mr ? s_red[0][rr + w] : mr;
+            S += s_red[1][rr + w]; Cs += s_red[2][rr + w]; Q += s_red[3][rr + w];
+        }
+        const uint32_t row = blk * ROWS + threadIdx.x;
+        if (row < d.n_blk) {
+            float* bs = a.blk_stats + (d.blk_off + row) * 4;
+            bs[0] = mr; bs[1] = S; bs[2] = Cs; bs[3] = Q;
+        }
     }
-    __syncthreads();
-    if (!__builtin_amdgcn_readfirstlane(s_last)) return;
-    __threadfence();
-    if (d.K >= MPPI_FP64_MIN_K) mppi_combine_stats<BLOCK, double>(a, agent, d);
-    else mppi_combine_stats<BLOCK, float>(a, agent, d);
 }
 
 template <int SAMPLES, int THREADS>
@@ -379,7 +399,7 @@ __device__ void mppi_rollout_tile(const mppi_step_args& a, uint32_t agent, uint3
     __shared__ __attribute__((aligned(16))) float s_un[MPPI_UN_LDS];
     const mppi_agent_desc d = a.desc[agent];
     const float total = mppi_rollout_costs<BLOCK>(a, d, blk, s_un);
-    mppi_block_stats<BLOCK>(a, agent, d, blk, blk * BLOCK + threadIdx.x < d.K, total);
+    mppi_block_stats<BLOCK>(a, d, blk, blk * BLOCK + threadIdx.x < d.K, total);
 }
 
 // ---------------------------------------------------------------------------
@@ -559,9 +579,10 @@ __device__ void mppi_fused_tile(const mppi_step_args& a, uint32_t agent, uint32_
 }
 
 // ---------------------------------------------------------------------------
-// Finalize (last reduce workgroup of an agent): sum the weighted-noise
-// partials over sample chunks, divide by the normaliser, write u_next and the
-// diagnostics; in closed loop also log history, step the plant and shift.
+// Finalize (mppi_finalize_agents, one workgroup per agent): sum the
+// weighted-noise partials over sample chunks, divide by the normaliser, write
+// u_next and the diagnostics; in closed loop also log history, step the plant
+// and shift.
 // ---------------------------------------------------------------------------
 // zs[col] = (sum_r scale_r * rows[r * W + col]) / S for col < W (W a
 // multiple of 4), summed in a fixed order; scale_r = scale[r * sstride], or
@@ -745,18 +766,17 @@ __device__ void mppi_apply_update(const mppi_step_args& a, uint32_t agent, const
             next[n] = uo[(t + 1 < H ? t + 1 : H - 1) * CDIM + i];
         }
     }
-    if (threadIdx.x == 0) {
-        ag->reduce_arrivals = 0u;
-        __threadfence();
-    }
 }
 
-template <int BLOCK>
-__device__ void mppi_finalize(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
-                              float* zs) {
-    const mppi_agent_state* ag = &a.agents[d.slot];
-    if (d.K >= MPPI_FP64_MIN_K) mppi_sum_partials<BLOCK, double>(a, d, ag->weight_sum, zs);
-    else mppi_sum_rows<BLOCK>(a.partials + d.part_off, d.n_chunks, d.NB * 4, nullptr, 1, float(ag->weight_sum), zs);
+// The update of one agent after its reduce tiles (mppi_finalize_agents, one
+// workgroup per agent): the normaliser into the agent's state, the reduce
+// partials summed over sample chunks and divided by it, the update applied.
+template <int BLOCK, typename T>
+__device__ void mppi_finalize(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d, float* zs) {
+    const mppi_norm<T> n = mppi_normaliser<BLOCK, T>(a, d);
+    mppi_store_normaliser(a, d, n);
+    if (sizeof(T) == 8) mppi_sum_partials<BLOCK, double>(a, d, double(n.S), zs);
+    else mppi_sum_rows<BLOCK>(a.partials + d.part_off, d.n_chunks, d.NB * 4, nullptr, 1, float(n.S), zs);
     __syncthreads();
     mppi_apply_update<BLOCK>(a, agent, d, zs);
 }
@@ -1048,8 +1068,17 @@ __device__ void mppi_reduce_tile(const mppi_step_args& a, uint32_t agent, uint32
     const uint32_t K = d.K, NB = d.NB;
     const mppi_agent_state* ag = &a.agents[d.slot];
     const uint32_t step = ag->step;
-    const float m = float(ag->min_cost);
-    const float Sf = float(ag->weight_sum);
+    // The agent's normaliser, formed here from the rollout's rows.
+    float m, Sf;
+    if (d.K >= MPPI_FP64_MIN_K) {
+        const mppi_norm<double> n = mppi_normaliser<BLOCK, double>(a, d);
+        m = n.m;
+        Sf = float(n.S);
+    } else {
+        const mppi_norm<float> n = mppi_normaliser<BLOCK, float>(a, d);
+        m = n.m;
+        Sf = n.S;
+    }
     const bool uniform = !mppi_isfinite(m);
     const uint32_t tag_agent = (MPPI_TAG_CONTROL_NOISE << 28) | (d.slot & MPPI_AGENT_MASK);
     const uint32_t b_begin = grp * MPPI_REDUCE_BPG;
@@ -1089,10 +1118,7 @@ __device__ void mppi_reduce_tile(const mppi_step_args& a, uint32_t agent, uint32
     }
 
     // ---- workgroup sum of the accumulators, transposed through LDS ----
-    constexpr int LDS = (BLOCK * (MPPI_NV + 1) > MPPI_MAX_HC + 4) ? BLOCK * (MPPI_NV + 1)
-                                                                  : MPPI_MAX_HC + 4;
-    __shared__ float s_acc[LDS];
-    __shared__ int s_last;
+    __shared__ float s_acc[BLOCK * (MPPI_NV + 1)];
 #pragma unroll
     for (int q = 0; q < MPPI_REDUCE_BPG; ++q)
 #pragma unroll
@@ -1115,16 +1141,7 @@ __device__ void mppi_reduce_tile(const mppi_step_args& a, uint32_t agent, uint32
             if (b < NB) a.partials[d.part_off + (size_t(chunk) * NB + b) * 4 + (threadIdx.x & 3)] = tot;
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        uint32_t prev = atomicAdd(&a.agents[d.slot].reduce_arrivals, 1u);
-        s_last = (prev == d.n_chunks * d.n_groups - 1);
-    }
-    __syncthreads();
-    if (!__builtin_amdgcn_readfirstlane(s_last)) return;
-    __threadfence();
-    mppi_finalize<BLOCK>(a, agent, d, s_acc);
+    // The rows are summed by mppi_finalize_agents, a later launch.
 }
 
 // ---------------------------------------------------------------------------
@@ -1201,12 +1218,11 @@ __device__ void mppi_tile_noise_sums(const mppi_step_args& a, const mppi_agent_d
 template <int BLOCK>
 __device__ void mppi_onelaunch_combine(const mppi_step_args& a, uint32_t agent, const mppi_agent_desc& d,
                                        float* zs) {
-    mppi_combine_stats<BLOCK, float>(a, agent, d);
-    __syncthreads();
-    const mppi_agent_state* ag = &a.agents[d.slot];
-    const float m = float(ag->min_cost);
+    const mppi_norm<float> n = mppi_normaliser<BLOCK, float>(a, d);
+    mppi_store_normaliser(a, d, n);
+    const float m = n.m;
     const bool uniform = !mppi_isfinite(m);
-    // Each tile row's factor f_g (the one mppi_combine_stats used for S)
+    // Each tile row's factor f_g (the one mppi_normaliser used for S)
     // replaces the row's m_g, which is no longer needed.
     float* st = a.blk_stats + d.blk_off * 4;
     for (uint32_t i = threadIdx.x; i < d.n_blk; i += BLOCK) {
@@ -1214,7 +1230,7 @@ __device__ void mppi_onelaunch_combine(const mppi_step_args& a, uint32_t agent, 
         st[i * 4] = uniform ? 1.0f : (mppi_isfinite(bm) ? mppi_expf(-(bm - m) / a.lambda) : 0.0f);
     }
     __syncthreads();
-    mppi_sum_rows<BLOCK>(a.partials + d.part_off, d.n_blk, d.NB * 4, st, 4, float(ag->weight_sum), zs);
+    mppi_sum_rows<BLOCK>(a.partials + d.part_off, d.n_blk, d.NB * 4, st, 4, n.S, zs);
     __syncthreads();
     mppi_apply_update<BLOCK>(a, agent, d, zs);
 }
@@ -1254,6 +1270,16 @@ extern "C" __global__ void __launch_bounds__(256) mppi_reduce(mppi_step_args a) 
     const uint32_t agent = blockIdx.z;
     if (blockIdx.x >= a.desc[agent].n_chunks || blockIdx.y >= a.desc[agent].n_groups) return;
     mppi_reduce_tile<256>(a, agent, blockIdx.x, blockIdx.y);
+}
+
+// The update of every agent after the reduce (grid N: one workgroup per
+// agent, whatever workgroup size the rollout and reduce tiles had).
+extern "C" __global__ void __launch_bounds__(256) mppi_finalize_agents(mppi_step_args a) {
+    __shared__ float s_zs[MPPI_MAX_HC + 4];
+    const uint32_t agent = blockIdx.x;
+    const mppi_agent_desc d = a.desc[agent];
+    if (d.K >= MPPI_FP64_MIN_K) mppi_finalize<256, double>(a, agent, d, s_zs);
+    else mppi_finalize<256, float>(a, agent, d, s_zs);
 }
 
 // Rollout of the one-pass step (grid as mppi_rollout, 512 threads per tile of
@@ -1567,9 +1593,7 @@ extern "C" __global__ void __launch_bounds__(MPPI_SUR_BLOCK) mppi_rollout_surrog
                                mppi_blob_ref{cost}, mppi_time{H, step});
         a.costs[d.cost_off + k] = total;
     }
-    // softmax partials per workgroup of MPPI_SUR_BLOCK samples
-    mppi_agent_desc ds = d;
-    ds.n_blk = (d.K + MPPI_SUR_BLOCK - 1) / MPPI_SUR_BLOCK;
-    mppi_block_stats<MPPI_SUR_BLOCK>(a, agent, ds, blk, active, total);
+    // softmax partials, one row per 256 samples (the rows the reduce reads)
+    mppi_block_stats<MPPI_SUR_BLOCK, 256>(a, d, blk, active, total);
 }
 #endif

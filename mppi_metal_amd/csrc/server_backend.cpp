@@ -216,6 +216,9 @@ int solve_impl(mppi_hip_server* s, uint32_t n, const uint32_t* ids, float* first
                                             ctx->stream, rp, nullptr));
     MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(s->mod->reduce_b[2][0], da.n_tiles, 1, 1, kServerBlock, 1, 1, 0,
                                             ctx->stream, dp, nullptr));
+    // the update: one workgroup of 256 threads per agent
+    MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(s->mod->finalize, uint32_t(n), 1, 1, 256, 1, 1, 0, ctx->stream, dp,
+                                            nullptr));
     MPPI_HIP_TRY(ctx, hipMemcpyAsync(s->down_host.ptr, out, down, hipMemcpyDeviceToHost, ctx->stream));
     MPPI_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     const float* r = static_cast<const float*>(s->down_host.ptr);
