@@ -60,8 +60,8 @@ sys.path.insert(0, str(ROOT))
 
 # Published MI355X peaks.  FP32 vector: 157.3 TFLOP/s, an FMA counted as two
 # flops, i.e. 78.6e12 lane-instructions/s (256 CUs x 4 SIMDs x 32 lanes x
-# 2.4 GHz); no VALU instruction issues faster (a v_fma_f32 stream measures
-# 22 lanes per cycle per SIMD).  Dense bf16 matrix cores: 2.5 PFLOP/s.
+# 2.4 GHz), the most any VALU instruction stream can issue.  Dense bf16
+# matrix cores: 2.5 PFLOP/s.
 VALU_PEAK_LANE_OPS = 157.3e12 / 2
 MFMA_PEAK_BF16 = 2.5e15
 HBM_PEAK_BPS = 8.0e12
@@ -284,7 +284,12 @@ def bench_batch(args):
     b = HipBatch(hip, sc.K, sc.H)
     b.set_params(mb, sc.cost_blob_packed(), 0, scenes.BICYCLE_COST_BYTES)
     b.reset(0, sc.x0)
-    tune_ms = b.autotune(3)
+    if args.batch_tuning:
+        f = [int(v) for v in args.batch_tuning.split(",")]
+        b.set_tuning(block=f[0], persistent=f[1], wgs_per_cu=f[2], reduce_spt=f[3], order=f[4], launches=f[5])
+        tune_ms = None
+    else:
+        tune_ms = b.autotune(3)
     tuning = b.get_tuning()
     b.reset(0, sc.x0)
     b.run(5, closed_loop=True)
@@ -511,6 +516,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--serial-steps", type=int, default=5)
+    ap.add_argument("--batch-tuning", default="",
+                    help="block,persistent,wgs_per_cu,reduce_spt,order,launches for the batch instead of "
+                         "the autotuner's choice (profiling runs fix it so that traced and untraced runs agree)")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-costly", action="store_true",
                     help="skip the surrogate against 64-substep dynamics")
@@ -568,8 +576,10 @@ def main():
     roll_ms, red_ms, launches = hip.profile_read()
     hip.profile(False)
     roll_s = roll_ms / launches / 1e3
-    roll_kernel = "mppi_rollout_fused" if fused else "mppi_rollout"
-    second_kernel = "mppi_fused_combine" if fused else "mppi_reduce"
+    # one agent: the one-pass kernels that take the agent's descriptor from
+    # their arguments
+    roll_kernel = "mppi_rollout_fused_one" if fused else "mppi_rollout"
+    second_kernel = "mppi_fused_combine_one" if fused else "mppi_reduce"
     second_s = red_ms / launches / 1e3     # the combine launch (fused) or the reduce
     w_ss = unicycle_w_ss(C, fused)
     achieved_ops = K * H * w_ss["total"] / roll_s
