@@ -145,6 +145,28 @@ MPPI_HD BicycleFrame bicycle_frame(const BicycleCost* c) {
     return f;
 }
 
+// The set of zones whose widened square meets column band `band` (col) or
+// row band `band` of the frame's grid; band BICYCLE_GRID is empty.
+MPPI_HD uint32_t bicycle_band_zones(const BicycleCost* c, const BicycleFrame& f, bool col, uint32_t band) {
+    const float o = col ? f.x0 : f.y0;
+    uint32_t m = 0;
+    if (band < BICYCLE_GRID) {
+        for (uint32_t j = 0; j < BICYCLE_MAX_ZONES; ++j) {
+            if (!((f.grid >> j) & 1u)) continue;
+            const float uc = (c->zones[j][col ? 0 : 1] - o) * f.inv;
+            const float ur = bicycle_square_half(c->zones[j][2]) * f.inv;
+            if (uc + ur >= float(band) - 0.125f && uc - ur <= float(band) + 1.125f) m |= 1u << j;
+        }
+    }
+    return m;
+}
+
+// The band of coordinate v (x with the frame's x0, y with its y0): the
+// cell index clamped to [0, BICYCLE_GRID]; NaN maps to 0.
+MPPI_HD uint32_t bicycle_band(float v, float o, float inv) {
+    return uint32_t(__builtin_fminf(__builtin_fmaxf((v - o) * inv, 0.0f), float(BICYCLE_GRID)));
+}
+
 #if defined(__HIP_DEVICE_COMPILE__)
 // Zone sets of the columns (x) and rows (y) of cells; entry BICYCLE_GRID,
 // empty, takes every coordinate at or past the last cell.  The frame (x0,
@@ -166,17 +188,7 @@ __device__ void bicycle_prepare(const uint8_t* cost_raw, uint32_t thread, uint32
     for (uint32_t i = thread; i < 2 * (BICYCLE_GRID + 1); i += threads) {
         const bool col = i <= BICYCLE_GRID;
         const uint32_t band = col ? i : i - (BICYCLE_GRID + 1);
-        const float o = col ? f.x0 : f.y0;
-        uint32_t m = 0;
-        if (band < BICYCLE_GRID) {
-            for (uint32_t j = 0; j < BICYCLE_MAX_ZONES; ++j) {
-                if (!((f.grid >> j) & 1u)) continue;
-                const float uc = (c->zones[j][col ? 0 : 1] - o) * f.inv;
-                const float ur = bicycle_square_half(c->zones[j][2]) * f.inv;
-                if (uc + ur >= float(band) - 0.125f && uc - ur <= float(band) + 1.125f) m |= 1u << j;
-            }
-        }
-        (col ? bicycle_cols : bicycle_rows)[band] = m;
+        (col ? bicycle_cols : bicycle_rows)[band] = bicycle_band_zones(c, f, col, band);
     }
 }
 #define MPPI_COST_PREPARE bicycle_prepare
@@ -185,9 +197,7 @@ __device__ void bicycle_prepare(const uint8_t* cost_raw, uint32_t thread, uint32
 // zones every wavefront tests.
 __device__ uint32_t bicycle_candidates(float x, float y) {
     const float4 f = *reinterpret_cast<const float4*>(bicycle_frame_lds);
-    const float u = __builtin_fminf(__builtin_fmaxf((x - f.x) * f.z, 0.0f), float(BICYCLE_GRID));
-    const float v = __builtin_fminf(__builtin_fmaxf((y - f.y) * f.z, 0.0f), float(BICYCLE_GRID));
-    return (bicycle_cols[uint32_t(u)] & bicycle_rows[uint32_t(v)]) | mppi_f2u(f.w);
+    return (bicycle_cols[bicycle_band(x, f.x, f.z)] & bicycle_rows[bicycle_band(y, f.y, f.z)]) | mppi_f2u(f.w);
 }
 #endif
 

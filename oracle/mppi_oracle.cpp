@@ -819,6 +819,41 @@ int oracle_rollout_range(const oracle_problem* p, uint32_t step, uint32_t agent,
 // Realised cost of a closed loop: out[i] = stage(x[i], u[i], time {0,
 // steps[i]}) for n (post-step state, applied control) pairs, the cost the
 // rollout's first timestep assigns to what the plant actually did.
+// The GPU pre-filters of the obstacle and zone costs, evaluated on the host
+// from the plugins' own (host-callable) functions, so that tests can check
+// that they flag every obstacle or zone a state counts.
+// unicycle: bit j of flags[i] is set when the square gate of slot j (scaled
+// coordinates, unicycle_gate) holds state i = (xy[2i], xy[2i+1]).
+int oracle_unicycle_gate_flags(const uint8_t* cost, uint32_t n, const float* xy, uint32_t* flags) {
+    const auto* p = reinterpret_cast<const plug_uni::UnicycleCost*>(cost);
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t f = 0;
+        for (uint32_t j = 0; j < UNICYCLE_MAX_OBSTACLES; ++j) {
+            const plug_uni::UnicycleGate g =
+                plug_uni::unicycle_gate(p->obstacles[j][0], p->obstacles[j][1], p->obstacles[j][2]);
+            const float ox = std::fma(xy[2 * i], g.a, g.bx), oy = std::fma(xy[2 * i + 1], g.a, g.by);
+            if (std::fmax(std::fabs(ox), std::fabs(oy)) < 1.0f) f |= 1u << j;
+        }
+        flags[i] = f;
+    }
+    return 0;
+}
+
+// bicycle: cand[i] = the zones the grid lists for state i (its column's and
+// row's sets and the zones every wavefront tests), as bicycle_candidates
+// forms them on the GPU from the same functions.
+int oracle_bicycle_candidates(const uint8_t* cost, uint32_t n, const float* xy, uint32_t* cand) {
+    const auto* c = reinterpret_cast<const plug_bic::BicycleCost*>(cost);
+    const plug_bic::BicycleFrame f = plug_bic::bicycle_frame(c);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t cu = plug_bic::bicycle_band(xy[2 * i], f.x0, f.inv);
+        const uint32_t cv = plug_bic::bicycle_band(xy[2 * i + 1], f.y0, f.inv);
+        cand[i] = (plug_bic::bicycle_band_zones(c, f, true, cu) & plug_bic::bicycle_band_zones(c, f, false, cv)) |
+                  f.always;
+    }
+    return 0;
+}
+
 int oracle_stage_costs(const oracle_problem* p, uint32_t n, const float* states, const float* controls,
                        const uint32_t* steps, float* out) {
     Resolved<float> r = resolve<float>(p);

@@ -208,6 +208,29 @@ class Problem:
         return st, ct, kt, tt
 
 
+def _prefilter(fn, cost_blob, xy):
+    blob = np.frombuffer(bytes(cost_blob), np.uint8).copy()
+    pts = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+    out = np.empty(pts.shape[0], np.uint32)
+    f = getattr(lib(), fn)
+    f.restype = ctypes.c_int
+    assert f(blob.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint32(pts.shape[0]), _p(pts),
+             out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))) == 0
+    return out
+
+
+def unicycle_gate_flags(cost_blob, xy):
+    """Bit j of row i: the unicycle obstacle cost's square gate of slot j
+    holds point i (the plugin's unicycle_gate, as the GPU evaluates it)."""
+    return _prefilter("oracle_unicycle_gate_flags", cost_blob, xy)
+
+
+def bicycle_candidates(cost_blob, xy):
+    """Bit j of row i: the bicycle pursuit cost's zone grid lists zone j for
+    point i (the plugin's frame, band sets and band mapping)."""
+    return _prefilter("oracle_bicycle_candidates", cost_blob, xy)
+
+
 def philox4x32(ctr, key, library=False):
     """library: False (this oracle's own generator), True (the library
     header's compiled for the host) or "split" (the header's split form)."""

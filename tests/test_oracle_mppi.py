@@ -335,3 +335,104 @@ def test_unicycle_obstacle_square_gate_misses_no_sample(obstacles):
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
     # the states straddle the circles: some pay the hinge, some do not
     assert 0 < np.count_nonzero(pen) < len(pts)
+
+
+def _counted(centres_radii, xy):
+    """Bit j of row i: point i counts circle j in the costs' FP32 arithmetic
+    (ox = x - cx, oy = y - cy, d2 = ox*ox + oy*oy, each rounded; r > 0 and
+    d2 < r*r)."""
+    f = np.float32
+    out = np.zeros(len(xy), np.uint32)
+    for j, (cx, cy, r) in enumerate(centres_radii):
+        r = f(r)
+        if not (r > 0):
+            continue
+        with np.errstate(all="ignore"):
+            ox = xy[:, 0].astype(f) - f(cx)
+            oy = xy[:, 1].astype(f) - f(cy)
+            d2 = ox * ox + oy * oy
+            hit = d2 < r * r
+        out |= hit.astype(np.uint32) << np.uint32(j)
+    return out
+
+
+def _boundary_points(circles, rng, n_random=2000, box=60.0):
+    """Points within a few ulps of every circle (on axes and diagonals, and
+    the float next to the axis points), near the corners of the squares
+    around them, and random points over the scene."""
+    pts = []
+    for cx, cy, r in circles:
+        if not (r > 0.0) or not np.isfinite(r) or r > 1e30:
+            r = 0.3
+        for ang in np.arange(8) * (np.pi / 4):
+            for k in range(-6, 7):
+                d = np.float64(r) * (1.0 + k * 2.0 ** -23)
+                pts.append((cx + d * np.cos(ang), cy + d * np.sin(ang)))
+                if ang % (np.pi / 2) == 0:
+                    px = np.float32(cx + d * np.cos(ang))
+                    pts.append((float(np.nextafter(px, np.float32(cx))), cy + d * np.sin(ang)))
+        for sx in (-1, 1):
+            for sy in (-1, 1):
+                pts.append((cx + sx * r, cy + sy * r))
+    pts += list(map(tuple, rng.uniform(-box, box, (n_random, 2))))
+    with np.errstate(all="ignore"):
+        return np.array(pts, np.float64).astype(np.float32)
+
+
+@pytest.mark.parametrize("obstacles", [
+    [(1.0, 2.0, 0.5), (3.0e4, -2.0e4, 0.01), (-7.5, 0.25, 3.0), (1e-3, 1e-3, 1e-3),
+     (2.0, -1.0, 0.0), (2.0, -1.0, -0.7), (4.0, 4.0, float("nan")), (0.5, 0.5, 0.25)],
+    [(0.0, 0.0, 2.0 ** -61), (1.0, 1.0, 2.0 ** -59), (-3.0, 2.0, 1e-12)],
+    [(5.0, 5.0, 2.0 ** 61)],
+])
+def test_unicycle_square_gate_flags_every_counted_obstacle(obstacles):
+    """The GPU's first pass of the obstacle cost (the square gate in scaled
+    coordinates, the plugin's unicycle_gate) flags, for every point near the
+    circles and over the scene, every slot whose hinge the point pays: the
+    exact pass it guards then sees them all, so the GPU's costs are the
+    definition's.  A gate that flags too few fails here."""
+    from mppi_metal_amd import scenes
+    sc = scenes.unicycle_scene(4, 42)
+    sc.obstacles = obstacles
+    xy = _boundary_points(obstacles, np.random.default_rng(5), box=20.0)
+    flags = ol.unicycle_gate_flags(sc.cost_blob(), xy)
+    counted = _counted(obstacles, xy)
+    assert np.all((counted & ~flags) == 0)
+    assert np.count_nonzero(counted) > 0
+
+
+def _zone_grid_case(kind):
+    from mppi_metal_amd import scenes
+    sc = scenes.bicycle_pursuit_scene(4, 42)
+    zones = list(sc.zones)
+    if kind == "edges":
+        nan, inf = float("nan"), float("inf")
+        zones = zones[:20] + [(1.5, 0.2, 0.0), (2.5, 0.3, -1.0), (4.0, 0.0, nan), (inf, 0.0, 1.0),
+                              (3.5, nan, 1.0), (1e-3, 1e-3, 1e-12), (-30.0, 10.0, 2.0 ** 61), (7.0, -7.0, 0.05)]
+    elif kind == "far":
+        # a tiny zone far from the origin: the frame cannot resolve it and
+        # every wavefront tests the zones instead
+        zones = [(3.0e6, -2.0e6, 0.01), (3.0e6 + 0.05, -2.0e6, 0.02)]
+    blob = bytearray(scenes.bicycle_cost_blob((0.0, 0.0), (0.0, 0.0), zones))
+    return zones, bytes(blob)
+
+
+@pytest.mark.parametrize("kind", ["scene", "edges", "far"])
+def test_bicycle_zone_grid_lists_every_counted_zone(kind):
+    """The GPU's zone pre-filter for the bicycle pursuit cost (per-column
+    and per-row zone sets of a grid over the zones' squares, built from the
+    plugin's host-callable frame and band functions) lists, for points
+    within a few ulps of every zone, near the squares' corners and over the
+    scene, every zone whose hinge the point pays; the exact pass runs for
+    the listed zones only, so the GPU's costs are the definition's.  On the
+    scene's own zones the grid is also selective (most random points list
+    no zone), so the check is not met by listing everything."""
+    zones, blob = _zone_grid_case(kind)
+    xy = _boundary_points(zones, np.random.default_rng(6))
+    cand = ol.bicycle_candidates(blob, xy)
+    counted = _counted(zones, xy)
+    assert np.all((counted & ~cand) == 0)
+    assert np.count_nonzero(counted) > 0
+    if kind == "scene":
+        far = cand[-2000:]
+        assert np.count_nonzero(far == 0) > 1000
