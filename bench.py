@@ -300,6 +300,24 @@ def bench_batch(args):
     batch_rate = work * T / el
     print(f"[bench] batch: {T} steps in {el:.3f}s, tuning {tuning}", file=sys.stderr, flush=True)
 
+    # The persistent one-launch form (a grid of 4 workgroups per CU takes
+    # tiles from a ticket counter; each agent's last tile forms its update),
+    # timed the same way beside the chosen shape.
+    forms = {}
+    for name, (pers, wgs, launches) in (("persistent_one_launch", (1, 4, 1)), ("table_one_launch", (0, 0, 1)),
+                                        ("persistent_two_launches", (1, 4, 2))):
+        b.set_tuning(block=256, persistent=pers, wgs_per_cu=wgs, reduce_spt=tuning["reduce_spt"],
+                     order=tuning["order"], launches=launches)
+        b.reset(0, sc.x0)
+        b.run(3, closed_loop=True)
+        t0 = time.perf_counter()
+        b.run(T, closed_loop=True)
+        el_f = time.perf_counter() - t0
+        forms[name] = {"value": work * T / el_f, "ms_per_step": el_f / T * 1e3, "tuning": b.get_tuning()}
+    b.set_tuning(**tuning)
+    b.reset(0, sc.x0)
+    b.run(3, closed_loop=True)
+
     # Kernel times from events attached to each dispatch (while profiling the
     # batch launches its steps directly rather than replaying its graph).
     hip.profile(True)
@@ -368,6 +386,7 @@ def bench_batch(args):
                                  "how": "simulate_batch with every agent at the largest K and H"},
         "speedup_vs_plain_per_agent": batch_rate / serial_rate,
         "speedup_vs_padded_batch": batch_rate / padded_rate,
+        "other_launch_forms": forms,
     }
 
 

@@ -148,3 +148,35 @@ def test_replay_serves_every_request(require_gpu, mode):
     assert 1 <= st["batches"] <= 96 and st["seconds"] >= rq.arrival_s[-1]
     assert srv.usage()[2] == 0             # every agent released
     srv.close()
+
+
+def test_full_size_admission_wave_matches_oracle(require_gpu):
+    """The serving benchmark's requests at full size (1024 quadrotor requests,
+    K log-uniform in [4096, 32768], H uniform in [30, 80]): the first
+    admission wave under the benchmark's 32 MB budget, solved as one batch;
+    the agents with the largest K, the largest H and the smallest K of the
+    wave against the oracle's step for their own K, H, cost blob and id
+    (sample costs bit-exact, best cost exact, first control within 1e-5 of
+    the update's largest magnitude)."""
+    hip, cov, mb = _ctx()
+    rq = scenes.quadrotor_requests(1024, seed=42, rate_hz=2000.0)
+    srv = HipServer(hip, 1024, 32 << 20)
+    ids, req = [], []
+    for i in range(len(rq.K)):
+        try:
+            ids.append(srv.admit(int(rq.K[i]), int(rq.H[i]), rq.x0[i], _hover(int(rq.H[i])), rq.cost_blob(i)))
+        except MppiHipError as e:
+            assert "HBM budget" in str(e)
+            break
+        req.append(i)
+    assert len(ids) > 100                   # a wave of a few hundred agents
+    u0, diag = srv.solve(ids)
+    Ks, Hs = rq.K[req], rq.H[req]
+    for j in sorted({int(np.argmax(Ks)), int(np.argmax(Hs)), int(np.argmin(Ks))}):
+        i, K, H = req[j], int(Ks[j]), int(Hs[j])
+        ora = _oracle(K, H, cov, mb, rq.cost_blob(i))
+        u_o, d_o, c_o, _ = ora.step(0, rq.x0[i], _hover(H), agent=ids[j], threads=8)
+        assert np.array_equal(hip.read_costs(j, K), c_o), f"request {i} (K={K}, H={H}): costs differ"
+        assert diag[j, 0] == d_o[0]
+        assert np.max(np.abs(u0[j] - u_o[0])) <= 1e-5 * float(np.max(np.abs(u_o))), f"request {i}"
+    srv.close()
