@@ -524,7 +524,7 @@ int mppi_hip_batch_autotune(mppi_hip_batch* b, uint32_t trial_steps, double* bes
     // table or persistent grid, block x order.
     std::vector<mppi_hip_batch_tuning> cands;
     for (uint32_t block : {64u, 128u, 256u}) {
-        for (uint32_t spt : {1u, 2u, 4u, 8u})
+        for (uint32_t spt : {1u, 2u, 4u, 8u, 16u, 32u})
             for (uint32_t order : {0u, 1u}) cands.push_back({block, 0, 0, spt, order, 2});
         for (uint32_t w : {2u, 4u}) cands.push_back({block, 1, w, 0, 0, 2});
         if (one_launch_ok(b))
