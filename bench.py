@@ -60,10 +60,10 @@ sys.path.insert(0, str(ROOT))
 
 # Published MI355X peaks.  FP32 vector: 157.3 TFLOP/s, an FMA counted as two
 # flops, i.e. 78.6e12 lane-instructions/s (256 CUs x 4 SIMDs x 32 lanes x
-# 2.4 GHz), the most any VALU instruction stream can issue.  Dense bf16
-# matrix cores: 2.5 PFLOP/s.
+# 2.4 GHz), the most any VALU instruction stream can issue.  Dense bf16 and
+# fp16 matrix cores: 2.5 PFLOP/s.
 VALU_PEAK_LANE_OPS = 157.3e12 / 2
-MFMA_PEAK_BF16 = 2.5e15
+MFMA_PEAK_16 = 2.5e15
 HBM_PEAK_BPS = 8.0e12
 COUNTERS = ROOT / "profiles" / "mppi_counters_unicycle_pursuit_surrogate.md"
 
@@ -449,8 +449,8 @@ def bench_surrogate(args, substeps=2):
              len(mb), len(cb), 42)
     hip.bind_params(mb, cb)
     t0 = time.perf_counter()
-    hist = hip.surrogate_train(**scenes.BICYCLE_SURROGATE_ENVELOPE, num_transitions=4 << 20, batch=4096,
-                               steps=args.train_steps, learning_rate=2e-3, seed=7)
+    recipe = dict(scenes.BICYCLE_SURROGATE_TRAINING, steps=args.train_steps)
+    hist = hip.surrogate_train(**scenes.BICYCLE_SURROGATE_ENVELOPE, **recipe)
     train_s = time.perf_counter() - t0
     _, rel = hip.surrogate_evaluate(1 << 16, seed=99)
     print(f"[bench] surrogate trained in {train_s:.2f}s, loss {hist[0]:.3e} -> {hist[-20:].mean():.3e}",
@@ -481,20 +481,22 @@ def bench_surrogate(args, substeps=2):
     flops = K * H * SURROGATE_FLOPS / sur_s
     key = "mppi_rollout_surrogate@surrogate"
     roofline = {"bound": "mfma", "kernel": "mppi_rollout_surrogate", "achieved": flops / 1e12,
-                "peak": MFMA_PEAK_BF16 / 1e12, "unit": "TFLOP/s", "frac": flops / MFMA_PEAK_BF16,
+                "peak": MFMA_PEAK_16 / 1e12, "unit": "TFLOP/s", "frac": flops / MFMA_PEAK_16,
                 "flops_per_sample_step": SURROGATE_FLOPS, "sample_steps_per_launch": K * H,
                 "avg_launch_us": sur["rollout_us"],
                 "issued_flops_per_sample_step": SURROGATE_FLOPS_ISSUED,
-                "issued_frac": K * H * SURROGATE_FLOPS_ISSUED / sur_s / MFMA_PEAK_BF16,
+                "issued_frac": K * H * SURROGATE_FLOPS_ISSUED / sur_s / MFMA_PEAK_16,
                 "valu_issued": issued(key, K * H, sur_s),
                 "lds_bank_conflict_frac": profile_line("lds_bank_conflict_cycles_over_lds_cycles").get(key),
                 "traffic": measured_traffic(key),
                 "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/" + COUNTERS.name}
     return {
-        "workload": "learned dynamic-bicycle surrogate (MLP 8-128-128-6 tanh, bf16 MFMA, fp32 accumulate), "
+        "workload": "learned dynamic-bicycle surrogate (MLP 8-128-128-6, smoothstep activation, fp16 MFMA, "
+                    "fp32 accumulate), "
                     f"pursuit, closed loop, plugin dynamics with {substeps} Euler substeps per step",
         "K": K, "H": H, "closed_loop_steps": T, "substeps": substeps,
-        "train": {"transitions": 4 << 20, "batch": 4096, "steps": args.train_steps, "seconds": train_s,
+        "train": {"transitions": recipe["num_transitions"], "batch": recipe["batch"], "steps": recipe["steps"],
+                  "learning_rate": recipe["learning_rate"], "seconds": train_s,
                   "loss_first": float(hist[0]), "loss_last": float(hist[-20:].mean()),
                   "one_step_relative_rmse": [float(v) for v in rel]},
         "plain": res["plain"], "surrogate": sur,
@@ -544,7 +546,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--requests", type=int, default=1024)
     ap.add_argument("--surrogate-steps", type=int, default=500)
-    ap.add_argument("--train-steps", type=int, default=4000)
+    ap.add_argument("--train-steps", type=int, default=40000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three device-to-device copies of 256 MiB (the reference kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")

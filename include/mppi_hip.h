@@ -294,8 +294,9 @@ int mppi_hip_batch_read(mppi_hip_batch* batch, float* x_out, float* u_out_packed
 
 /* ---------------------------------------------------------------------------
  * Learned-dynamics surrogate.  No reference counterpart.  An MLP
- * (state_dim + control_dim) -> 128 -> 128 -> state_dim with tanh hidden
- * layers, BF16 weights on the matrix cores and FP32 accumulation, trained on
+ * (state_dim + control_dim) -> 128 -> 128 -> state_dim with a saturating
+ * hidden activation T (mppi_hip_surrogate_activation), FP16 weights on the
+ * matrix cores and FP32 accumulation (BF16 in training's backward pass), trained on
  * the GPU on transitions that the linked plugin's own dynamics generates
  * (states drawn uniformly from [state_lo, state_hi], controls from the
  * control bounds), predicting the state change of one step.  When enabled,
@@ -357,6 +358,15 @@ int mppi_hip_surrogate_import(mppi_hip_ctx* ctx, const float* params, size_t n_p
  * batch MSE in grads (n_params floats) and the loss. */
 int mppi_hip_surrogate_gradients(mppi_hip_ctx* ctx, uint32_t n, const float* x, const float* targets,
                                  float* grads, float* loss);
+
+/* The network's hidden activation T of n pre-activations z, evaluated on the
+ * device: T(z) = S(y), y = clamp01(r16(z)) (z rounded to binary16 toward
+ * zero, clamped to [0, 1], NaN to 0), S(y) = y^2 (6 - 4 y) - 1 evaluated in
+ * binary16 as t = fma(y, -4, 6), u = y * y, T = fma(u, t, -1), each rounded
+ * to nearest even.  out receives the n binary16 results as floats.  No
+ * reference counterpart: it lets an application reproduce the network on
+ * the CPU bit for bit in its activations. */
+int mppi_hip_surrogate_activation(mppi_hip_ctx* ctx, uint32_t n, const float* z, float* out);
 
 /* ---------------------------------------------------------------------------
  * Serving: agents admitted against an HBM budget and solved in batches of any

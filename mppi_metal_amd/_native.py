@@ -144,6 +144,7 @@ SIGNATURES = [
      [_CTX, _F, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _F]),
     ("mppi_hip_surrogate_import", ctypes.c_int, [_CTX, _F, ctypes.c_size_t, _F]),
     ("mppi_hip_surrogate_gradients", ctypes.c_int, [_CTX, ctypes.c_uint32, _F, _F, _F, _F]),
+    ("mppi_hip_surrogate_activation", ctypes.c_int, [_CTX, ctypes.c_uint32, _F, _F]),
     ("mppi_hip_server_create", ctypes.c_int, [_CTX, ctypes.c_uint32, ctypes.c_size_t, ctypes.POINTER(_VP)]),
     ("mppi_hip_server_destroy", None, [_VP]),
     ("mppi_hip_server_workspace_bytes", ctypes.c_int,

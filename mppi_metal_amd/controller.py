@@ -251,6 +251,14 @@ class HipMPPI:
                                                           _fptr(loss)))
         return g, float(loss[0])
 
+    def surrogate_activation(self, z):
+        """The network's hidden activation T of float32 pre-activations,
+        evaluated on the device (binary16 results as float32)."""
+        zs = _f32(z).ravel()
+        out = np.empty_like(zs)
+        self._check(self.lib.mppi_hip_surrogate_activation(self.ctx, zs.size, _fptr(zs), _fptr(out)))
+        return out
+
     def profile(self, enable=True):
         self._check(self.lib.mppi_hip_profile_enable(self.ctx, 1 if enable else 0))
 

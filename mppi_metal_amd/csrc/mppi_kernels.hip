@@ -1439,14 +1439,14 @@ extern "C" __global__ void __launch_bounds__(256) mppi_surrogate_data(mppi_surro
 // workgroups, grid (ceil(K / 512), N); the reduce kernel is shared).  Per
 // timestep every wavefront evaluates the network for its 64 samples as four
 // 16-sample column blocks:
-//   layer 1   v_mfma_f32_16x16x16_bf16, one per (m-block, column block), the
+//   layer 1   v_mfma_f32_16x16x16_f16, one per (m-block, column block), the
 //             inputs staged per sample in the wavefront's LDS rows;
-//   layer 2   v_mfma_f32_16x16x32_bf16: each A fragment is read from LDS
+//   layer 2   v_mfma_f32_16x16x32_f16: each A fragment is read from LDS
 //             once and applied to the four column blocks; its B operands are
 //             the layer-1 results a lane already holds (mppi_mlp.h);
 //   layer 3   the same, accumulated as layer 2's m-blocks come out in pairs,
 //             so that only two m-blocks of the second hidden layer are live;
-// activations through the LDS table (mppi_act2).  The staging rows are
+// activations on packed binary16 pairs in registers (mppi_act2).  The staging rows are
 // private to a wavefront, whose LDS operations complete in order, so a
 // timestep needs wavefront-level ordering only (no workgroup barrier).
 constexpr int MPPI_SUR_BLOCK = 512;
@@ -1457,7 +1457,6 @@ extern "C" __global__ void __launch_bounds__(MPPI_SUR_BLOCK) mppi_rollout_surrog
     static_assert(IN <= 16 && SDIM <= 16, "surrogate network: inputs and outputs of at most 16");
     __shared__ __attribute__((aligned(16))) short s_w1[8 * MPPI_MLP_FRAG_ELEMS];
     __shared__ __attribute__((aligned(16))) short s_wx[MPPI_MLP_X32_FRAGS * MPPI_MLP_X32_ELEMS];
-    __shared__ unsigned short s_tab[MPPI_MLP_TANH_KEYS];
     __shared__ __attribute__((aligned(16))) float s_b[2 * MPPI_MLP_HID + 16];
     __shared__ __attribute__((aligned(16))) short s_in[WAVES][64][16];
     __shared__ float s_dx[WAVES][64][SDIM + 1];
@@ -1469,7 +1468,6 @@ extern "C" __global__ void __launch_bounds__(MPPI_SUR_BLOCK) mppi_rollout_surrog
         reinterpret_cast<uint4*>(s_w1)[e] = reinterpret_cast<const uint4*>(sa.frags + MPPI_MLP_FRAG_W1 * MPPI_MLP_FRAG_ELEMS)[e];
     for (uint32_t e = threadIdx.x; e < MPPI_MLP_X32_FRAGS * MPPI_MLP_X32_ELEMS / 8; e += MPPI_SUR_BLOCK)
         reinterpret_cast<uint4*>(s_wx)[e] = reinterpret_cast<const uint4*>(sa.frags + MPPI_MLP_X32_BASE)[e];
-    for (uint32_t e = threadIdx.x; e < MPPI_MLP_TANH_KEYS; e += MPPI_SUR_BLOCK) s_tab[e] = sa.tanh_table[e];
     for (uint32_t e = threadIdx.x; e < MPPI_MLP_HID; e += MPPI_SUR_BLOCK) {
         s_b[e] = sa.params[sa.b1 + e];
         s_b[MPPI_MLP_HID + e] = sa.params[sa.b2 + e];
@@ -1510,7 +1508,7 @@ extern "C" __global__ void __launch_bounds__(MPPI_SUR_BLOCK) mppi_rollout_surrog
             if (t < H) {
                 float u[CDIM];
                 mppi_make_control(un + t * CDIM, z + gi * CDIM, a.noise_scale, a.u_min, a.u_max, u);
-                // this lane's sample: normalised inputs as bf16, zeros past IN
+                // this lane's sample: normalised inputs as FP16, zeros past IN
                 mppi_v4f xin[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
@@ -1525,35 +1523,35 @@ extern "C" __global__ void __launch_bounds__(MPPI_SUR_BLOCK) mppi_rollout_surrog
                     }
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    *reinterpret_cast<mppi_v4bf*>(&s_in[wave][lane][4 * q]) = mppi_pack_bf16(xin[q]);
+                    *reinterpret_cast<mppi_v4s*>(&s_in[wave][lane][4 * q]) = mppi_pack_f16(xin[q]);
                 __builtin_amdgcn_wave_barrier();
                 // layer 1: h1[nb][mb] = T(W1 x + b1) for column block nb
-                mppi_v4bf h1[4][8];
+                mppi_v4s h1[4][8];
 #pragma unroll
                 for (int nb = 0; nb < 4; ++nb) {
-                    const mppi_v4bf xb = *reinterpret_cast<const mppi_v4bf*>(&s_in[wave][16 * nb + c][4 * g]);
+                    const mppi_v4s xb = *reinterpret_cast<const mppi_v4s*>(&s_in[wave][16 * nb + c][4 * g]);
                     // the eight products first, then their activations, so
-                    // that table reads overlap the matrix cores
+                    // that the activations overlap the matrix cores
                     mppi_v4f acc[8];
 #pragma unroll
                     for (int mb = 0; mb < 8; ++mb)
                         acc[mb] = mppi_mfma(mppi_frag(s_w1, mb, lane), xb,
                                             *reinterpret_cast<const mppi_v4f*>(s_b + 16 * mb + 4 * g));
 #pragma unroll
-                    for (int mb = 0; mb < 8; ++mb) h1[nb][mb] = mppi_act4(acc[mb], s_tab);
+                    for (int mb = 0; mb < 8; ++mb) h1[nb][mb] = mppi_act4(acc[mb]);
                 }
                 // layers 2 and 3
                 mppi_v4f y[4] = {zero, zero, zero, zero};
 #pragma unroll
                 for (int kb3 = 0; kb3 < 4; ++kb3) {
-                    mppi_v4bf h2[4][2];
+                    mppi_v4s h2[4][2];
 #pragma unroll
                     for (int half = 0; half < 2; ++half) {
                         const int mb = 2 * kb3 + half;
-                        mppi_v8bf w[4];
+                        mppi_v8s w[4];
 #pragma unroll
                         for (int kb = 0; kb < 4; ++kb)
-                            w[kb] = *reinterpret_cast<const mppi_v8bf*>(
+                            w[kb] = *reinterpret_cast<const mppi_v8s*>(
                                 s_wx + (MPPI_MLP_X32_W2 + mb * 4 + kb) * MPPI_MLP_X32_ELEMS + lane * 8);
                         const mppi_v4f bias = *reinterpret_cast<const mppi_v4f*>(s_b + MPPI_MLP_HID + 16 * mb + 4 * g);
                         // four independent accumulation chains (column blocks)
@@ -1564,9 +1562,9 @@ extern "C" __global__ void __launch_bounds__(MPPI_SUR_BLOCK) mppi_rollout_surrog
                             for (int nb = 0; nb < 4; ++nb)
                                 acc[nb] = mppi_mfma32(w[kb], mppi_cat8(h1[nb][2 * kb], h1[nb][2 * kb + 1]), acc[nb]);
 #pragma unroll
-                        for (int nb = 0; nb < 4; ++nb) h2[nb][half] = mppi_act4(acc[nb], s_tab);
+                        for (int nb = 0; nb < 4; ++nb) h2[nb][half] = mppi_act4(acc[nb]);
                     }
-                    const mppi_v8bf w3 = *reinterpret_cast<const mppi_v8bf*>(
+                    const mppi_v8s w3 = *reinterpret_cast<const mppi_v8s*>(
                         s_wx + (MPPI_MLP_X32_W3 + kb3) * MPPI_MLP_X32_ELEMS + lane * 8);
 #pragma unroll
                     for (int nb = 0; nb < 4; ++nb) y[nb] = mppi_mfma32(w3, mppi_cat8(h2[nb][0], h2[nb][1]), y[nb]);

@@ -123,8 +123,7 @@ struct mppi_surrogate_data_args {
 
 struct mppi_surrogate_rollout_args {
     mppi_step_args s;
-    const short* frags;          // bf16 A fragments (mppi_mlp.h)
-    const unsigned short* tanh_table;  // activation table (mppi_mlp.h), MPPI_MLP_TANH_KEYS entries
+    const short* frags;          // A fragments (mppi_mlp.h)
     const float* params;         // FP32 parameters (biases are read from here)
     uint32_t b1, b2, b3;         // bias offsets into params
     uint32_t pad;

@@ -7,19 +7,21 @@
 //
 // One optimiser step over a batch of n samples (n a multiple of 64) is four
 // launches:
-//   mppi_mlp_train_fwd_bwd  per 16-sample block: forward pass (MFMA), MSE
-//                           error against the normalised targets, backward
-//                           pass through the layers (MFMA with the transposed
-//                           weight fragments), writing the bf16 activations
-//                           and pre-activation gradients feature-major
-//                           ([feature][sample]) plus per-wave loss partials.
+//   mppi_mlp_train_fwd_bwd  per 16-sample block: forward pass (FP16 MFMA),
+//                           MSE error against the normalised targets,
+//                           backward pass through the layers (BF16 MFMA with
+//                           the transposed weight fragments), writing the
+//                           inputs, activations and pre-activation gradients
+//                           as bf16, feature-major ([feature][sample]), plus
+//                           per-wave loss partials.
 //   mppi_mlp_weight_grad    weight and bias gradients as MFMA GEMMs that
 //                           contract over samples (A = dZ^T, B = activations;
 //                           biases with B = ones), split over sample chunks;
 //                           FP32 partials in parameter order.
 //   mppi_mlp_adam           sums the chunk partials in a fixed order and
 //                           applies Adam to the FP32 master parameters.
-//   mppi_mlp_pack           rebuilds the bf16 A fragments from the masters.
+//   mppi_mlp_pack           rebuilds the A fragments (FP16 forward, BF16
+//                           backward) from the masters.
 #include <hip/hip_runtime.h>
 
 #include "backend.h"
@@ -36,7 +38,6 @@ constexpr int HID = MPPI_MLP_HID;
 
 struct train_args {
     const short* frags;
-    const unsigned short* tab;   // activation table
     const float* params;      // FP32 masters
     mppi_mlp_dims dims;
     const float* x;           // [n][in] normalised inputs
@@ -92,7 +93,6 @@ struct pack_args {
 
 struct forward_args {
     const short* frags;
-    const unsigned short* tab;   // activation table
     const float* params;
     mppi_mlp_dims dims;
     const float* x;           // [n][in] normalised
@@ -110,26 +110,27 @@ extern "C" __global__ void __launch_bounds__(256) mppi_mlp_train_fwd_bwd(train_a
     const mppi_mlp_dims& d = a.dims;
     const mppi_v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
 
-    mppi_v4bf xb;
+    mppi_v4s xb;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const uint32_t k = 4 * g + j;
-        xb[j] = mppi_f2bf(k < d.in ? a.x[size_t(s) * d.in + k] : 0.0f);
-        a.xt[size_t(k) * n + s] = xb[j];
+        const float v = k < d.in ? a.x[size_t(s) * d.in + k] : 0.0f;
+        xb[j] = mppi_f2h(v);
+        a.xt[size_t(k) * n + s] = mppi_f2bf(v);
     }
-    mppi_v4bf h1b[8], h2b[8];
-    mppi_v4f y = mppi_mlp_block(a.frags, a.params + d.b1, a.params + d.b2, a.tab, xb, lane, h1b, h2b);
+    mppi_v4s h1b[8], h2b[8], y1b[8], y2b[8];
+    mppi_v4f y = mppi_mlp_block(a.frags, a.params + d.b1, a.params + d.b2, xb, lane, h1b, h2b, y1b, y2b);
 #pragma unroll
     for (int mb = 0; mb < 8; ++mb)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const size_t row = 16 * mb + 4 * g + j;
-            a.h1t[row * n + s] = h1b[mb][j];
-            a.h2t[row * n + s] = h2b[mb][j];
+            a.h1t[row * n + s] = mppi_f2bf(mppi_h2f(h1b[mb][j]));
+            a.h2t[row * n + s] = mppi_f2bf(mppi_h2f(h2b[mb][j]));
         }
     // MSE against normalised targets.
     float loss = 0.0f;
-    mppi_v4bf dyb;
+    mppi_v4s dyb;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const uint32_t r = 4 * g + j;
@@ -144,29 +145,29 @@ extern "C" __global__ void __launch_bounds__(256) mppi_mlp_train_fwd_bwd(train_a
         dyb[j] = mppi_f2bf(dy);
         a.dyt[size_t(r) * n + s] = dyb[j];
     }
-    // dH2 = W3^T dY, dZ2 = dH2 (1 - h2^2)
-    mppi_v4bf dz2b[8];
+    // dH2 = W3^T dY, dZ2 = dH2 T'(z2), T' = 12 y (1 - y)
+    mppi_v4s dz2b[8];
 #pragma unroll
     for (int mb = 0; mb < 8; ++mb) {
-        mppi_v4f acc = mppi_mfma(mppi_frag(a.frags, MPPI_MLP_FRAG_W3T + mb, lane), dyb, zero);
+        mppi_v4f acc = mppi_mfma_bf16(mppi_frag(a.frags, MPPI_MLP_FRAG_W3T + mb, lane), dyb, zero);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const float h = mppi_bf2f(h2b[mb][j]);
-            dz2b[mb][j] = mppi_f2bf(acc[j] * (1.0f - h * h));
+            const float yv = mppi_h2f(y2b[mb][j]);
+            dz2b[mb][j] = mppi_f2bf(acc[j] * (12.0f * yv * (1.0f - yv)));
             a.dz2t[size_t(16 * mb + 4 * g + j) * n + s] = dz2b[mb][j];
         }
     }
-    // dH1 = W2^T dZ2, dZ1 = dH1 (1 - h1^2)
+    // dH1 = W2^T dZ2, dZ1 = dH1 T'(z1)
 #pragma unroll
     for (int mb = 0; mb < 8; ++mb) {
         mppi_v4f acc = zero;
 #pragma unroll
         for (int kb = 0; kb < 8; ++kb)
-            acc = mppi_mfma(mppi_frag(a.frags, MPPI_MLP_FRAG_W2T + mb * 8 + kb, lane), dz2b[kb], acc);
+            acc = mppi_mfma_bf16(mppi_frag(a.frags, MPPI_MLP_FRAG_W2T + mb * 8 + kb, lane), dz2b[kb], acc);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const float h = mppi_bf2f(h1b[mb][j]);
-            a.dz1t[size_t(16 * mb + 4 * g + j) * n + s] = mppi_f2bf(acc[j] * (1.0f - h * h));
+            const float yv = mppi_h2f(y1b[mb][j]);
+            a.dz1t[size_t(16 * mb + 4 * g + j) * n + s] = mppi_f2bf(acc[j] * (12.0f * yv * (1.0f - yv)));
         }
     }
     // loss partial of this 16-sample block
@@ -193,15 +194,15 @@ extern "C" __global__ void __launch_bounds__(64) mppi_mlp_weight_grad(grad_args 
     const uint32_t k_begin = ks * a.kchunk;
     const uint32_t k_end = k_begin + a.kchunk < a.n ? k_begin + a.kchunk : a.n;
     mppi_v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    mppi_v4bf ones;
+    mppi_v4s ones;
 #pragma unroll
     for (int j = 0; j < 4; ++j) ones[j] = short(0x3f80);
     const short* arow = at + size_t(16 * ob + c) * a.n;
     const short* brow = bt ? bt + size_t(16 * ib + c) * a.n : nullptr;
     for (uint32_t k0 = k_begin; k0 < k_end; k0 += 16) {
-        const mppi_v4bf av = *reinterpret_cast<const mppi_v4bf*>(arow + k0 + 4 * g);
-        const mppi_v4bf bv = brow ? *reinterpret_cast<const mppi_v4bf*>(brow + k0 + 4 * g) : ones;
-        acc = mppi_mfma(av, bv, acc);
+        const mppi_v4s av = *reinterpret_cast<const mppi_v4s*>(arow + k0 + 4 * g);
+        const mppi_v4s bv = brow ? *reinterpret_cast<const mppi_v4s*>(brow + k0 + 4 * g) : ones;
+        acc = mppi_mfma_bf16(av, bv, acc);
     }
     float* out = a.partial + size_t(ks) * d.n_params;
 #pragma unroll
@@ -270,7 +271,7 @@ extern "C" __global__ void __launch_bounds__(256) mppi_mlp_pack(pack_args a) {
         float v = 0.0f;
         if (f < MPPI_MLP_X32_W3) v = P[d.w2 + (16 * (f / 4) + i) * HID + unit];      // W2X, mb = f / 4
         else if (i < d.out) v = P[d.w3 + i * HID + unit];                              // W3X
-        a.frags[e] = mppi_f2bf(v);
+        a.frags[e] = mppi_f2h(v);
         return;
     }
     const uint32_t f = e / MPPI_MLP_FRAG_ELEMS, lane = (e / 4) % 64, j = e % 4;
@@ -291,7 +292,7 @@ extern "C" __global__ void __launch_bounds__(256) mppi_mlp_pack(pack_args a) {
         const uint32_t mb = f - MPPI_MLP_FRAG_W3T;
         if (k < d.out) v = P[d.w3 + k * HID + 16 * mb + i];
     }
-    a.frags[e] = mppi_f2bf(v);
+    a.frags[e] = f < MPPI_MLP_FRAG_W2T ? mppi_f2h(v) : mppi_f2bf(v);
 }
 
 // Inference over n samples (n a multiple of 16): y = MLP(x) + b3.
@@ -301,13 +302,14 @@ extern "C" __global__ void __launch_bounds__(256) mppi_mlp_forward(forward_args 
     if (s0 >= a.n) return;
     const uint32_t s = s0 + c;
     const mppi_mlp_dims& d = a.dims;
-    mppi_v4bf xb;
+    mppi_v4s xb;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const uint32_t k = 4 * g + j;
-        xb[j] = mppi_f2bf(k < d.in ? a.x[size_t(s) * d.in + k] : 0.0f);
+        xb[j] = mppi_f2h(k < d.in ? a.x[size_t(s) * d.in + k] : 0.0f);
     }
-    mppi_v4f y = mppi_mlp_block(a.frags, a.params + d.b1, a.params + d.b2, a.tab, xb, lane, nullptr, nullptr);
+    mppi_v4f y = mppi_mlp_block(a.frags, a.params + d.b1, a.params + d.b2, xb, lane, nullptr, nullptr, nullptr,
+                                nullptr);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const uint32_t r = 4 * g + j;
@@ -315,21 +317,11 @@ extern "C" __global__ void __launch_bounds__(256) mppi_mlp_forward(forward_args 
     }
 }
 
-// Layout check of v_mfma_f32_16x16x16_bf16: D = A B for one 16x16x16 tile,
-// A and B given row-major in float (rounded to bf16), D row-major.
-extern "C" __global__ void __launch_bounds__(64) mppi_mfma_layout_probe(const float* A, const float* B,
-                                                                        float* D) {
-    const uint32_t lane = threadIdx.x, g = lane >> 4, c = lane & 15;
-    mppi_v4bf av, bv;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        av[j] = mppi_f2bf(A[c * 16 + 4 * g + j]);
-        bv[j] = mppi_f2bf(B[(4 * g + j) * 16 + c]);
-    }
-    const mppi_v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
-    mppi_v4f dv = mppi_mfma(av, bv, zero);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) D[(4 * g + j) * 16 + c] = dv[j];
+// The hidden activation T (mppi_mlp.h) of n pre-activations, as binary16
+// bits (one thread per pair; n even).
+extern "C" __global__ void __launch_bounds__(256) mppi_mlp_activation(const float* z, uint32_t* out, uint32_t pairs) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p < pairs) out[p] = mppi_act2(z[2 * p], z[2 * p + 1]);
 }
 
 // ===========================================================================
@@ -344,12 +336,12 @@ struct SurrogateState {
     float box_lo[16] = {}, box_hi[16] = {};   // sampling box of the training inputs
     bool ready = false, enabled = false;
     uint32_t adam_t = 0;
-    DeviceBuffer params, m, v, frags, partial, grad_sum, loss_part, loss_hist, out_norm, tanh_table;
+    DeviceBuffer params, m, v, frags, partial, grad_sum, loss_part, loss_hist, out_norm;
     DeviceBuffer xt, h1t, h2t, dz1t, dz2t, dyt;
     DeviceBuffer data_x, data_t, io_x, io_y;
     ~SurrogateState() {
         for (DeviceBuffer* b : {&params, &m, &v, &frags, &partial, &grad_sum, &loss_part, &loss_hist,
-                                &out_norm, &tanh_table, &xt, &h1t, &h2t, &dz1t, &dz2t, &dyt, &data_x,
+                                &out_norm, &xt, &h1t, &h2t, &dz1t, &dz2t, &dyt, &data_x,
                                 &data_t, &io_x, &io_y})
             b->release();
     }
@@ -375,34 +367,6 @@ mppi_mlp_dims dims_for(uint32_t in, uint32_t out) {
     return d;
 }
 
-// The activation table (mppi_mlp.h): entry `key` is the bf16 (nearest even)
-// of tanh of the binary16 value whose bits are min(key, 0x4400) (magnitudes
-// capped at 4.0), computed in FP64, for every key below 0x8000.
-uint16_t bf16_nearest_even(double t) {
-    float f = float(t);
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    const uint32_t lo = u & 0xffff0000u, hi = lo + 0x10000u;   // bf16 neighbours of t (t >= 0)
-    float flo, fhi;
-    std::memcpy(&flo, &lo, 4);
-    std::memcpy(&fhi, &hi, 4);
-    double dlo = t - double(flo), dhi = double(fhi) - t;
-    if (dlo < 0.0) return uint16_t(lo >> 16);            // t was rounded up past lo: t == flo - tiny
-    if (dlo < dhi || (dlo == dhi && ((lo >> 16) & 1u) == 0)) return uint16_t(lo >> 16);
-    return uint16_t(hi >> 16);
-}
-
-std::vector<uint16_t> tanh_table_host() {
-    std::vector<uint16_t> t(MPPI_MLP_TANH_KEYS);
-    for (uint32_t key = 0; key <= 0x4400u; ++key) {
-        const uint32_t ex = key >> 10, mant = key & 0x3ffu;
-        const double v = ex == 0 ? std::ldexp(double(mant), -24) : std::ldexp(double(1024 + mant), int(ex) - 25);
-        t[key] = bf16_nearest_even(std::tanh(v));
-    }
-    for (uint32_t key = 0x4401u; key < MPPI_MLP_TANH_KEYS; ++key) t[key] = t[0x4400u];
-    return t;
-}
-
 int ensure_state(mppi_hip_ctx* ctx) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
@@ -420,11 +384,6 @@ int ensure_state(mppi_hip_ctx* ctx) {
     if ((rc = ensure_device_buffer(ctx, &s.partial, sizeof(float) * P * kSplit))) return rc;
     if ((rc = ensure_device_buffer(ctx, &s.frags, sizeof(short) * MPPI_MLP_TOTAL_ELEMS))) return rc;
     if ((rc = ensure_device_buffer(ctx, &s.out_norm, sizeof(float) * 32))) return rc;
-    if (!s.tanh_table.ptr) {
-        const std::vector<uint16_t> t = tanh_table_host();
-        if ((rc = ensure_device_buffer(ctx, &s.tanh_table, sizeof(uint16_t) * t.size()))) return rc;
-        MPPI_HIP_TRY(ctx, hipMemcpy(s.tanh_table.ptr, t.data(), sizeof(uint16_t) * t.size(), hipMemcpyHostToDevice));
-    }
     return MPPI_HIP_OK;
 }
 
@@ -474,7 +433,6 @@ int train_step(mppi_hip_ctx* ctx, const float* x, const float* t, uint32_t n, bo
     const float* onorm = static_cast<const float*>(s.out_norm.ptr);
     train_args ta{};
     ta.frags = static_cast<const short*>(s.frags.ptr);
-    ta.tab = static_cast<const unsigned short*>(s.tanh_table.ptr);
     ta.params = static_cast<const float*>(s.params.ptr);
     ta.dims = s.dims;
     ta.x = x;
@@ -584,7 +542,6 @@ bool surrogate_rollout(mppi_hip_ctx* ctx, const mppi_step_args& a, hipFunction_t
     mppi_surrogate_rollout_args ra{};
     ra.s = a;
     ra.frags = static_cast<const short*>(s.frags.ptr);
-    ra.tanh_table = static_cast<const unsigned short*>(s.tanh_table.ptr);
     ra.params = static_cast<const float*>(s.params.ptr);
     ra.b1 = s.dims.b1;
     ra.b2 = s.dims.b2;
@@ -651,17 +608,20 @@ int mppi_hip_surrogate_train(mppi_hip_ctx* ctx, const mppi_hip_surrogate_config*
         s.norm.out_mu[k] = float(m);
         s.norm.out_sd[k] = float(std::max(std::sqrt(v / ns), 1e-6));
     }
-    // Initialisation: uniform +-sqrt(6 / (fan_in + fan_out)), zero biases.
+    // Initialisation: weights uniform +-sqrt(6 / (fan_in + fan_out)), those
+    // of the hidden layers divided by 3 (the slope of T at the centre of its
+    // linear region); hidden biases 1/2 (that centre), output biases 0.
     std::vector<float> P(s.dims.n_params, 0.0f);
     std::mt19937_64 gen(cfg->seed ^ 0x5eed5eedULL);
-    auto init = [&](uint32_t off, uint32_t rows, uint32_t cols) {
-        const float a = std::sqrt(6.0f / float(rows + cols));
+    auto init = [&](uint32_t off, uint32_t rows, uint32_t cols, float scale) {
+        const float a = scale * std::sqrt(6.0f / float(rows + cols));
         std::uniform_real_distribution<float> U(-a, a);
         for (uint32_t i = 0; i < rows * cols; ++i) P[off + i] = U(gen);
     };
-    init(s.dims.w1, HID, in);
-    init(s.dims.w2, HID, HID);
-    init(s.dims.w3, out, HID);
+    init(s.dims.w1, HID, in, 1.0f / 3.0f);
+    init(s.dims.w2, HID, HID, 1.0f / 3.0f);
+    init(s.dims.w3, out, HID, 1.0f);
+    for (uint32_t i = 0; i < HID; ++i) P[s.dims.b1 + i] = P[s.dims.b2 + i] = 0.5f;
     MPPI_HIP_TRY(ctx, hipMemcpyAsync(s.params.ptr, P.data(), sizeof(float) * P.size(), hipMemcpyHostToDevice, ctx->stream));
     MPPI_HIP_TRY(ctx, hipMemsetAsync(s.m.ptr, 0, sizeof(float) * P.size(), ctx->stream));
     MPPI_HIP_TRY(ctx, hipMemsetAsync(s.v.ptr, 0, sizeof(float) * P.size(), ctx->stream));
@@ -691,8 +651,7 @@ namespace {
 // Normalised network outputs (without denormalisation) for n inputs in io_x.
 int forward_io(mppi_hip_ctx* ctx, uint32_t n) {
     SurrogateState& s = *ctx->sur;
-    forward_args fa{static_cast<const short*>(s.frags.ptr), static_cast<const unsigned short*>(s.tanh_table.ptr),
-                    static_cast<const float*>(s.params.ptr), s.dims,
+    forward_args fa{static_cast<const short*>(s.frags.ptr), static_cast<const float*>(s.params.ptr), s.dims,
                     static_cast<const float*>(s.io_x.ptr), static_cast<float*>(s.io_y.ptr), n};
     hipLaunchKernelGGL(mppi_mlp_forward, dim3((n / 16 + 3) / 4), dim3(256), 0, ctx->stream, fa);
     MPPI_HIP_TRY(ctx, hipGetLastError());
@@ -846,6 +805,36 @@ int mppi_hip_surrogate_gradients(mppi_hip_ctx* ctx, uint32_t n, const float* x, 
     (void)hipStreamSynchronize(ctx->stream);
     cleanup();
     return rc;
+}
+
+int mppi_hip_surrogate_activation(mppi_hip_ctx* ctx, uint32_t n, const float* z, float* out) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (n == 0 || !z || !out) return fail(ctx, MPPI_HIP_E_INVALID, "surrogate activation: bad arguments.");
+    const uint32_t pairs = (n + 1) / 2;
+    std::vector<float> zin(size_t(pairs) * 2, 0.0f);
+    std::memcpy(zin.data(), z, sizeof(float) * n);
+    std::vector<uint32_t> bits(pairs);
+    DeviceBuffer dz, dt;
+    auto cleanup = [&]() { dz.release(); dt.release(); };
+    if ((rc = ensure_device_buffer(ctx, &dz, sizeof(float) * zin.size())) ||
+        (rc = ensure_device_buffer(ctx, &dt, sizeof(uint32_t) * pairs))) { cleanup(); return rc; }
+    hipError_t e = hipMemcpyAsync(dz.ptr, zin.data(), sizeof(float) * zin.size(), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(mppi_mlp_activation, dim3((pairs + 255) / 256), dim3(256), 0, ctx->stream,
+                           static_cast<const float*>(dz.ptr), static_cast<uint32_t*>(dt.ptr), pairs);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(bits.data(), dt.ptr, sizeof(uint32_t) * pairs, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
+    cleanup();
+    if (e != hipSuccess) return fail(ctx, MPPI_HIP_E_DEVICE, "surrogate activation: device error.");
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint16_t h = uint16_t(bits[i / 2] >> (16 * (i % 2)));
+        out[i] = float(__builtin_bit_cast(_Float16, h));
+    }
+    return MPPI_HIP_OK;
 }
 
 }  // extern "C"

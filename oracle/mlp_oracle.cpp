@@ -1,22 +1,33 @@
 // mlp_oracle.cpp — TEST INFRASTRUCTURE: CPU oracle of the learned-dynamics
-// surrogate (an MLP 8 -> 128 -> 128 -> 6 with tanh hidden layers).
+// surrogate (an MLP 8 -> 128 -> 128 -> 6 with a saturating hidden
+// activation).
 //
 // No reference counterpart: the reference has no learned model.  The
 // GPU kernels (mppi_metal_amd/csrc/surrogate.hip, mppi_mlp.h) compute the
-// forward pass with BF16 operands and FP32 accumulation and train with MSE
-// loss and Adam; this file states the same mathematics in FP64:
-//   z1 = W1 x + b1, h1 = tanh(z1), z2 = W2 h1 + b2, h2 = tanh(z2),
+// forward pass with binary16 (FP16) operands and FP32 accumulation, the
+// backward pass of training with bfloat16 operands, and train with MSE loss
+// and Adam; this file states the same mathematics in FP64:
+//   z1 = W1 x + b1, h1 = f(z1), z2 = W2 h1 + b2, h2 = f(z2),
 //   y = W3 h2 + b3, loss = mean over batch and outputs of (y - t)^2,
-// and its analytic gradients by backpropagation.  `round_bf16` makes the
-// FP64 computation see the same rounded operands as the GPU (weights and
-// activations rounded to bfloat16, nearest-even), so the remaining difference
-// is FP32-versus-FP64 accumulation.
-// With bf16 the hidden activations are T(z) = bf16(tanh(r16(float(z)))), the
-// GPU's activation (mppi_metal_amd/csrc/mppi_mlp.h): r16 rounds the FP32
-// pre-activation to binary16 toward zero, magnitudes above 4 (and
-// infinities, NaN) are capped at 4, bf16 is nearest-even of the FP64 tanh.
+//   f(z) = S(clamp01(z)), S(y) = y^2 (6 - 4 y) - 1, f'(z) = 12 y (1 - y),
+// and its analytic gradients by backpropagation.  With `rounded` set the
+// FP64 computation sees the GPU's rounded operands: weights and inputs of
+// the forward pass rounded to binary16 (nearest even), the hidden
+// activations T(z) of the GPU (below) in place of f, and what the backward
+// pass keeps in bfloat16 (nearest even): the transposed weights, dL/dy, dL/dz
+// of both hidden layers and the inputs and activations of the
+// weight-gradient products; the remaining difference is FP32-versus-FP64
+// accumulation.
+// The GPU's activation is T(z) = S16(y), y = clamp01(r16(float(z))): r16
+// rounds the FP32 pre-activation to binary16 toward zero, clamp01 maps NaN
+// to 0, and S16 evaluates S as three binary16 operations, each rounded to
+// nearest even: t = fma(y, -4, 6), u = y * y, T = fma(u, t, -1).  Each of
+// those is exact in FP64 before its rounding (binary16 operands), so one
+// rounding of the FP64 value is the binary16 operation's result.
 // Pinned in tests/test_oracle_mlp.py by central finite differences (FP64
-// path) and, for T, against numpy over every binary16 magnitude.
+// path) and, for T, against numpy's binary16 conversions over every
+// binary16 value.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -38,40 +49,55 @@ double round_bf16(double v) {
     return double(f);
 }
 
-// bf16 nearest-even of t >= 0, by scaling to 8 significant bits.
-double bf16_rne(double t) {
-    if (t == 0.0) return 0.0;
+// Binary16 nearest-even of a finite v (values beyond 65504 are not used
+// here): 11 significant bits above 2^-14, multiples of 2^-24 below.
+double round_f16(double v) {
+    if (v == 0.0 || !std::isfinite(v)) return v;
     int e;
-    const double m = std::frexp(t, &e);            // t = m 2^e, m in [0.5, 1)
-    return std::ldexp(std::nearbyint(std::ldexp(m, 8)), e - 8);
+    std::frexp(std::fabs(v), &e);                  // |v| in [2^(e-1), 2^e)
+    const int q = std::max(e - 11, -24);           // exponent of the last kept bit
+    return std::ldexp(std::nearbyint(std::ldexp(v, -q)), q);
 }
 
-// Binary16 bits of a >= 0 rounded toward zero (65504 for larger finite
-// values, 0x7c00 for infinity and NaN).
-uint32_t f16_rtz_bits(float a) {
-    uint32_t u;
-    std::memcpy(&u, &a, 4);
-    if (u >= 0x7f800000u) return 0x7c00u;
-    const int e = int(u >> 23) - 127;
-    if (e >= 16) return 0x7bffu;
-    if (e >= -14) return uint32_t((e + 15) << 10) | ((u >> 13) & 0x3ffu);
-    if (e < -24) return 0u;
-    return uint32_t(std::floor(std::ldexp(double(a), 24)));   // subnormal: m 2^-24
+// Binary16 value of z rounded toward zero (finite magnitudes above 65504
+// give 65504; infinities and NaN pass through).
+double f16_rtz(float z) {
+    const double v = z;
+    if (!std::isfinite(v)) return v;
+    const double a = std::fabs(v);
+    if (a >= 65504.0) return std::copysign(65504.0, v);
+    if (a == 0.0) return v;
+    int e;
+    std::frexp(a, &e);
+    const int q = std::max(e - 11, -24);
+    return std::copysign(std::ldexp(std::floor(std::ldexp(a, -q)), q), v);
+}
+
+// y of the GPU's activation: clamp01(r16(z)), NaN to 0.
+double act_y(float z) {
+    const double h = f16_rtz(z);
+    if (!(h > 0.0)) return 0.0;                    // also NaN and -0
+    return h < 1.0 ? h : 1.0;
+}
+
+// S evaluated in binary16 (three rounded operations).
+double s16(double y) {
+    const double t = round_f16(6.0 - 4.0 * y);
+    const double u = round_f16(y * y);
+    return round_f16(u * t - 1.0);
 }
 
 // The GPU's activation T of an FP32 pre-activation.
-double act(float z) {
-    uint32_t u;
-    std::memcpy(&u, &z, 4);
-    float a;
-    const uint32_t au = u & 0x7fffffffu;
-    std::memcpy(&a, &au, 4);
-    uint32_t key = f16_rtz_bits(a);
-    if (key > 0x4400u) key = 0x4400u;
-    const uint32_t ex = key >> 10, mant = key & 0x3ffu;
-    const double v = ex == 0 ? std::ldexp(double(mant), -24) : std::ldexp(double(1024 + mant), int(ex) - 25);
-    const double t = bf16_rne(std::tanh(v));
-    return (u >> 31) ? -t : t;
+double act(float z) { return s16(act_y(z)); }
+
+// The exact activation f and its derivative (FP64 path).
+double f_exact(double z) {
+    const double y = z <= 0.0 ? 0.0 : z >= 1.0 ? 1.0 : z;
+    return y * y * (6.0 - 4.0 * y) - 1.0;
+}
+double df_exact(double z) {
+    const double y = z <= 0.0 ? 0.0 : z >= 1.0 ? 1.0 : z;
+    return 12.0 * y * (1.0 - y);
 }
 
 struct Params {
@@ -82,23 +108,24 @@ struct Params {
 
 extern "C" {
 
-// Forward pass for n samples.  x: [n][8]; y: [n][6].  bf16 != 0 rounds
-// weights, inputs and hidden activations to bfloat16 as the GPU does.
+// Forward pass for n samples.  x: [n][8]; y: [n][6].  rounded != 0 uses
+// the GPU's binary16 operands and activation T.
 void oracle_mlp_forward(const float* w1, const float* b1, const float* w2, const float* b2,
-                        const float* w3, const float* b3, const float* x, int n, int bf16,
+                        const float* w3, const float* b3, const float* x, int n, int rounded,
                         double* y) {
-    auto r = [&](double v) { return bf16 ? round_bf16(v) : v; };
+    auto r = [&](double v) { return rounded ? round_f16(v) : v; };
+    auto f = [&](double z) { return rounded ? act(float(z)) : f_exact(z); };
     std::vector<double> h1(HID), h2(HID);
     for (int s = 0; s < n; ++s) {
         for (int o = 0; o < HID; ++o) {
             double z = b1[o];
             for (int i = 0; i < IN; ++i) z += r(w1[o * IN + i]) * r(x[s * IN + i]);
-            h1[o] = bf16 ? act(float(z)) : std::tanh(z);
+            h1[o] = f(z);
         }
         for (int o = 0; o < HID; ++o) {
             double z = b2[o];
             for (int i = 0; i < HID; ++i) z += r(w2[o * HID + i]) * h1[i];
-            h2[o] = bf16 ? act(float(z)) : std::tanh(z);
+            h2[o] = f(z);
         }
         for (int o = 0; o < OUT; ++o) {
             double z = b3[o];
@@ -109,22 +136,21 @@ void oracle_mlp_forward(const float* w1, const float* b1, const float* w2, const
 }
 
 // Loss and gradients for a batch: loss = sum_{s,o} (y - t)^2 / (n * 6).
-// Gradient outputs have the parameters' shapes.  bf16 != 0 rounds what the
-// GPU keeps in bfloat16: weights, inputs and activations as in the forward
-// pass, and the backward operands dL/dy and dL/dz of both hidden layers (the
-// GPU stores them in bf16 for the weight-gradient products); sums in FP64.
+// Gradient outputs have the parameters' shapes.  rounded != 0 rounds as the
+// GPU does (header); sums in FP64.
 double oracle_mlp_grad(const float* w1, const float* b1, const float* w2, const float* b2,
                        const float* w3, const float* b3, const float* x, const float* t, int n,
-                       int bf16, double* gw1, double* gb1, double* gw2, double* gb2, double* gw3,
+                       int rounded, double* gw1, double* gb1, double* gw2, double* gb2, double* gw3,
                        double* gb3) {
-    auto r = [&](double v) { return bf16 ? round_bf16(v) : v; };
+    auto r = [&](double v) { return rounded ? round_f16(v) : v; };      // forward operands
+    auto rb = [&](double v) { return rounded ? round_bf16(v) : v; };    // backward operands
     std::memset(gw1, 0, sizeof(double) * HID * IN);
     std::memset(gb1, 0, sizeof(double) * HID);
     std::memset(gw2, 0, sizeof(double) * HID * HID);
     std::memset(gb2, 0, sizeof(double) * HID);
     std::memset(gw3, 0, sizeof(double) * OUT * HID);
     std::memset(gb3, 0, sizeof(double) * OUT);
-    std::vector<double> xr(IN), h1(HID), h2(HID), y(OUT), dy(OUT), dh2(HID), dz2(HID), dh1(HID), dz1(HID);
+    std::vector<double> xr(IN), h1(HID), h2(HID), d1(HID), d2(HID), y(OUT), dy(OUT), dz2(HID), dz1(HID);
     double loss = 0.0;
     const double scale = 2.0 / (double(n) * OUT);
     for (int s = 0; s < n; ++s) {
@@ -132,12 +158,26 @@ double oracle_mlp_grad(const float* w1, const float* b1, const float* w2, const 
         for (int o = 0; o < HID; ++o) {
             double z = b1[o];
             for (int i = 0; i < IN; ++i) z += r(w1[o * IN + i]) * xr[i];
-            h1[o] = bf16 ? act(float(z)) : std::tanh(z);
+            if (rounded) {
+                const double yv = act_y(float(z));
+                h1[o] = s16(yv);
+                d1[o] = 12.0 * yv * (1.0 - yv);
+            } else {
+                h1[o] = f_exact(z);
+                d1[o] = df_exact(z);
+            }
         }
         for (int o = 0; o < HID; ++o) {
             double z = b2[o];
             for (int i = 0; i < HID; ++i) z += r(w2[o * HID + i]) * h1[i];
-            h2[o] = bf16 ? act(float(z)) : std::tanh(z);
+            if (rounded) {
+                const double yv = act_y(float(z));
+                h2[o] = s16(yv);
+                d2[o] = 12.0 * yv * (1.0 - yv);
+            } else {
+                h2[o] = f_exact(z);
+                d2[o] = df_exact(z);
+            }
         }
         for (int o = 0; o < OUT; ++o) {
             double z = b3[o];
@@ -145,31 +185,29 @@ double oracle_mlp_grad(const float* w1, const float* b1, const float* w2, const 
             y[o] = z;
             const double e = z - t[s * OUT + o];
             loss += e * e;
-            dy[o] = r(scale * e);
+            dy[o] = rb(scale * e);
         }
         for (int o = 0; o < OUT; ++o) {
             gb3[o] += dy[o];
-            for (int i = 0; i < HID; ++i) gw3[o * HID + i] += dy[o] * h2[i];
+            for (int i = 0; i < HID; ++i) gw3[o * HID + i] += dy[o] * rb(h2[i]);
         }
         for (int i = 0; i < HID; ++i) {
             double a = 0.0;
-            for (int o = 0; o < OUT; ++o) a += r(w3[o * HID + i]) * dy[o];
-            dh2[i] = a;
-            dz2[i] = r(a * (1.0 - h2[i] * h2[i]));
+            for (int o = 0; o < OUT; ++o) a += rb(w3[o * HID + i]) * dy[o];
+            dz2[i] = rb(a * d2[i]);
         }
         for (int o = 0; o < HID; ++o) {
             gb2[o] += dz2[o];
-            for (int i = 0; i < HID; ++i) gw2[o * HID + i] += dz2[o] * h1[i];
+            for (int i = 0; i < HID; ++i) gw2[o * HID + i] += dz2[o] * rb(h1[i]);
         }
         for (int i = 0; i < HID; ++i) {
             double a = 0.0;
-            for (int o = 0; o < HID; ++o) a += r(w2[o * HID + i]) * dz2[o];
-            dh1[i] = a;
-            dz1[i] = r(a * (1.0 - h1[i] * h1[i]));
+            for (int o = 0; o < HID; ++o) a += rb(w2[o * HID + i]) * dz2[o];
+            dz1[i] = rb(a * d1[i]);
         }
         for (int o = 0; o < HID; ++o) {
             gb1[o] += dz1[o];
-            for (int i = 0; i < IN; ++i) gw1[o * IN + i] += dz1[o] * xr[i];
+            for (int i = 0; i < IN; ++i) gw1[o * IN + i] += dz1[o] * rb(double(x[s * IN + i]));
         }
     }
     return loss / (double(n) * OUT);
@@ -177,7 +215,7 @@ double oracle_mlp_grad(const float* w1, const float* b1, const float* w2, const 
 
 // One sample through a network of any input / output width (<= 16) with
 // the parameter layout [W1 | b1 | W2 | b2 | W3 | b3] (row-major [out][in]),
-// as the GPU rollout evaluates it: bfloat16 weights and inputs, sums
+// as the GPU rollout evaluates it: binary16 weights and inputs, sums
 // rounded to FP32 before the FP32 bias is added (the matrix cores
 // accumulate in FP32), hidden activations T (above).
 void oracle_mlp_forward1(const float* P, uint32_t in, uint32_t out, const float* x, float* y) {
@@ -190,17 +228,17 @@ void oracle_mlp_forward1(const float* P, uint32_t in, uint32_t out, const float*
     double h1[HID], h2[HID];
     for (int o = 0; o < HID; ++o) {
         double z = 0.0;
-        for (uint32_t i = 0; i < in; ++i) z += round_bf16(w1[o * in + i]) * round_bf16(x[i]);
+        for (uint32_t i = 0; i < in; ++i) z += round_f16(w1[o * in + i]) * round_f16(x[i]);
         h1[o] = act(float(z) + b1[o]);
     }
     for (int o = 0; o < HID; ++o) {
         double z = 0.0;
-        for (int i = 0; i < HID; ++i) z += round_bf16(w2[o * HID + i]) * h1[i];
+        for (int i = 0; i < HID; ++i) z += round_f16(w2[o * HID + i]) * h1[i];
         h2[o] = act(float(z) + b2[o]);
     }
     for (uint32_t o = 0; o < out; ++o) {
         double z = 0.0;
-        for (int i = 0; i < HID; ++i) z += round_bf16(w3[o * HID + i]) * h2[i];
+        for (int i = 0; i < HID; ++i) z += round_f16(w3[o * HID + i]) * h2[i];
         y[o] = float(z) + b3[o];
     }
 }

@@ -63,6 +63,7 @@ def bicycle(K, H, seed=SEED, agent=0, substeps=2, lam=None):
 
 
 BICYCLE_ENVELOPE = scenes.BICYCLE_SURROGATE_ENVELOPE
+BICYCLE_TRAINING = scenes.BICYCLE_SURROGATE_TRAINING
 
 
 def quadrotor(K, H, numeric_mode=0, seed=SEED, goal=(2.0, -1.0, 1.5)):

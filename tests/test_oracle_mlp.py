@@ -10,10 +10,18 @@ import oracle_lib as ol
 IN, HID, OUT = 8, 128, 6
 
 
-def _params(seed=0):
+def _params(seed=0, spread=(0.4, 0.1, 0.1, 0.1, 0.1, 0.1)):
+    """Random parameters; hidden biases centred on 1/2, the centre of the
+    activation's linear region."""
     rng = np.random.default_rng(seed)
-    return [rng.normal(0, s, shape).astype(np.float32) for shape, s in
-            (((HID, IN), 0.4), ((HID,), 0.1), ((HID, HID), 0.1), ((HID,), 0.1), ((OUT, HID), 0.1), ((OUT,), 0.1))]
+    shapes = ((HID, IN), (HID,), (HID, HID), (HID,), (OUT, HID), (OUT,))
+    centre = (0.0, 0.5, 0.0, 0.5, 0.0, 0.0)
+    return [rng.normal(c, s, shape).astype(np.float32) for shape, c, s in zip(shapes, centre, spread)]
+
+
+def _f_exact(z):
+    y = np.clip(z, 0.0, 1.0)
+    return y * y * (6.0 - 4.0 * y) - 1.0
 
 
 def _fwd(p, x, bf16=0):
@@ -42,14 +50,16 @@ def test_forward_matches_numpy():
     p = _params()
     x = np.random.default_rng(1).uniform(-1, 1, (32, IN)).astype(np.float32)
     w1, b1, w2, b2, w3, b3 = [a.astype(np.float64) for a in p]
-    h1 = np.tanh(x @ w1.T + b1)
-    h2 = np.tanh(h1 @ w2.T + b2)
+    h1 = _f_exact(x @ w1.T + b1)
+    h2 = _f_exact(h1 @ w2.T + b2)
     y = h2 @ w3.T + b3
     assert np.max(np.abs(_fwd(p, x) - y)) < 1e-12
 
 
 def test_gradients_match_finite_differences():
-    p = _params(2)
+    # Pre-activations well inside (0, 1) or beyond it, none within reach of
+    # the perturbation of a clamp corner (where f has no second derivative).
+    p = _params(2, spread=(0.05, 0.05, 0.04, 0.05, 0.1, 0.1))
     rng = np.random.default_rng(3)
     x = rng.uniform(-1, 1, (16, IN)).astype(np.float32)
     t = rng.normal(0, 1, (16, OUT)).astype(np.float32)
@@ -71,7 +81,7 @@ def test_gradients_match_finite_differences():
         assert abs(fd - gs[k][idx]) <= 1e-5 * max(1.0, abs(fd)) + 1e-8, (k, idx, fd, gs[k][idx])
 
 
-def test_bf16_rounding_changes_little():
+def test_gpu_rounding_changes_little():
     p = _params(4)
     x = np.random.default_rng(5).uniform(-1, 1, (64, IN)).astype(np.float32)
     y64, y16 = _fwd(p, x), _fwd(p, x, bf16=1)
@@ -82,64 +92,42 @@ def test_bf16_rounding_changes_little():
 # The surrogate rollout's pieces (oracle_mlp_forward1, oracle_surrogate_dynamics,
 # oracle_rollout_surrogate_range), pinned against numpy restatements.
 # ---------------------------------------------------------------------------
-def _bf16(a):
-    """Round float32 values to bfloat16 (nearest even) in numpy."""
-    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
-    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
-    return u.astype(np.uint32).view(np.float32)
-
-
 def _flat(p):
     return np.concatenate([a.ravel() for a in p]).astype(np.float32)
 
 
-def _f16_rtz_keys(z):
-    """Binary16 bits of |z| (float32) rounded toward zero, capped at 4.0
-    (0x4400), infinities and NaN capped too; via numpy's binary16 and a
-    correction of its nearest rounding."""
-    a = np.abs(np.asarray(z, np.float32)).astype(np.float64)
-    big = ~(a < 4.0)                           # also NaN
-    a = np.where(big, 0.0, a)
-    h = a.astype(np.float16)                   # nearest
-    over = h.astype(np.float64) > a            # rounded up: step down one binary16 place
-    keys = h.view(np.uint16).astype(np.int64) - over.astype(np.int64)
-    return np.where(big, 0x4400, np.minimum(keys, 0x4400))
+def _f16_rtz(z):
+    """Binary16 values of float32 z rounded toward zero, as float64 (finite
+    magnitudes past 65504 give 65504; infinities and NaN pass), via numpy's
+    nearest rounding and a step back where it rounded away from zero."""
+    z = np.asarray(z, np.float32)
+    with np.errstate(over="ignore", invalid="ignore"):
+        a = np.abs(z).astype(np.float64)
+        h = a.astype(np.float16)
+        up = h.astype(np.float64) > a
+    bits = h.view(np.uint16).astype(np.int64) - up.astype(np.int64)
+    v = bits.astype(np.uint16).view(np.float16).astype(np.float64)
+    return np.where(np.signbit(z), -v, v)
 
 
-def _bf16_nearest_even_exact(t):
-    """bf16 nearest-even of float64 values, exact (ties decided with
-    Fractions)."""
-    from fractions import Fraction
-    out = np.empty(t.shape, np.float64)
-    for i, v in enumerate(t.ravel()):
-        if v == 0.0:
-            out.flat[i] = 0.0
-            continue
-        f = np.float32(v)
-        u = int(np.array(f).view(np.uint32)) & 0xFFFF0000
-        lo = float(np.array(u, np.uint32).view(np.float32))
-        hi = float(np.array(u + 0x10000, np.uint32).view(np.float32))
-        if lo > v:                             # float rounding went up past a bf16 value
-            hi, lo = lo, float(np.array(u - 0x10000, np.uint32).view(np.float32))
-        d_lo, d_hi = Fraction(v) - Fraction(lo), Fraction(hi) - Fraction(v)
-        even_lo = ((int(np.array(lo, np.float32).view(np.uint32)) >> 16) & 1) == 0
-        out.flat[i] = lo if (d_lo < d_hi or (d_lo == d_hi and even_lo)) else hi
-    return out
+def _f16(v):
+    """Nearest-even binary16 of float64 values (numpy's direct conversion)."""
+    return np.asarray(v, np.float64).astype(np.float16).astype(np.float64)
 
 
 def _np_act(z):
-    """The GPU's hidden activation T(z) = bf16(tanh(r16(z)))."""
-    z = np.asarray(z, np.float32)
-    keys = _f16_rtz_keys(z)
-    v = keys.astype(np.uint16).view(np.float16).astype(np.float64)
-    t = _bf16_nearest_even_exact(np.tanh(v))
-    return np.where(np.signbit(z), -t, t)
+    """The GPU's hidden activation T(z) = S16(clamp01(r16(z))), every binary16
+    operation of S16 rounded once (its FP64 value is exact)."""
+    h = _f16_rtz(z)
+    with np.errstate(invalid="ignore"):
+        y = np.where(h > 0, np.minimum(h, 1.0), 0.0)      # NaN, -0 and negatives to 0
+    return _f16(_f16(y * y) * _f16(6.0 - 4.0 * y) - 1.0)
 
 
 def _np_forward1(p, x):
-    """bf16 operands, FP64 sums rounded to FP32, FP32 bias, activation T."""
+    """binary16 operands, FP64 sums rounded to FP32, FP32 bias, activation T."""
     w1, b1, w2, b2, w3, b3 = p
-    f64 = lambda a: _bf16(a).astype(np.float64)
+    f64 = lambda a: np.asarray(a, np.float32).astype(np.float16).astype(np.float64)
     z1 = (f64(x) @ f64(w1).T).astype(np.float32) + b1
     h1 = _np_act(z1)
     z2 = (h1 @ f64(w2).T).astype(np.float32) + b2
@@ -156,34 +144,49 @@ def _oracle_act(z):
     return out
 
 
-def test_activation_over_every_binary16_magnitude():
-    """T at every binary16 value in [0, 4] (each table entry) and just above
-    and below each, signed, against numpy's binary16 and exact bf16
-    rounding; plus the properties that pin it without numpy: odd, monotone,
-    within half a bf16 step of tanh, saturated to 1 from 4 on."""
-    keys = np.arange(0x4401, dtype=np.uint16)
-    v = keys.view(np.float16).astype(np.float32)
-    up = np.nextafter(v, np.float32(np.inf))
-    down = np.nextafter(v, np.float32(0))
-    for z in (v, up, down, -v, -up):
-        assert np.array_equal(_oracle_act(z), _np_act(z).astype(np.float32))
-    t = _oracle_act(v).astype(np.float64)
-    assert np.all(np.diff(t) >= 0) and t[0] == 0.0 and t[-1] == 1.0
-    ulp = np.array([np.spacing(np.float32(x)) * 65536 for x in t[1:]])   # bf16 step at each value
-    assert np.all(np.abs(t[1:] - np.tanh(v[1:].astype(np.float64))) <= 0.5 * ulp + 1e-300)
-    big = np.array([4.0, 4.5, 1e4, 7e4, 3e38, np.inf, np.nan], np.float32)
-    assert np.array_equal(_oracle_act(big), np.ones(7, np.float32))
-    assert np.array_equal(_oracle_act(-big[:-1]), -np.ones(6, np.float32))
+def test_activation_over_every_binary16_value():
+    """T at every binary16 value (infinities and NaN included), just above and
+    below each and at random float32 values, against numpy's binary16
+    conversions; plus what pins T without numpy: its exact values at y = 0,
+    1/4, 1/2, 3/4 and 1, saturation outside [0, 1], and within the rounding
+    bound of its three binary16 operations of the exact cubic (hence
+    monotone and antisymmetric about 1/2 up to that bound)."""
+    v = np.arange(0x10000, dtype=np.uint32).astype(np.uint16).view(np.float16).astype(np.float32)
+    fin = v[np.isfinite(v)]
+    rng = np.random.default_rng(11)
+    for z in (v, np.nextafter(fin, np.float32(np.inf)), np.nextafter(fin, np.float32(-np.inf)),
+              rng.uniform(-2, 2, 100000).astype(np.float32),
+              np.array([3e38, -3e38, 7e4, -7e4, 65519.0, 65520.0, 1e-30, -1e-30], np.float32)):
+        assert np.array_equal(_oracle_act(z), _np_act(z).astype(np.float32), equal_nan=True)
+    exact = {0.0: -1.0, 0.25: -0.6875, 0.5: 0.0, 0.75: 0.6875, 1.0: 1.0}
+    assert np.array_equal(_oracle_act(np.array(list(exact), np.float32)), np.array(list(exact.values()), np.float32))
+    out = np.array([-1e-30, -0.5, -65504.0, -np.inf, np.nan, 1.5, 65504.0, np.inf, 3e38], np.float32)
+    assert np.array_equal(_oracle_act(out), np.array([-1] * 5 + [1] * 4, np.float32))
+    # every binary16 y in [0, 1] in increasing order
+    y = np.arange(0x3C01, dtype=np.uint16).view(np.float16).astype(np.float32)
+    t = _oracle_act(y).astype(np.float64)
+    yd = y.astype(np.float64)
+    s_exact = yd * yd * (6.0 - 4.0 * yd) - 1.0
+    # rounding bound: half a binary16 step of u = y^2 times t, of t = 6 - 4 y
+    # times u, and of the result
+    ulp = lambda v: np.maximum(np.spacing(np.abs(v).astype(np.float16)).astype(np.float64), 2.0 ** -24)
+    u, tt = yd * yd, 6.0 - 4.0 * yd
+    bound = 0.5 * (tt * ulp(u) + u * ulp(tt) + ulp(s_exact)) * (1 + 1e-9)
+    assert np.all(np.abs(t - s_exact) <= bound)
+    assert np.all(np.diff(t) >= -(bound[1:] + bound[:-1]))
+    mirror = _oracle_act((1.0 - y).astype(np.float32)).astype(np.float64)       # 1 - y is exact in float32
+    mb = 0.5 * ((2.0 + 4.0 * yd) * ulp((1 - yd) ** 2) + (1 - yd) ** 2 * ulp(2.0 + 4.0 * yd) + ulp(s_exact))
+    assert np.all(np.abs(t + mirror) <= bound + mb)
     # a value between binary16 neighbours takes the lower one (toward zero)
-    z = np.float32(1.0 + 2.0 ** -11)           # halfway between binary16 1.0 and 1.0 + 2^-10
-    assert _oracle_act(np.array([z]))[0] == _oracle_act(np.array([np.float32(1.0)]))[0]
+    z = np.float32(0.5 + 2.0 ** -12)           # between binary16 0.5 and 0.5 + 2^-11
+    assert _oracle_act(np.array([z]))[0] == 0.0
 
 
 def _fp(a):
     return np.ascontiguousarray(a, np.float32).ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
-def test_forward1_matches_numpy_bf16():
+def test_forward1_matches_numpy_f16():
     p = _params(6)
     x = np.random.default_rng(7).uniform(-1, 1, (64, IN)).astype(np.float32)
     L = ol.lib()
@@ -193,10 +196,10 @@ def test_forward1_matches_numpy_bf16():
     for s in range(x.shape[0]):
         y = np.empty(OUT, np.float32)
         L.oracle_mlp_forward1(_fp(P), ctypes.c_uint32(IN), ctypes.c_uint32(OUT), _fp(x[s]), _fp(y))
-        # tanh is rounded to float before bf16 in numpy and not in C: one bf16
-        # step of an activation can differ in rare ties, hence the tolerance.
+        # numpy sums in another order: a pre-activation can round to the
+        # neighbouring float32 and move T by a binary16 step, hence the tolerance.
         assert np.max(np.abs(y - ref[s])) <= 1e-3, s
-    # and the FP64 oracle_mlp_forward with bf16 = 1 agrees to the bias rounding
+    # and the FP64 oracle_mlp_forward with rounding agrees to the bias rounding
     assert np.max(np.abs(_fwd(p, x, bf16=1) - ref)) <= 1e-3
 
 
