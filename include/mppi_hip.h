@@ -360,8 +360,8 @@ int mppi_hip_surrogate_gradients(mppi_hip_ctx* ctx, uint32_t n, const float* x, 
                                  float* grads, float* loss);
 
 /* The network's hidden activation T of n pre-activations z, evaluated on the
- * device: T(z) = S(y), y = clamp01(r16(z)) (z rounded to binary16 toward
- * zero, clamped to [0, 1], NaN to 0), S(y) = y^2 (6 - 4 y) - 1 evaluated in
+ * device: T(z) = S(y), y = clamp01(r16(z)) (z rounded to binary16, nearest
+ * even, clamped to [0, 1], NaN to 0), S(y) = y^2 (6 - 4 y) - 1 evaluated in
  * binary16 as t = fma(y, -4, 6), u = y * y, T = fma(u, t, -1), each rounded
  * to nearest even.  out receives the n binary16 results as floats.  No
  * reference counterpart: it lets an application reproduce the network on

@@ -625,8 +625,8 @@ int launch_iteration(mppi_hip_ctx* ctx, const mppi_agent_desc& d, uint32_t N, co
     }
     if (sur) {
         void* sparams[] = {sur_args};
-        // 512-sample workgroups (mppi_rollout_surrogate)
-        MPPI_HIP_TRY(ctx, launch_kernel(st, sur_fn, (d.K + 511) / 512, N, 1, sparams, e0, e1, 512));
+        MPPI_HIP_TRY(ctx, launch_kernel(st, sur_fn, (d.K + MPPI_SUR_BLOCK - 1) / MPPI_SUR_BLOCK, N, 1, sparams,
+                                        e0, e1, MPPI_SUR_BLOCK));
     } else {
         MPPI_HIP_TRY(ctx, launch_kernel(st, ctx->mod->rollout, d.n_blk, N, 1, params, e0, e1));
     }

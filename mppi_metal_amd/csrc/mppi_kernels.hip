@@ -1435,8 +1435,8 @@ extern "C" __global__ void __launch_bounds__(256) mppi_surrogate_data(mppi_surro
     for (int k = 0; k < SDIM; ++k) a.t[size_t(i) * SDIM + k] = xs[k] - v[k];
 }
 
-// Rollout with the MLP as dynamics (one sample per lane, 512-thread
-// workgroups, grid (ceil(K / 512), N); the reduce kernel is shared).  Per
+// Rollout with the MLP as dynamics (one sample per lane, MPPI_SUR_BLOCK-thread
+// workgroups, grid (ceil(K / MPPI_SUR_BLOCK), N); the reduce kernel is shared).  Per
 // timestep every wavefront evaluates the network for its 64 samples as four
 // 16-sample column blocks:
 //   layer 1   v_mfma_f32_16x16x16_f16, one per (m-block, column block), the
@@ -1449,7 +1449,6 @@ extern "C" __global__ void __launch_bounds__(256) mppi_surrogate_data(mppi_surro
 // activations on packed binary16 pairs in registers (mppi_act2).  The staging rows are
 // private to a wavefront, whose LDS operations complete in order, so a
 // timestep needs wavefront-level ordering only (no workgroup barrier).
-constexpr int MPPI_SUR_BLOCK = 512;
 
 extern "C" __global__ void __launch_bounds__(MPPI_SUR_BLOCK) mppi_rollout_surrogate(mppi_surrogate_rollout_args sa) {
     constexpr int IN = SDIM + CDIM;

@@ -121,6 +121,11 @@ struct mppi_surrogate_data_args {
     float center[16], inv_half[16], period[16];   // input normalisation (mppi_mlp.h)
 };
 
+// Samples (threads) per workgroup of the surrogate rollout: eight
+// wavefronts, two per SIMD.  (Twelve, with registers cut to three per SIMD,
+// ran slower at K = 2^18: 342 workgroups take two rounds of 256 CUs.)
+constexpr uint32_t MPPI_SUR_BLOCK = 512;
+
 struct mppi_surrogate_rollout_args {
     mppi_step_args s;
     const short* frags;          // A fragments (mppi_mlp.h)

@@ -33,7 +33,7 @@
 //   W3X  : 4 fragments, kb:                    A[i][p] = W3[i][u(kb, p)]       (i < out)
 //
 // Activation: T(z) = S(y), y = clamp01(r16(z)), S(y) = y^2 (6 - 4 y) - 1, with
-// r16 the rounding of the FP32 pre-activation to binary16 toward zero and
+// r16 the rounding of the FP32 pre-activation to binary16, nearest even, and
 // every operation of S one binary16 operation rounded to nearest even, in
 // this order:
 //   t = fma(y, -4, 6),  u = y * y,  T = fma(u, t, -1).
@@ -42,10 +42,10 @@
 // smoothstep), so T is continuous with a continuous derivative
 // dT/dz = 12 y (1 - y) inside (0, 1) and 0 outside: a saturating activation
 // like tanh, whose linear region is centred on z = 1/2 (the biases start
-// there).  On packed binary16 pairs T costs five vector instructions per two
-// values (the conversion, a clamp, two fused multiply-adds and a multiply)
-// and no memory access, and its values are the FP16 B operands of the next
-// layer as they stand.
+// there).  On packed binary16 pairs T costs four vector instructions per two
+// values (the conversion with its output clamp, two fused multiply-adds and
+// a multiply) and no memory access, and its values are the FP16 B operands
+// of the next layer as they stand.
 #ifndef MPPI_HIP_MLP_H
 #define MPPI_HIP_MLP_H
 
@@ -123,14 +123,14 @@ __device__ inline mppi_v4s mppi_pack_f16(mppi_v4f v) {
 }
 
 // y = clamp01(r16(z)) of two pre-activations as a binary16 pair (z0 in the
-// low half): the packed conversion toward zero, then a packed maximum with
-// its output clamp (NaN to 0).  Written out because the compiler would
-// otherwise move the clamp onto the conversion, which does not clamp.
+// low half): one packed conversion to nearest even with its output clamp
+// (v_cvt_pk_f16_f32 ... clamp; NaN clamps to 0).  The maximum and minimum
+// below are that clamp: they fold into the conversion, and have its meaning
+// where they do not.
 __device__ inline mppi_v2h mppi_act_y2(float z0, float z1) {
-    const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(z0, z1));
-    uint32_t y;
-    asm("v_pk_max_f16 %0, %1, %1 clamp" : "=v"(y) : "v"(h));
-    return __builtin_bit_cast(mppi_v2h, y);
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    const mppi_v2h h = __builtin_convertvector((f2){z0, z1}, mppi_v2h);
+    return __builtin_elementwise_min(__builtin_elementwise_max(h, (mppi_v2h){0, 0}), (mppi_v2h){1, 1});
 }
 // S(y) of a binary16 pair.
 __device__ inline uint32_t mppi_act_s2(mppi_v2h y) {

@@ -19,8 +19,8 @@
 // weight-gradient products; the remaining difference is FP32-versus-FP64
 // accumulation.
 // The GPU's activation is T(z) = S16(y), y = clamp01(r16(float(z))): r16
-// rounds the FP32 pre-activation to binary16 toward zero, clamp01 maps NaN
-// to 0, and S16 evaluates S as three binary16 operations, each rounded to
+// rounds the FP32 pre-activation to binary16 (nearest even), clamp01 maps
+// NaN to 0, and S16 evaluates S as three binary16 operations, each rounded to
 // nearest even: t = fma(y, -4, 6), u = y * y, T = fma(u, t, -1).  Each of
 // those is exact in FP64 before its rounding (binary16 operands), so one
 // rounding of the FP64 value is the binary16 operation's result.
@@ -49,8 +49,9 @@ double round_bf16(double v) {
     return double(f);
 }
 
-// Binary16 nearest-even of a finite v (values beyond 65504 are not used
-// here): 11 significant bits above 2^-14, multiples of 2^-24 below.
+// Binary16 nearest-even of a finite v below 65520 in magnitude (larger ones
+// overflow and are handled by the caller): 11 significant bits above 2^-14,
+// multiples of 2^-24 below.
 double round_f16(double v) {
     if (v == 0.0 || !std::isfinite(v)) return v;
     int e;
@@ -59,24 +60,13 @@ double round_f16(double v) {
     return std::ldexp(std::nearbyint(std::ldexp(v, -q)), q);
 }
 
-// Binary16 value of z rounded toward zero (finite magnitudes above 65504
-// give 65504; infinities and NaN pass through).
-double f16_rtz(float z) {
-    const double v = z;
-    if (!std::isfinite(v)) return v;
-    const double a = std::fabs(v);
-    if (a >= 65504.0) return std::copysign(65504.0, v);
-    if (a == 0.0) return v;
-    int e;
-    std::frexp(a, &e);
-    const int q = std::max(e - 11, -24);
-    return std::copysign(std::ldexp(std::floor(std::ldexp(a, -q)), q), v);
-}
-
-// y of the GPU's activation: clamp01(r16(z)), NaN to 0.
+// y of the GPU's activation: clamp01(r16(z)), r16 nearest even (magnitudes
+// from 65520 on overflow to infinity), NaN to 0.
 double act_y(float z) {
-    const double h = f16_rtz(z);
-    if (!(h > 0.0)) return 0.0;                    // also NaN and -0
+    const double v = z;
+    if (!(std::fabs(v) < 65520.0)) return v > 0.0 ? 1.0 : 0.0;   // infinities, overflow, NaN
+    const double h = round_f16(v);
+    if (!(h > 0.0)) return 0.0;                    // also -0
     return h < 1.0 ? h : 1.0;
 }
 

@@ -8,8 +8,8 @@ the forward pass, bf16 operands in the backward pass):
   forward pass      |y_gpu - y_oracle| <= 1e-3 per output, mean <= 1e-4; the
                     oracle rounds the same operands to binary16, so what
                     remains is FP32-versus-FP64 accumulation, which in rare
-                    cases moves a pre-activation across a binary16 boundary
-                    of the conversion toward zero and T by one step
+                    cases moves a pre-activation across a rounding boundary
+                    of its binary16 conversion and T by one step
   gradients         relative L2 error per parameter tensor <= 1e-3, loss
                     <= 1e-4 (same roundings, including the bf16 backward
                     operands)

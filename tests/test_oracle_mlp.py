@@ -96,30 +96,17 @@ def _flat(p):
     return np.concatenate([a.ravel() for a in p]).astype(np.float32)
 
 
-def _f16_rtz(z):
-    """Binary16 values of float32 z rounded toward zero, as float64 (finite
-    magnitudes past 65504 give 65504; infinities and NaN pass), via numpy's
-    nearest rounding and a step back where it rounded away from zero."""
-    z = np.asarray(z, np.float32)
-    with np.errstate(over="ignore", invalid="ignore"):
-        a = np.abs(z).astype(np.float64)
-        h = a.astype(np.float16)
-        up = h.astype(np.float64) > a
-    bits = h.view(np.uint16).astype(np.int64) - up.astype(np.int64)
-    v = bits.astype(np.uint16).view(np.float16).astype(np.float64)
-    return np.where(np.signbit(z), -v, v)
-
-
 def _f16(v):
     """Nearest-even binary16 of float64 values (numpy's direct conversion)."""
     return np.asarray(v, np.float64).astype(np.float16).astype(np.float64)
 
 
 def _np_act(z):
-    """The GPU's hidden activation T(z) = S16(clamp01(r16(z))), every binary16
+    """The GPU's hidden activation T(z) = S16(clamp01(r16(z))): numpy's
+    binary16 conversion (nearest even, overflow to infinity), every binary16
     operation of S16 rounded once (its FP64 value is exact)."""
-    h = _f16_rtz(z)
-    with np.errstate(invalid="ignore"):
+    with np.errstate(over="ignore", invalid="ignore"):
+        h = np.asarray(z, np.float32).astype(np.float16).astype(np.float64)
         y = np.where(h > 0, np.minimum(h, 1.0), 0.0)      # NaN, -0 and negatives to 0
     return _f16(_f16(y * y) * _f16(6.0 - 4.0 * y) - 1.0)
 
@@ -177,9 +164,11 @@ def test_activation_over_every_binary16_value():
     mirror = _oracle_act((1.0 - y).astype(np.float32)).astype(np.float64)       # 1 - y is exact in float32
     mb = 0.5 * ((2.0 + 4.0 * yd) * ulp((1 - yd) ** 2) + (1 - yd) ** 2 * ulp(2.0 + 4.0 * yd) + ulp(s_exact))
     assert np.all(np.abs(t + mirror) <= bound + mb)
-    # a value between binary16 neighbours takes the lower one (toward zero)
-    z = np.float32(0.5 + 2.0 ** -12)           # between binary16 0.5 and 0.5 + 2^-11
-    assert _oracle_act(np.array([z]))[0] == 0.0
+    # a value between binary16 neighbours takes the nearer one, ties the even
+    half = np.float32(0.5 + 2.0 ** -12)        # halfway between binary16 0.5 and 0.5 + 2^-11
+    above = np.nextafter(half, np.float32(1))
+    t = _oracle_act(np.array([half, above, np.float32(0.5 + 2.0 ** -11)], np.float32))
+    assert t[0] == 0.0 and t[1] == t[2] > 0.0
 
 
 def _fp(a):
