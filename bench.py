@@ -438,44 +438,69 @@ def bench_serving(args):
 
 def bench_surrogate(args, substeps=2):
     """Learned bicycle dynamics in the rollout vs plugin dynamics (the
-    bicycle integrated with `substeps` Euler steps per MPPI step)."""
+    bicycle integrated with `substeps` Euler steps per MPPI step).  Speed on
+    pursuit agent 0; the closed-loop cost gap on agents 0..n-1 of the scene
+    (own start, target and threat zones; plant = plugin dynamics, same seed),
+    each agent's gap (surrogate - plain) / plain and their mean."""
     from mppi_metal_amd import HipMPPI, scenes
-    sc = scenes.bicycle_pursuit_scene(1, 42)
+    n_agents = max(1, args.surrogate_agents)
+    sc = scenes.bicycle_pursuit_scene(n_agents, 42)
     sc.model.substeps = substeps
-    mb, cb = sc.model.blob(), sc.cost_blobs[0]
+    mb = sc.model.blob()
     K, H, T = 262144, 64, args.surrogate_steps
-    hip = HipMPPI(6, 2, H, K, sc.lam, sc.sigma, sc.u_min, sc.u_max)
-    hip.link("bicycle.hip", "bicycle_dynamics", "bicycle_pursuit_cost", "bicycle_terminal_cost",
-             len(mb), len(cb), 42)
-    hip.bind_params(mb, cb)
+
+    def context(agent):
+        cb = sc.cost_blobs[agent]
+        h = HipMPPI(6, 2, H, K, sc.lam, sc.sigma, sc.u_min, sc.u_max)
+        h.link("bicycle.hip", "bicycle_dynamics", "bicycle_pursuit_cost", "bicycle_terminal_cost",
+               len(mb), len(cb), 42)
+        h.bind_params(mb, cb)
+        return h
+
+    hip = context(0)
     t0 = time.perf_counter()
     recipe = dict(scenes.BICYCLE_SURROGATE_TRAINING, steps=args.train_steps)
+    if args.train_transitions:
+        recipe["num_transitions"] = args.train_transitions
     hist = hip.surrogate_train(**scenes.BICYCLE_SURROGATE_ENVELOPE, **recipe)
     train_s = time.perf_counter() - t0
     _, rel = hip.surrogate_evaluate(1 << 16, seed=99)
-    print(f"[bench] surrogate trained in {train_s:.2f}s, loss {hist[0]:.3e} -> {hist[-20:].mean():.3e}",
-          file=sys.stderr, flush=True)
-    x0 = sc.x0[0]
+    print(f"[bench] surrogate ({substeps} substeps) trained in {train_s:.2f}s, loss {hist[0]:.3e} -> "
+          f"{hist[-20:].mean():.3e}", file=sys.stderr, flush=True)
+    P, norm = hip.surrogate_export()
     un = np.zeros((H, 2), np.float32)
-    res = {}
-    for name, use in (("plain", False), ("surrogate", True)):
-        hip.surrogate_use(use)
-        hip.simulate_batch(1, 3, 0, x0, un, outputs=())
-        hip.profile(True)
-        t0 = time.perf_counter()
-        st, ct, _, tt = hip.simulate_batch(1, T, 0, x0, un)
-        el = time.perf_counter() - t0
-        roll_ms, red_ms, launches = hip.profile_read()
-        hip.profile(False)
-        nxt = np.concatenate([st[1:, 0], tt])
-        cost = scenes.pursuit_realised_cost(sc, 0, nxt, ct[:, 0], np.arange(T))
-        res[name] = {"value": K * H * T / el, "ms_per_step": el / T * 1e3,
-                     "rollout_us": roll_ms / launches * 1e3, "reduce_us": red_ms / launches * 1e3,
-                     "mean_realised_cost": float(cost.mean())}
-        print(f"[bench] {name}: {T} steps in {el:.3f}s, mean cost {cost.mean():.3f}", file=sys.stderr,
-              flush=True)
-    hip.surrogate_use(False)
+    res, gaps = {}, []
+    for agent in range(n_agents):
+        h = hip if agent == 0 else context(agent)
+        if agent:
+            h.surrogate_import(P, norm)
+        x0 = sc.x0[agent]
+        cost = {}
+        for name, use in (("plain", False), ("surrogate", True)):
+            h.surrogate_use(use)
+            if agent == 0:
+                h.simulate_batch(1, 3, 0, x0, un, outputs=())
+                h.profile(True)
+            t0 = time.perf_counter()
+            st, ct, _, tt = h.simulate_batch(1, T, 0, x0, un)
+            el = time.perf_counter() - t0
+            nxt = np.concatenate([st[1:, 0], tt])
+            cost[name] = float(scenes.pursuit_realised_cost(sc, agent, nxt, ct[:, 0], np.arange(T)).mean())
+            if agent == 0:
+                roll_ms, red_ms, launches = h.profile_read()
+                h.profile(False)
+                res[name] = {"value": K * H * T / el, "ms_per_step": el / T * 1e3,
+                             "rollout_us": roll_ms / launches * 1e3, "reduce_us": red_ms / launches * 1e3,
+                             "mean_realised_cost": cost[name]}
+                print(f"[bench] {name}: {T} steps in {el:.3f}s, mean cost {cost[name]:.3f}", file=sys.stderr,
+                      flush=True)
+        h.surrogate_use(False)
+        if agent:
+            h.close()
+        gaps.append((cost["surrogate"] - cost["plain"]) / cost["plain"])
     hip.close()
+    print(f"[bench] closed-loop gaps over {n_agents} agents: mean {np.mean(gaps):+.4f}, "
+          + " ".join(f"{g:+.4f}" for g in gaps), file=sys.stderr, flush=True)
     sur = res["surrogate"]
     sur_s = sur["rollout_us"] * 1e-6
     flops = K * H * SURROGATE_FLOPS / sur_s
@@ -491,8 +516,8 @@ def bench_surrogate(args, substeps=2):
                 "traffic": measured_traffic(key),
                 "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/" + COUNTERS.name}
     return {
-        "workload": "learned dynamic-bicycle surrogate (MLP 8-128-128-6, smoothstep activation, fp16 MFMA, "
-                    "fp32 accumulate), "
+        "workload": "learned dynamic-bicycle surrogate (MLP 8-128-128-6, tanh hidden layers as a binary16 "
+                    "polynomial, fp16 MFMA, fp32 accumulate), "
                     f"pursuit, closed loop, plugin dynamics with {substeps} Euler substeps per step",
         "K": K, "H": H, "closed_loop_steps": T, "substeps": substeps,
         "train": {"transitions": recipe["num_transitions"], "batch": recipe["batch"], "steps": recipe["steps"],
@@ -501,8 +526,10 @@ def bench_surrogate(args, substeps=2):
                   "one_step_relative_rmse": [float(v) for v in rel]},
         "plain": res["plain"], "surrogate": sur,
         "speed_ratio_surrogate_vs_plain": sur["value"] / res["plain"]["value"],
-        "closed_loop_cost_gap": (sur["mean_realised_cost"] - res["plain"]["mean_realised_cost"])
-                                / res["plain"]["mean_realised_cost"],
+        "closed_loop_cost_gap": float(np.mean(gaps)),
+        "closed_loop_cost_gap_per_agent": [float(g) for g in gaps],
+        "closed_loop_cost_gap_how": f"mean over pursuit agents 0..{n_agents - 1} of (surrogate - plain) / plain, "
+                                    f"mean realised cost of {T} closed-loop steps each, plant = plugin dynamics",
         "roofline": roofline if substeps == 2 else None,
     }
 
@@ -547,6 +574,9 @@ def main():
     ap.add_argument("--requests", type=int, default=1024)
     ap.add_argument("--surrogate-steps", type=int, default=500)
     ap.add_argument("--train-steps", type=int, default=40000)
+    ap.add_argument("--train-transitions", type=int, default=0, help="0: the workload's recipe")
+    ap.add_argument("--surrogate-agents", type=int, default=8,
+                    help="pursuit agents over which the closed-loop cost gap is averaged")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three device-to-device copies of 256 MiB (the reference kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")

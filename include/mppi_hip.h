@@ -360,10 +360,13 @@ int mppi_hip_surrogate_gradients(mppi_hip_ctx* ctx, uint32_t n, const float* x, 
                                  float* grads, float* loss);
 
 /* The network's hidden activation T of n pre-activations z, evaluated on the
- * device: T(z) = S(y), y = clamp01(r16(z)) (z rounded to binary16, nearest
- * even, clamped to [0, 1], NaN to 0), S(y) = y^2 (6 - 4 y) - 1 evaluated in
- * binary16 as t = fma(y, -4, 6), u = y * y, T = fma(u, t, -1), each rounded
- * to nearest even.  out receives the n binary16 results as floats.  No
+ * device: tanh as T(z) = Q(y), y = clamp01(r16(z)) (z rounded to binary16,
+ * nearest even, clamped to [0, 1], NaN to 0), Q = s P(v), s = 2 y - 1,
+ * v = s^2 - 1/2, P of degree 4 with binary16 coefficients, evaluated as
+ * seven binary16 operations each rounded to nearest even (s = fma(y, 2, -1),
+ * v = fma(s, s, -1/2), four Horner fused multiply-adds, T = s * p); T is
+ * within 4.31e-3 of tanh(5.54 (r16(z) - 1/2)) for every z.  out receives the
+ * n binary16 results as floats.  No
  * reference counterpart: it lets an application reproduce the network on
  * the CPU bit for bit in its activations. */
 int mppi_hip_surrogate_activation(mppi_hip_ctx* ctx, uint32_t n, const float* z, float* out);

@@ -1425,7 +1425,7 @@ extern "C" __global__ void __launch_bounds__(256) mppi_surrogate_data(mppi_surro
         }
     }
 #pragma unroll
-    for (int k = 0; k < IN; ++k) a.x[size_t(i) * IN + k] = mppi_mlp_input(v[k], a.center[k], a.inv_half[k], a.period[k]);
+    for (int k = 0; k < IN; ++k) a.x[size_t(i) * IN + k] = mppi_mlp_input(v[k], a.center[k], a.inv_half[k], a.period[k], a.inv_period[k]);
     float xs[SDIM];
 #pragma unroll
     for (int k = 0; k < SDIM; ++k) xs[k] = v[k];
@@ -1515,9 +1515,9 @@ extern "C" __global__ void __launch_bounds__(MPPI_SUR_BLOCK) mppi_rollout_surrog
                     for (int j = 0; j < 4; ++j) {
                         const int kk = 4 * q + j;
                         xin[q][j] = kk < SDIM ? mppi_mlp_input(x[kk < SDIM ? kk : 0], sa.in_center[kk], sa.in_inv_half[kk],
-                                                               sa.in_period[kk])
+                                                               sa.in_period[kk], sa.in_inv_period[kk])
                                   : kk < IN ? mppi_mlp_input(u[kk - SDIM < CDIM ? kk - SDIM : 0], sa.in_center[kk],
-                                                             sa.in_inv_half[kk], sa.in_period[kk])
+                                                             sa.in_inv_half[kk], sa.in_period[kk], sa.in_inv_period[kk])
                                             : 0.0f;
                     }
 #pragma unroll

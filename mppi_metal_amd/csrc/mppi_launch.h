@@ -118,7 +118,7 @@ struct mppi_surrogate_data_args {
     uint32_t key0, key1;
     uint32_t stream;             // counter word 2: data stream id
     float lo[16], hi[16];        // sampling box of (state, control)
-    float center[16], inv_half[16], period[16];   // input normalisation (mppi_mlp.h)
+    float center[16], inv_half[16], period[16], inv_period[16];   // input normalisation (mppi_mlp.h)
 };
 
 // Samples (threads) per workgroup of the surrogate rollout: eight
@@ -132,7 +132,7 @@ struct mppi_surrogate_rollout_args {
     const float* params;         // FP32 parameters (biases are read from here)
     uint32_t b1, b2, b3;         // bias offsets into params
     uint32_t pad;
-    float in_center[16], in_inv_half[16], in_period[16], out_mu[16], out_sd[16];
+    float in_center[16], in_inv_half[16], in_period[16], in_inv_period[16], out_mu[16], out_sd[16];
 };
 
 #endif  // MPPI_HIP_LAUNCH_H

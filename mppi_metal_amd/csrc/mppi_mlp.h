@@ -32,20 +32,32 @@
 //   W2X  : 32 fragments, mb * 4 + kb:          A[i][p] = W2[16mb + i][u(kb, p)]
 //   W3X  : 4 fragments, kb:                    A[i][p] = W3[i][u(kb, p)]       (i < out)
 //
-// Activation: T(z) = S(y), y = clamp01(r16(z)), S(y) = y^2 (6 - 4 y) - 1, with
-// r16 the rounding of the FP32 pre-activation to binary16, nearest even, and
-// every operation of S one binary16 operation rounded to nearest even, in
+// Activation: tanh, as T(z) = Q(y), y = clamp01(r16(z)), with r16 the rounding
+// of the FP32 pre-activation to binary16, nearest even (clamp01 maps NaN to
+// 0: the hardware's output clamp), and Q the odd polynomial
+//   Q = s P(v),  s = 2 y - 1,  v = s^2 - 1/2,
+//   P(v) = c0 + v (c1 + v (c2 + v (c3 + v c4))),
+// degree 9 in s, its five coefficients binary16 values (MPPI_MLP_TANH_C0..4)
+// fitted (Lawson's iteration toward minimax, then a search over their
+// last bits with every binary16 rounding in place) to tanh(A s), A = 2.77.
+// So T(z) = tanh(A (2 z - 1)) = tanh(5.54 (z - 1/2)) for z in [0, 1], and
+// T saturates at Q(0) = -Q(1) beyond; over every binary16 z, T is within
+// 4.31e-3 of tanh(5.54 (r16(z) - 1/2)) (inside and beyond the clamp).  The
+// centring on z = 1/2 and the slope 5.54 are a change of variable that the
+// first two layers' weights and biases absorb (z_tanh = 5.54 (z - 1/2)): the
+// network is an MLP with tanh hidden layers, whose biases start at 1/2.
+// Every operation is one binary16 operation rounded to nearest even, in
 // this order:
-//   t = fma(y, -4, 6),  u = y * y,  T = fma(u, t, -1).
-// clamp01 maps NaN to 0 (the hardware's output clamp).  S is the cubic that
-// rises from -1 at y = 0 to 1 at y = 1 with zero slope at both ends (a
-// smoothstep), so T is continuous with a continuous derivative
-// dT/dz = 12 y (1 - y) inside (0, 1) and 0 outside: a saturating activation
-// like tanh, whose linear region is centred on z = 1/2 (the biases start
-// there).  On packed binary16 pairs T costs four vector instructions per two
-// values (the conversion with its output clamp, two fused multiply-adds and
-// a multiply) and no memory access, and its values are the FP16 B operands
-// of the next layer as they stand.
+//   s = fma(y, 2, -1),  v = fma(s, s, -1/2),
+//   p = fma(v, c4, c3),  p = fma(v, p, c2),  p = fma(v, p, c1),  p = fma(v, p, c0),
+//   T = s * p.
+// The variable v is centred on its range [-1/2, 1/2], which keeps the
+// Horner sums free of the cancellation that monomials of s^2 would have in
+// binary16.  On packed binary16 pairs T costs eight vector instructions per
+// two values (the conversion with its output clamp, six fused multiply-adds
+// and a multiply) and no memory access, and its values are the FP16 B
+// operands of the next layer as they stand.  dT/dz = 2 (P(v) + 2 s^2 P'(v))
+// inside (0, 1), 0 outside.
 #ifndef MPPI_HIP_MLP_H
 #define MPPI_HIP_MLP_H
 
@@ -54,6 +66,12 @@
 #endif
 
 #define MPPI_MLP_HID 128
+// tanh polynomial (above): binary16 coefficients, exact as written.
+#define MPPI_MLP_TANH_C0 1.3583984375f
+#define MPPI_MLP_TANH_C1 (-1.0908203125f)
+#define MPPI_MLP_TANH_C2 1.298828125f
+#define MPPI_MLP_TANH_C3 (-2.537109375f)
+#define MPPI_MLP_TANH_C4 2.80859375f
 #define MPPI_MLP_FRAG_W1 0
 #define MPPI_MLP_FRAG_W2 8
 #define MPPI_MLP_FRAG_W3 72
@@ -78,8 +96,9 @@ struct mppi_mlp_dims {
 
 // Input normalisation x_n = wrap(x - center) * inv_half, where wrap reduces
 // into [-period/2, period/2] when period > 0 (inputs the dynamics is periodic
-// in) and inv_half = 0 drops an input (one the dynamics is invariant to);
-// output (state delta) delta = y * out_sd + out_mu.
+// in), as d - period * rint(d * inv_period) with inv_period = 1 / period
+// rounded to FP32 once, and inv_half = 0 drops an input (one the dynamics is
+// invariant to); output (state delta) delta = y * out_sd + out_mu.
 struct mppi_mlp_norm {
     float in_center[16];
     float in_inv_half[16];
@@ -97,9 +116,9 @@ typedef _Float16 mppi_v2h __attribute__((ext_vector_type(2)));
 typedef _Float16 mppi_v4h __attribute__((ext_vector_type(4)));
 typedef _Float16 mppi_v8h __attribute__((ext_vector_type(8)));
 
-__device__ inline float mppi_mlp_input(float v, float center, float inv_half, float period) {
+__device__ inline float mppi_mlp_input(float v, float center, float inv_half, float period, float inv_period) {
     float d = v - center;
-    if (period > 0.0f) d -= period * __builtin_rintf(d / period);
+    if (period > 0.0f) d -= period * __builtin_rintf(d * inv_period);
     return d * inv_half;
 }
 
@@ -132,10 +151,25 @@ __device__ inline mppi_v2h mppi_act_y2(float z0, float z1) {
     const mppi_v2h h = __builtin_convertvector((f2){z0, z1}, mppi_v2h);
     return __builtin_elementwise_min(__builtin_elementwise_max(h, (mppi_v2h){0, 0}), (mppi_v2h){1, 1});
 }
-// S(y) of a binary16 pair.
+// Q(y) of a binary16 pair.
 __device__ inline uint32_t mppi_act_s2(mppi_v2h y) {
-    const mppi_v2h t = __builtin_elementwise_fma(y, (mppi_v2h){-4, -4}, (mppi_v2h){6, 6});
-    return __builtin_bit_cast(uint32_t, __builtin_elementwise_fma(y * y, t, (mppi_v2h){-1, -1}));
+    const mppi_v2h s = __builtin_elementwise_fma(y, (mppi_v2h){2, 2}, (mppi_v2h){-1, -1});
+    const mppi_v2h v = __builtin_elementwise_fma(s, s, (mppi_v2h){-0.5, -0.5});
+    const _Float16 c0 = MPPI_MLP_TANH_C0, c1 = MPPI_MLP_TANH_C1, c2 = MPPI_MLP_TANH_C2, c3 = MPPI_MLP_TANH_C3,
+                   c4 = MPPI_MLP_TANH_C4;
+    mppi_v2h p = __builtin_elementwise_fma(v, (mppi_v2h){c4, c4}, (mppi_v2h){c3, c3});
+    p = __builtin_elementwise_fma(v, p, (mppi_v2h){c2, c2});
+    p = __builtin_elementwise_fma(v, p, (mppi_v2h){c1, c1});
+    p = __builtin_elementwise_fma(v, p, (mppi_v2h){c0, c0});
+    return __builtin_bit_cast(uint32_t, s * p);
+}
+// dT/dz at a binary16 y (training's backward pass, FP32): 2 (P(v) + 2 s^2 P'(v))
+// for y in (0, 1), 0 at the clamp.
+__device__ inline float mppi_act_deriv(float y) {
+    const float s = 2.0f * y - 1.0f, v = s * s - 0.5f;
+    const float p = MPPI_MLP_TANH_C0 + v * (MPPI_MLP_TANH_C1 + v * (MPPI_MLP_TANH_C2 + v * (MPPI_MLP_TANH_C3 + v * MPPI_MLP_TANH_C4)));
+    const float dp = MPPI_MLP_TANH_C1 + v * (2.0f * MPPI_MLP_TANH_C2 + v * (3.0f * MPPI_MLP_TANH_C3 + v * (4.0f * MPPI_MLP_TANH_C4)));
+    return (y > 0.0f && y < 1.0f) ? 2.0f * (p + 2.0f * s * s * dp) : 0.0f;
 }
 // T of two pre-activations, as a pair of binary16 (z0 in the low half).
 __device__ inline uint32_t mppi_act2(float z0, float z1) {
@@ -147,7 +181,7 @@ __device__ inline mppi_v4s mppi_act4(mppi_v4f z) {
     return __builtin_bit_cast(mppi_v4s, (uint64_t(b) << 32) | a);
 }
 
-// T of four pre-activations and their y (for the derivative 12 y (1 - y)).
+// T of four pre-activations and their y (for the derivative, mppi_act_deriv).
 __device__ inline mppi_v4s mppi_act4_y(mppi_v4f z, mppi_v4s* y_out) {
     const mppi_v2h ya = mppi_act_y2(z[0], z[1]), yb = mppi_act_y2(z[2], z[3]);
     *y_out = __builtin_bit_cast(mppi_v4s, __builtin_shufflevector(ya, yb, 0, 1, 2, 3));

@@ -145,7 +145,7 @@ extern "C" __global__ void __launch_bounds__(256) mppi_mlp_train_fwd_bwd(train_a
         dyb[j] = mppi_f2bf(dy);
         a.dyt[size_t(r) * n + s] = dyb[j];
     }
-    // dH2 = W3^T dY, dZ2 = dH2 T'(z2), T' = 12 y (1 - y)
+    // dH2 = W3^T dY, dZ2 = dH2 T'(z2), T' from the stored y (mppi_act_deriv)
     mppi_v4s dz2b[8];
 #pragma unroll
     for (int mb = 0; mb < 8; ++mb) {
@@ -153,7 +153,7 @@ extern "C" __global__ void __launch_bounds__(256) mppi_mlp_train_fwd_bwd(train_a
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const float yv = mppi_h2f(y2b[mb][j]);
-            dz2b[mb][j] = mppi_f2bf(acc[j] * (12.0f * yv * (1.0f - yv)));
+            dz2b[mb][j] = mppi_f2bf(acc[j] * mppi_act_deriv(yv));
             a.dz2t[size_t(16 * mb + 4 * g + j) * n + s] = dz2b[mb][j];
         }
     }
@@ -167,7 +167,7 @@ extern "C" __global__ void __launch_bounds__(256) mppi_mlp_train_fwd_bwd(train_a
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const float yv = mppi_h2f(y1b[mb][j]);
-            a.dz1t[size_t(16 * mb + 4 * g + j) * n + s] = mppi_f2bf(acc[j] * (12.0f * yv * (1.0f - yv)));
+            a.dz1t[size_t(16 * mb + 4 * g + j) * n + s] = mppi_f2bf(acc[j] * mppi_act_deriv(yv));
         }
     }
     // loss partial of this 16-sample block
@@ -398,10 +398,15 @@ int ensure_activations(mppi_hip_ctx* ctx, uint32_t n) {
     return MPPI_HIP_OK;
 }
 
+// 1 / period in FP32 (0 for an input that is not periodic).
+float inv_period(float period) {
+    return period > 0.0f ? 1.0f / period : 0.0f;
+}
+
 // Host form of mppi_mlp_input (same operations, FP32).
 float normalise_input(const mppi_mlp_norm& n, uint32_t k, float v) {
     float d = v - n.in_center[k];
-    if (n.in_period[k] > 0.0f) d -= n.in_period[k] * std::nearbyint(d / n.in_period[k]);
+    if (n.in_period[k] > 0.0f) d -= n.in_period[k] * std::nearbyint(d * inv_period(n.in_period[k]));
     return d * n.in_inv_half[k];
 }
 
@@ -514,6 +519,7 @@ int generate(mppi_hip_ctx* ctx, uint32_t M, uint32_t stream, uint64_t seed, cons
         da.center[k] = s.norm.in_center[k];
         da.inv_half[k] = s.norm.in_inv_half[k];
         da.period[k] = s.norm.in_period[k];
+        da.inv_period[k] = inv_period(s.norm.in_period[k]);
     }
     void* params[] = {&da};
     MPPI_HIP_TRY(ctx, hipModuleLaunchKernel(s.mod->surrogate_data, (M + 255) / 256, 1, 1, 256, 1, 1, 0,
@@ -550,6 +556,7 @@ bool surrogate_rollout(mppi_hip_ctx* ctx, const mppi_step_args& a, hipFunction_t
         ra.in_center[k] = s.norm.in_center[k];
         ra.in_inv_half[k] = s.norm.in_inv_half[k];
         ra.in_period[k] = s.norm.in_period[k];
+        ra.in_inv_period[k] = inv_period(s.norm.in_period[k]);
         ra.out_mu[k] = s.norm.out_mu[k];
         ra.out_sd[k] = s.norm.out_sd[k];
     }
@@ -609,8 +616,9 @@ int mppi_hip_surrogate_train(mppi_hip_ctx* ctx, const mppi_hip_surrogate_config*
         s.norm.out_sd[k] = float(std::max(std::sqrt(v / ns), 1e-6));
     }
     // Initialisation: weights uniform +-sqrt(6 / (fan_in + fan_out)), those
-    // of the hidden layers divided by 3 (the slope of T at the centre of its
-    // linear region); hidden biases 1/2 (that centre), output biases 0.
+    // of the hidden layers divided by 5.5 (about the slope of T at the centre
+    // of its linear region, 2 A = 5.54); hidden biases 1/2 (that centre),
+    // output biases 0.
     std::vector<float> P(s.dims.n_params, 0.0f);
     std::mt19937_64 gen(cfg->seed ^ 0x5eed5eedULL);
     auto init = [&](uint32_t off, uint32_t rows, uint32_t cols, float scale) {
@@ -618,8 +626,8 @@ int mppi_hip_surrogate_train(mppi_hip_ctx* ctx, const mppi_hip_surrogate_config*
         std::uniform_real_distribution<float> U(-a, a);
         for (uint32_t i = 0; i < rows * cols; ++i) P[off + i] = U(gen);
     };
-    init(s.dims.w1, HID, in, 1.0f / 3.0f);
-    init(s.dims.w2, HID, HID, 1.0f / 3.0f);
+    init(s.dims.w1, HID, in, 1.0f / 5.5f);
+    init(s.dims.w2, HID, HID, 1.0f / 5.5f);
     init(s.dims.w3, out, HID, 1.0f);
     for (uint32_t i = 0; i < HID; ++i) P[s.dims.b1 + i] = P[s.dims.b2 + i] = 0.5f;
     MPPI_HIP_TRY(ctx, hipMemcpyAsync(s.params.ptr, P.data(), sizeof(float) * P.size(), hipMemcpyHostToDevice, ctx->stream));

@@ -206,10 +206,10 @@ def bicycle_pursuit_scene(n_agents=256, seed=42, k_log2=(10, 14), h_range=(16, 9
 BICYCLE_SURROGATE_ENVELOPE = dict(state_lo=[-50.0, -50.0, -np.pi, 0.0, -4.0, -4.0],
                                   state_hi=[50.0, 50.0, np.pi, 20.0, 4.0, 4.0],
                                   state_input=[2, 2, 1, 0, 0, 0])
-# The training of the learned-surrogate workload (K = 2^18, H = 64): 16M
-# transitions, 40000 Adam steps of 4096 (about ten passes over the data),
+# The training of the learned-surrogate workload (K = 2^18, H = 64): 4M
+# transitions, 40000 Adam steps of 4096 (about forty passes over the data),
 # peak step size 1e-3.
-BICYCLE_SURROGATE_TRAINING = dict(num_transitions=16 << 20, batch=4096, steps=40000, learning_rate=1e-3,
+BICYCLE_SURROGATE_TRAINING = dict(num_transitions=4 << 20, batch=4096, steps=40000, learning_rate=1e-3,
                                   seed=7)
 
 

@@ -1,6 +1,5 @@
 // mlp_oracle.cpp — TEST INFRASTRUCTURE: CPU oracle of the learned-dynamics
-// surrogate (an MLP 8 -> 128 -> 128 -> 6 with a saturating hidden
-// activation).
+// surrogate (an MLP 8 -> 128 -> 128 -> 6 with tanh hidden layers).
 //
 // No reference counterpart: the reference has no learned model.  The
 // GPU kernels (mppi_metal_amd/csrc/surrogate.hip, mppi_mlp.h) compute the
@@ -9,8 +8,12 @@
 // and Adam; this file states the same mathematics in FP64:
 //   z1 = W1 x + b1, h1 = f(z1), z2 = W2 h1 + b2, h2 = f(z2),
 //   y = W3 h2 + b3, loss = mean over batch and outputs of (y - t)^2,
-//   f(z) = S(clamp01(z)), S(y) = y^2 (6 - 4 y) - 1, f'(z) = 12 y (1 - y),
-// and its analytic gradients by backpropagation.  With `rounded` set the
+//   f(z) = Q(clamp01(z)), Q(y) = s P(v), s = 2 y - 1, v = s^2 - 1/2,
+//   P(v) = c0 + v (c1 + v (c2 + v (c3 + v c4))), f'(z) = 2 (P(v) + 2 s^2 P'(v))
+//   inside (0, 1) and 0 outside,
+// with the device's binary16 coefficients (mppi_mlp.h: Q is tanh(2.77 s)
+// within 4.31e-3 once rounded), and its analytic gradients by
+// backpropagation.  With `rounded` set the
 // FP64 computation sees the GPU's rounded operands: weights and inputs of
 // the forward pass rounded to binary16 (nearest even), the hidden
 // activations T(z) of the GPU (below) in place of f, and what the backward
@@ -18,15 +21,17 @@
 // of both hidden layers and the inputs and activations of the
 // weight-gradient products; the remaining difference is FP32-versus-FP64
 // accumulation.
-// The GPU's activation is T(z) = S16(y), y = clamp01(r16(float(z))): r16
+// The GPU's activation is T(z) = Q16(y), y = clamp01(r16(float(z))): r16
 // rounds the FP32 pre-activation to binary16 (nearest even), clamp01 maps
-// NaN to 0, and S16 evaluates S as three binary16 operations, each rounded to
-// nearest even: t = fma(y, -4, 6), u = y * y, T = fma(u, t, -1).  Each of
-// those is exact in FP64 before its rounding (binary16 operands), so one
-// rounding of the FP64 value is the binary16 operation's result.
+// NaN to 0, and Q16 evaluates Q as seven binary16 operations, each rounded
+// to nearest even: s = fma(y, 2, -1), v = fma(s, s, -1/2), p = fma(v, c4,
+// c3), p = fma(v, p, c2), p = fma(v, p, c1), p = fma(v, p, c0), T = s * p.
+// Each of those is exact in FP64 before its rounding (binary16 operands, whose
+// products and sums span far fewer than 53 bits), so one rounding of the FP64
+// value is the binary16 operation's result.
 // Pinned in tests/test_oracle_mlp.py by central finite differences (FP64
 // path) and, for T, against numpy's binary16 conversions over every
-// binary16 value.
+// binary16 value and against tanh itself.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -70,24 +75,42 @@ double act_y(float z) {
     return h < 1.0 ? h : 1.0;
 }
 
-// S evaluated in binary16 (three rounded operations).
-double s16(double y) {
-    const double t = round_f16(6.0 - 4.0 * y);
-    const double u = round_f16(y * y);
-    return round_f16(u * t - 1.0);
+// The tanh polynomial's binary16 coefficients (mppi_mlp.h).
+constexpr double TC0 = 1.3583984375, TC1 = -1.0908203125, TC2 = 1.298828125, TC3 = -2.537109375,
+                 TC4 = 2.80859375;
+
+// Q evaluated in binary16 (seven rounded operations).
+double q16(double y) {
+    const double s = round_f16(2.0 * y - 1.0);
+    const double v = round_f16(s * s - 0.5);
+    double p = round_f16(v * TC4 + TC3);
+    p = round_f16(v * p + TC2);
+    p = round_f16(v * p + TC1);
+    p = round_f16(v * p + TC0);
+    return round_f16(s * p);
+}
+
+// dT/dz at y (the backward pass's factor): 2 (P(v) + 2 s^2 P'(v)) inside
+// (0, 1), 0 at the clamp.
+double dq(double y) {
+    if (!(y > 0.0 && y < 1.0)) return 0.0;
+    const double s = 2.0 * y - 1.0, v = s * s - 0.5;
+    const double p = TC0 + v * (TC1 + v * (TC2 + v * (TC3 + v * TC4)));
+    const double dp = TC1 + v * (2.0 * TC2 + v * (3.0 * TC3 + v * 4.0 * TC4));
+    return 2.0 * (p + 2.0 * s * s * dp);
 }
 
 // The GPU's activation T of an FP32 pre-activation.
-double act(float z) { return s16(act_y(z)); }
+double act(float z) { return q16(act_y(z)); }
 
 // The exact activation f and its derivative (FP64 path).
 double f_exact(double z) {
     const double y = z <= 0.0 ? 0.0 : z >= 1.0 ? 1.0 : z;
-    return y * y * (6.0 - 4.0 * y) - 1.0;
+    const double s = 2.0 * y - 1.0, v = s * s - 0.5;
+    return s * (TC0 + v * (TC1 + v * (TC2 + v * (TC3 + v * TC4))));
 }
 double df_exact(double z) {
-    const double y = z <= 0.0 ? 0.0 : z >= 1.0 ? 1.0 : z;
-    return 12.0 * y * (1.0 - y);
+    return z <= 0.0 || z >= 1.0 ? 0.0 : dq(z);
 }
 
 struct Params {
@@ -150,8 +173,8 @@ double oracle_mlp_grad(const float* w1, const float* b1, const float* w2, const 
             for (int i = 0; i < IN; ++i) z += r(w1[o * IN + i]) * xr[i];
             if (rounded) {
                 const double yv = act_y(float(z));
-                h1[o] = s16(yv);
-                d1[o] = 12.0 * yv * (1.0 - yv);
+                h1[o] = q16(yv);
+                d1[o] = dq(yv);
             } else {
                 h1[o] = f_exact(z);
                 d1[o] = df_exact(z);
@@ -162,8 +185,8 @@ double oracle_mlp_grad(const float* w1, const float* b1, const float* w2, const 
             for (int i = 0; i < HID; ++i) z += r(w2[o * HID + i]) * h1[i];
             if (rounded) {
                 const double yv = act_y(float(z));
-                h2[o] = s16(yv);
-                d2[o] = 12.0 * yv * (1.0 - yv);
+                h2[o] = q16(yv);
+                d2[o] = dq(yv);
             } else {
                 h2[o] = f_exact(z);
                 d2[o] = df_exact(z);

@@ -567,7 +567,7 @@ void surrogate_dynamics(const Surrogate& s, uint32_t D, uint32_t C, float* x, co
     float xin[16], y[16];
     for (uint32_t k = 0; k < D + C; ++k) {
         float d = (k < D ? x[k] : u[k - D]) - center[k];
-        if (period[k] > 0.0f) d -= period[k] * std::nearbyint(d / period[k]);
+        if (period[k] > 0.0f) d -= period[k] * std::nearbyint(d * (1.0f / period[k]));
         xin[k] = d * inv_half[k];
     }
     oracle_mlp_forward1(s.params, D + C, D, xin, y);

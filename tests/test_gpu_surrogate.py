@@ -340,29 +340,40 @@ def test_surrogate_gains_on_costly_dynamics(require_gpu):
     the network -- the same bicycle integrated with 64 substeps per step, a
     high-fidelity integration -- the surrogate rollout runs more sample-steps
     per second than the plugin rollout at the full K = 2^18, H = 64, and the
-    closed loop (plant: the plugin dynamics) keeps its mean realised cost
-    within 1% above plain MPPI's."""
+    closed loop (plant: the plugin dynamics) keeps its mean realised cost,
+    averaged over four pursuit agents (200 steps each), within 1% above plain
+    MPPI's."""
     K, H, T = 1 << 18, 64, 200
     hip, ora, sc = bicycle(K, H, substeps=64)
     hip.surrogate_train(**BICYCLE_ENVELOPE, **BICYCLE_TRAINING)
-    x0 = sc.x0[0]
+    P, norm = hip.surrogate_export()
     un = np.zeros((H, 2), np.float32)
-    res = {}
-    try:
-        for use in (False, True):
-            hip.surrogate_use(use)
-            hip.simulate_batch(1, 3, 0, x0, un, outputs=())
-            hip.profile(True)
-            st, ct, _, tt = hip.simulate_batch(1, T, 0, x0, un)
-            roll_ms, _, n = hip.profile_read()
-            hip.profile(False)
-            cost = ora.closed_loop_cost(st[:, 0], ct[:, 0], tt[0])
-            res[use] = (roll_ms / n, float(cost.mean()))
-    finally:
-        hip.surrogate_use(False)
-    speed = res[False][0] / res[True][0]
-    gap = (res[True][1] - res[False][1]) / res[False][1]
-    _report("gain on costly dynamics", plugin_rollout_ms=res[False][0], surrogate_rollout_ms=res[True][0],
-            speedup=speed, plain_cost=res[False][1], surrogate_cost=res[True][1], gap=gap)
+    times, gaps = {}, []
+    for agent in range(4):
+        h, o, s = (hip, ora, sc) if agent == 0 else bicycle(K, H, agent=agent, substeps=64)
+        if agent:
+            h.surrogate_import(P, norm)
+        x0 = s.x0[agent]
+        cost = {}
+        try:
+            for use in (False, True):
+                h.surrogate_use(use)
+                if agent == 0:
+                    h.simulate_batch(1, 3, 0, x0, un, outputs=())
+                    h.profile(True)
+                st, ct, _, tt = h.simulate_batch(1, T, 0, x0, un)
+                if agent == 0:
+                    roll_ms, _, n = h.profile_read()
+                    h.profile(False)
+                    times[use] = roll_ms / n
+                cost[use] = float(o.closed_loop_cost(st[:, 0], ct[:, 0], tt[0]).mean())
+        finally:
+            h.surrogate_use(False)
+        gaps.append((cost[True] - cost[False]) / cost[False])
+        if agent:
+            h.close()
+    speed = times[False] / times[True]
+    _report("gain on costly dynamics", plugin_rollout_ms=times[False], surrogate_rollout_ms=times[True],
+            speedup=speed, mean_gap=float(np.mean(gaps)), gaps=" ".join(f"{g:+.4f}" for g in gaps))
     assert speed > 1.2
-    assert gap <= 0.01
+    assert np.mean(gaps) <= 0.01

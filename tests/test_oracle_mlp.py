@@ -19,9 +19,17 @@ def _params(seed=0, spread=(0.4, 0.1, 0.1, 0.1, 0.1, 0.1)):
     return [rng.normal(c, s, shape).astype(np.float32) for shape, c, s in zip(shapes, centre, spread)]
 
 
+# The tanh polynomial's binary16 coefficients (mppi_mlp.h) and the slope it
+# is fitted to: Q(y) = s P(s^2 - 1/2), s = 2 y - 1, approximates tanh(A s).
+TANH_C = (1.3583984375, -1.0908203125, 1.298828125, -2.537109375, 2.80859375)
+TANH_A = 2.77
+
+
 def _f_exact(z):
-    y = np.clip(z, 0.0, 1.0)
-    return y * y * (6.0 - 4.0 * y) - 1.0
+    s = 2.0 * np.clip(z, 0.0, 1.0) - 1.0
+    v = s * s - 0.5
+    c0, c1, c2, c3, c4 = TANH_C
+    return s * (c0 + v * (c1 + v * (c2 + v * (c3 + v * c4))))
 
 
 def _fwd(p, x, bf16=0):
@@ -102,13 +110,19 @@ def _f16(v):
 
 
 def _np_act(z):
-    """The GPU's hidden activation T(z) = S16(clamp01(r16(z))): numpy's
+    """The GPU's hidden activation T(z) = Q16(clamp01(r16(z))): numpy's
     binary16 conversion (nearest even, overflow to infinity), every binary16
-    operation of S16 rounded once (its FP64 value is exact)."""
+    operation of Q16 rounded once (its FP64 value is exact)."""
     with np.errstate(over="ignore", invalid="ignore"):
         h = np.asarray(z, np.float32).astype(np.float16).astype(np.float64)
         y = np.where(h > 0, np.minimum(h, 1.0), 0.0)      # NaN, -0 and negatives to 0
-    return _f16(_f16(y * y) * _f16(6.0 - 4.0 * y) - 1.0)
+    c0, c1, c2, c3, c4 = TANH_C
+    s = _f16(2.0 * y - 1.0)
+    v = _f16(s * s - 0.5)
+    p = _f16(v * c4 + c3)
+    for c in (c2, c1, c0):
+        p = _f16(v * p + c)
+    return _f16(s * p)
 
 
 def _np_forward1(p, x):
@@ -134,10 +148,10 @@ def _oracle_act(z):
 def test_activation_over_every_binary16_value():
     """T at every binary16 value (infinities and NaN included), just above and
     below each and at random float32 values, against numpy's binary16
-    conversions; plus what pins T without numpy: its exact values at y = 0,
-    1/4, 1/2, 3/4 and 1, saturation outside [0, 1], and within the rounding
-    bound of its three binary16 operations of the exact cubic (hence
-    monotone and antisymmetric about 1/2 up to that bound)."""
+    conversions; plus what pins T without numpy: within 4.32e-3 of
+    tanh(A (2 y - 1)) at every binary16 y in [0, 1] and of tanh beyond the
+    clamp, exactly 0 at y = 1/2, exactly odd about 1/2, saturated outside
+    [0, 1], and monotone up to two binary16 steps."""
     v = np.arange(0x10000, dtype=np.uint32).astype(np.uint16).view(np.float16).astype(np.float32)
     fin = v[np.isfinite(v)]
     rng = np.random.default_rng(11)
@@ -145,30 +159,47 @@ def test_activation_over_every_binary16_value():
               rng.uniform(-2, 2, 100000).astype(np.float32),
               np.array([3e38, -3e38, 7e4, -7e4, 65519.0, 65520.0, 1e-30, -1e-30], np.float32)):
         assert np.array_equal(_oracle_act(z), _np_act(z).astype(np.float32), equal_nan=True)
-    exact = {0.0: -1.0, 0.25: -0.6875, 0.5: 0.0, 0.75: 0.6875, 1.0: 1.0}
-    assert np.array_equal(_oracle_act(np.array(list(exact), np.float32)), np.array(list(exact.values()), np.float32))
-    out = np.array([-1e-30, -0.5, -65504.0, -np.inf, np.nan, 1.5, 65504.0, np.inf, 3e38], np.float32)
-    assert np.array_equal(_oracle_act(out), np.array([-1] * 5 + [1] * 4, np.float32))
     # every binary16 y in [0, 1] in increasing order
     y = np.arange(0x3C01, dtype=np.uint16).view(np.float16).astype(np.float32)
     t = _oracle_act(y).astype(np.float64)
     yd = y.astype(np.float64)
-    s_exact = yd * yd * (6.0 - 4.0 * yd) - 1.0
-    # rounding bound: half a binary16 step of u = y^2 times t, of t = 6 - 4 y
-    # times u, and of the result
-    ulp = lambda v: np.maximum(np.spacing(np.abs(v).astype(np.float16)).astype(np.float64), 2.0 ** -24)
-    u, tt = yd * yd, 6.0 - 4.0 * yd
-    bound = 0.5 * (tt * ulp(u) + u * ulp(tt) + ulp(s_exact)) * (1 + 1e-9)
-    assert np.all(np.abs(t - s_exact) <= bound)
-    assert np.all(np.diff(t) >= -(bound[1:] + bound[:-1]))
-    mirror = _oracle_act((1.0 - y).astype(np.float32)).astype(np.float64)       # 1 - y is exact in float32
-    mb = 0.5 * ((2.0 + 4.0 * yd) * ulp((1 - yd) ** 2) + (1 - yd) ** 2 * ulp(2.0 + 4.0 * yd) + ulp(s_exact))
-    assert np.all(np.abs(t + mirror) <= bound + mb)
+    assert np.max(np.abs(t - np.tanh(TANH_A * (2.0 * yd - 1.0)))) <= 4.32e-3
+    top = float(t[-1])
+    assert abs(top - 1.0) <= 4.32e-3 and abs(top - np.tanh(TANH_A)) <= 4.32e-3
+    assert _oracle_act(np.array([0.5], np.float32))[0] == 0.0
+    # odd about 1/2: for y in [1/2, 1], 1 - y is a binary16 value and T(1 - y) = -T(y)
+    hi = y[y >= 0.5]
+    assert np.array_equal(_oracle_act((1.0 - hi).astype(np.float32)), -_oracle_act(hi))
+    out = np.array([-1e-30, -0.5, -65504.0, -np.inf, np.nan, 1.5, 65504.0, np.inf, 3e38], np.float32)
+    assert np.array_equal(_oracle_act(out), np.array([-top] * 5 + [top] * 4, np.float32))
+    step = np.spacing(np.abs(t).astype(np.float16)).astype(np.float64)
+    # the fit ripples about tanh where tanh is nearly flat: never down by more
+    # than two binary16 steps
+    assert np.all(np.diff(t) >= -2 * step[1:])
     # a value between binary16 neighbours takes the nearer one, ties the even
     half = np.float32(0.5 + 2.0 ** -12)        # halfway between binary16 0.5 and 0.5 + 2^-11
     above = np.nextafter(half, np.float32(1))
     t = _oracle_act(np.array([half, above, np.float32(0.5 + 2.0 ** -11)], np.float32))
     assert t[0] == 0.0 and t[1] == t[2] > 0.0
+
+
+def test_activation_derivative_matches_the_function():
+    """The backward pass's factor dT/dz (the oracle's FP64 gradients use it)
+    against central differences of the exact polynomial across (0, 1), and 0
+    outside; tanh's own slope 2 A (1 - tanh^2) within 7 % of its peak 2 A (a
+    minimax fit's derivative ripples more than the fit itself)."""
+    z = np.linspace(0.01, 0.99, 99)
+    h = 1e-6
+    fd = (_f_exact(z + h) - _f_exact(z - h)) / (2 * h)
+    c0, c1, c2, c3, c4 = TANH_C
+    s = 2 * z - 1
+    v = s * s - 0.5
+    P = c0 + v * (c1 + v * (c2 + v * (c3 + v * c4)))
+    dP = c1 + v * (2 * c2 + v * (3 * c3 + v * 4 * c4))
+    d = 2 * (P + 2 * s * s * dP)
+    assert np.max(np.abs(d - fd)) <= 1e-6
+    assert np.max(np.abs(d - 2 * TANH_A * (1 - np.tanh(TANH_A * s) ** 2))) <= 0.07 * 2 * TANH_A
+    assert _f_exact(np.array([-0.5, 1.5])).tolist() == _f_exact(np.array([0.0, 1.0])).tolist()
 
 
 def _fp(a):
