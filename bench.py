@@ -13,7 +13,9 @@ Printed as one JSON line:
   step_p50/p99_ms  host wall time of a blocking MPPIDriver-style step()
                    (uploads x0 and the nominal, downloads u_next), 200 steps
   roofline         the rollout kernel (the dominant one) against the VALU
-                   lane-op peak, from kernel-exact HIP event times; reduce_roofline
+                   lane-op peak, from kernel-exact HIP event times, and against
+                   the plain-instruction issue ceiling measured on the device at
+                   start-up (mppi_hip_calibrate); reduce_roofline
                    the reduce kernel against HBM bandwidth
   cpu_baseline     the CPU oracle (scalar C++, same algorithm and noise
                    stream) on the host's cores over a bounded sample of steps
@@ -66,6 +68,34 @@ VALU_PEAK_LANE_OPS = 157.3e12 / 2
 MFMA_PEAK_16 = 2.5e15
 HBM_PEAK_BPS = 8.0e12
 COUNTERS = ROOT / "profiles" / "mppi_counters_unicycle_pursuit_surrogate.md"
+# Issue ceilings measured on this device at start-up (mppi_hip_calibrate):
+# plain FP32 VALU lane-ops/s and FP16 16x16x32 matrix flops/s.
+CEILING = {"valu": None, "mfma": None}
+
+
+def calibrate():
+    """Measure the device's issue ceilings once (mppi_hip_calibrate)."""
+    import ctypes
+    from mppi_metal_amd import _native
+    lib = _native.load()
+    v, m = ctypes.c_double(), ctypes.c_double()
+    rc = lib.mppi_hip_calibrate(0, ctypes.byref(v), ctypes.byref(m))
+    if rc != 0:
+        raise RuntimeError(f"mppi_hip_calibrate failed ({rc})")
+    CEILING["valu"], CEILING["mfma"] = v.value, m.value
+    print(f"[bench] measured ceilings: VALU {v.value / 1e12:.2f} T lane-op/s (plain FP32), "
+          f"MFMA {m.value / 1e12:.1f} TFLOP/s (fp16 16x16x32)", file=sys.stderr, flush=True)
+
+
+def against_ceiling(rate: float, kind: str) -> dict:
+    """`rate` (lane-op/s or flop/s) against the measured issue ceiling of
+    `kind` ("valu" or "mfma")."""
+    c = CEILING[kind]
+    unit = "T lane-op/s" if kind == "valu" else "TFLOP/s"
+    how = ("plain (unpacked) FP32 FMAs, independent streams, 4 wavefronts per SIMD" if kind == "valu"
+           else "v_mfma_f32_16x16x32_f16, independent accumulators, 4 wavefronts per SIMD")
+    return {"measured": None if c is None else c / 1e12, "unit": unit,
+            "frac": None if c is None else rate / c, "how": "mppi_hip_calibrate: " + how}
 
 
 def noise_ops(C: int) -> dict:
@@ -141,7 +171,8 @@ def issued(kernel: str, sample_steps: float, seconds: float):
         return None
     rate = slots * sample_steps / seconds
     return {"slots_per_sample_step": slots, "achieved": rate / 1e12, "peak": VALU_PEAK_LANE_OPS / 1e12,
-            "unit": "T lane-op/s", "frac": rate / VALU_PEAK_LANE_OPS, "source": "profiles/" + COUNTERS.name}
+            "unit": "T lane-op/s", "frac": rate / VALU_PEAK_LANE_OPS,
+            "issue_ceiling": against_ceiling(rate, "valu"), "source": "profiles/" + COUNTERS.name}
 
 
 def reduce_bytes(K: int, H: int, C: int, n_chunks: int) -> int:
@@ -331,7 +362,8 @@ def bench_batch(args):
     k_s = r_ms / n_l / 1e3
     rate = work * w["total"] / k_s
     roofline = {"bound": "valu", "kernel": kern, "achieved": rate / 1e12, "peak": VALU_PEAK_LANE_OPS / 1e12,
-                "unit": "T lane-op/s", "frac": rate / VALU_PEAK_LANE_OPS, "ops_per_sample_step": w["total"],
+                "unit": "T lane-op/s", "frac": rate / VALU_PEAK_LANE_OPS,
+                "issue_ceiling": against_ceiling(rate, "valu"), "ops_per_sample_step": w["total"],
                 "ops_breakdown": w, "sample_steps_per_launch": work, "avg_launch_us": k_s * 1e6,
                 "second_launch_us": None if one else d_ms / n_l * 1e3,
                 "issued": issued(kern + "@pursuit", work, k_s),
@@ -507,6 +539,7 @@ def bench_surrogate(args, substeps=2):
     key = "mppi_rollout_surrogate@surrogate"
     roofline = {"bound": "mfma", "kernel": "mppi_rollout_surrogate", "achieved": flops / 1e12,
                 "peak": MFMA_PEAK_16 / 1e12, "unit": "TFLOP/s", "frac": flops / MFMA_PEAK_16,
+                "issue_ceiling": against_ceiling(K * H * SURROGATE_FLOPS_ISSUED / sur_s, "mfma"),
                 "flops_per_sample_step": SURROGATE_FLOPS, "sample_steps_per_launch": K * H,
                 "avg_launch_us": sur["rollout_us"],
                 "issued_flops_per_sample_step": SURROGATE_FLOPS_ISSUED,
@@ -592,6 +625,7 @@ def main():
 
     from mppi_metal_amd import HipMPPI, scenes
 
+    calibrate()
     K, H, C, n_obs = args.K, args.H, 2, args.obstacles
     default_workload = (K, H, n_obs) == (65536, 64, 16)
     sc = scenes.unicycle_scene(n_obs, 42)
@@ -707,6 +741,7 @@ def main():
         "roofline": {"bound": "valu", "kernel": roll_kernel,
                      "achieved": achieved_ops / 1e12, "peak": VALU_PEAK_LANE_OPS / 1e12,
                      "unit": "T lane-op/s", "frac": achieved_ops / VALU_PEAK_LANE_OPS,
+                     "issue_ceiling": against_ceiling(achieved_ops, "valu"),
                      "ops_per_sample_step": w_ss["total"], "ops_breakdown": w_ss,
                      "sample_steps_per_launch": K * H, "avg_launch_us": roll_s * 1e6,
                      "issued": issued(roll_kernel + "@unicycle", K * H, roll_s) if default_workload else None,

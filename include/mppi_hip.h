@@ -453,6 +453,15 @@ int mppi_hip_server_replay(mppi_hip_server* server, uint32_t n, const mppi_hip_r
 /* Library version string. */
 const char* mppi_hip_version(void);
 
+/* Measured issue ceilings of `device` for roofline figures: the rate at
+ * which its SIMDs retire plain (unpacked) FP32 vector instructions, in lane
+ * operations per second (an FMA counts once), and 16x16x32 FP16 matrix
+ * instructions, in flops per second, each from long streams of independent
+ * instructions of that kind at four wavefronts per SIMD (about 10 ms).  No
+ * reference counterpart (measurement only).  The published FP32 vector peak
+ * counts packed FP32 instructions, which this library does not emit. */
+int mppi_hip_calibrate(int device, double* valu_lane_ops_per_s, double* mfma_f16_flops_per_s);
+
 /* Compile a plugin with the library kernels for gfx950 without a device
  * (build hosts, CI): the same sources and options as linking.  No reference
  * counterpart (the reference compiles its shaders ahead of time with xcrun,

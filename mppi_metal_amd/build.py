@@ -97,7 +97,7 @@ def check_kernels(lib_path: Path):
 def build_library() -> Path:
     out = PKG / "libmppi_hip.so"
     srcs = [CSRC / "hip_backend.cpp", CSRC / "batch_backend.cpp", CSRC / "server_backend.cpp",
-            CSRC / "surrogate.hip", CSRC / "noalloc.hip", CSRC / "mppi_driver.cpp"]
+            CSRC / "surrogate.hip", CSRC / "noalloc.hip", CSRC / "calibrate.hip", CSRC / "mppi_driver.cpp"]
     _run([HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-fPIC", "-shared",
           *NO_PACKED_FP32, "-Wall", f"-I{INC}", f"-I{INC / 'mppi_hip'}", f"-I{CSRC}",
           *srcs, "-lhiprtc", "-o", out])
