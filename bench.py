@@ -70,7 +70,7 @@ HBM_PEAK_BPS = 8.0e12
 COUNTERS = ROOT / "profiles" / "mppi_counters_unicycle_pursuit_surrogate.md"
 # Issue ceilings measured on this device at start-up (mppi_hip_calibrate):
 # plain FP32 VALU lane-ops/s and FP16 16x16x32 matrix flops/s.
-CEILING = {"valu": None, "mfma": None}
+CEILING = {"valu": None, "mfma": None, "mixed_over_sum": None}
 
 
 def calibrate():
@@ -78,13 +78,14 @@ def calibrate():
     import ctypes
     from mppi_metal_amd import _native
     lib = _native.load()
-    v, m = ctypes.c_double(), ctypes.c_double()
-    rc = lib.mppi_hip_calibrate(0, ctypes.byref(v), ctypes.byref(m))
+    v, m, x = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    rc = lib.mppi_hip_calibrate(0, ctypes.byref(v), ctypes.byref(m), ctypes.byref(x))
     if rc != 0:
         raise RuntimeError(f"mppi_hip_calibrate failed ({rc})")
-    CEILING["valu"], CEILING["mfma"] = v.value, m.value
+    CEILING["valu"], CEILING["mfma"], CEILING["mixed_over_sum"] = v.value, m.value, x.value
     print(f"[bench] measured ceilings: VALU {v.value / 1e12:.2f} T lane-op/s (plain FP32), "
-          f"MFMA {m.value / 1e12:.1f} TFLOP/s (fp16 16x16x32)", file=sys.stderr, flush=True)
+          f"MFMA {m.value / 1e12:.1f} TFLOP/s (fp16 16x16x32), VALU+MFMA in one stream "
+          f"{x.value:.2f} of their separate times", file=sys.stderr, flush=True)
 
 
 def against_ceiling(rate: float, kind: str) -> dict:
@@ -92,8 +93,8 @@ def against_ceiling(rate: float, kind: str) -> dict:
     `kind` ("valu" or "mfma")."""
     c = CEILING[kind]
     unit = "T lane-op/s" if kind == "valu" else "TFLOP/s"
-    how = ("plain (unpacked) FP32 FMAs, independent streams, 4 wavefronts per SIMD" if kind == "valu"
-           else "v_mfma_f32_16x16x32_f16, independent accumulators, 4 wavefronts per SIMD")
+    how = ("plain (unpacked) FP32 FMAs, independent streams, 8 wavefronts per SIMD" if kind == "valu"
+           else "v_mfma_f32_16x16x32_f16, independent accumulators, 8 wavefronts per SIMD")
     return {"measured": None if c is None else c / 1e12, "unit": unit,
             "frac": None if c is None else rate / c, "how": "mppi_hip_calibrate: " + how}
 
@@ -128,12 +129,14 @@ def bicycle_w_ss(substeps: int = 2, zones: int = 32, C: int = 2, one_launch: boo
     pursuit rollout: noise; per Euler substep 40 (sincos x4 each, the speed
     floor 1, 1/u x4, the two tyre forces 7, the body-frame rates 12, six
     state FMAs, the clamp 2); the tracking and control cost 9 (the target's
-    position is uniform); the zone test 5 per zone (two subtractions, max,
-    subtract the radius, running min); accumulation 1.  The one-launch step
-    also regenerates the noise for its weighted sum: the noise again and 2 per
-    control."""
+    position is uniform); the zone test through the workgroup's zone grid 14
+    (two band indices of 5 each, the set intersection with the zones every
+    wavefront tests 2, the wavefront's any-candidate test 2; the exact pass,
+    which runs only in wavefronts where some sample has a candidate zone, is
+    not counted); accumulation 1.  The one-launch step also regenerates the
+    noise for its weighted sum: the noise again and 2 per control."""
     w = noise_ops(C)
-    w.update(dynamics=40 * substeps, cost=9, zone_test=5 * zones, accumulate=1)
+    w.update(dynamics=40 * substeps, cost=9, zone_test=14, accumulate=1)
     if one_launch:
         w["noise_sums"] = sum(noise_ops(C).values()) + 2 * C
     w["total"] = sum(w.values())
@@ -160,19 +163,18 @@ def profile_line(key: str) -> dict:
 
 
 def issued(kernel: str, sample_steps: float, seconds: float):
-    """Counter-derived VALU issue of `kernel` (named kernel@workload): its issue slots per
-    sample-step from the committed rocprofv3 summary (SQ_INSTS_VALU, with
-    SQ_INSTS_VALU_TRANS_F32 and SQ_INSTS_VALU_INT64 weighted x4, times 64
-    lanes; an MFMA instruction, if the counter includes it, counts once),
-    times the sample-steps of a launch over the launch's duration measured
-    here, against the FP32 vector peak; or None without a summary."""
-    slots = profile_line("issue_slots_per_sample_step").get(kernel)
-    if slots is None:
+    """Counter-derived VALU issue of `kernel` (named kernel@workload): the
+    VALU instructions it executes per sample-step (SQ_INSTS_VALU x 64 lanes
+    over the sample-steps of a launch, from the committed rocprofv3 summary;
+    MFMA instructions are counted separately there), times the sample-steps of
+    a launch over the launch's duration measured here, against the FP32
+    vector peak; or None without a summary."""
+    per = profile_line("valu_instructions_per_sample_step").get(kernel)
+    if per is None:
         return None
-    rate = slots * sample_steps / seconds
-    return {"slots_per_sample_step": slots, "achieved": rate / 1e12, "peak": VALU_PEAK_LANE_OPS / 1e12,
-            "unit": "T lane-op/s", "frac": rate / VALU_PEAK_LANE_OPS,
-            "issue_ceiling": against_ceiling(rate, "valu"), "source": "profiles/" + COUNTERS.name}
+    rate = per * sample_steps / seconds
+    return {"valu_per_sample_step": per, "achieved": rate / 1e12, "peak": VALU_PEAK_LANE_OPS / 1e12,
+            "unit": "T lane-op/s", "frac": rate / VALU_PEAK_LANE_OPS, "source": "profiles/" + COUNTERS.name}
 
 
 def reduce_bytes(K: int, H: int, C: int, n_chunks: int) -> int:

@@ -457,10 +457,14 @@ const char* mppi_hip_version(void);
  * which its SIMDs retire plain (unpacked) FP32 vector instructions, in lane
  * operations per second (an FMA counts once), and 16x16x32 FP16 matrix
  * instructions, in flops per second, each from long streams of independent
- * instructions of that kind at four wavefronts per SIMD (about 10 ms).  No
- * reference counterpart (measurement only).  The published FP32 vector peak
- * counts packed FP32 instructions, which this library does not emit. */
-int mppi_hip_calibrate(int device, double* valu_lane_ops_per_s, double* mfma_f16_flops_per_s);
+ * instructions of that kind at eight wavefronts per SIMD; and, in
+ * mixed_over_sum (may be NULL), the time of one stream holding both kinds
+ * over the sum of the times of its two parts alone (1: the SIMD does not
+ * overlap them; 0.5: fully).  About 40 ms.  No reference counterpart
+ * (measurement only).  The published FP32 vector peak counts packed FP32
+ * instructions, which this library does not emit. */
+int mppi_hip_calibrate(int device, double* valu_lane_ops_per_s, double* mfma_f16_flops_per_s,
+                       double* mixed_over_sum);
 
 /* Compile a plugin with the library kernels for gfx950 without a device
  * (build hosts, CI): the same sources and options as linking.  No reference

@@ -161,7 +161,8 @@ SIGNATURES = [
       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(MppiHipReplayStats)]),
     ("mppi_hip_version", ctypes.c_char_p, []),
     ("mppi_hip_calibrate", ctypes.c_int,
-     [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+     [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+      ctypes.POINTER(ctypes.c_double)]),
     ("mppi_hip_compile_check", ctypes.c_int,
      [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
       ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_char_p,

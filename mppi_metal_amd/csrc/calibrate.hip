@@ -1,12 +1,12 @@
 // calibrate.hip — measured issue ceilings of one MI355X for the roofline
 // figures: the rate at which the SIMDs retire plain (unpacked) FP32 VALU
-// instructions and 16x16x32 FP16 matrix instructions.  Each probe kernel
-// runs long unrolled streams of independent instructions of one kind from
-// four wavefronts per SIMD, so the figure is the hardware's issue rate for
-// that kind, not a latency.  The published FP32 vector peak counts packed
-// FP32 instructions (two FMAs per lane per instruction), which this library
-// never emits; the plain-instruction ceiling is the one its VALU-bound
-// kernels can reach.
+// instructions and 16x16x32 FP16 matrix instructions, and whether a SIMD
+// overlaps the two.  Each probe kernel runs long unrolled streams of
+// independent instructions from eight wavefronts per SIMD, so the figure is
+// the hardware's issue rate for that kind, not a latency.  The published FP32
+// vector peak counts packed FP32 instructions (two FMAs per lane per
+// instruction), which this library never emits; the plain-instruction
+// ceiling is the one its VALU-bound kernels are measured against.
 #include "backend.h"
 
 #include <hip/hip_runtime.h>
@@ -15,30 +15,15 @@ namespace {
 
 constexpr int kProbeThreads = 256;
 constexpr int kValuChains = 16;      // independent FMA chains per lane
-constexpr int kValuUnroll = 8;       // FMAs per chain per iteration
 constexpr int kMfmaChains = 8;       // independent accumulators per wavefront
-constexpr int kMfmaUnroll = 2;       // matrix instructions per accumulator per iteration
-
-__global__ void __launch_bounds__(kProbeThreads) mppi_probe_valu(float* out, int iters, float a, float b) {
-    float x[kValuChains];
-#pragma unroll
-    for (int i = 0; i < kValuChains; ++i) x[i] = float(threadIdx.x) * 1e-3f + float(i);
-    for (int it = 0; it < iters; ++it) {
-#pragma unroll
-        for (int u = 0; u < kValuUnroll; ++u)
-#pragma unroll
-            for (int i = 0; i < kValuChains; ++i) x[i] = __builtin_fmaf(x[i], a, b);
-    }
-    float s = 0.0f;
-#pragma unroll
-    for (int i = 0; i < kValuChains; ++i) s += x[i];
-    out[size_t(blockIdx.x) * kProbeThreads + threadIdx.x] = s;
-}
 
 typedef _Float16 probe_v8h __attribute__((ext_vector_type(8)));
 typedef float probe_v4f __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(kProbeThreads) mppi_probe_mfma(float* out, int iters, float a) {
+// Per iteration: NM matrix instructions (v_mfma_f32_16x16x32_f16) over
+// kMfmaChains accumulators and NV FP32 FMAs over kValuChains chains.
+template <int NM, int NV>
+__global__ void __launch_bounds__(kProbeThreads) mppi_probe(float* out, int iters, float a, float b) {
     probe_v4f acc[kMfmaChains];
 #pragma unroll
     for (int i = 0; i < kMfmaChains; ++i) acc[i] = probe_v4f{0.0f, 0.0f, 0.0f, 0.0f};
@@ -48,29 +33,36 @@ __global__ void __launch_bounds__(kProbeThreads) mppi_probe_mfma(float* out, int
         A[j] = _Float16(a * float(threadIdx.x & 15) + float(j));
         B[j] = _Float16(a - 0.01f * float(j));
     }
+    float x[kValuChains];
+#pragma unroll
+    for (int i = 0; i < kValuChains; ++i) x[i] = float(threadIdx.x) * 1e-3f + float(i);
     for (int it = 0; it < iters; ++it) {
 #pragma unroll
-        for (int u = 0; u < kMfmaUnroll; ++u)
+        for (int m = 0; m < NM; ++m)
+            acc[m % kMfmaChains] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A, B, acc[m % kMfmaChains], 0, 0, 0);
 #pragma unroll
-            for (int i = 0; i < kMfmaChains; ++i) acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A, B, acc[i], 0, 0, 0);
+        for (int v = 0; v < NV; ++v) x[v % kValuChains] = __builtin_fmaf(x[v % kValuChains], a, b);
     }
     float s = 0.0f;
 #pragma unroll
     for (int i = 0; i < kMfmaChains; ++i) s += acc[i][0] + acc[i][1] + acc[i][2] + acc[i][3];
+#pragma unroll
+    for (int i = 0; i < kValuChains; ++i) s += x[i];
     out[size_t(blockIdx.x) * kProbeThreads + threadIdx.x] = s;
 }
 
 }  // namespace
 
-extern "C" int mppi_hip_calibrate(int device, double* valu_lane_ops_per_s, double* mfma_f16_flops_per_s) {
+extern "C" int mppi_hip_calibrate(int device, double* valu_lane_ops_per_s, double* mfma_f16_flops_per_s,
+                                  double* mixed_over_sum) {
     if (!valu_lane_ops_per_s || !mfma_f16_flops_per_s) return MPPI_HIP_E_INVALID;
     int n_dev = 0;
     if (hipGetDeviceCount(&n_dev) != hipSuccess || device < 0 || device >= n_dev) return MPPI_HIP_E_DEVICE;
     if (hipSetDevice(device) != hipSuccess) return MPPI_HIP_E_DEVICE;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return MPPI_HIP_E_DEVICE;
-    // four wavefronts of 64 lanes per SIMD on every compute unit, twice over
-    const int blocks = prop.multiProcessorCount * 4 * 2;
+    // one wavefront per SIMD per workgroup, eight workgroups per compute unit
+    const int blocks = prop.multiProcessorCount * 8;
     float* out = nullptr;
     if (hipMalloc(&out, sizeof(float) * size_t(blocks) * kProbeThreads) != hipSuccess) return MPPI_HIP_E_NO_MEMORY;
     hipStream_t st = nullptr;
@@ -78,30 +70,29 @@ extern "C" int mppi_hip_calibrate(int device, double* valu_lane_ops_per_s, doubl
     int rc = MPPI_HIP_OK;
     if (hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         rc = MPPI_HIP_E_DEVICE;
-    auto timed = [&](auto launch) -> double {
-        launch();                                      // warm-up
+    auto timed = [&](auto kernel, int iters) -> double {
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kProbeThreads), 0, st, out, iters, 0.9999f, 1e-4f);   // warm-up
         if (hipEventRecord(e0, st) != hipSuccess) return -1.0;
-        launch();
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kProbeThreads), 0, st, out, iters, 0.9999f, 1e-4f);
         if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return -1.0;
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
         return double(ms) * 1e-3;
     };
     if (rc == MPPI_HIP_OK) {
-        const int vi = 4000, mi = 2000;
-        const double tv = timed([&] {
-            hipLaunchKernelGGL(mppi_probe_valu, dim3(blocks), dim3(kProbeThreads), 0, st, out, vi, 0.9999f, 1e-4f);
-        });
-        const double tm = timed([&] {
-            hipLaunchKernelGGL(mppi_probe_mfma, dim3(blocks), dim3(kProbeThreads), 0, st, out, mi, 0.5f);
-        });
-        if (tv <= 0.0 || tm <= 0.0 || hipGetLastError() != hipSuccess) {
+        const int it = 4000;
+        const double tv = timed(mppi_probe<0, 128>, it);     // VALU only
+        const double tm = timed(mppi_probe<16, 0>, it);      // matrix only
+        const double tv2 = timed(mppi_probe<0, 64>, it);     // the mixed probe's VALU part alone
+        const double tmv = timed(mppi_probe<16, 64>, it);    // both in one stream
+        if (tv <= 0.0 || tm <= 0.0 || tv2 <= 0.0 || tmv <= 0.0 || hipGetLastError() != hipSuccess) {
             rc = MPPI_HIP_E_DEVICE;
         } else {
             const double lanes = double(blocks) * kProbeThreads;
-            *valu_lane_ops_per_s = lanes * vi * kValuUnroll * kValuChains / tv;
+            *valu_lane_ops_per_s = lanes * double(it) * 128.0 / tv;
             const double waves = double(blocks) * (kProbeThreads / 64);
-            *mfma_f16_flops_per_s = waves * mi * kMfmaUnroll * kMfmaChains * (2.0 * 16 * 16 * 32) / tm;
+            *mfma_f16_flops_per_s = waves * double(it) * 16.0 * (2.0 * 16 * 16 * 32) / tm;
+            if (mixed_over_sum) *mixed_over_sum = tmv / (tm + tv2);
         }
     }
     if (e0) (void)hipEventDestroy(e0);
