@@ -932,6 +932,27 @@ int oracle_reduce64(const oracle_problem* p, uint32_t step, uint32_t agent, cons
     return 0;
 }
 
+// The magnitude of the update's sum, per control: sum_k w_k |eps_k| in FP64
+// (the conditioning of u_nom + sum_k w_k eps_k, against which the tests
+// state the update's elementwise error).
+int oracle_reduce_abs(const oracle_problem* p, uint32_t step, uint32_t agent, const double* w,
+                      double* out) {
+    Resolved<double> r = resolve<double>(p);
+    if (!r.ok) return -1;
+    const uint32_t C = p->C, H = p->H, HC = H * C;
+    std::vector<double> z(HC), eps(C);
+    std::fill(out, out + HC, 0.0);
+    for (uint32_t k = 0; k < p->K; ++k) {
+        if (w[k] == 0.0) continue;
+        sample_normals<double>(p, k, step, agent, z.data());
+        for (uint32_t t = 0; t < H; ++t) {
+            noise_eps<double>(p, r, z.data(), t, eps.data());
+            for (uint32_t i = 0; i < C; ++i) out[t * C + i] += w[k] * std::fabs(eps[i]);
+        }
+    }
+    return 0;
+}
+
 int oracle_step(const oracle_problem* p, uint32_t step, uint32_t agent, const float* x0,
                 const float* u_nom, float* u_out, float* diag3, float* costs_out,
                 float* weights_out, int threads) {

@@ -57,6 +57,7 @@ def lib():
             "oracle_weights64": (ctypes.c_int, [P, _D, _D, _D]),
             "oracle_reduce": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, _F, _F, _F]),
             "oracle_reduce64": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, _D, _D, _D]),
+            "oracle_reduce_abs": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, _D, _D]),
             "oracle_step": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, _F, _F, _F, _F, _F, _F,
                                            ctypes.c_int]),
             "oracle_step64": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, _F, _D, _D, _D, _D, _D,
@@ -179,6 +180,14 @@ class Problem:
             un = np.ascontiguousarray(u_nom, np.float32).ravel()
             out = np.empty(self.H * self.C, np.float32)
             assert lib().oracle_reduce(self.ref, step, agent, _p(w), _p(un), _p(out)) == 0
+        return out.reshape(self.H, self.C)
+
+    def reduce_abs(self, step, w, agent=0):
+        """sum_k w_k |eps_k| per control [H, C] in FP64: the magnitude of the
+        update's sum."""
+        w = np.ascontiguousarray(w, np.float64)
+        out = np.empty(self.H * self.C, np.float64)
+        assert lib().oracle_reduce_abs(self.ref, step, agent, _p(w, _D), _p(out, _D)) == 0
         return out.reshape(self.H, self.C)
 
     def step(self, step, x0, u_nom, agent=0, threads=1, fp64=False):

@@ -127,11 +127,24 @@ def seq_rel_err(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
 
 
+def update_rel_err(u, u_ref, u_nom, mag):
+    """Elementwise error of an update u = clamp(u_nom + sum_k w_k eps_k)
+    relative to the magnitude of what was summed, |u_nom| + sum_k w_k
+    |eps_k| (`mag`, Problem.reduce_abs): the conditioning of the sum, so
+    the bound holds for components that cancel to near zero as for any
+    other."""
+    u = np.asarray(u, np.float64)
+    r = np.asarray(u_ref, np.float64)
+    scale = np.abs(np.asarray(u_nom, np.float64)).reshape(r.shape) + np.asarray(mag, np.float64).reshape(r.shape)
+    return float(np.max(np.abs(u - r) / np.maximum(scale, 1e-30)))
+
+
 # ------------------------------------------------------ GPU-vs-oracle step --
 # FP32 bars: sample costs and the best cost bit-exact; weights 1e-5 relative
 # elementwise; u_next 1e-5 of the sequence's largest magnitude (tree versus
-# sequential sums); mean cost and ESS 1e-5 relative.  Errors against the FP64
-# oracle are printed and the costs' bounded.
+# sequential sums) and, elementwise, 1e-5 relative to the magnitude of its
+# sum (update_rel_err); mean cost and ESS 1e-5 relative.  Errors against the
+# FP64 oracle are printed and the costs' bounded.
 def report(name, **errs):
     print(f"[{name}] " + " ".join(f"{k}={v:.3e}" for k, v in errs.items()), flush=True)
 
@@ -147,8 +160,10 @@ def check_step(hip, ora, step, x0, u_nom, name, check_fp64=True):
     assert abs(w_g.sum() - 1.0) <= 1e-5
     assert seq_rel_err(u_g, u_o) <= 1e-5
     assert rel_err(d_g[1:], d_o[1:]) <= 1e-5
-    errs = dict(u_vs_fp32=seq_rel_err(u_g, u_o), w_vs_fp32=rel_err(w_g, w_o, 1e-37),
-                ess_vs_fp32=rel_err(d_g[2:], d_o[2:]))
+    u_elem = update_rel_err(u_g, u_o, u_nom, ora.reduce_abs(step, w_o.astype(np.float64)))
+    assert u_elem <= 1e-5
+    errs = dict(u_vs_fp32=seq_rel_err(u_g, u_o), u_elem_vs_fp32=u_elem,
+                w_vs_fp32=rel_err(w_g, w_o, 1e-37), ess_vs_fp32=rel_err(d_g[2:], d_o[2:]))
     if check_fp64:
         u_d, d_d, c_d, w_d = ora.step(step, x0, np.asarray(u_nom, np.float64), threads=8, fp64=True)
         errs.update(cost_vs_fp64=rel_err(c_g, c_d), u_vs_fp64=seq_rel_err(u_g, u_d),

@@ -6,7 +6,9 @@ Tolerances (FP32):
   best cost (min)   bit-exact
   weights           1e-5 relative, elementwise
   u_next            1e-5 of the sequence's largest magnitude (the GPU sums
-                    w_k z_k in a tree, the oracle sequentially in k)
+                    w_k z_k in a tree, the oracle sequentially in k), and
+                    elementwise 1e-5 relative to the magnitude of each
+                    component's sum, |u_nom| + sum_k w_k |eps_k|
   mean cost, ESS    1e-5 relative
 Every result's error against the FP64 oracle is printed (pytest -s) and
 bounded.  The single-step and closed-loop cases run on both step paths (two
@@ -21,7 +23,8 @@ import pytest
 
 from mppi_metal_amd import HipMPPI, scenes
 from problems import check_step as _check_step
-from problems import double_integrator, point_mass3, rel_err, report as _report, seq_rel_err, unicycle
+from problems import (double_integrator, point_mass3, rel_err, report as _report, seq_rel_err, unicycle,
+                      update_rel_err)
 
 pytestmark = pytest.mark.gpu
 
@@ -211,8 +214,11 @@ def test_unicycle_k65536_h64_obs16_full_size(require_gpu, path):
     u_o, d_o, c_o, w_o = ora.step(0, x0, u_nom, threads=8)
     assert np.array_equal(c_g, c_o)
     assert seq_rel_err(u_g, u_o) <= 1e-5
+    u_elem = update_rel_err(u_g, u_o, u_nom, ora.reduce_abs(0, w_o.astype(np.float64)))
+    assert u_elem <= 1e-5
     assert rel_err(w_g, w_o, 1e-37) <= 1e-5
-    _report("unicycle C2 full size", u_vs_fp32=seq_rel_err(u_g, u_o), w_vs_fp32=rel_err(w_g, w_o, 1e-37))
+    _report("unicycle C2 full size", u_vs_fp32=seq_rel_err(u_g, u_o), u_elem_vs_fp32=u_elem,
+            w_vs_fp32=rel_err(w_g, w_o, 1e-37))
 
 
 def test_k_2_18_uses_fp64_combine(require_gpu):
@@ -225,8 +231,11 @@ def test_k_2_18_uses_fp64_combine(require_gpu):
     u_d, d_d, c_d, w_d = ora.step(2, x0, np.zeros((8, 2)), threads=8, fp64=True)
     u_o, d_o, c_o, w_o = ora.step(2, x0, u_nom, threads=8)
     e64, e32 = seq_rel_err(u_g, u_d), seq_rel_err(u_o, u_d)
-    _report("K=2^18", gpu_u_vs_fp64=e64, oracle32_u_vs_fp64=e32, ess_vs_fp64=rel_err(d_g[2:], d_d[2:]))
+    u_elem = update_rel_err(u_g, u_d, u_nom, ora.reduce_abs(2, w_d))
+    _report("K=2^18", gpu_u_vs_fp64=e64, u_elem_vs_fp64=u_elem, oracle32_u_vs_fp64=e32,
+            ess_vs_fp64=rel_err(d_g[2:], d_d[2:]))
     assert e64 <= 1e-5
+    assert u_elem <= 1e-5
     assert rel_err(d_g[2:], d_d[2:]) <= 1e-5
     assert abs(float(w_g.astype(np.float64).sum()) - 1.0) <= 1e-5
 

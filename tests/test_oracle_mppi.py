@@ -111,6 +111,24 @@ def test_reduce_one_hot_weights_and_clamp():
         assert np.max(np.abs(u - want)) < 1e-12
 
 
+def test_reduce_abs_is_the_weighted_noise_magnitude():
+    """oracle_reduce_abs (the scale of the tests' elementwise update bound):
+    sum_k w_k |eps_k| is |eps_j| for one-hot weights, the mean of |eps| for
+    uniform weights, and bounds |u - u_nom| (triangle inequality)."""
+    K, H = 48, 5
+    p = di(K=K, H=H, lo=np.array([-50, -50], np.float32), hi=np.array([50, 50], np.float32))
+    sig = SIG.astype(np.float64)
+    eps = np.array([noise64(p, k, 6) * sig for k in range(K)])
+    for j in (0, 23, 47):
+        w = np.zeros(K)
+        w[j] = 1.0
+        assert np.max(np.abs(p.reduce_abs(6, w) - np.abs(eps[j]))) < 1e-12
+    assert np.max(np.abs(p.reduce_abs(6, np.full(K, 1.0 / K)) - np.abs(eps).mean(axis=0))) < 1e-12
+    w = np.random.default_rng(4).dirichlet(np.ones(K))
+    u_nom = np.zeros((H, 2))
+    assert np.all(np.abs(p.reduce(6, w, u_nom, fp64=True) - u_nom) <= p.reduce_abs(6, w) + 1e-12)
+
+
 # ---------------------------------------------------------------- rollout --
 def closed_form_di_costs(p, x0, u_nom, step, with_terminal=True):
     """Costs of the double integrator in matrix form: x_{t+1} = A x_t + B u_t."""
