@@ -121,10 +121,34 @@ MPPI_HD float mppi_sqrt_boundedf(float x) {
 #endif
 }
 
-// Clamp with explicit comparisons (NaN passes through unchanged on both
-// targets, unlike min/max instructions whose NaN rules differ).
+// Maximum and minimum with the rules of the GPU's v_max_f32 / v_min_f32 in
+// IEEE mode, stated for the host: a NaN operand yields the other operand,
+// and -0 orders below +0.  On the device each is one instruction.
+MPPI_HD float mppi_maxf(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_fmaxf(a, b);
+#else
+    if (a != a) return b;
+    if (b != b) return a;
+    if (a == b) return mppi_f2u(a) >> 31 ? b : a;
+    return a > b ? a : b;
+#endif
+}
+MPPI_HD float mppi_minf(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_fminf(a, b);
+#else
+    if (a != a) return b;
+    if (b != b) return a;
+    if (a == b) return mppi_f2u(a) >> 31 ? a : b;
+    return a < b ? a : b;
+#endif
+}
+
+// clamp(x, lo, hi) = min(max(x, lo), hi), the reference's Metal clamp (a NaN
+// x gives lo): two instructions on the device.
 MPPI_HD float mppi_clampf(float x, float lo, float hi) {
-    return x < lo ? lo : (x > hi ? hi : x);
+    return mppi_minf(mppi_maxf(x, lo), hi);
 }
 
 // Round to nearest integer (ties to even) for |x| < 2^22.

@@ -99,7 +99,23 @@ void normals4(uint32_t block, uint32_t sample, uint32_t step, uint32_t tag, uint
     }
 }
 
-template <typename T> T clampv(T x, T lo, T hi) { return x < lo ? lo : (x > hi ? hi : x); }
+// clamp(x, lo, hi) = min(max(x, lo), hi) as Metal's clamp
+// (shaders/mppi_rollout.metal:79-86, shaders/mppi_reduce.metal:167): a NaN
+// operand of max or min yields the other operand, -0 orders below +0 (the
+// GPU's v_max_f32 / v_min_f32; mppi_maxf / mppi_minf).
+template <typename T> T max_v(T a, T b) {
+    if (a != a) return b;
+    if (b != b) return a;
+    if (a == b) return std::signbit(a) ? b : a;
+    return a > b ? a : b;
+}
+template <typename T> T min_v(T a, T b) {
+    if (a != a) return b;
+    if (b != b) return a;
+    if (a == b) return std::signbit(a) ? a : b;
+    return a < b ? a : b;
+}
+template <typename T> T clampv(T x, T lo, T hi) { return min_v(max_v(x, lo), hi); }
 
 // x rounded to the nearest binary16 value, ties to even (IEEE 754 binary16:
 // 11 significant bits, quantum 2^-24 below 2^-14, overflow at 65520), by
@@ -790,6 +806,12 @@ void oracle_mathf(int fn, const float* x, float* y, size_t n) {
 }
 void oracle_fmodf(const float* x, const float* y, float* out, size_t n) {
     for (size_t i = 0; i < n; ++i) out[i] = mppi_fmodf(x[i], y[i]);
+}
+// The library's two-argument helpers: 0 mppi_maxf, 1 mppi_minf, 2
+// mppi_clampf(x, -1/2, y).
+void oracle_math2f(int fn, const float* x, const float* y, float* out, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        out[i] = fn == 0 ? mppi_maxf(x[i], y[i]) : fn == 1 ? mppi_minf(x[i], y[i]) : mppi_clampf(x[i], -0.5f, y[i]);
 }
 
 // The library header's Philox4x32-10 (include/mppi_hip/mppi_philox.h)

@@ -49,6 +49,7 @@ def lib():
             "oracle_normals4_64": (None, [ctypes.c_uint32] * 5 + [ctypes.c_uint64, _D]),
             "oracle_mathf": (None, [ctypes.c_int, _F, _F, ctypes.c_size_t]),
             "oracle_fmodf": (None, [_F, _F, _F, ctypes.c_size_t]),
+            "oracle_math2f": (None, [ctypes.c_int, _F, _F, _F, ctypes.c_size_t]),
             "oracle_rollout": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, _F, _F, _F, ctypes.c_int]),
             "oracle_rollout_range": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, _F, _F,
                                                     ctypes.c_uint32, ctypes.c_uint32, _F]),
@@ -274,6 +275,15 @@ def mathf(fn, x):
     y = np.empty_like(x)
     lib().oracle_mathf(fn, _p(x), _p(y), x.size)
     return y
+
+
+def math2f(fn, x, y):
+    """mppi_maxf (0), mppi_minf (1), mppi_clampf(x, -1/2, y) (2), host build."""
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    out = np.empty_like(x)
+    lib().oracle_math2f(fn, _p(x), _p(y), _p(out), x.size)
+    return out
 
 
 def fmodf(x, y):

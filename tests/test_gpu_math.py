@@ -5,6 +5,9 @@ compiled for the host (the oracle), over random arguments and special values
 long-division fmod quotients).  The range-restricted forms used by Box-Muller
 (log of a positive normal number, square root of [2^-96, 2^126] or zero)
 are checked over their domains against mppi_logf and the IEEE square root.
+The maximum, minimum and clamp (one instruction each on the device, stated
+for the host in the header) are checked over random pairs and every pair of
+special values, a zero result turned into an infinity of its sign.
 
 A test plugin evaluates function `fn` on input `sample` of its model blob in
 the dynamics callable, and the terminal cost returns it, so with H = 1 and a
@@ -39,9 +42,13 @@ MPPI_CALLABLE void eval_fn(float* s, const float* u, const uint8_t* m, mppi_rng 
         case 7: v = mppi_sqrtf(x); break;
         case 8: v = mppi_fmodf(x, y); break;
         case 9: v = mppi_log_normalf(x); break;
-        default: v = mppi_sqrt_boundedf(x); break;
+        case 10: v = mppi_sqrt_boundedf(x); break;
+        case 11: v = mppi_maxf(x, y); break;
+        case 12: v = mppi_minf(x, y); break;
+        default: v = mppi_clampf(x, -0.5f, y); break;
     }
-    s[0] = v;
+    // a zero result as an infinity of its sign, which the cost's addition keeps
+    s[0] = fn >= 11 && v == 0.0f ? 1.0f / v : v;
 }
 MPPI_CALLABLE float zero_stage(const float* s, const float* u, const uint8_t* m, const uint8_t* c) {
     return 0.0f;
@@ -76,6 +83,10 @@ def _inputs(fn, rng):
         x = np.concatenate([2.0 ** rng.uniform(-96, 126, 4000), rng.uniform(0.0, 34.0, 20000),
                             [0.0, -0.0, 2.0 ** -96, 2.0 ** 126, 1.0, 4.0, 33.27]])
         return x.astype(F), np.zeros(x.size, F)
+    elif fn >= 11:                      # max, min, clamp: random pairs and every pair of specials
+        gx, gy = (a.ravel() for a in np.meshgrid(SPECIAL, SPECIAL))
+        return (np.concatenate([rng.uniform(-3, 3, 2000).astype(F), gx]),
+                np.concatenate([rng.uniform(-3, 3, 2000).astype(F), gy]))
     else:                               # fmod: random pairs and every pair of specials
         gx, gy = (a.ravel() for a in np.meshgrid(SPECIAL, SPECIAL))
         yr = rng.uniform(0.05, 60, 2000) * rng.choice([-1, 1], 2000)
@@ -92,11 +103,15 @@ def _host(fn, x, y):
         return ol.mathf(1, x)          # mppi_logf: the same core for normal arguments
     if fn == 10:
         return np.sqrt(x)              # IEEE, correctly rounded
+    if fn >= 11:
+        v = ol.math2f(fn - 11, x, y)
+        with np.errstate(divide="ignore"):
+            return np.where(v == 0, F(1) / v, v).astype(F)
     return ol.mathf(fn, x)
 
 
-@pytest.mark.parametrize("fn", range(11), ids=["exp", "log", "sin", "cos", "sinpi", "cospi", "tanh", "sqrt",
-                                               "fmod", "log_normal", "sqrt_bounded"])
+@pytest.mark.parametrize("fn", range(14), ids=["exp", "log", "sin", "cos", "sinpi", "cospi", "tanh", "sqrt",
+                                               "fmod", "log_normal", "sqrt_bounded", "max", "min", "clamp"])
 def test_math_bit_for_bit(require_gpu, fn):
     x, y = _inputs(fn, np.random.default_rng(100 + fn))
     K = x.size

@@ -106,3 +106,27 @@ def test_fmod_special_values_bit_for_bit():
     nan = np.isnan(want)
     assert np.array_equal(np.isnan(got), nan)
     assert np.array_equal(got[~nan].view(np.uint32), want[~nan].view(np.uint32))
+
+
+def test_max_min_clamp_rules():
+    """mppi_maxf / mppi_minf / mppi_clampf on the host: numpy's fmax / fmin
+    (a NaN operand yields the other) away from signed zeros, -0 below +0,
+    and clamp = min(max(x, lo), hi) with a NaN x giving lo."""
+    rng = np.random.default_rng(21)
+    x = rng.uniform(-3, 3, 5000).astype(np.float32)
+    y = rng.uniform(-3, 3, 5000).astype(np.float32)
+    x[::7] = np.nan
+    y[::11] = np.nan
+    assert np.array_equal(ol.math2f(0, x, y), np.fmax(x, y), equal_nan=True)
+    assert np.array_equal(ol.math2f(1, x, y), np.fmin(x, y), equal_nan=True)
+    z = np.array([0.0, -0.0], np.float32)
+    for a in z:
+        for b in z:
+            mx = ol.math2f(0, np.array([a]), np.array([b]))[0]
+            mn = ol.math2f(1, np.array([a]), np.array([b]))[0]
+            assert np.signbit(mx) == (np.signbit(a) and np.signbit(b))
+            assert np.signbit(mn) == (np.signbit(a) or np.signbit(b))
+    hi = np.full(5000, 1.25, np.float32)
+    c = ol.math2f(2, x, hi)
+    want = np.where(np.isnan(x), np.float32(-0.5), np.clip(x, -0.5, 1.25))
+    assert np.array_equal(c, want)
