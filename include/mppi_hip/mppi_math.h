@@ -272,23 +272,32 @@ MPPI_HD float mppi_cos_poly(float r) {
     return mppi_fma(r2, p, 1.0f);
 }
 
-// Select (sin, cos) of r + q*pi/2 from the kernels' values, without
-// branches: odd quadrants swap sin and cos, quadrants 2 and 3 negate sin,
-// quadrants 1 and 2 negate cos.
-MPPI_HD void mppi_quadrant(int q, float s, float c, float* so, float* co) {
-    const bool swap = (q & 1) != 0;
-    const float sa = swap ? c : s;
-    const float ca = swap ? s : c;
-    *so = (q & 2) ? -sa : sa;
-    *co = ((q + 1) & 2) ? -ca : ca;
+// Select (sin, cos) of r + q*pi/2 from the kernels' values in bit
+// operations only (no comparison, so no condition-mask hazards on the GPU):
+// odd quadrants swap sin and cos, quadrants 2 and 3 negate sin, quadrants 1
+// and 2 negate cos.  Only the low two bits of q are used.
+MPPI_HD void mppi_quadrant(uint32_t q, float s, float c, float* so, float* co) {
+    const uint32_t swap = 0u - (q & 1u);                 // all ones in odd quadrants
+    const uint32_t su = mppi_f2u(s), cu = mppi_f2u(c);
+    const uint32_t sa = (cu & swap) | (su & ~swap);
+    const uint32_t ca = (su & swap) | (cu & ~swap);
+    const uint32_t t = q << 30;                          // bit 1 of q at the sign bit
+    *so = mppi_u2f(sa ^ (t & 0x80000000u));
+    *co = mppi_u2f(ca ^ ((t + 0x40000000u) & 0x80000000u));
 }
 
-// The quadrant (mod 4) of an integral float k.  Converting a float beyond
-// the int range is undefined in C++ and differs between targets (x86 gives
-// INT_MIN, the GPU saturates), so such k (and NaN) map to 0: every float
-// with |k| >= 2^26 is a multiple of 4 anyway.
-MPPI_HD int mppi_quadrant_index(float k) {
-    return mppi_fabsf(k) < 67108864.0f ? int(k) : 0;
+// k = round(v) (ties to even) by adding and subtracting 1.5 * 2^23, with the
+// intermediate sum y returned as well: while |v| < 2^22, y lies in
+// [2^23, 2^24), where floats are the integers, and its low mantissa bits are
+// 2^22 + k, so mppi_f2u(y) holds k mod 4 in its low two bits (2^22 is a
+// multiple of 4).  Beyond, k and those bits are what the same operations
+// give on every target (no float-to-int conversion, whose out-of-range
+// result differs between targets).
+MPPI_HD float mppi_rint_bits(float v, uint32_t* bits) {
+    const float magic = 12582912.0f;  // 1.5 * 2^23
+    const float y = v + magic;
+    *bits = mppi_f2u(y);
+    return y - magic;
 }
 
 // sin(x), cos(x).  Accurate for |x| <= 1e5 (the three-part pi/2 keeps the
@@ -310,11 +319,12 @@ MPPI_HD void mppi_sincosf(float x, float* s, float* c) {
     const float pio2_1 = 1.57079625129699707031e+00f;  // 0x3fc90fda
     const float pio2_2 = 7.54978941586159635335e-08f;  // 0x33a22168
     const float pio2_3 = 5.39030252995776476554e-15f;  // 0x27c234c4
-    float kf = mppi_rintf(x * two_over_pi);
+    uint32_t q;
+    const float kf = mppi_rint_bits(x * two_over_pi, &q);
     float r = mppi_fma(-kf, pio2_1, x);
     r = mppi_fma(-kf, pio2_2, r);
     r = mppi_fma(-kf, pio2_3, r);
-    mppi_quadrant(mppi_quadrant_index(kf), mppi_sin_poly(r), mppi_cos_poly(r), s, c);
+    mppi_quadrant(q, mppi_sin_poly(r), mppi_cos_poly(r), s, c);
 }
 
 MPPI_HD float mppi_sinf(float x) { float s, c; mppi_sincosf(x, &s, &c); return s; }
@@ -329,10 +339,11 @@ MPPI_HD void mppi_sincospif(float x, float* s, float* c) {
         return;
     }
 #endif
-    float n = mppi_rintf(x * 2.0f);                // exact scaling
+    uint32_t q;
+    const float n = mppi_rint_bits(x * 2.0f, &q);  // exact scaling
     float f = x - n * 0.5f;                        // exact: f in [-1/4, 1/4]
     const float pi_f = 3.14159274101257324219f;
-    mppi_quadrant(mppi_quadrant_index(n), mppi_sin_poly(f * pi_f), mppi_cos_poly(f * pi_f), s, c);
+    mppi_quadrant(q, mppi_sin_poly(f * pi_f), mppi_cos_poly(f * pi_f), s, c);
 }
 
 // ---------------------------------------------------------------------------

@@ -182,7 +182,7 @@ MPPI_HD void mppi_box_muller(uint32_t a, uint32_t b, float* z0, float* z1) {
     const uint32_t m = (b >> 8) | 1u;
     const uint32_t n = (m + 0x200000u) >> 22;
     const float fpi = float(int32_t(m) - int32_t(n << 22)) * (3.14159274101257324219f * 0x1p-23f);
-    mppi_quadrant(int(n), mppi_sin_poly(fpi), mppi_cos_poly(fpi), &s, &c);
+    mppi_quadrant(n, mppi_sin_poly(fpi), mppi_cos_poly(fpi), &s, &c);
 #endif
     *z0 = r * c;
     *z1 = r * s;
