@@ -16,7 +16,7 @@
 //   Fyf  = Cf * (steer - (vy + lf r) / u),  Fyr = Cr * (-(vy - lr r) / u),
 //   u    = max(vx, v_min)  (linearised slip angles)
 // integrated with `substeps` explicit Euler steps of dt / substeps; vx is
-// clamped to [0, v_max].
+// clamped to [0, v_max] (min(max(vx, 0), v_max): a NaN gives 0).
 //
 // Stage cost (pursuit of a moving target among circular threat zones):
 //   w_pos |p - g(t)|^2 + w_u |u|^2 + w_zone * sum_i max(0, r_i - |p - c_i|)
@@ -56,7 +56,7 @@ MPPI_CALLABLE void bicycle_dynamics(float* s, const float* u, const uint8_t* mod
         float sp, cp;
         mppi_sincosf(s[2], &sp, &cp);
         const float vx = s[3], vy = s[4], r = s[5];
-        const float uu = vx > p->v_min ? vx : p->v_min;
+        const float uu = mppi_maxf(vx, p->v_min);
         const float inv_u = 1.0f / uu;
         const float fyf = p->cf * (steer - (vy + p->lf * r) * inv_u);
         const float fyr = p->cr * (-(vy - p->lr * r) * inv_u);
@@ -71,8 +71,7 @@ MPPI_CALLABLE void bicycle_dynamics(float* s, const float* u, const uint8_t* mod
         s[3] += h * dvx;
         s[4] += h * dvy;
         s[5] += h * dr;
-        if (s[3] < 0.0f) s[3] = 0.0f;
-        if (s[3] > p->v_max) s[3] = p->v_max;
+        s[3] = mppi_clampf(s[3], 0.0f, p->v_max);
     }
 }
 
