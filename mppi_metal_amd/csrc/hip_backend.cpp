@@ -1003,10 +1003,16 @@ int mppi_hip_step(mppi_hip_ctx* ctx, uint64_t step_index, const float* x0, const
     a.one = 1;
     a.desc0 = desc[0];
     a.step0 = uint32_t(step_index);
-    if ((rc = launch_iteration(ctx, desc[0], uint32_t(desc.size()), a))) return rc;
+    // The update, the diagnostics and the status word go straight to pinned
+    // host memory (the same layout as the device block): the device writes
+    // them over the bus, and they are visible to the host once the stream has
+    // synchronised, so a step needs no device-to-host copy.
     const size_t hc = size_t(ctx->H) * ctx->C;
-    MPPI_HIP_TRY(ctx, hipMemcpyAsync(ctx->down_host.ptr, ctx->down.ptr, sizeof(float) * (hc + 4),
-                                     hipMemcpyDeviceToHost, ctx->stream));
+    float* hout = static_cast<float*>(ctx->down_host.ptr);
+    a.u_out = hout;
+    a.diag = hout + hc;
+    a.status_out = reinterpret_cast<uint32_t*>(hout + hc + 3);
+    if ((rc = launch_iteration(ctx, desc[0], uint32_t(desc.size()), a))) return rc;
     if ((rc = finish(ctx))) return rc;
     const float* h = static_cast<const float*>(ctx->down_host.ptr);
     if ((rc = check_status(ctx, reinterpret_cast<const uint32_t*>(h + hc + 3), 1))) return rc;
