@@ -1537,7 +1537,7 @@ extern "C" __global__ void __launch_bounds__(MPPI_SUR_BLOCK) mppi_rollout_surrog
                         acc[mb] = mppi_mfma(mppi_frag(s_w1, mb, lane), xb,
                                             *reinterpret_cast<const mppi_v4f*>(s_b + 16 * mb + 4 * g));
 #pragma unroll
-                    for (int mb = 0; mb < 8; ++mb) h1[nb][mb] = mppi_act4(acc[mb]);
+                    for (int mb = 0; mb < 8; mb += 2) mppi_act8(acc[mb], acc[mb + 1], &h1[nb][mb], &h1[nb][mb + 1]);
                 }
                 // layers 2 and 3
                 mppi_v4f y[4] = {zero, zero, zero, zero};
@@ -1561,7 +1561,7 @@ extern "C" __global__ void __launch_bounds__(MPPI_SUR_BLOCK) mppi_rollout_surrog
                             for (int nb = 0; nb < 4; ++nb)
                                 acc[nb] = mppi_mfma32(w[kb], mppi_cat8(h1[nb][2 * kb], h1[nb][2 * kb + 1]), acc[nb]);
 #pragma unroll
-                        for (int nb = 0; nb < 4; ++nb) h2[nb][half] = mppi_act4(acc[nb]);
+                        for (int nb = 0; nb < 4; nb += 2) mppi_act8(acc[nb], acc[nb + 1], &h2[nb][half], &h2[nb + 1][half]);
                     }
                     const mppi_v8s w3 = *reinterpret_cast<const mppi_v8s*>(
                         s_wx + (MPPI_MLP_X32_W3 + kb3) * MPPI_MLP_X32_ELEMS + lane * 8);

@@ -141,27 +141,28 @@ __device__ inline mppi_v4s mppi_pack_f16(mppi_v4f v) {
     return __builtin_bit_cast(mppi_v4s, __builtin_convertvector(v, mppi_v4h));
 }
 
-// y = clamp01(r16(z)) of two pre-activations as a binary16 pair (z0 in the
-// low half): one packed conversion to nearest even with its output clamp
-// (v_cvt_pk_f16_f32 ... clamp; NaN clamps to 0).  The maximum and minimum
-// below are that clamp: they fold into the conversion, and have its meaning
-// where they do not.
-__device__ inline mppi_v2h mppi_act_y2(float z0, float z1) {
-    typedef float f2 __attribute__((ext_vector_type(2)));
-    const mppi_v2h h = __builtin_convertvector((f2){z0, z1}, mppi_v2h);
-    return __builtin_elementwise_min(__builtin_elementwise_max(h, (mppi_v2h){0, 0}), (mppi_v2h){1, 1});
+// y = clamp01(r16(z)) of N pre-activations as binary16 (N even): packed
+// conversions to nearest even with their output clamp (v_cvt_pk_f16_f32 ...
+// clamp; NaN clamps to 0).  The maximum and minimum below are that clamp:
+// they fold into the conversion, and have its meaning where they do not.
+template <typename VF, typename VH>
+__device__ inline VH mppi_act_y(VF z) {
+    const VH h = __builtin_convertvector(z, VH);
+    return __builtin_elementwise_min(__builtin_elementwise_max(h, (VH)(_Float16)0), (VH)(_Float16)1);
 }
-// Q(y) of a binary16 pair.
-__device__ inline uint32_t mppi_act_s2(mppi_v2h y) {
-    const mppi_v2h s = __builtin_elementwise_fma(y, (mppi_v2h){2, 2}, (mppi_v2h){-1, -1});
-    const mppi_v2h v = __builtin_elementwise_fma(s, s, (mppi_v2h){-0.5, -0.5});
-    const _Float16 c0 = MPPI_MLP_TANH_C0, c1 = MPPI_MLP_TANH_C1, c2 = MPPI_MLP_TANH_C2, c3 = MPPI_MLP_TANH_C3,
-                   c4 = MPPI_MLP_TANH_C4;
-    mppi_v2h p = __builtin_elementwise_fma(v, (mppi_v2h){c4, c4}, (mppi_v2h){c3, c3});
-    p = __builtin_elementwise_fma(v, p, (mppi_v2h){c2, c2});
-    p = __builtin_elementwise_fma(v, p, (mppi_v2h){c1, c1});
-    p = __builtin_elementwise_fma(v, p, (mppi_v2h){c0, c0});
-    return __builtin_bit_cast(uint32_t, s * p);
+// Q(y) of N binary16 values.  With N = 4 or 8 every step is two or four
+// independent packed instructions, so a dependent one never follows right
+// after the instruction that writes its operand (which on gfx950 costs a
+// wait state per packed binary16 instruction of a single pair's chain).
+template <typename VH>
+__device__ inline VH mppi_act_q(VH y) {
+    const VH s = __builtin_elementwise_fma(y, (VH)(_Float16)2, (VH)(_Float16)-1);
+    const VH v = __builtin_elementwise_fma(s, s, (VH)(_Float16)-0.5);
+    VH p = __builtin_elementwise_fma(v, (VH)(_Float16)MPPI_MLP_TANH_C4, (VH)(_Float16)MPPI_MLP_TANH_C3);
+    p = __builtin_elementwise_fma(v, p, (VH)(_Float16)MPPI_MLP_TANH_C2);
+    p = __builtin_elementwise_fma(v, p, (VH)(_Float16)MPPI_MLP_TANH_C1);
+    p = __builtin_elementwise_fma(v, p, (VH)(_Float16)MPPI_MLP_TANH_C0);
+    return s * p;
 }
 // dT/dz at a binary16 y (training's backward pass, FP32): 2 (P(v) + 2 s^2 P'(v))
 // for y in (0, 1), 0 at the clamp.
@@ -171,22 +172,31 @@ __device__ inline float mppi_act_deriv(float y) {
     const float dp = MPPI_MLP_TANH_C1 + v * (2.0f * MPPI_MLP_TANH_C2 + v * (3.0f * MPPI_MLP_TANH_C3 + v * (4.0f * MPPI_MLP_TANH_C4)));
     return (y > 0.0f && y < 1.0f) ? 2.0f * (p + 2.0f * s * s * dp) : 0.0f;
 }
+typedef float mppi_v2f __attribute__((ext_vector_type(2)));
+typedef float mppi_v8f __attribute__((ext_vector_type(8)));
 // T of two pre-activations, as a pair of binary16 (z0 in the low half).
 __device__ inline uint32_t mppi_act2(float z0, float z1) {
-    return mppi_act_s2(mppi_act_y2(z0, z1));
+    return __builtin_bit_cast(uint32_t, mppi_act_q(mppi_act_y<mppi_v2f, mppi_v2h>((mppi_v2f){z0, z1})));
 }
 
 __device__ inline mppi_v4s mppi_act4(mppi_v4f z) {
-    const uint32_t a = mppi_act2(z[0], z[1]), b = mppi_act2(z[2], z[3]);
-    return __builtin_bit_cast(mppi_v4s, (uint64_t(b) << 32) | a);
+    return __builtin_bit_cast(mppi_v4s, mppi_act_q(mppi_act_y<mppi_v4f, mppi_v4h>(z)));
+}
+
+// T of two tiles of four pre-activations at once (four independent packed
+// instructions per step).
+__device__ inline void mppi_act8(mppi_v4f za, mppi_v4f zb, mppi_v4s* ta, mppi_v4s* tb) {
+    const mppi_v8f z = __builtin_shufflevector(za, zb, 0, 1, 2, 3, 4, 5, 6, 7);
+    const mppi_v8h t = mppi_act_q(mppi_act_y<mppi_v8f, mppi_v8h>(z));
+    *ta = __builtin_bit_cast(mppi_v4s, __builtin_shufflevector(t, t, 0, 1, 2, 3));
+    *tb = __builtin_bit_cast(mppi_v4s, __builtin_shufflevector(t, t, 4, 5, 6, 7));
 }
 
 // T of four pre-activations and their y (for the derivative, mppi_act_deriv).
 __device__ inline mppi_v4s mppi_act4_y(mppi_v4f z, mppi_v4s* y_out) {
-    const mppi_v2h ya = mppi_act_y2(z[0], z[1]), yb = mppi_act_y2(z[2], z[3]);
-    *y_out = __builtin_bit_cast(mppi_v4s, __builtin_shufflevector(ya, yb, 0, 1, 2, 3));
-    const uint32_t a = mppi_act_s2(ya), b = mppi_act_s2(yb);
-    return __builtin_bit_cast(mppi_v4s, (uint64_t(b) << 32) | a);
+    const mppi_v4h y = mppi_act_y<mppi_v4f, mppi_v4h>(z);
+    *y_out = __builtin_bit_cast(mppi_v4s, y);
+    return __builtin_bit_cast(mppi_v4s, mppi_act_q(y));
 }
 
 // v_mfma_f32_16x16x16_f16 (forward pass).
