@@ -1437,45 +1437,45 @@ extern "C" __global__ void __launch_bounds__(256) mppi_surrogate_data(mppi_surro
 
 // Rollout with the MLP as dynamics (one sample per lane, MPPI_SUR_BLOCK-thread
 // workgroups, grid (ceil(K / MPPI_SUR_BLOCK), N); the reduce kernel is shared).  Per
-// timestep every wavefront evaluates the network for its 64 samples as four
-// 16-sample column blocks:
-//   layer 1   v_mfma_f32_16x16x16_f16, one per (m-block, column block), the
-//             inputs staged per sample in the wavefront's LDS rows;
-//   layer 2   v_mfma_f32_16x16x32_f16: each A fragment is read from LDS
-//             once and applied to the four column blocks; its B operands are
-//             the layer-1 results a lane already holds (mppi_mlp.h);
-//   layer 3   the same, accumulated as layer 2's m-blocks come out in pairs,
-//             so that only two m-blocks of the second hidden layer are live;
-// activations on packed binary16 pairs in registers (mppi_act2).  The staging rows are
-// private to a wavefront, whose LDS operations complete in order, so a
-// timestep needs wavefront-level ordering only (no workgroup barrier).
+// timestep every wavefront evaluates the network for its 64 samples as two
+// 32-sample column blocks on 32x32 tiles (mppi_mlp_net in mppi_mlp.h):
+//   layer 1   v_mfma_f32_32x32x8_f16, one per (m-block of 32 units, column
+//             block, eight inputs), its B operands the lanes' own inputs
+//             after one exchange of lane halves;
+//   layer 2   v_mfma_f32_32x32x16_f16: each A fragment is read from LDS once
+//             and applied to both column blocks; its B operands are the
+//             layer-1 results a lane already holds;
+//   layer 3   the same (v_mfma_f32_16x16x32_f16 with at most eight outputs),
+//             accumulated as each m-block of layer 2 comes out;
+// activations on packed binary16 values in registers, which the vector unit
+// runs while the matrix core works through the products they do not feed.
+// The state deltas go through the wavefront's own LDS rows (s_dx), whose
+// operations complete in order, so a timestep needs wavefront-level ordering
+// only (no workgroup barrier).
 
 extern "C" __global__ void __launch_bounds__(MPPI_SUR_BLOCK) mppi_rollout_surrogate(mppi_surrogate_rollout_args sa) {
     constexpr int IN = SDIM + CDIM;
     constexpr int WAVES = MPPI_SUR_BLOCK / 64;
+    constexpr int KB1 = IN <= 8 ? 1 : 2;          // layer-1 K-blocks of eight inputs
     static_assert(IN <= 16 && SDIM <= 16, "surrogate network: inputs and outputs of at most 16");
-    __shared__ __attribute__((aligned(16))) short s_w1[8 * MPPI_MLP_FRAG_ELEMS];
-    __shared__ __attribute__((aligned(16))) short s_wx[MPPI_MLP_X32_FRAGS * MPPI_MLP_X32_ELEMS];
-    __shared__ __attribute__((aligned(16))) float s_b[2 * MPPI_MLP_HID + 16];
-    __shared__ __attribute__((aligned(16))) short s_in[WAVES][64][16];
+    __shared__ __attribute__((aligned(16))) short s_wy[MPPI_MLP_Y_ELEMS];
+    __shared__ __attribute__((aligned(16))) float s_b[2 * MPPI_MLP_HID + 32];
     __shared__ float s_dx[WAVES][64][SDIM + 1];
     const mppi_step_args& a = sa.s;
     const uint32_t agent = blockIdx.y;
     const mppi_agent_desc d = a.desc[agent];
     if (blockIdx.x * MPPI_SUR_BLOCK >= d.K) return;
-    for (uint32_t e = threadIdx.x; e < 8 * MPPI_MLP_FRAG_ELEMS / 8; e += MPPI_SUR_BLOCK)
-        reinterpret_cast<uint4*>(s_w1)[e] = reinterpret_cast<const uint4*>(sa.frags + MPPI_MLP_FRAG_W1 * MPPI_MLP_FRAG_ELEMS)[e];
-    for (uint32_t e = threadIdx.x; e < MPPI_MLP_X32_FRAGS * MPPI_MLP_X32_ELEMS / 8; e += MPPI_SUR_BLOCK)
-        reinterpret_cast<uint4*>(s_wx)[e] = reinterpret_cast<const uint4*>(sa.frags + MPPI_MLP_X32_BASE)[e];
+    for (uint32_t e = threadIdx.x; e < MPPI_MLP_Y_ELEMS / 8; e += MPPI_SUR_BLOCK)
+        reinterpret_cast<uint4*>(s_wy)[e] = reinterpret_cast<const uint4*>(sa.frags + MPPI_MLP_Y_BASE)[e];
     for (uint32_t e = threadIdx.x; e < MPPI_MLP_HID; e += MPPI_SUR_BLOCK) {
         s_b[e] = sa.params[sa.b1 + e];
         s_b[MPPI_MLP_HID + e] = sa.params[sa.b2 + e];
     }
-    if (threadIdx.x < 16) s_b[2 * MPPI_MLP_HID + threadIdx.x] = threadIdx.x < SDIM ? sa.params[sa.b3 + threadIdx.x] : 0.0f;
+    if (threadIdx.x < 32) s_b[2 * MPPI_MLP_HID + threadIdx.x] = threadIdx.x < SDIM ? sa.params[sa.b3 + threadIdx.x] : 0.0f;
     mppi_prepare_costs(a.cost + d.slot * a.cost_stride, MPPI_SUR_BLOCK);
     __syncthreads();
 
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, h = lane >> 5, c = lane & 31;
     const uint32_t blk = blockIdx.x;
     const uint32_t k = blk * MPPI_SUR_BLOCK + threadIdx.x;
     const uint32_t H = d.H;
@@ -1486,7 +1486,6 @@ extern "C" __global__ void __launch_bounds__(MPPI_SUR_BLOCK) mppi_rollout_surrog
     const float* __restrict__ un = a.u_nom + d.seq_off;
     const uint8_t* model = a.model + slot * a.model_stride;
     const uint8_t* cost = a.cost + slot * a.cost_stride;
-    const mppi_v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
 
     float x[SDIM];
 #pragma unroll
@@ -1508,9 +1507,9 @@ extern "C" __global__ void __launch_bounds__(MPPI_SUR_BLOCK) mppi_rollout_surrog
                 float u[CDIM];
                 mppi_make_control(un + t * CDIM, z + gi * CDIM, a.noise_scale, a.u_min, a.u_max, u);
                 // this lane's sample: normalised inputs as FP16, zeros past IN
-                mppi_v4f xin[4];
+                mppi_v4f xin[2 * KB1];
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
+                for (int q = 0; q < 2 * KB1; ++q)
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int kk = 4 * q + j;
@@ -1520,62 +1519,48 @@ extern "C" __global__ void __launch_bounds__(MPPI_SUR_BLOCK) mppi_rollout_surrog
                                                              sa.in_inv_half[kk], sa.in_period[kk], sa.in_inv_period[kk])
                                             : 0.0f;
                     }
+                // layer-1 B operands: lane l holds inputs 8 kb1 + 4 h .. + 3 of sample
+                // 32 nb + c, so lanes 32-63 of column block 0 take the second four
+                // inputs of samples 0-31, and lanes 0-31 of column block 1 the first
+                // four of samples 32-63: one exchange of lane halves per register
+                // (v_permlane32_swap: the upper half of the first operand with the
+                // lower half of the second)
+                mppi_v4s xb[2][KB1];
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    *reinterpret_cast<mppi_v4s*>(&s_in[wave][lane][4 * q]) = mppi_pack_f16(xin[q]);
-                __builtin_amdgcn_wave_barrier();
-                // layer 1: h1[nb][mb] = T(W1 x + b1) for column block nb
-                mppi_v4s h1[4][8];
-#pragma unroll
-                for (int nb = 0; nb < 4; ++nb) {
-                    const mppi_v4s xb = *reinterpret_cast<const mppi_v4s*>(&s_in[wave][16 * nb + c][4 * g]);
-                    // the eight products first, then their activations, so
-                    // that the activations overlap the matrix cores
-                    mppi_v4f acc[8];
-#pragma unroll
-                    for (int mb = 0; mb < 8; ++mb)
-                        acc[mb] = mppi_mfma(mppi_frag(s_w1, mb, lane), xb,
-                                            *reinterpret_cast<const mppi_v4f*>(s_b + 16 * mb + 4 * g));
-#pragma unroll
-                    for (int mb = 0; mb < 8; mb += 2) mppi_act8(acc[mb], acc[mb + 1], &h1[nb][mb], &h1[nb][mb + 1]);
+                for (int kb1 = 0; kb1 < KB1; ++kb1) {
+                    const mppi_v2u lo = __builtin_bit_cast(mppi_v2u, mppi_pack_f16(xin[2 * kb1]));
+                    const mppi_v2u hi = __builtin_bit_cast(mppi_v2u, mppi_pack_f16(xin[2 * kb1 + 1]));
+                    const auto r0 = __builtin_amdgcn_permlane32_swap(lo.x, hi.x, false, false);
+                    const auto r1 = __builtin_amdgcn_permlane32_swap(lo.y, hi.y, false, false);
+                    xb[0][kb1] = __builtin_bit_cast(mppi_v4s, mppi_v2u{r0[0], r1[0]});
+                    xb[1][kb1] = __builtin_bit_cast(mppi_v4s, mppi_v2u{r0[1], r1[1]});
                 }
-                // layers 2 and 3
-                mppi_v4f y[4] = {zero, zero, zero, zero};
+                if constexpr (SDIM <= 8) {
+                    // the state deltas: register i holds output row 4 (g % 2) + i of
+                    // sample 32 nb + lane % 16 + 16 (g / 2), g = lane / 16
+                    mppi_v4f y[2];
+                    mppi_mlp_net<KB1, true>(s_wy, s_b, xb, lane, y);
+                    const uint32_t g = lane >> 4;
 #pragma unroll
-                for (int kb3 = 0; kb3 < 4; ++kb3) {
-                    mppi_v4s h2[4][2];
+                    for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
-                    for (int half = 0; half < 2; ++half) {
-                        const int mb = 2 * kb3 + half;
-                        mppi_v8s w[4];
+                        for (int i = 0; i < 4; ++i) {
+                            const uint32_t r = 4 * (g & 1) + i;
+                            if (r < SDIM) s_dx[wave][32 * nb + (lane & 15) + 16 * (g >> 1)][r] = y[nb][i];
+                        }
+                } else {
+                    // register i holds output row 8 (i / 4) + 4 h + i % 4 of sample 32 nb + c
+                    mppi_v16f y[2];
+                    mppi_mlp_net<KB1, false>(s_wy, s_b, xb, lane, y);
 #pragma unroll
-                        for (int kb = 0; kb < 4; ++kb)
-                            w[kb] = *reinterpret_cast<const mppi_v8s*>(
-                                s_wx + (MPPI_MLP_X32_W2 + mb * 4 + kb) * MPPI_MLP_X32_ELEMS + lane * 8);
-                        const mppi_v4f bias = *reinterpret_cast<const mppi_v4f*>(s_b + MPPI_MLP_HID + 16 * mb + 4 * g);
-                        // four independent accumulation chains (column blocks)
-                        mppi_v4f acc[4] = {bias, bias, bias, bias};
+                    for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
-                        for (int kb = 0; kb < 4; ++kb)
-#pragma unroll
-                            for (int nb = 0; nb < 4; ++nb)
-                                acc[nb] = mppi_mfma32(w[kb], mppi_cat8(h1[nb][2 * kb], h1[nb][2 * kb + 1]), acc[nb]);
-#pragma unroll
-                        for (int nb = 0; nb < 4; nb += 2) mppi_act8(acc[nb], acc[nb + 1], &h2[nb][half], &h2[nb + 1][half]);
-                    }
-                    const mppi_v8s w3 = *reinterpret_cast<const mppi_v8s*>(
-                        s_wx + (MPPI_MLP_X32_W3 + kb3) * MPPI_MLP_X32_ELEMS + lane * 8);
-#pragma unroll
-                    for (int nb = 0; nb < 4; ++nb) y[nb] = mppi_mfma32(w3, mppi_cat8(h2[nb][0], h2[nb][1]), y[nb]);
+                        for (int i = 0; i < 16; ++i) {
+                            if (8 * (i / 4) >= SDIM) continue;
+                            const uint32_t r = 8 * (i / 4) + 4 * h + (i % 4);
+                            if (r < SDIM) s_dx[wave][32 * nb + c][r] = y[nb][i];
+                        }
                 }
-                // the state deltas: output row 4 g + j of sample 16 nb + c
-#pragma unroll
-                for (int nb = 0; nb < 4; ++nb)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const int r = 4 * g + j;
-                        if (r < SDIM) s_dx[wave][16 * nb + c][r] = y[nb][j] + s_b[2 * MPPI_MLP_HID + r];
-                    }
                 __builtin_amdgcn_wave_barrier();
 #pragma unroll
                 for (int i = 0; i < SDIM; ++i) x[i] += s_dx[wave][lane][i] * sa.out_sd[i] + sa.out_mu[i];

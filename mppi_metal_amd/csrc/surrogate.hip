@@ -255,22 +255,39 @@ extern "C" __global__ void __launch_bounds__(256) mppi_mlp_adam(adam_args a) {
     a.params[p] -= a.lr_t * m / (__builtin_sqrtf(v) + a.eps);
 }
 
-// One thread per fragment element (152 x16 fragments of 256, then 36 x32
-// fragments of 512).
+// Column order of K-block kb of the 32x32 fragments: the order in which a
+// 32x32 result tile's registers hold its rows (mppi_mlp.h).
+__device__ inline uint32_t mlp_y_unit(uint32_t kb, uint32_t p) {
+    return 16 * kb + 8 * ((p % 8) / 4) + 4 * (p / 8) + p % 4;
+}
+
+// One thread per fragment element (152 x16 fragments of 256, then the 32x32
+// set: 8 layer-1 fragments of 256, 32 + 8 fragments of 512).
 extern "C" __global__ void __launch_bounds__(256) mppi_mlp_pack(pack_args a) {
     const uint32_t e = blockIdx.x * 256 + threadIdx.x;
     if (e >= MPPI_MLP_TOTAL_ELEMS) return;
     const mppi_mlp_dims& d = a.dims;
     const float* P = a.params;
-    if (e >= MPPI_MLP_X32_BASE) {
-        const uint32_t q = e - MPPI_MLP_X32_BASE;
-        const uint32_t f = q / MPPI_MLP_X32_ELEMS, lane = (q / 8) % 64, j = q % 8;
-        const uint32_t i = lane % 16, g = lane / 16;          // A[i][8 g + j] of the fragment
-        const uint32_t kb = f % 4;
-        const uint32_t unit = 32 * kb + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
+    if (e >= MPPI_MLP_Y_BASE) {
+        const uint32_t q = e - MPPI_MLP_Y_BASE;
         float v = 0.0f;
-        if (f < MPPI_MLP_X32_W3) v = P[d.w2 + (16 * (f / 4) + i) * HID + unit];      // W2X, mb = f / 4
-        else if (i < d.out) v = P[d.w3 + i * HID + unit];                              // W3X
+        if (q < MPPI_MLP_Y_W2) {                      // W1Y, kb1 * 4 + mb
+            const uint32_t f = q / 256, lane = (q / 4) % 64, j = q % 4;
+            const uint32_t i = lane % 32, k = 8 * (f / 4) + 4 * (lane / 32) + j;
+            if (k < d.in) v = P[d.w1 + (32 * (f % 4) + i) * d.in + k];
+        } else {
+            const uint32_t r = q < MPPI_MLP_Y_W3 ? q - MPPI_MLP_Y_W2 : q - MPPI_MLP_Y_W3;
+            const uint32_t f = r / 512, lane = (r / 8) % 64, j = r % 8;
+            const uint32_t i = lane % 32, p = 8 * (lane / 32) + j;
+            if (q < MPPI_MLP_Y_W3) {
+                v = P[d.w2 + (32 * (f / 8) + i) * HID + mlp_y_unit(f % 8, p)];                     // W2Y, mb * 8 + kb
+            } else if (d.out <= 8) {                                                               // W3Z, kb
+                const uint32_t m = lane % 16, g = lane / 16;
+                if (m / 8 == g % 2 && m % 8 < d.out) v = P[d.w3 + (m % 8) * HID + mlp_y_unit(f, 8 * (g / 2) + j)];
+            } else if (i < d.out) {
+                v = P[d.w3 + i * HID + mlp_y_unit(f, p)];                                          // W3Y, kb
+            }
+        }
         a.frags[e] = mppi_f2h(v);
         return;
     }
