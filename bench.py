@@ -69,7 +69,7 @@ MFMA_PEAK_16 = 2.5e15
 HBM_PEAK_BPS = 8.0e12
 COUNTERS = ROOT / "profiles" / "mppi_counters_unicycle_pursuit_surrogate.md"
 # Issue ceilings measured on this device at start-up (mppi_hip_calibrate):
-# plain FP32 VALU lane-ops/s and FP16 16x16x32 matrix flops/s.
+# plain FP32 VALU lane-ops/s and FP16 32x32x16 matrix flops/s.
 CEILING = {"valu": None, "mfma": None, "mixed_over_sum": None}
 
 
@@ -84,7 +84,7 @@ def calibrate():
         raise RuntimeError(f"mppi_hip_calibrate failed ({rc})")
     CEILING["valu"], CEILING["mfma"], CEILING["mixed_over_sum"] = v.value, m.value, x.value
     print(f"[bench] measured ceilings: VALU {v.value / 1e12:.2f} T lane-op/s (plain FP32), "
-          f"MFMA {m.value / 1e12:.1f} TFLOP/s (fp16 16x16x32), VALU+MFMA in one stream "
+          f"MFMA {m.value / 1e12:.1f} TFLOP/s (fp16 32x32x16), one MFMA per 16 FMAs in one stream "
           f"{x.value:.2f} of their separate times", file=sys.stderr, flush=True)
 
 
@@ -94,7 +94,7 @@ def against_ceiling(rate: float, kind: str) -> dict:
     c = CEILING[kind]
     unit = "T lane-op/s" if kind == "valu" else "TFLOP/s"
     how = ("plain (unpacked) FP32 FMAs, independent streams, 8 wavefronts per SIMD" if kind == "valu"
-           else "v_mfma_f32_16x16x32_f16, independent accumulators, 8 wavefronts per SIMD")
+           else "v_mfma_f32_32x32x16_f16, independent accumulators, 8 wavefronts per SIMD")
     return {"measured": None if c is None else c / 1e12, "unit": unit,
             "frac": None if c is None else rate / c, "how": "mppi_hip_calibrate: " + how}
 

@@ -455,14 +455,15 @@ const char* mppi_hip_version(void);
 
 /* Measured issue ceilings of `device` for roofline figures: the rate at
  * which its SIMDs retire plain (unpacked) FP32 vector instructions, in lane
- * operations per second (an FMA counts once), and 16x16x32 FP16 matrix
- * instructions, in flops per second, each from long streams of independent
- * instructions of that kind at eight wavefronts per SIMD; and, in
- * mixed_over_sum (may be NULL), the time of one stream holding both kinds
- * over the sum of the times of its two parts alone (1: the SIMD does not
- * overlap them; 0.5: fully).  About 40 ms.  No reference counterpart
- * (measurement only).  The published FP32 vector peak counts packed FP32
- * instructions, which this library does not emit. */
+ * operations per second (an FMA counts once), and 32x32x16 FP16 matrix
+ * instructions (the surrogate rollout's products), in flops per second, each
+ * from long streams of independent instructions of that kind at eight
+ * wavefronts per SIMD; and, in mixed_over_sum (may be NULL), the time of a
+ * stream with one matrix instruction after every sixteen FP32 FMAs over the
+ * sum of the times of its two parts alone (1: the SIMD does not overlap
+ * them; 0.5: fully).  About 40 ms.  No reference counterpart (measurement
+ * only).  The published FP32 vector peak counts packed FP32 instructions,
+ * which this library does not emit. */
 int mppi_hip_calibrate(int device, double* valu_lane_ops_per_s, double* mfma_f16_flops_per_s,
                        double* mixed_over_sum);
 

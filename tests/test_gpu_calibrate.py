@@ -16,9 +16,10 @@ def test_calibrate_reports_plausible_ceilings(require_gpu):
     print(f"[calibrate] valu={v.value / 1e12:.2f} T lane-op/s mfma={m.value / 1e12:.1f} TFLOP/s "
           f"mixed/sum={x.value:.3f}", flush=True)
     # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3 T; the published packed
-    # peak 78.65 T lane-ops and 2.5 PFLOP/s bound them from above
+    # peak 78.65 T lane-ops and 2.5 PFLOP/s (at 2.4 GHz, so with a little slack
+    # for the clock) bound them from above
     assert 10e12 < v.value <= 78.65e12
-    assert 300e12 < m.value <= 2.5e15
+    assert 300e12 < m.value <= 2.6e15
     # a stream of both kinds takes between the longer part and the sum of both
     assert 0.45 < x.value < 1.1
     assert _native.STATUS_NAMES[lib.mppi_hip_calibrate(0, None, ctypes.byref(m), None)] == "E_INVALID"
