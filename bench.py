@@ -144,10 +144,12 @@ def bicycle_w_ss(substeps: int = 2, zones: int = 32, C: int = 2, one_launch: boo
 
 
 # Matrix-core flops per sample-step of the 8-128-128-6 surrogate: the
-# algorithm's 2 (8*128 + 128*128 + 128*6) and, with the input and output
-# padded to 16-wide tiles, the issued 2 (16*128 + 128*128 + 128*16).
+# algorithm's 2 (8*128 + 128*128 + 128*6) and the issued ones: layers 1 and 2
+# exactly (32x32x8 and 32x32x16 tiles), the output layer as 16x16x32
+# products that hold 8 output rows and, per sample, 16 of their 32 K values
+# (2 * 16 * 128).
 SURROGATE_FLOPS = 2 * (8 * 128 + 128 * 128 + 128 * 6)
-SURROGATE_FLOPS_ISSUED = 2 * (16 * 128 + 128 * 128 + 128 * 16)
+SURROGATE_FLOPS_ISSUED = 2 * (8 * 128 + 128 * 128 + 16 * 128)
 
 
 def profile_line(key: str) -> dict:
