@@ -15,7 +15,12 @@ the forward pass, bf16 operands in the backward pass):
                     operands)
   surrogate rollout sample costs within 2e-4 relative for 99% of samples and
                     1e-3 for all, over a 16-step horizon (the same rare
-                    boundary crossings, compounded)
+                    boundary crossings, compounded); the same bars for
+                    imported random networks on the unicycle (6 inputs, 4
+                    outputs) and the quadrotor (16 inputs, two layer-1
+                    K-blocks; 12 outputs, the 32x32 output layer; full
+                    covariance), whose state deltas are scaled down so that
+                    an untrained network keeps the rollout in range
   training          the loss falls by 50x and the one-step relative RMSE of
                     every state component on unseen transitions is < 0.1
   closed loop       at the full workload (K = 2^18, H = 64, 500 steps, the
@@ -38,7 +43,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as ol
-from problems import BICYCLE_ENVELOPE, BICYCLE_TRAINING, bicycle, seq_rel_err
+from problems import BICYCLE_ENVELOPE, BICYCLE_TRAINING, bicycle, quadrotor, seq_rel_err, unicycle
 
 pytestmark = pytest.mark.gpu
 
@@ -377,3 +382,49 @@ def test_surrogate_gains_on_costly_dynamics(require_gpu):
             speedup=speed, mean_gap=float(np.mean(gaps)), gaps=" ".join(f"{g:+.4f}" for g in gaps))
     assert speed > 1.2
     assert np.mean(gaps) <= 0.01
+
+
+def _random_import(hip, D, C, seed):
+    """A random network of the plugin's shape with inputs scaled by 1/2, one
+    periodic input (component 2, period 2 pi) and state deltas of 0.02 per
+    unit network output."""
+    rng = np.random.default_rng(seed)
+    IN_, OUT_ = D + C, D
+    parts = []
+    for (o, i), scale, centre in (((HID, IN_), 1 / 3, 0.5), ((HID, HID), 1 / 3, 0.5), ((OUT_, HID), 1.0, 0.0)):
+        a = scale * np.sqrt(6.0 / (o + i))
+        parts += [rng.uniform(-a, a, (o, i)).astype(np.float32), rng.normal(centre, 0.1, o).astype(np.float32)]
+    P = _flat(parts)
+    norm = np.zeros(80, np.float32)
+    norm[0:IN_] = rng.uniform(-0.2, 0.2, IN_)
+    norm[16:16 + IN_] = 0.5
+    norm[32 + 2] = 2.0 * np.pi
+    norm[64:64 + OUT_] = 0.02
+    hip.surrogate_import(P, norm)
+    return P, norm
+
+
+@pytest.mark.parametrize("shape", ["unicycle", "quadrotor"])
+def test_surrogate_rollout_other_shapes(require_gpu, shape):
+    if shape == "unicycle":
+        hip, ora, sc = unicycle(1024, 16, 8)
+        x0, C = np.array(sc.x0, np.float32), 2
+    else:
+        hip, ora = quadrotor(1024, 16)
+        x0, C = np.zeros(12, np.float32), 4
+    D = x0.size
+    P, norm = _random_import(hip, D, C, 21)
+    rng = np.random.default_rng(8)
+    u_nom = (rng.standard_normal((16, C)) * 0.2).astype(np.float32)
+    hip.surrogate_use(True)
+    try:
+        hip.step(2, x0, u_nom)
+        c_g = hip.read_costs()
+    finally:
+        hip.surrogate_use(False)
+    c_o = ora.rollout_surrogate_range(2, x0, u_nom, 0, 1024, P, norm)
+    rel = np.abs(c_g.astype(np.float64) - c_o) / np.maximum(np.abs(c_o), 1e-30)
+    _report(f"surrogate rollout {shape}", q50=float(np.quantile(rel, 0.5)), q99=float(np.quantile(rel, 0.99)),
+            max=float(rel.max()))
+    assert np.all(np.isfinite(c_g)) and np.all(np.isfinite(c_o))
+    assert np.quantile(rel, 0.99) <= 2e-4 and rel.max() <= 1e-3
